@@ -1,0 +1,564 @@
+// Elemental's C++ and C interfaces for the dense factor-and-solve path, on top of the device
+// ABI (elemental_amd.h). Host matrices are staged in HBM per call; results, pivots and
+// failures come back with the reference's types and exceptions:
+//   El::LU                    src/lapack_like/factor/LU.cpp:104-142
+//   El::lu::SolveAfter        src/lapack_like/factor/LU/SolveAfter.hpp:66-92
+//   El::LinearSolve           src/lapack_like/solve/Linear.cpp:212-218
+//   El::Cholesky              src/lapack_like/factor/Cholesky.cpp:26-38
+//   El::cholesky::SolveAfter  src/lapack_like/factor/Cholesky/SolveAfter.hpp:15-43
+//   El::HPDSolve              src/lapack_like/solve/HPD.cpp:47-57
+//   C wrappers + EL_TRY       src/lapack_like/factor-C.cpp:70-135,268-323,
+//                             src/lapack_like/solve-C.cpp:18-46, include/El/core/CReflect.hpp:12-26
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/El.h"
+#include "../../include/El.hpp"
+#include "elx_common.hpp"
+
+namespace El {
+
+const Range ALL(0, END);
+
+void LogicError(const std::string& msg) { throw std::logic_error(msg); }
+void RuntimeError(const std::string& msg) { throw std::runtime_error(msg); }
+
+// ------------------------------------------------------------------ Matrix<T>
+template <typename T>
+Matrix<T>::Matrix(const Matrix<T>& A) : Matrix() {
+    Resize(A.Height(), A.Width());
+    for (Int j = 0; j < width_; ++j)
+        std::memcpy(buf_ + (int64_t)j * ldim_, A.LockedBuffer(0, j), sizeof(T) * height_);
+}
+
+template <typename T>
+Matrix<T>& Matrix<T>::operator=(const Matrix<T>& A) {
+    if (this == &A) return *this;
+    if (viewing_ && (height_ != A.Height() || width_ != A.Width()))
+        LogicError("Cannot resize a view in assignment");
+    if (!viewing_) Resize(A.Height(), A.Width());
+    for (Int j = 0; j < width_; ++j)
+        std::memcpy(Buffer() + (int64_t)j * ldim_, A.LockedBuffer(0, j), sizeof(T) * height_);
+    return *this;
+}
+
+template <typename T>
+void Matrix<T>::Empty() {
+    mem_.clear();
+    buf_ = nullptr;
+    height_ = width_ = 0;
+    ldim_ = 1;
+    viewing_ = locked_ = false;
+}
+
+template <typename T>
+void Matrix<T>::Resize(Int height, Int width) {
+    Resize(height, width, std::max(height, 1));
+}
+
+template <typename T>
+void Matrix<T>::Resize(Int height, Int width, Int ldim) {
+    if (height < 0 || width < 0) LogicError("Height and width must be non-negative");
+    if (ldim < std::max(height, 1)) LogicError("Leading dimension must be >= max(height,1)");
+    if (viewing_) {
+        if (height != height_ || width != width_) LogicError("Cannot resize a view");
+        return;
+    }
+    height_ = height;
+    width_ = width;
+    ldim_ = ldim;
+    mem_.assign((size_t)ldim * (size_t)width, T(0));
+    buf_ = mem_.empty() ? nullptr : mem_.data();
+}
+
+template <typename T>
+void Matrix<T>::Attach(Int height, Int width, T* buffer, Int ldim) {
+    if (ldim < std::max(height, 1)) LogicError("Leading dimension must be >= max(height,1)");
+    mem_.clear();
+    height_ = height;
+    width_ = width;
+    ldim_ = ldim;
+    buf_ = buffer;
+    viewing_ = true;
+    locked_ = false;
+}
+
+template <typename T>
+void Matrix<T>::LockedAttach(Int height, Int width, const T* buffer, Int ldim) {
+    Attach(height, width, const_cast<T*>(buffer), ldim);
+    locked_ = true;
+}
+
+template <typename T>
+Matrix<T> Matrix<T>::operator()(Range I, Range J) {
+    const Int i1 = (I.end == END) ? height_ : I.end, j1 = (J.end == END) ? width_ : J.end;
+    if (I.beg < 0 || i1 > height_ || I.beg > i1 || J.beg < 0 || j1 > width_ || J.beg > j1)
+        LogicError("View out of bounds");
+    Matrix<T> V;
+    V.Attach(i1 - I.beg, j1 - J.beg, buf_ + I.beg + (int64_t)J.beg * ldim_, ldim_);
+    V.locked_ = locked_;
+    return V;
+}
+
+template <typename T>
+const Matrix<T> Matrix<T>::operator()(Range I, Range J) const {
+    return const_cast<Matrix<T>*>(this)->operator()(I, J);
+}
+
+template class Matrix<double>;
+template class Matrix<Int>;
+
+// ------------------------------------------------------------------ Permutation
+void Permutation::Empty() {
+    size_ = 0;
+    parity_ = false;
+    dests_.clear();
+}
+void Permutation::MakeIdentity(Int size) {
+    size_ = size;
+    parity_ = false;
+    dests_.clear();
+}
+void Permutation::ReserveSwaps(Int maxSwaps) { dests_.reserve(maxSwaps); }
+void Permutation::Swap(Int origin, Int dest) {
+    if (origin < 0 || origin >= size_ || dest < 0 || dest >= size_)
+        LogicError("Attempted swap out of range");
+    if (origin != (Int)dests_.size())
+        LogicError("Only swap sequences with origins 0,1,2,... are supported");
+    if (origin != dest) parity_ = !parity_;
+    dests_.push_back(dest);
+}
+void Permutation::SetSwapDestinations(Int size, const std::vector<Int>& dests) {
+    size_ = size;
+    dests_ = dests;
+    parity_ = false;
+    for (size_t j = 0; j < dests_.size(); ++j)
+        if (dests_[j] != (Int)j) parity_ = !parity_;
+}
+void Permutation::ExplicitVector(Matrix<Int>& p) const {
+    p.Resize(size_, 1);
+    for (Int i = 0; i < size_; ++i) p(i, 0) = i;
+    for (size_t j = 0; j < dests_.size(); ++j)
+        if (dests_[j] != (Int)j) std::swap(p((Int)j, 0), p(dests_[j], 0));
+}
+void Permutation::ExplicitMatrix(Matrix<Int>& P) const {
+    Matrix<Int> p;
+    ExplicitVector(p);
+    P.Resize(size_, size_);
+    for (Int i = 0; i < size_; ++i) P(i, p(i, 0)) = 1;
+}
+
+// ------------------------------------------------------------------ blocksize stack
+static std::vector<Int> g_blocksizes{128};
+Int Blocksize() { return g_blocksizes.back(); }
+void SetBlocksize(Int b) { g_blocksizes.back() = b; }
+void PushBlocksizeStack(Int b) { g_blocksizes.push_back(b); }
+void PopBlocksizeStack() {
+    if (g_blocksizes.size() > 1) g_blocksizes.pop_back();
+}
+
+// ------------------------------------------------------------------ device staging
+namespace {
+
+void check(elx_status_t s, const char* what) {
+    if (s == ELX_SUCCESS) return;
+    if (s == ELX_ERROR_ALLOC) throw std::bad_alloc();
+    if (s == ELX_ERROR_INVALID_ARG) LogicError(std::string(what) + ": invalid argument");
+    RuntimeError(std::string(what) + ": device error " + std::to_string((int)s));
+}
+
+void hcheck(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+    RuntimeError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) {
+        if (bytes) hcheck(hipMalloc(&p, bytes), "hipMalloc");
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Device copy of a host matrix with leading dimension max(height,1).
+template <typename T>
+struct DevMatrix {
+    Int m, n;
+    int64_t ld;
+    DevBuf buf;
+    explicit DevMatrix(const Matrix<T>& A)
+        : m(A.Height()), n(A.Width()), ld(std::max(A.Height(), 1)),
+          buf(sizeof(T) * (size_t)ld * (size_t)A.Width()) {
+        if (m > 0 && n > 0)
+            hcheck(hipMemcpy2D(buf.p, sizeof(T) * ld, A.LockedBuffer(), sizeof(T) * A.LDim(),
+                               sizeof(T) * m, n, hipMemcpyHostToDevice),
+                   "hipMemcpy2D");
+    }
+    T* ptr() { return static_cast<T*>(buf.p); }
+    void to_host(Matrix<T>& A) {
+        if (m > 0 && n > 0)
+            hcheck(hipMemcpy2D(A.Buffer(), sizeof(T) * A.LDim(), buf.p, sizeof(T) * ld,
+                               sizeof(T) * m, n, hipMemcpyDeviceToHost),
+                   "hipMemcpy2D");
+    }
+};
+
+int read_info(const DevBuf& info) {
+    int32_t v = 0;
+    hcheck(hipMemcpy(&v, info.p, sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy");
+    return v;
+}
+
+void check_blocksize() {
+    if (Blocksize() < 1 || Blocksize() > 128)
+        LogicError("Blocksize() must be in [1,128] for the GPU factorizations");
+}
+
+}  // namespace
+
+template <typename T>
+void Permutation::PermuteRows(Matrix<T>& A, Int offset) const {
+    if (A.Height() == 0 || A.Width() == 0 || dests_.empty()) return;
+    if (offset != 0) LogicError("Only offset 0 is supported");
+    if ((Int)dests_.size() > A.Height()) LogicError("Permutation larger than matrix");
+    DevMatrix<double> dA(A);
+    DevBuf dd(sizeof(int32_t) * dests_.size());
+    hcheck(hipMemcpy(dd.p, dests_.data(), sizeof(int32_t) * dests_.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    check(elx_dpermute_rows(nullptr, 0, static_cast<int32_t*>(dd.p), (int64_t)dests_.size(), dA.m,
+                            dA.n, dA.ptr(), dA.ld),
+          "PermuteRows");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dA.to_host(A);
+}
+
+template <typename T>
+void Permutation::InversePermuteRows(Matrix<T>& A, Int offset) const {
+    if (A.Height() == 0 || A.Width() == 0 || dests_.empty()) return;
+    if (offset != 0) LogicError("Only offset 0 is supported");
+    if ((Int)dests_.size() > A.Height()) LogicError("Permutation larger than matrix");
+    DevMatrix<double> dA(A);
+    DevBuf dd(sizeof(int32_t) * dests_.size());
+    hcheck(hipMemcpy(dd.p, dests_.data(), sizeof(int32_t) * dests_.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    check(elx_dpermute_rows(nullptr, 1, static_cast<int32_t*>(dd.p), (int64_t)dests_.size(), dA.m,
+                            dA.n, dA.ptr(), dA.ld),
+          "InversePermuteRows");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dA.to_host(A);
+}
+template void Permutation::PermuteRows(Matrix<double>&, Int) const;
+template void Permutation::InversePermuteRows(Matrix<double>&, Int) const;
+
+// ------------------------------------------------------------------ level 3
+template <>
+void Gemm(Orientation oA, Orientation oB, double alpha, const Matrix<double>& A,
+          const Matrix<double>& B, double beta, Matrix<double>& C) {
+    const Int m = C.Height(), n = C.Width();
+    const Int k = (oA == NORMAL) ? A.Width() : A.Height();
+    if (((oA == NORMAL) ? A.Height() : A.Width()) != m || ((oB == NORMAL) ? B.Width() : B.Height()) != n ||
+        ((oB == NORMAL) ? B.Height() : B.Width()) != k)
+        LogicError("Nonconformal Gemm");
+    DevMatrix<double> dA(A), dB(B), dC(C);
+    check(elx_dgemm(nullptr, oA, oB, m, n, k, alpha, dA.ptr(), dA.ld, dB.ptr(), dB.ld, beta, dC.ptr(), dC.ld), "Gemm");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dC.to_host(C);
+}
+
+template <>
+void Trsm(LeftOrRight side, UpperOrLower uplo, Orientation orientation, UnitOrNonUnit diag,
+          double alpha, const Matrix<double>& A, Matrix<double>& B, bool checkIfSingular) {
+    if (A.Height() != A.Width()) LogicError("Triangular matrix must be square");
+    if ((side == LEFT ? B.Height() : B.Width()) != A.Height()) LogicError("Nonconformal Trsm");
+    if (checkIfSingular && diag != UNIT)
+        for (Int j = 0; j < A.Height(); ++j)
+            if (A.Get(j, j) == 0.0) throw SingularMatrixException();
+    DevMatrix<double> dA(A), dB(B);
+    check(elx_dtrsm(nullptr, side, uplo, orientation, diag, B.Height(), B.Width(), alpha, dA.ptr(),
+                    dA.ld, dB.ptr(), dB.ld),
+          "Trsm");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dB.to_host(B);
+}
+
+template <>
+void Syrk(UpperOrLower uplo, Orientation orientation, double alpha, const Matrix<double>& A,
+          double beta, Matrix<double>& C, bool) {
+    const Int n = C.Height();
+    const Int k = (orientation == NORMAL) ? A.Width() : A.Height();
+    if (C.Width() != n || ((orientation == NORMAL) ? A.Height() : A.Width()) != n)
+        LogicError("Nonconformal Syrk");
+    DevMatrix<double> dA(A), dC(C);
+    check(elx_dsyrk(nullptr, uplo, orientation, n, k, alpha, dA.ptr(), dA.ld, beta, dC.ptr(), dC.ld), "Syrk");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dC.to_host(C);
+}
+
+template <>
+void Herk(UpperOrLower uplo, Orientation orientation, double alpha, const Matrix<double>& A,
+          double beta, Matrix<double>& C) {
+    Syrk(uplo, orientation, alpha, A, beta, C, true);
+}
+
+// ------------------------------------------------------------------ factorizations
+template <>
+void LU(Matrix<double>& A, Permutation& P) {
+    check_blocksize();
+    const Int m = A.Height(), n = A.Width(), minDim = std::min(m, n);
+    P.MakeIdentity(m);
+    if (minDim == 0) return;
+    DevMatrix<double> dA(A);
+    DevBuf dests(sizeof(int32_t) * minDim), info(sizeof(int32_t));
+    check(elx_dgetrf(nullptr, m, n, dA.ptr(), dA.ld, Blocksize(), static_cast<int32_t*>(dests.p),
+                     static_cast<int32_t*>(info.p)),
+          "LU");
+    hcheck(hipDeviceSynchronize(), "sync");
+    const int inf = read_info(info);
+    std::vector<Int> d(minDim);
+    hcheck(hipMemcpy(d.data(), dests.p, sizeof(int32_t) * minDim, hipMemcpyDeviceToHost), "hipMemcpy");
+    dA.to_host(A);
+    if (inf > 0) {
+        d.resize(inf);
+        P.SetSwapDestinations(m, d);
+        throw SingularMatrixException();
+    }
+    if (inf < 0) RuntimeError("LU: device panel did not complete");
+    P.SetSwapDestinations(m, d);
+}
+
+namespace lu {
+template <>
+void SolveAfter(Orientation orientation, const Matrix<double>& A, const Permutation& P,
+                Matrix<double>& B) {
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    if (A.Height() != B.Height()) LogicError("A and B must be the same height");
+    const Int n = A.Height();
+    if (n == 0 || B.Width() == 0) return;
+    const std::vector<Int>& d = P.SwapDestinations();
+    DevMatrix<double> dA(A), dB(B);
+    DevBuf dd(sizeof(int32_t) * std::max<size_t>(d.size(), 1));
+    if (!d.empty())
+        hcheck(hipMemcpy(dd.p, d.data(), sizeof(int32_t) * d.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if ((Int)d.size() != n) {
+        // A partial swap sequence: pad with identity swaps.
+        std::vector<Int> full(n);
+        for (Int i = 0; i < n; ++i) full[i] = (i < (Int)d.size()) ? d[i] : i;
+        DevBuf df(sizeof(int32_t) * n);
+        hcheck(hipMemcpy(df.p, full.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice), "hipMemcpy");
+        check(elx_dgetrs(nullptr, orientation, n, B.Width(), dA.ptr(), dA.ld,
+                         static_cast<int32_t*>(df.p), dB.ptr(), dB.ld),
+              "lu::SolveAfter");
+        hcheck(hipDeviceSynchronize(), "sync");
+    } else {
+        check(elx_dgetrs(nullptr, orientation, n, B.Width(), dA.ptr(), dA.ld,
+                         static_cast<int32_t*>(dd.p), dB.ptr(), dB.ld),
+              "lu::SolveAfter");
+        hcheck(hipDeviceSynchronize(), "sync");
+    }
+    dB.to_host(B);
+}
+}  // namespace lu
+
+template <>
+void Cholesky(UpperOrLower uplo, Matrix<double>& A) {
+    check_blocksize();
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    const Int n = A.Height();
+    if (n == 0) return;
+    DevMatrix<double> dA(A);
+    DevBuf info(sizeof(int32_t));
+    check(elx_dpotrf(nullptr, uplo, n, dA.ptr(), dA.ld, Blocksize(), static_cast<int32_t*>(info.p)),
+          "Cholesky");
+    hcheck(hipDeviceSynchronize(), "sync");
+    const int inf = read_info(info);
+    dA.to_host(A);
+    if (inf > 0) throw NonHPDMatrixException("A was not numerically HPD");
+}
+
+namespace cholesky {
+template <>
+void SolveAfter(UpperOrLower uplo, Orientation orientation, const Matrix<double>& A,
+                Matrix<double>& B) {
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    if (A.Height() != B.Height()) LogicError("A and B must be the same height");
+    if (A.Height() == 0 || B.Width() == 0) return;
+    DevMatrix<double> dA(A), dB(B);
+    check(elx_dpotrs(nullptr, uplo, orientation, A.Height(), B.Width(), dA.ptr(), dA.ld, dB.ptr(),
+                     dB.ld),
+          "cholesky::SolveAfter");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dB.to_host(B);
+}
+}  // namespace cholesky
+
+template <>
+void LinearSolve(const Matrix<double>& A, Matrix<double>& B) {
+    check_blocksize();
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    if (A.Height() != B.Height()) LogicError("A and B must be the same height");
+    const Int n = A.Height();
+    if (n == 0) return;
+    DevMatrix<double> dA(A), dB(B);  // the device copy plays the role of ACopy
+    DevBuf info(sizeof(int32_t));
+    check(elx_dgesv(nullptr, n, B.Width(), dA.ptr(), dA.ld, dB.ptr(), dB.ld, Blocksize(), nullptr,
+                    static_cast<int32_t*>(info.p)),
+          "LinearSolve");
+    hcheck(hipDeviceSynchronize(), "sync");
+    if (read_info(info) > 0) throw SingularMatrixException();
+    dB.to_host(B);
+}
+
+template <>
+void HPDSolve(UpperOrLower uplo, Orientation orientation, const Matrix<double>& A,
+              Matrix<double>& B) {
+    check_blocksize();
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    if (A.Height() != B.Height()) LogicError("A and B must be the same height");
+    const Int n = A.Height();
+    if (n == 0) return;
+    DevMatrix<double> dA(A), dB(B);
+    DevBuf info(sizeof(int32_t));
+    check(elx_dposv(nullptr, uplo, orientation, n, B.Width(), dA.ptr(), dA.ld, dB.ptr(), dB.ld,
+                    Blocksize(), static_cast<int32_t*>(info.p)),
+          "HPDSolve");
+    hcheck(hipDeviceSynchronize(), "sync");
+    if (read_info(info) > 0) throw NonHPDMatrixException("A was not numerically HPD");
+    dB.to_host(B);
+}
+
+}  // namespace El
+
+// ====================================================================== C interface
+using El::Int;
+using El::Matrix;
+using El::Permutation;
+
+#define EL_CATCH                                                     \
+    catch (std::bad_alloc & e) {                                     \
+        std::fprintf(stderr, "[elemental_amd] %s\n", e.what());      \
+        return EL_ALLOC_ERROR;                                       \
+    }                                                                \
+    catch (El::ArgException & e) {                                   \
+        std::fprintf(stderr, "[elemental_amd] %s\n", e.what());      \
+        return EL_ARG_ERROR;                                         \
+    }                                                                \
+    catch (std::logic_error & e) {                                   \
+        std::fprintf(stderr, "[elemental_amd] %s\n", e.what());      \
+        return EL_LOGIC_ERROR;                                       \
+    }                                                                \
+    catch (std::runtime_error & e) {                                 \
+        std::fprintf(stderr, "[elemental_amd] %s\n", e.what());      \
+        return EL_RUNTIME_ERROR;                                     \
+    }                                                                \
+    catch (std::exception & e) {                                     \
+        std::fprintf(stderr, "[elemental_amd] %s\n", e.what());      \
+        return EL_ERROR;                                             \
+    }
+#define EL_TRY(payload) \
+    try {               \
+        payload;        \
+    }                   \
+    EL_CATCH            \
+    return EL_SUCCESS;
+
+static Matrix<double>* M(ElMatrix_d A) { return reinterpret_cast<Matrix<double>*>(A); }
+static const Matrix<double>* M(ElConstMatrix_d A) { return reinterpret_cast<const Matrix<double>*>(A); }
+static Matrix<Int>* Mi(ElMatrix_i A) { return reinterpret_cast<Matrix<Int>*>(A); }
+static const Matrix<Int>* Mi(ElConstMatrix_i A) { return reinterpret_cast<const Matrix<Int>*>(A); }
+static Permutation* Pm(ElPermutation P) { return reinterpret_cast<Permutation*>(P); }
+static const Permutation* Pm(ElConstPermutation P) { return reinterpret_cast<const Permutation*>(P); }
+static El::Orientation O(ElOrientation o) { return static_cast<El::Orientation>(o); }
+static El::UpperOrLower U(ElUpperOrLower u) { return static_cast<El::UpperOrLower>(u); }
+
+extern "C" {
+
+const char* ElErrorString(ElError error) {
+    switch (error) {
+        case EL_SUCCESS: return "EL_SUCCESS";
+        case EL_ALLOC_ERROR: return "EL_ALLOC_ERROR";
+        case EL_OUT_OF_BOUNDS_ERROR: return "EL_OUT_OF_BOUNDS_ERROR";
+        case EL_ARG_ERROR: return "EL_ARG_ERROR";
+        case EL_LOGIC_ERROR: return "EL_LOGIC_ERROR";
+        case EL_RUNTIME_ERROR: return "EL_RUNTIME_ERROR";
+        default: return "EL_ERROR";
+    }
+}
+
+ElError ElMatrixCreate_d(ElMatrix_d* A) { EL_TRY(*A = reinterpret_cast<ElMatrix_d>(new Matrix<double>())) }
+ElError ElMatrixDestroy_d(ElConstMatrix_d A) { EL_TRY(delete M(A)) }
+ElError ElMatrixResize_d(ElMatrix_d A, ElInt h, ElInt w) { EL_TRY(M(A)->Resize(h, w)) }
+ElError ElMatrixResizeWithLDim_d(ElMatrix_d A, ElInt h, ElInt w, ElInt ld) { EL_TRY(M(A)->Resize(h, w, ld)) }
+ElError ElMatrixAttach_d(ElMatrix_d A, ElInt h, ElInt w, double* buf, ElInt ld) { EL_TRY(M(A)->Attach(h, w, buf, ld)) }
+ElError ElMatrixHeight_d(ElConstMatrix_d A, ElInt* h) { EL_TRY(*h = M(A)->Height()) }
+ElError ElMatrixWidth_d(ElConstMatrix_d A, ElInt* w) { EL_TRY(*w = M(A)->Width()) }
+ElError ElMatrixLDim_d(ElConstMatrix_d A, ElInt* ld) { EL_TRY(*ld = M(A)->LDim()) }
+ElError ElMatrixBuffer_d(ElMatrix_d A, double** b) { EL_TRY(*b = M(A)->Buffer()) }
+ElError ElMatrixLockedBuffer_d(ElConstMatrix_d A, const double** b) { EL_TRY(*b = M(A)->LockedBuffer()) }
+
+ElError ElMatrixCreate_i(ElMatrix_i* A) { EL_TRY(*A = reinterpret_cast<ElMatrix_i>(new Matrix<Int>())) }
+ElError ElMatrixDestroy_i(ElConstMatrix_i A) { EL_TRY(delete Mi(A)) }
+ElError ElMatrixHeight_i(ElConstMatrix_i A, ElInt* h) { EL_TRY(*h = Mi(A)->Height()) }
+ElError ElMatrixLockedBuffer_i(ElConstMatrix_i A, const ElInt** b) { EL_TRY(*b = Mi(A)->LockedBuffer()) }
+
+ElError ElPermutationCreate(ElPermutation* P) { EL_TRY(*P = reinterpret_cast<ElPermutation>(new Permutation())) }
+ElError ElPermutationDestroy(ElConstPermutation P) { EL_TRY(delete Pm(P)) }
+ElError ElPermutationMakeIdentity(ElPermutation P, ElInt size) { EL_TRY(Pm(P)->MakeIdentity(size)) }
+ElError ElPermutationSwap(ElPermutation P, ElInt o, ElInt d) { EL_TRY(Pm(P)->Swap(o, d)) }
+ElError ElPermutationHeight(ElConstPermutation P, ElInt* h) { EL_TRY(*h = Pm(P)->Height()) }
+ElError ElPermutationParity(ElConstPermutation P, bool* par) { EL_TRY(*par = Pm(P)->Parity()) }
+ElError ElPermutationSwapDestinations(ElConstPermutation P, ElInt* dests, ElInt* numSwaps) {
+    EL_TRY({
+        const auto& d = Pm(P)->SwapDestinations();
+        *numSwaps = (ElInt)d.size();
+        if (dests) std::copy(d.begin(), d.end(), dests);
+    })
+}
+ElError ElPermutationExplicitVector(ElConstPermutation P, ElMatrix_i p) { EL_TRY(Pm(P)->ExplicitVector(*Mi(p))) }
+ElError ElPermutationPermuteRows_d(ElConstPermutation P, ElMatrix_d A) { EL_TRY(Pm(P)->PermuteRows(*M(A))) }
+ElError ElPermutationInversePermuteRows_d(ElConstPermutation P, ElMatrix_d A) {
+    EL_TRY(Pm(P)->InversePermuteRows(*M(A)))
+}
+
+ElError ElBlocksize(ElInt* b) { EL_TRY(*b = El::Blocksize()) }
+ElError ElSetBlocksize(ElInt b) { EL_TRY(El::SetBlocksize(b)) }
+ElError ElPushBlocksizeStack(ElInt b) { EL_TRY(El::PushBlocksizeStack(b)) }
+ElError ElPopBlocksizeStack(void) { EL_TRY(El::PopBlocksizeStack()) }
+
+ElError ElLUPartialPiv_d(ElMatrix_d A, ElPermutation P) { EL_TRY(El::LU(*M(A), *Pm(P))) }
+ElError ElSolveAfterLUPartialPiv_d(ElOrientation o, ElConstMatrix_d A, ElConstPermutation P,
+                                   ElMatrix_d B) {
+    EL_TRY(El::lu::SolveAfter(O(o), *M(A), *Pm(P), *M(B)))
+}
+ElError ElCholesky_d(ElUpperOrLower u, ElMatrix_d A) { EL_TRY(El::Cholesky(U(u), *M(A))) }
+ElError ElSolveAfterCholesky_d(ElUpperOrLower u, ElOrientation o, ElConstMatrix_d A, ElMatrix_d B) {
+    EL_TRY(El::cholesky::SolveAfter(U(u), O(o), *M(A), *M(B)))
+}
+ElError ElLinearSolve_d(ElConstMatrix_d A, ElMatrix_d B) { EL_TRY(El::LinearSolve(*M(A), *M(B))) }
+ElError ElHPDSolve_d(ElUpperOrLower u, ElOrientation o, ElConstMatrix_d A, ElMatrix_d B) {
+    EL_TRY(El::HPDSolve(U(u), O(o), *M(A), *M(B)))
+}
+
+ElError ElGemm_d(ElOrientation oA, ElOrientation oB, double alpha, ElConstMatrix_d A,
+                 ElConstMatrix_d B, double beta, ElMatrix_d C) {
+    EL_TRY(El::Gemm(O(oA), O(oB), alpha, *M(A), *M(B), beta, *M(C)))
+}
+ElError ElTrsm_d(ElLeftOrRight side, ElUpperOrLower u, ElOrientation o, ElUnitOrNonUnit diag,
+                 double alpha, ElConstMatrix_d A, ElMatrix_d B) {
+    EL_TRY(El::Trsm(static_cast<El::LeftOrRight>(side), U(u), O(o),
+                    static_cast<El::UnitOrNonUnit>(diag), alpha, *M(A), *M(B)))
+}
+ElError ElHerk_d(ElUpperOrLower u, ElOrientation o, double alpha, ElConstMatrix_d A, double beta,
+                 ElMatrix_d C) {
+    EL_TRY(El::Herk(U(u), O(o), alpha, *M(A), beta, *M(C)))
+}
+ElError ElSyrk_d(ElUpperOrLower u, ElOrientation o, double alpha, ElConstMatrix_d A, double beta,
+                 ElMatrix_d C) {
+    EL_TRY(El::Syrk(U(u), O(o), alpha, *M(A), beta, *M(C)))
+}
+
+}  // extern "C"

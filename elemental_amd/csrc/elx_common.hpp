@@ -1,0 +1,121 @@
+// Shared helpers of the HIP library: status macros, device properties, scratch pools.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "../../include/elemental_amd.h"
+
+typedef int64_t i64;
+
+#define ELX_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            std::fprintf(stderr, "elemental_amd: %s failed: %s (%s:%d)\n", #call,          \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                      \
+            return ELX_ERROR_HIP;                                                         \
+        }                                                                                 \
+    } while (0)
+
+#define ELX_CHECK(status)                         \
+    do {                                          \
+        elx_status_t s_ = (status);               \
+        if (s_ != ELX_SUCCESS) return s_;         \
+    } while (0)
+
+#define ELX_ARG(cond)                                                                  \
+    do {                                                                               \
+        if (!(cond)) {                                                                 \
+            std::fprintf(stderr, "elemental_amd: invalid argument: %s (%s:%d)\n", #cond, \
+                         __FILE__, __LINE__);                                          \
+            return ELX_ERROR_INVALID_ARG;                                              \
+        }                                                                              \
+    } while (0)
+
+namespace elx {
+
+// Enum values of Elemental's C interface (include/El/core/types.h:185-225).
+enum { kNormal = 0, kTranspose = 1, kAdjoint = 2 };
+enum { kLower = 0, kUpper = 1 };
+enum { kLeft = 0, kRight = 1 };
+enum { kNonUnit = 0, kUnit = 1 };
+
+struct DeviceInfo {
+    int device = -1;
+    int num_cus = 0;
+    int max_lds_per_block = 0;
+    int coop = 0;
+};
+
+// Properties of the current device (cached per device).
+const DeviceInfo& device_info();
+
+// Per-device scratch buffer that grows on demand and is kept for reuse. Growth
+// synchronizes the device (the old buffer may still be in use by queued work).
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+elx_status_t scratch_get(int slot, size_t bytes, void** out);
+
+// Per-device persistent state of the cooperative panel kernel's exchange area.
+struct PanelSync {
+    int64_t* flags = nullptr;  // one per workgroup, monotonically increasing
+    void* slots = nullptr;     // candidate exchange area
+    size_t slot_bytes = 0;
+    int64_t epoch = 0;         // host-side launch counter (flags compare against it)
+    int max_groups = 0;
+};
+elx_status_t panel_sync_get(int max_groups, size_t slot_bytes, PanelSync** out);
+
+inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace elx
+
+// Kernel launchers shared between translation units (all enqueue on `stream`).
+namespace elx {
+
+// C := alpha op(A) op(B) + beta C  (FP64, MFMA). tri: 0 full, 1 lower only, 2 upper only.
+elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
+                          const double* A, i64 lda, const double* B, i64 ldb, double beta,
+                          double* C, i64 ldc, int tri);
+
+// Small triangular solves for many independent vectors (triangle order <= 128).
+// side/uplo/orient/diag as in Elemental. B is m x n.
+elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, int diag, i64 m,
+                               i64 n, const double* A, i64 lda, double* B, i64 ldb);
+elx_status_t launch_trsm_small_f32(hipStream_t s, int side, int uplo, int orient, int diag,
+                                   i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb);
+
+// General triangular solve (blocked by 128 with GEMM updates for larger triangles).
+elx_status_t dtrsm(hipStream_t s, int side, int uplo, int orient, int diag, i64 m, i64 n,
+                   double alpha, const double* A, i64 lda, double* B, i64 ldb);
+
+// LU panel of height mp, width w (<= 128) at A (column-major, lda). Records panel-relative
+// pivots (plus offset `goff`) into dests, the row-move list into pairs (2*w int2), and the
+// first zero pivot (global column + 1) into info.
+elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
+                                 int32_t* dests, i64 goff, int2* pairs, int32_t* info);
+
+// Apply a row-move list (pairs[i] = {dst, src}, panel-relative, -1 = none) to columns
+// [c0,c1) and [c2,c3) of A starting at row `r0`.
+elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, double* A,
+                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3);
+
+// Explicit permutation vector from a swap sequence (p[i] = source row of row i).
+elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswaps, i64 size,
+                                  int32_t* perm, int inverse);
+
+// B_out(i,:) = B_in(p[i],:)
+elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                    const double* Bin, i64 ldin, double* Bout, i64 ldout);
+
+// Unblocked Cholesky of the diagonal block (n <= 128) in one workgroup.
+elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,
+                              int32_t* info);
+
+}  // namespace elx

@@ -1,0 +1,352 @@
+// Row permutation, small triangular solve and unblocked Cholesky kernels.
+#include <algorithm>
+
+#include "elx_common.hpp"
+
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// Row moves: the net effect of one panel's swap sequence on the columns outside the panel
+// (PB.PermuteRows(AB0) / PB.PermuteRows(AB2), src/lapack_like/factor/LU.cpp:136-137).
+// One wave per column. All sources of a column are read (and staged in LDS) before any
+// destination is written, so overlapping source/destination rows are safe.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void row_moves_kernel(const int2* __restrict__ pairs,
+                                                        int npairs, T* A, i64 lda, i64 r0,
+                                                        i64 c0, i64 c1, i64 c2, i64 c3) {
+    __shared__ T stage[4][256];
+    __shared__ int2 pr[256];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int i = t; i < 256; i += 256) pr[i] = (i < npairs) ? pairs[i] : int2{-1, -1};
+    __syncthreads();
+    const i64 n1 = c1 - c0, total = n1 + (c3 - c2);
+    const i64 cidx = (i64)blockIdx.x * 4 + wave;
+    const bool active = cidx < total;
+    T* col = nullptr;
+    if (active) {
+        const i64 j = (cidx < n1) ? (c0 + cidx) : (c2 + (cidx - n1));
+        col = A + r0 + j * lda;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = lane + 64 * q;
+            if (pr[i].x >= 0) stage[wave][i] = col[pr[i].y];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = lane + 64 * q;
+            if (pr[i].x >= 0) col[pr[i].x] = stage[wave][i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Explicit permutation of a swap sequence (Permutation::ExplicitVector,
+// src/lapack_like/perm/Permutation.cpp:731-745). The swaps are sequential by nature; one
+// thread replays them on an LDS copy of the index vector when it fits, else in global
+// memory.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void swaps_to_perm_kernel(const int32_t* __restrict__ dests,
+                                                            i64 nswaps, i64 size, int32_t* perm,
+                                                            int inverse, int in_lds) {
+    extern __shared__ int32_t sp[];
+    const int t = threadIdx.x;
+    int32_t* p = in_lds ? sp : perm;
+    for (i64 i = t; i < size; i += 256) p[i] = (int32_t)i;
+    __syncthreads();
+    if (t == 0) {
+        if (!inverse) {
+            for (i64 j = 0; j < nswaps; ++j) {
+                const i64 d = dests[j];
+                if (d != j && d >= 0 && d < size) {
+                    const int32_t a = p[j];
+                    p[j] = p[d];
+                    p[d] = a;
+                }
+            }
+        } else {
+            for (i64 j = nswaps - 1; j >= 0; --j) {
+                const i64 d = dests[j];
+                if (d != j && d >= 0 && d < size) {
+                    const int32_t a = p[j];
+                    p[j] = p[d];
+                    p[d] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (in_lds)
+        for (i64 i = t; i < size; i += 256) perm[i] = p[i];
+}
+
+template <typename T>
+__global__ void gather_rows_kernel(const int32_t* __restrict__ perm, i64 m, i64 n,
+                                   const T* __restrict__ Bin, i64 ldin, T* __restrict__ Bout,
+                                   i64 ldout) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    const i64 j = blockIdx.y;
+    if (i >= m || j >= n) return;
+    Bout[i + j * ldout] = Bin[(i64)perm[i] + j * ldin];
+}
+
+// ---------------------------------------------------------------------------------------
+// Triangular solves M x = b for many independent vectors with an order <= 128 triangle.
+// Each thread owns one vector, staged in LDS; M is read with wave-uniform (scalar) loads.
+// Left side: the vectors are columns of B and M = op(A). Right side (X op(A) = B): the
+// vectors are rows of B and M = op(A)^T. The update order is the column-oriented (axpy)
+// substitution of the reference BLAS dtrsm: x_i -= M(i,j) x_j for j = 0,1,... in turn, one
+// FMA each, with division by the diagonal on the left side and multiplication by its
+// reciprocal on the right side (the two forms the reference BLAS uses).
+// ---------------------------------------------------------------------------------------
+constexpr int TV = 64;  // vectors per workgroup
+
+template <typename T>
+__global__ __launch_bounds__(TV) void trsm_small_kernel(int m, i64 nvec, const T* __restrict__ A,
+                                                        i64 lda, T* B, i64 ldb, int right,
+                                                        int lower_eff, int mtrans, int unit) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* x = reinterpret_cast<T*>(smem_raw);  // x[i * TV + v]
+    const int t = threadIdx.x;
+    const i64 v0 = (i64)blockIdx.x * TV;
+    // Coalesced staging: left side walks down each column, right side across rows.
+    for (int idx = t; idx < m * TV; idx += TV) {
+        int i, vv;
+        if (!right) {
+            i = idx % m;
+            vv = idx / m;
+        } else {
+            vv = idx % TV;
+            i = idx / TV;
+        }
+        const i64 v = v0 + vv;
+        T val = T(0);
+        if (v < nvec) val = right ? B[v + (i64)i * ldb] : B[i + v * ldb];
+        x[i * TV + vv] = val;
+    }
+    __syncthreads();
+    // M(i,j) = A(i,j) or A(j,i)
+    auto M = [&](int i, int j) -> T { return mtrans ? A[j + (i64)i * lda] : A[i + (i64)j * lda]; };
+    if (lower_eff) {
+        for (int j = 0; j < m; ++j) {
+            T xj = x[j * TV + t];
+            if (!unit) {
+                if (right) xj = xj * (T(1) / M(j, j));
+                else xj = xj / M(j, j);
+                x[j * TV + t] = xj;
+            }
+            for (int i = j + 1; i < m; ++i) x[i * TV + t] = __builtin_fma(-M(i, j), xj, x[i * TV + t]);
+        }
+    } else if (!mtrans) {
+        // backward, axpy form: x_i receives its updates in descending j
+        for (int j = m - 1; j >= 0; --j) {
+            T xj = x[j * TV + t];
+            if (!unit) {
+                if (right) xj = xj * (T(1) / M(j, j));
+                else xj = xj / M(j, j);
+                x[j * TV + t] = xj;
+            }
+            for (int i = 0; i < j; ++i) x[i * TV + t] = __builtin_fma(-M(i, j), xj, x[i * TV + t]);
+        }
+    } else {
+        // backward, dot form (the reference BLAS order for transposed-left and
+        // untransposed-right solves): x_i accumulates its terms in ascending k
+        for (int i = m - 1; i >= 0; --i) {
+            T temp = x[i * TV + t];
+            for (int k = i + 1; k < m; ++k) temp = __builtin_fma(-M(i, k), x[k * TV + t], temp);
+            if (!unit) {
+                if (right) temp = temp * (T(1) / M(i, i));
+                else temp = temp / M(i, i);
+            }
+            x[i * TV + t] = temp;
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < m * TV; idx += TV) {
+        int i, vv;
+        if (!right) {
+            i = idx % m;
+            vv = idx / m;
+        } else {
+            vv = idx % TV;
+            i = idx / TV;
+        }
+        const i64 v = v0 + vv;
+        if (v < nvec) {
+            if (right) B[v + (i64)i * ldb] = x[i * TV + vv];
+            else B[i + v * ldb] = x[i * TV + vv];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Unblocked Cholesky of an n x n (n <= 128) diagonal block in one workgroup, in LDS.
+// Lower: cholesky::LowerVariant3Unblocked (src/lapack_like/factor/Cholesky/LowerVariant3.hpp:
+// 15-41): alpha = sqrt(a_jj) (a_jj <= 0 -> NonHPD), a21 *= 1/alpha, dsyr('L', -1, a21).
+// Upper: UpperVariant3Unblocked (UpperVariant3.hpp:15-46) on the row instead of the column.
+// The dsyr update is a(i,l) = fma(x_i, -x_l, a(i,l)), skipped when x_l == 0, as in the
+// reference BLAS.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void potf2_kernel(int n, T* A, i64 lda, int upper, i64 goff,
+                                                    int32_t* info) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* S = reinterpret_cast<T*>(smem_raw);  // column-major, leading dimension n+1
+    __shared__ int fail;
+    const int LD = n + 1, t = threadIdx.x;
+    for (int idx = t; idx < n * n; idx += 256) {
+        const int i = idx % n, j = idx / n;
+        S[i + j * LD] = A[i + (i64)j * lda];
+    }
+    if (t == 0) fail = 0;
+    __syncthreads();
+    int j = 0;
+    for (; j < n; ++j) {
+        T alpha = S[j + j * LD];
+        if (alpha <= T(0)) {
+            if (t == 0) fail = 1;
+            break;
+        }
+        alpha = sqrt(alpha);
+        const T inv = T(1) / alpha;
+        __syncthreads();
+        if (t == 0) S[j + j * LD] = alpha;
+        const int h = n - (j + 1);
+        // scale the column (lower) or row (upper) of the block
+        for (int q = t; q < h; q += 256) {
+            if (!upper) S[(j + 1 + q) + j * LD] = inv * S[(j + 1 + q) + j * LD];
+            else S[j + (j + 1 + q) * LD] = inv * S[j + (j + 1 + q) * LD];
+        }
+        __syncthreads();
+        // rank-1 update of the trailing triangle: column l, rows i
+        for (int idx = t; idx < h * h; idx += 256) {
+            const int il = idx % h, ll = idx / h;
+            const int i = j + 1 + il, l = j + 1 + ll;
+            if (!upper) {
+                if (i < l) continue;
+                const T xl = S[l + j * LD];
+                if (xl == T(0)) continue;
+                S[i + l * LD] = __builtin_fma(S[i + j * LD], -xl, S[i + l * LD]);
+            } else {
+                if (i > l) continue;
+                const T xl = S[j + l * LD];
+                if (xl == T(0)) continue;
+                S[i + l * LD] = __builtin_fma(S[j + i * LD], -xl, S[i + l * LD]);
+            }
+        }
+        __syncthreads();
+    }
+    if (fail && t == 0) atomicCAS(info, 0, (int32_t)(goff + j + 1));
+    __syncthreads();
+    for (int idx = t; idx < n * n; idx += 256) {
+        const int i = idx % n, jj = idx / n;
+        if (!upper ? (i >= jj) : (i <= jj)) A[i + (i64)jj * lda] = S[i + jj * LD];
+    }
+}
+
+}  // namespace
+
+namespace elx {
+
+elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, double* A,
+                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3) {
+    ELX_ARG(npairs <= 256);
+    const i64 total = std::max<i64>(c1 - c0, 0) + std::max<i64>(c3 - c2, 0);
+    if (total <= 0 || npairs <= 0) return ELX_SUCCESS;
+    if (c1 < c0) c1 = c0;
+    if (c3 < c2) c3 = c2;
+    hipLaunchKernelGGL(row_moves_kernel<double>, dim3(cdiv(total, 4)), dim3(256), 0, s, pairs,
+                       npairs, A, lda, r0, c0, c1, c2, c3);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswaps, i64 size,
+                                  int32_t* perm, int inverse) {
+    if (size <= 0) return ELX_SUCCESS;
+    const DeviceInfo& di = device_info();
+    const size_t need = sizeof(int32_t) * (size_t)size;
+    const int in_lds = need <= (size_t)di.max_lds_per_block ? 1 : 0;
+    static bool attr = false;
+    if (!attr) {
+        ELX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&swaps_to_perm_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    di.max_lds_per_block));
+        attr = true;
+    }
+    hipLaunchKernelGGL(swaps_to_perm_kernel, dim3(1), dim3(256), in_lds ? need : 0, s, dests,
+                       nswaps, size, perm, inverse, in_lds);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                    const double* Bin, i64 ldin, double* Bout, i64 ldout) {
+    if (m <= 0 || n <= 0) return ELX_SUCCESS;
+    ELX_ARG(n <= 65535);
+    hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(cdiv(m, 256), (unsigned)n), dim3(256), 0,
+                       s, perm, m, n, Bin, ldin, Bout, ldout);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+template <typename T>
+static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orient, int diag,
+                                    i64 m, i64 n, const T* A, i64 lda, T* B, i64 ldb) {
+    if (m <= 0 || n <= 0) return ELX_SUCCESS;
+    const bool right = (side == kRight);
+    const i64 order = right ? n : m;
+    const i64 nvec = right ? m : n;
+    ELX_ARG(order <= 128);
+    const bool trans = (orient != kNormal);
+    const bool lower = (uplo == kLower);
+    // Left: M = op(A). Right: M = op(A)^T.
+    const bool mtrans = right ? !trans : trans;
+    const bool lower_eff = lower != mtrans;
+    const size_t lds = sizeof(T) * (size_t)order * TV;
+    static bool attr = false;
+    if (!attr) {
+        ELX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_small_kernel<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(T) * 128 * TV)));
+        attr = true;
+    }
+    hipLaunchKernelGGL(trsm_small_kernel<T>, dim3(cdiv(nvec, TV)), dim3(TV), lds, s, (int)order,
+                       nvec, A, lda, B, ldb, right ? 1 : 0, lower_eff ? 1 : 0, mtrans ? 1 : 0,
+                       diag == kUnit ? 1 : 0);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, int diag, i64 m,
+                               i64 n, const double* A, i64 lda, double* B, i64 ldb) {
+    return trsm_small_impl<double>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb);
+}
+
+elx_status_t launch_trsm_small_f32(hipStream_t s, int side, int uplo, int orient, int diag,
+                                   i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb) {
+    return trsm_small_impl<float>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb);
+}
+
+elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,
+                              int32_t* info) {
+    if (n <= 0) return ELX_SUCCESS;
+    ELX_ARG(n <= 128);
+    const size_t lds = sizeof(double) * (size_t)n * (n + 1);
+    static bool attr = false;
+    if (!attr) {
+        ELX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&potf2_kernel<double>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(double) * 128 * 129)));
+        attr = true;
+    }
+    hipLaunchKernelGGL(potf2_kernel<double>, dim3(1), dim3(256), lds, s, (int)n, A, lda,
+                       uplo == kUpper ? 1 : 0, goff, info);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+}  // namespace elx

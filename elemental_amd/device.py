@@ -1,0 +1,175 @@
+"""Device-resident interface over the C ABI, for column-major torch tensors in HBM.
+
+A column-major m x n matrix is a 2-D tensor with stride (1, ld), ld >= max(m, 1) -- e.g.
+`colmajor_empty(m, n)` or `to_device(numpy_array)`. Pivots are int32 swap destinations
+(Elemental's Permutation::SwapDestinations()). torch provides memory and streams only; all
+arithmetic runs in the HIP kernels of libelemental_amd.so.
+"""
+import numpy as np
+import torch
+
+from ._lib import IrStats, call, load
+
+NORMAL, TRANSPOSE, ADJOINT = 0, 1, 2
+LOWER, UPPER = 0, 1
+LEFT, RIGHT = 0, 1
+NON_UNIT, UNIT = 0, 1
+
+FACTOR = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+REFINE = {"none": 0, "ir": 1, "gmres": 2, "auto": 3}
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes_void(s.cuda_stream)
+
+
+def ctypes_void(x):
+    import ctypes
+    return ctypes.c_void_p(x)
+
+
+def _mat(t, dtype=torch.float64):
+    if t.dim() != 2 or t.dtype != dtype or not t.is_cuda:
+        raise ValueError(f"expected a 2-D {dtype} CUDA tensor, got {t.dtype} {tuple(t.shape)}")
+    m, n = t.shape
+    if m > 1 and t.stride(0) != 1:
+        raise ValueError("expected a column-major tensor (stride(0) == 1)")
+    ld = t.stride(1) if n > 1 else max(m, 1)
+    if ld < max(m, 1):
+        raise ValueError("leading dimension smaller than the height")
+    return t.data_ptr(), ld
+
+
+def colmajor_empty(m, n, dtype=torch.float64, device="cuda"):
+    return torch.empty((n, m), dtype=dtype, device=device).t()
+
+
+def to_device(a, dtype=torch.float64, device="cuda"):
+    a = np.asarray(a)
+    if a.ndim == 1:
+        a = a[:, None]
+    t = colmajor_empty(a.shape[0], a.shape[1], dtype=dtype, device=device)
+    t.copy_(torch.from_numpy(np.ascontiguousarray(a)))
+    return t
+
+
+def to_host(t):
+    return np.asfortranarray(t.detach().cpu().numpy())
+
+
+def getrf(A, nb=128, stream=None):
+    """In-place PA = LU. Returns (dests int32 tensor, info int32 tensor)."""
+    pA, lda = _mat(A)
+    m, n = A.shape
+    dests = torch.zeros(max(min(m, n), 1), dtype=torch.int32, device=A.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call("elx_dgetrf", _stream(stream), m, n, pA, lda, nb, dests.data_ptr(), info.data_ptr())
+    return dests[: min(m, n)], info
+
+
+def getrs(A, dests, B, orient=NORMAL, stream=None):
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    call("elx_dgetrs", _stream(stream), orient, A.shape[0], B.shape[1], pA, lda,
+         dests.data_ptr(), pB, ldb)
+    return B
+
+
+def gesv(A, B, nb=128, stream=None, want_dests=False):
+    """Row-echelon LU on [A|B] and upper solve (A and B overwritten)."""
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    n = A.shape[0]
+    dests = torch.zeros(max(n, 1), dtype=torch.int32, device=A.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call("elx_dgesv", _stream(stream), n, B.shape[1], pA, lda, pB, ldb, nb, dests.data_ptr(),
+         info.data_ptr())
+    return (dests[:n], info) if want_dests else info
+
+
+def potrf(A, uplo=LOWER, nb=128, stream=None):
+    pA, lda = _mat(A)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call("elx_dpotrf", _stream(stream), uplo, A.shape[0], pA, lda, nb, info.data_ptr())
+    return info
+
+
+def potrs(A, B, uplo=LOWER, orient=NORMAL, stream=None):
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    call("elx_dpotrs", _stream(stream), uplo, orient, A.shape[0], B.shape[1], pA, lda, pB, ldb)
+    return B
+
+
+def posv(A, B, uplo=LOWER, orient=NORMAL, nb=128, stream=None):
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call("elx_dposv", _stream(stream), uplo, orient, A.shape[0], B.shape[1], pA, lda, pB, ldb,
+         nb, info.data_ptr())
+    return info
+
+
+def gemm(transA, transB, alpha, A, B, beta, C, stream=None):
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    pC, ldc = _mat(C)
+    m, n = C.shape
+    k = A.shape[1] if transA == NORMAL else A.shape[0]
+    call("elx_dgemm", _stream(stream), transA, transB, m, n, k, alpha, pA, lda, pB, ldb, beta,
+         pC, ldc)
+    return C
+
+
+def trsm(side, uplo, orient, diag, alpha, A, B, stream=None):
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    m, n = B.shape
+    call("elx_dtrsm", _stream(stream), side, uplo, orient, diag, m, n, alpha, pA, lda, pB, ldb)
+    return B
+
+
+def syrk(uplo, orient, alpha, A, beta, C, stream=None):
+    pA, lda = _mat(A)
+    pC, ldc = _mat(C)
+    n = C.shape[0]
+    k = A.shape[1] if orient == NORMAL else A.shape[0]
+    call("elx_dsyrk", _stream(stream), uplo, orient, n, k, alpha, pA, lda, beta, pC, ldc)
+    return C
+
+
+def permute_rows(dests, A, inverse=False, stream=None):
+    pA, lda = _mat(A)
+    call("elx_dpermute_rows", _stream(stream), 1 if inverse else 0, dests.data_ptr(),
+         dests.numel(), A.shape[0], A.shape[1], pA, lda)
+    return A
+
+
+def gesv_mixed(A, B, X, factor="fp16", refine="auto", tol=0.0, max_iters=0, stream=None):
+    """Low-precision LU + FP64 refinement; returns the stats dict (synchronous)."""
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    pX, ldx = _mat(X)
+    st = IrStats()
+    import ctypes
+    call("elx_dgesv_mixed", _stream(stream), A.shape[0], B.shape[1], pA, lda, pB, ldb, pX, ldx,
+         FACTOR[factor], REFINE[refine], tol, max_iters, ctypes.byref(st))
+    return st.as_dict()
+
+
+def posv_mixed(A, B, X, factor="fp32", refine="ir", tol=0.0, max_iters=0, stream=None):
+    pA, lda = _mat(A)
+    pB, ldb = _mat(B)
+    pX, ldx = _mat(X)
+    st = IrStats()
+    import ctypes
+    call("elx_dposv_mixed", _stream(stream), A.shape[0], B.shape[1], pA, lda, pB, ldb, pX, ldx,
+         FACTOR[factor], REFINE[refine], tol, max_iters, ctypes.byref(st))
+    return st.as_dict()
+
+
+__all__ = [
+    "load", "getrf", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
+    "permute_rows", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
+]

@@ -1,0 +1,157 @@
+/* elemental_amd.h -- C ABI of the MI355X-native dense factor-and-solve path.
+ *
+ * Every entry point takes DEVICE pointers to column-major data (element (i,j) at
+ * A[i + j*lda], lda >= max(m,1)) and a HIP stream (hipStream_t passed as void*; NULL is
+ * the default stream). Work is enqueued on that stream; calls return after enqueueing,
+ * except where noted. Scratch memory is allocated by the library on first use per device
+ * and kept until elx_finalize().
+ *
+ * Return value: ELX_SUCCESS or an elx_status_t error detected on the host (bad argument,
+ * HIP error). Numerical failure is reported through the device word *dInfo, with the
+ * LAPACK convention used by the reference's exceptions:
+ *   *dInfo == 0      success
+ *   *dInfo == j+1    column j (0-based, global) had an exactly zero pivot (LU) or a
+ *                    non-positive diagonal (Cholesky) -- the point at which Elemental throws
+ *                    SingularMatrixException / NonHPDMatrixException.
+ * Pivots use Elemental's swap-sequence format: dests[j] is the 0-based row swapped with
+ * row j at step j, i.e. Permutation::SwapDestinations() (LAPACK ipiv - 1).
+ */
+#ifndef ELEMENTAL_AMD_H
+#define ELEMENTAL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ELX_SUCCESS = 0,
+    ELX_ERROR_INVALID_ARG = 1,
+    ELX_ERROR_HIP = 2,
+    ELX_ERROR_ALLOC = 3,
+    ELX_ERROR_NOT_SUPPORTED = 4,
+    ELX_ERROR_NUMERIC = 5 /* returned only by the synchronous solvers below */
+} elx_status_t;
+
+/* Orientation, UpperOrLower, LeftOrRight and UnitOrNonUnit use Elemental's C enum values
+ * (include/El/core/types.h:185-225): NORMAL=0, TRANSPOSE=1, ADJOINT=2; LOWER=0, UPPER=1;
+ * LEFT=0, RIGHT=1; NON_UNIT=0, UNIT=1. */
+
+/* Library information / lifetime. */
+const char* elx_version(void);
+int elx_device_count(void);
+elx_status_t elx_finalize(void);
+
+/* ------------------------------------------------------------------------------------
+ * Plain FP64 path (the drop-in)
+ * ---------------------------------------------------------------------------------- */
+
+/* PA = LU, blocked right-looking with block size nb (Elemental default 128).
+ * Replaces El::LU(Matrix<double>&, Permutation&) -- src/lapack_like/factor/LU.cpp:104-142,
+ * with lu::Panel -- src/lapack_like/factor/LU/Panel.hpp:15-60.
+ * dDests: min(m,n) int32 (device). dInfo: one int32 (device). */
+elx_status_t elx_dgetrf(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
+                        int64_t nb, int32_t* dDests, int32_t* dInfo);
+
+/* B := op(A)^{-1} P B (orient NORMAL) or P^T op(A)^{-1} B (TRANSPOSE/ADJOINT) with the
+ * factors of elx_dgetrf. Replaces lu::SolveAfter(Orientation, const Matrix<double>&,
+ * const Permutation&, Matrix<double>&) -- src/lapack_like/factor/LU/SolveAfter.hpp:66-92. */
+elx_status_t elx_dgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const double* dA,
+                        int64_t lda, const int32_t* dDests, double* dB, int64_t ldb);
+
+/* Short-circuited LU on [A|B] followed by one upper Trsm: B := A^{-1} B, A overwritten
+ * with its (row-echelon) factors. Replaces lin_solve::Overwrite / RowEchelon --
+ * src/lapack_like/solve/Linear.cpp:30-76,176-183 (El::LinearSolve, :212-218, is this after
+ * a copy of A). dDests may be NULL. */
+elx_status_t elx_dgesv(void* stream, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                       double* dB, int64_t ldb, int64_t nb, int32_t* dDests, int32_t* dInfo);
+
+/* A = L L^T (uplo LOWER) or U^T U (UPPER), right-looking variant 3 with block size nb.
+ * Replaces El::Cholesky(UpperOrLower, Matrix<double>&) -- src/lapack_like/factor/
+ * Cholesky.cpp:26-38, Cholesky/LowerVariant3.hpp:15-68, Cholesky/UpperVariant3.hpp:15-73. */
+elx_status_t elx_dpotrf(void* stream, int uplo, int64_t n, double* dA, int64_t lda, int64_t nb,
+                        int32_t* dInfo);
+
+/* B := A^{-1} B with the Cholesky factor. Replaces cholesky::SolveAfter(UpperOrLower,
+ * Orientation, const Matrix<double>&, Matrix<double>&) --
+ * src/lapack_like/factor/Cholesky/SolveAfter.hpp:15-43. */
+elx_status_t elx_dpotrs(void* stream, int uplo, int orient, int64_t n, int64_t nrhs,
+                        const double* dA, int64_t lda, double* dB, int64_t ldb);
+
+/* Cholesky factor + solve in place (A overwritten). Replaces hpd_solve::Overwrite --
+ * src/lapack_like/solve/HPD.cpp:15-25 (El::HPDSolve, :47-57, is this after a copy of A). */
+elx_status_t elx_dposv(void* stream, int uplo, int orient, int64_t n, int64_t nrhs, double* dA,
+                       int64_t lda, double* dB, int64_t ldb, int64_t nb, int32_t* dInfo);
+
+/* Level-3 pieces on device matrices. Replace El::Gemm / El::Trsm / El::Herk on
+ * Matrix<double> -- src/blas_like/level3/Gemm.cpp:19-72, Trsm.cpp:24-64, Herk.cpp:14-21 ->
+ * Syrk.cpp:20-55 (only the uplo triangle of C is written). */
+elx_status_t elx_dgemm(void* stream, int transA, int transB, int64_t m, int64_t n, int64_t k,
+                       double alpha, const double* dA, int64_t lda, const double* dB,
+                       int64_t ldb, double beta, double* dC, int64_t ldc);
+elx_status_t elx_dtrsm(void* stream, int side, int uplo, int orient, int diag, int64_t m,
+                       int64_t n, double alpha, const double* dA, int64_t lda, double* dB,
+                       int64_t ldb);
+elx_status_t elx_dsyrk(void* stream, int uplo, int orient, int64_t n, int64_t k, double alpha,
+                       const double* dA, int64_t lda, double beta, double* dC, int64_t ldc);
+
+/* Row permutation of a device matrix by a swap sequence (forward or inverse order).
+ * Replaces Permutation::PermuteRows / InversePermuteRows on Matrix<double> --
+ * src/lapack_like/perm/Permutation.cpp:544-631. */
+elx_status_t elx_dpermute_rows(void* stream, int inverse, const int32_t* dDests,
+                               int64_t nswaps, int64_t m, int64_t n, double* dA, int64_t lda);
+
+/* ------------------------------------------------------------------------------------
+ * Mixed precision: low-precision LU + FP64 iterative refinement (no reference
+ * counterpart for dense matrices; the refinement loop follows the semantics of
+ * El::RefinedSolve, include/El/lapack_like/solve/Refined.hpp:44-119, and the Krylov
+ * variant follows El::FGMRES, include/El/lapack_like/solve/FGMRES.hpp:37-291).
+ * ---------------------------------------------------------------------------------- */
+typedef enum {
+    ELX_FACTOR_FP64 = 0,
+    ELX_FACTOR_FP32 = 1,
+    ELX_FACTOR_FP16 = 2,
+    ELX_FACTOR_BF16 = 3
+} elx_factor_type_t;
+
+typedef enum {
+    ELX_REFINE_NONE = 0,
+    ELX_REFINE_IR = 1,      /* classical iterative refinement */
+    ELX_REFINE_GMRES = 2,   /* GMRES-IR: each correction by GMRES preconditioned with the LU */
+    ELX_REFINE_AUTO = 3     /* IR, switching to GMRES-IR when IR stagnates */
+} elx_refine_t;
+
+typedef struct {
+    int32_t converged;        /* 1 when every column met the tolerance */
+    int32_t outer_iters;      /* refinement steps (residual evaluations after the first) */
+    int32_t inner_iters;      /* total GMRES iterations */
+    int32_t fell_back;        /* 1 when the FP64 LU fallback was used */
+    int32_t factor_info;      /* info of the low-precision factorization */
+    int32_t near_tie_pivots;  /* unused by the mixed path (0) */
+    double backward_error;    /* max over columns of ||b-Ax||inf/(||A||inf||x||inf+||b||inf) */
+    double factor_seconds;    /* device time of the factorization */
+    double refine_seconds;    /* device time of the refinement */
+} elx_ir_stats_t;
+
+/* Solve A X = B to FP64 backward error <= tol (tol <= 0 selects n * 2^-53) with a
+ * factorization in factor_type and refinement in FP64. A (FP64, not modified) and B stay
+ * on the device; X receives the solution. Synchronous: returns after the solve, with
+ * *stats filled (may be NULL). Returns ELX_ERROR_NUMERIC if neither refinement nor the
+ * FP64 fallback reached tol. */
+elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double* dA,
+                             int64_t lda, const double* dB, int64_t ldb, double* dX,
+                             int64_t ldx, int factor_type, int refine, double tol,
+                             int32_t max_iters, elx_ir_stats_t* stats);
+
+/* Same for symmetric positive definite A: low-precision Cholesky + FP64 refinement. */
+elx_status_t elx_dposv_mixed(void* stream, int64_t n, int64_t nrhs, const double* dA,
+                             int64_t lda, const double* dB, int64_t ldb, double* dX,
+                             int64_t ldx, int factor_type, int refine, double tol,
+                             int32_t max_iters, elx_ir_stats_t* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ELEMENTAL_AMD_H */

