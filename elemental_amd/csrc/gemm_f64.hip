@@ -128,7 +128,8 @@ __global__ __launch_bounds__(NT) void dgemm_kernel(i64 M, i64 N, i64 K, double a
         __syncthreads();
     }
 
-    // Lane holds C(m0+wm+mi*16+r16, n0+wn+ni*16+4*kq+r), r = 0..3.
+    // v_mfma_f64_16x16x4f64 leaves D(i,j) with i = lane/16 + 4r, j = lane%16 in register r,
+    // so the lane holds C(m0+wm+mi*16+r16, n0+wn+ni*16+kq+4r), r = 0..3.
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
         const i64 m = m0 + wm + mi * 16 + r16;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(NT) void dgemm_kernel(i64 M, i64 N, i64 K, double a
         for (int ni = 0; ni < 4; ++ni) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const i64 n = n0 + wn + ni * 16 + 4 * kq + r;
+                const i64 n = n0 + wn + ni * 16 + kq + 4 * r;
                 if (n >= N) continue;
                 if (tri == 1 && m < n) continue;
                 if (tri == 2 && m > n) continue;

@@ -272,9 +272,10 @@ elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswap
     const int in_lds = need <= (size_t)di.max_lds_per_block ? 1 : 0;
     static bool attr = false;
     if (!attr) {
-        ELX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&swaps_to_perm_kernel),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&swaps_to_perm_kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    di.max_lds_per_block));
+                                    di.max_lds_per_block);
+        (void)hipGetLastError();
         attr = true;
     }
     hipLaunchKernelGGL(swaps_to_perm_kernel, dim3(1), dim3(256), in_lds ? need : 0, s, dests,
@@ -309,9 +310,10 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     const size_t lds = sizeof(T) * (size_t)order * TV;
     static bool attr = false;
     if (!attr) {
-        ELX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_small_kernel<T>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_small_kernel<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(sizeof(T) * 128 * TV)));
+                                    (int)(sizeof(T) * 128 * TV));
+        (void)hipGetLastError();
         attr = true;
     }
     hipLaunchKernelGGL(trsm_small_kernel<T>, dim3(cdiv(nvec, TV)), dim3(TV), lds, s, (int)order,
@@ -338,9 +340,10 @@ elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda
     const size_t lds = sizeof(double) * (size_t)n * (n + 1);
     static bool attr = false;
     if (!attr) {
-        ELX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&potf2_kernel<double>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&potf2_kernel<double>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(sizeof(double) * 128 * 129)));
+                                    (int)(sizeof(double) * 128 * 129));
+        (void)hipGetLastError();
         attr = true;
     }
     hipLaunchKernelGGL(potf2_kernel<double>, dim3(1), dim3(256), lds, s, (int)n, A, lda,

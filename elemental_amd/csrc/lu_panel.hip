@@ -290,8 +290,9 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     a.slot_lbl = reinterpret_cast<int*>(a.slot_row + (size_t)2 * di.num_cus * WMAX);
     static bool attr_set = false;
     if (!attr_set) {
-        ELX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cap));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cap);
+        (void)hipGetLastError();
         attr_set = true;
     }
     void* args[] = {&a};
