@@ -60,21 +60,26 @@ elx_status_t scratch_get(int slot, size_t bytes, void** out) {
     return ELX_SUCCESS;
 }
 
+// The exchange area of the cooperative panel kernel is uncached device memory (coherent
+// across XCDs without cache maintenance). Uncached memory that a kernel has read is never
+// freed before the process ends; a larger area replaces a smaller one and the old one is
+// kept.
 elx_status_t panel_sync_get(int max_groups, size_t slot_bytes, PanelSync** out) {
     int dev = 0;
     ELX_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_mu);
     PanelSync& ps = g_psync[dev];
     if (ps.max_groups < max_groups || ps.slot_bytes < slot_bytes) {
-        if (ps.flags) {
-            ELX_HIP(hipDeviceSynchronize());
-            ELX_HIP(hipFree(ps.flags));
-            ELX_HIP(hipFree(ps.slots));
-        }
-        ELX_HIP(hipMalloc(reinterpret_cast<void**>(&ps.flags), sizeof(int64_t) * max_groups));
-        ELX_HIP(hipMemset(ps.flags, 0, sizeof(int64_t) * max_groups));
-        ELX_HIP(hipMalloc(&ps.slots, slot_bytes));
+        if (ps.flags) ELX_HIP(hipDeviceSynchronize());
+        int64_t* flags = nullptr;
+        void* slots = nullptr;
+        ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&flags),
+                                      sizeof(int64_t) * max_groups, hipDeviceMallocUncached));
+        ELX_HIP(hipMemset(flags, 0, sizeof(int64_t) * max_groups));
+        ELX_HIP(hipExtMallocWithFlags(&slots, slot_bytes, hipDeviceMallocUncached));
         ELX_HIP(hipDeviceSynchronize());
+        ps.flags = flags;
+        ps.slots = slots;
         ps.max_groups = max_groups;
         ps.slot_bytes = slot_bytes;
         ps.epoch = 0;
@@ -89,11 +94,7 @@ elx_status_t release_all() {
     for (auto& kv : g_scratch)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     g_scratch.clear();
-    for (auto& kv : g_psync) {
-        if (kv.second.flags) (void)hipFree(kv.second.flags);
-        if (kv.second.slots) (void)hipFree(kv.second.slots);
-    }
-    g_psync.clear();
+    // the panel exchange areas (uncached memory) stay allocated until the process ends
     return ELX_SUCCESS;
 }
 

@@ -101,6 +101,11 @@ elx_status_t dtrsm(hipStream_t s, int side, int uplo, int orient, int diag, i64 
 elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
                                  int32_t* dests, i64 goff, int2* pairs, int32_t* info);
 
+elx_status_t launch_lu_panel_f32(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
+                                 int32_t* dests, i64 goff, int2* pairs, int32_t* info);
+elx_status_t launch_row_moves_f32(hipStream_t s, const int2* pairs, int npairs, float* A,
+                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3);
+
 // Apply a row-move list (pairs[i] = {dst, src}, panel-relative, -1 = none) to columns
 // [c0,c1) and [c2,c3) of A starting at row `r0`.
 elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, double* A,
@@ -113,6 +118,20 @@ elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswap
 // B_out(i,:) = B_in(p[i],:)
 elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i64 n,
                                     const double* Bin, i64 ldin, double* Bout, i64 ldout);
+
+// C(fp32, column-major) += alpha * 2^-(ea+eb) * A * B^T with A (m x k) and B (n x k) fp16
+// (prec 0) or bf16 (prec 1) stored K-major (ld a multiple of 8, >= k rounded up to 32,
+// zero-padded); MFMA with fp32 accumulation. ea_dev / eb_dev: device ints, may be null.
+elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, float alpha,
+                                  const uint16_t* A, i64 lda, const uint16_t* B, i64 ldb,
+                                  float* C, i64 ldc, const int* ea_dev, const int* eb_dev);
+// H(r,k) = cvt(sign * 2^e * S(r,k)), S(r,k) = S[r + k*lds] (trans false) or S[k + r*lds].
+elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
+                         const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
+                         i64 ldh);
+// e (device) with max|S| * 2^e < 2^15; bits points at a zeroed unsigned.
+elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
+                                   unsigned* bits, int* e_dev);
 
 // Unblocked Cholesky of the diagonal block (n <= 128) in one workgroup.
 elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,

@@ -49,21 +49,22 @@ __device__ __forceinline__ void load_slab(const double* __restrict__ P, i64 ld, 
 }
 
 template <bool KCONTIG>
-__device__ __forceinline__ void store_slab(double (*S)[BM + PAD], const double (&r)[8]) {
+__device__ __forceinline__ void store_slab(double (*S)[BM + PAD], const double (&r)[8],
+                                           double sgn = 1.0) {
     const int t = threadIdx.x;
     if (!KCONTIG) {
         const int kk = t >> 4, ii = (t & 15) * 8;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) S[kk][ii + q] = r[q];
+        for (int q = 0; q < 8; ++q) S[kk][ii + q] = sgn * r[q];
     } else {
         const int ii = t >> 1, kk = (t & 1) * 8;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) S[kk + q][ii] = r[q];
+        for (int q = 0; q < 8; ++q) S[kk + q][ii] = sgn * r[q];
     }
 }
 
 template <bool TA, bool TB>
-__global__ __launch_bounds__(NT) void dgemm_kernel(i64 M, i64 N, i64 K, double alpha,
+__global__ __launch_bounds__(NT, 2) void dgemm_kernel(i64 M, i64 N, i64 K, double alpha,
                                                    const double* __restrict__ A, i64 lda,
                                                    const double* __restrict__ B, i64 ldb,
                                                    double beta, double* __restrict__ C,
@@ -93,17 +94,30 @@ __global__ __launch_bounds__(NT) void dgemm_kernel(i64 M, i64 N, i64 K, double a
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     const int r16 = lane & 15, kq = lane >> 4;
 
+    // With beta == 1 and alpha == +-1 (every update of the factorizations) the accumulators
+    // start from C and alpha is folded into the A operand (negation is exact): the C tile is
+    // read while the first slabs load, and the epilogue is a plain store.
+    const bool fold = (beta == 1.0) && (alpha == 1.0 || alpha == -1.0);
+    const double sgn = fold ? alpha : 1.0;
     d4 acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int mi = 0; mi < 4; ++mi) {
+        const i64 m = m0 + wm + mi * 16 + r16;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 n = n0 + wn + ni * 16 + kq + 4 * r;
+                const bool ok = fold && m < M && n < N && !(tri == 1 && m < n) && !(tri == 2 && m > n);
+                acc[mi][ni][r] = ok ? C[m + n * ldc] : 0.0;
+            }
+    }
 
     double ra[8], rb[8];
     load_slab<TA>(A, lda, m0, M, 0, K, ra);
     load_slab<!TB>(B, ldb, n0, N, 0, K, rb);
     for (i64 k0 = 0; k0 < K; k0 += BK) {
-        store_slab<TA>(As, ra);
+        store_slab<TA>(As, ra, sgn);
         store_slab<!TB>(Bs, rb);
         __syncthreads();
         if (k0 + BK < K) {
@@ -144,7 +158,8 @@ __global__ __launch_bounds__(NT) void dgemm_kernel(i64 M, i64 N, i64 K, double a
                 if (tri == 2 && m > n) continue;
                 double* c = C + m + n * ldc;
                 const double v = acc[mi][ni][r];
-                *c = (beta == 0.0) ? alpha * v : __builtin_fma(alpha, v, beta * *c);
+                if (fold) *c = v;
+                else *c = (beta == 0.0) ? alpha * v : __builtin_fma(alpha, v, beta * *c);
             }
         }
     }

@@ -1,0 +1,294 @@
+// Low-precision GEMM for the trailing updates of the mixed-precision factorizations:
+//   C(fp32, column-major) += alpha * A * B^T
+// where A (M x K) and B (N x K) are fp16 or bf16 matrices stored K-major (row r of A is
+// K contiguous halves at A + r*lda), and the products accumulate in fp32 on the CDNA4
+// matrix cores (v_mfma_f32_16x16x32_f16 / _bf16).
+//
+// Tiling: one 512-thread workgroup owns a 256 x 256 tile of C; its eight waves (2 along M,
+// 4 along N) each own 128 x 64 as 8 x 4 MFMA accumulators (128 fp32 per lane). K advances 32
+// at a time through a double-buffered LDS stage (rows padded to 40 halves so the 16-byte
+// fragment reads of 16 consecutive rows hit disjoint banks), with the next slab's 16-byte
+// global loads in flight while the current one feeds the MFMAs. The MFMA operands are fed as
+// (B^T, A^T) so a lane's four results lie in one column of C: stores go out as 64-byte runs.
+//
+// Pack kernels produce the K-major operands from fp32 column-major data, with an exact
+// power-of-two scale (and sign) applied before rounding; absmax / exponent kernels choose the
+// scale that keeps an fp16 operand below 2^15.
+#include <algorithm>
+
+#include "elx_common.hpp"
+
+namespace {
+
+constexpr int BM = 256, BN = 256, BK = 32, LDK = BK + 8, NT = 512;
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+template <int PREC>
+struct HalfT;
+template <>
+struct HalfT<0> {
+    typedef _Float16 T;
+};
+template <>
+struct HalfT<1> {
+    typedef __bf16 T;
+};
+
+template <int PREC>
+__device__ __forceinline__ f4 mfma(u4 a, u4 b, f4 c) {
+    if constexpr (PREC == 0)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                      __builtin_bit_cast(h8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
+                                                       __builtin_bit_cast(b8, b), c, 0, 0, 0);
+}
+
+// One K-slab of a K-major operand: rows r0..r0+255, k0..k0+31; thread t loads 32 bytes of
+// row t/2 (halves 16*(t%2) .. +15). Rows >= R load zeros. The operand's K extent is padded
+// to a multiple of 32 with zeros by the pack kernels, so k never runs past the row.
+__device__ __forceinline__ void load_slab(const uint16_t* __restrict__ P, i64 ld, i64 r0, i64 R,
+                                          i64 k0, u4 (&r)[2]) {
+    const int t = threadIdx.x;
+    const i64 row = r0 + (t >> 1);
+    if (row < R) {
+        const u4* src = reinterpret_cast<const u4*>(P + row * ld + k0 + 16 * (t & 1));
+        r[0] = src[0];
+        r[1] = src[1];
+    } else {
+        r[0] = u4{0, 0, 0, 0};
+        r[1] = u4{0, 0, 0, 0};
+    }
+}
+
+__device__ __forceinline__ void store_slab(uint16_t* S, const u4 (&r)[2]) {
+    const int t = threadIdx.x;
+    u4* dst = reinterpret_cast<u4*>(S + (t >> 1) * LDK + 16 * (t & 1));
+    dst[0] = r[0];
+    dst[1] = r[1];
+}
+
+template <int PREC>
+__global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float alpha,
+                                                    const uint16_t* __restrict__ A, i64 lda,
+                                                    const uint16_t* __restrict__ B, i64 ldb,
+                                                    float* __restrict__ C, i64 ldc,
+                                                    const int* ea_dev, const int* eb_dev,
+                                                    int tiles_m, int tiles_n) {
+    __shared__ __align__(16) uint16_t As[2][BM * LDK];
+    __shared__ __align__(16) uint16_t Bs[2][BN * LDK];
+
+    const int total = tiles_m * tiles_n;
+    const int per_xcd = (total + 7) / 8;
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (lin >= total) return;
+    constexpr int GROUP = 4;
+    const int group = lin / (GROUP * tiles_n);
+    const int first_m = group * GROUP;
+    const int gsize = min(tiles_m - first_m, GROUP);
+    const int tm = first_m + (lin % (GROUP * tiles_n)) % gsize;
+    const int tn = (lin % (GROUP * tiles_n)) / gsize;
+    const i64 m0 = (i64)tm * BM, n0 = (i64)tn * BN;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave & 1) * 128, wn = (wave >> 1) * 64;
+    const int r16 = lane & 15, kq = lane >> 4;
+
+    f4 acc[4][8];
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int mi = 0; mi < 8; ++mi) acc[ni][mi] = f4{0.f, 0.f, 0.f, 0.f};
+
+    u4 ra[2], rb[2];
+    load_slab(A, lda, m0, M, 0, ra);
+    load_slab(B, ldb, n0, N, 0, rb);
+    store_slab(As[0], ra);
+    store_slab(Bs[0], rb);
+    __syncthreads();
+    int cur = 0;
+    for (i64 k0 = 0; k0 < K; k0 += BK) {
+        const bool more = (k0 + BK < K);
+        if (more) {
+            load_slab(A, lda, m0, M, k0 + BK, ra);
+            load_slab(B, ldb, n0, N, k0 + BK, rb);
+        }
+        const uint16_t* as = As[cur];
+        const uint16_t* bs = Bs[cur];
+        u4 af[8];
+#pragma unroll
+        for (int mi = 0; mi < 8; ++mi)
+            af[mi] = *reinterpret_cast<const u4*>(as + (wm + mi * 16 + r16) * LDK + 8 * kq);
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+            const u4 bf = *reinterpret_cast<const u4*>(bs + (wn + ni * 16 + r16) * LDK + 8 * kq);
+#pragma unroll
+            for (int mi = 0; mi < 8; ++mi) acc[ni][mi] = mfma<PREC>(bf, af[mi], acc[ni][mi]);
+        }
+        if (more) {
+            store_slab(As[cur ^ 1], ra);
+            store_slab(Bs[cur ^ 1], rb);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+
+    // Lane holds D(i = n, j = m) with n = n0+wn+ni*16+4*kq+r, m = m0+wm+mi*16+r16.
+    const int ea = ea_dev ? *ea_dev : 0, eb = eb_dev ? *eb_dev : 0;
+    const float a = alpha * __builtin_ldexpf(1.0f, -(ea + eb));
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const i64 n = n0 + wn + ni * 16 + 4 * kq + r;
+            if (n >= N) continue;
+#pragma unroll
+            for (int mi = 0; mi < 8; ++mi) {
+                const i64 m = m0 + wm + mi * 16 + r16;
+                if (m >= M) continue;
+                float* c = C + m + n * ldc;
+                *c = __builtin_fmaf(a, acc[ni][mi][r], *c);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------------------
+// Pack: H(r, k) = cvt(sign * 2^e * S(r, k)) for r < R, k < K; zeros for K <= k < Kpad.
+// S(r, k) = S[r + k*lds] (column-major, r fastest: TRANS = false) or S[k + r*lds]
+// (TRANS = true). The column-major case goes through a 64x64 LDS transpose so both the reads
+// and the K-major writes are coalesced.
+// ---------------------------------------------------------------------------------------
+template <int PREC, bool TRANS>
+__global__ __launch_bounds__(256) void pack_kernel(i64 R, i64 K, i64 Kpad, const float* __restrict__ S,
+                                                   i64 lds, float sign, const int* e_dev,
+                                                   uint16_t* __restrict__ H, i64 ldh) {
+    typedef typename HalfT<PREC>::T HT;
+    __shared__ float tile[64][65];
+    const float sc = sign * __builtin_ldexpf(1.0f, e_dev ? *e_dev : 0);
+    const i64 r0 = (i64)blockIdx.x * 64, k0 = (i64)blockIdx.y * 64;
+    const int t = threadIdx.x;
+    if (!TRANS) {
+        // read: 64 k-columns x 64 consecutive rows
+        for (int idx = t; idx < 64 * 64; idx += 256) {
+            const int rr = idx & 63, kk = idx >> 6;
+            const i64 r = r0 + rr, k = k0 + kk;
+            tile[kk][rr] = (r < R && k < K) ? S[r + k * lds] : 0.0f;
+        }
+        __syncthreads();
+        for (int idx = t; idx < 64 * 64; idx += 256) {
+            const int kk = idx & 63, rr = idx >> 6;
+            const i64 r = r0 + rr, k = k0 + kk;
+            if (r < R && k < Kpad) {
+                const HT h = (HT)(sc * tile[kk][rr]);
+                H[r * ldh + k] = __builtin_bit_cast(uint16_t, h);
+            }
+        }
+    } else {
+        for (int idx = t; idx < 64 * 64; idx += 256) {
+            const int kk = idx & 63, rr = idx >> 6;
+            const i64 r = r0 + rr, k = k0 + kk;
+            if (r < R && k < Kpad) {
+                const float v = (k < K) ? S[k + r * lds] : 0.0f;
+                const HT h = (HT)(sc * v);
+                H[r * ldh + k] = __builtin_bit_cast(uint16_t, h);
+            }
+        }
+    }
+}
+
+// max |S(i,j)| over an m x n column-major block, accumulated into *out (as uint bits of a
+// non-negative float; *out must start at 0).
+__global__ __launch_bounds__(256) void absmax_kernel(i64 m, i64 n, const float* __restrict__ S,
+                                                     i64 lds, unsigned* out) {
+    float mx = 0.0f;
+    const i64 total = m * n;
+    for (i64 idx = (i64)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (i64)gridDim.x * 256) {
+        const i64 i = idx % m, j = idx / m;
+        mx = fmaxf(mx, fabsf(S[i + j * lds]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    __shared__ float w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = fmaxf(fmaxf(w[0], w[1]), fmaxf(w[2], w[3]));
+        atomicMax(out, __float_as_uint(mx));
+    }
+}
+
+// e such that max * 2^e < 2^15 (fp16 headroom for K-term fp32 accumulation), clamped.
+__global__ void scale_exp_kernel(const unsigned* mx_bits, int* e_out) {
+    const float mx = __uint_as_float(*mx_bits);
+    int e = 0;
+    if (mx > 0.0f && isfinite(mx)) {
+        int p;
+        (void)frexpf(mx, &p);  // mx < 2^p
+        e = 15 - p;
+        e = max(-100, min(100, e));
+    }
+    *e_out = e;
+}
+
+}  // namespace
+
+namespace elx {
+
+elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, float alpha,
+                                  const uint16_t* A, i64 lda, const uint16_t* B, i64 ldb,
+                                  float* C, i64 ldc, const int* ea_dev, const int* eb_dev) {
+    if (m <= 0 || n <= 0 || k <= 0) return ELX_SUCCESS;
+    ELX_ARG(lda % 8 == 0 && ldb % 8 == 0 && lda >= ((k + BK - 1) / BK) * BK &&
+            ldb >= ((k + BK - 1) / BK) * BK);
+    const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    ELX_ARG(tm * tn < ((i64)1 << 30));
+    const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
+    if (prec == 0)
+        hipLaunchKernelGGL(lpgemm_kernel<0>, dim3(blocks), dim3(NT), 0, s, m, n, k, alpha, A, lda,
+                           B, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
+    else
+        hipLaunchKernelGGL(lpgemm_kernel<1>, dim3(blocks), dim3(NT), 0, s, m, n, k, alpha, A, lda,
+                           B, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
+                         const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
+                         i64 ldh) {
+    if (R <= 0 || Kpad <= 0) return ELX_SUCCESS;
+    ELX_ARG(ldh >= Kpad);
+    dim3 grid(cdiv(R, 64), cdiv(Kpad, 64));
+    ELX_ARG(grid.y <= 65535);
+    if (prec == 0) {
+        if (trans)
+            hipLaunchKernelGGL((pack_kernel<0, true>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
+        else
+            hipLaunchKernelGGL((pack_kernel<0, false>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
+    } else {
+        if (trans)
+            hipLaunchKernelGGL((pack_kernel<1, true>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
+        else
+            hipLaunchKernelGGL((pack_kernel<1, false>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
+    }
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+// Exponent e (device int) with max|S| * 2^e < 2^15; bits must point at a zeroed unsigned.
+elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
+                                   unsigned* bits, int* e_dev) {
+    if (m > 0 && n > 0) {
+        const unsigned blocks = (unsigned)std::min<i64>((m * n + 255) / 256, 2048);
+        hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(256), 0, s, m, n, S, lds, bits);
+        ELX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(scale_exp_kernel, dim3(1), dim3(1), 0, s, bits, e_dev);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+}  // namespace elx

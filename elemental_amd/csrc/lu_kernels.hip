@@ -95,25 +95,32 @@ __global__ void gather_rows_kernel(const int32_t* __restrict__ perm, i64 m, i64 
 
 // ---------------------------------------------------------------------------------------
 // Triangular solves M x = b for many independent vectors with an order <= 128 triangle.
-// Each thread owns one vector, staged in LDS; M is read with wave-uniform (scalar) loads.
 // Left side: the vectors are columns of B and M = op(A). Right side (X op(A) = B): the
-// vectors are rows of B and M = op(A)^T. The update order is the column-oriented (axpy)
-// substitution of the reference BLAS dtrsm: x_i -= M(i,j) x_j for j = 0,1,... in turn, one
-// FMA each, with division by the diagonal on the left side and multiplication by its
-// reciprocal on the right side (the two forms the reference BLAS uses).
+// vectors are rows of B and M = op(A)^T. The vectors are staged in LDS; each vector is worked
+// on by the TP lanes of one 16-lane quarter-wave, which split the rows of every update step.
+// The update order is the reference BLAS dtrsm's: in forward and column-oriented backward
+// substitution x_i -= M(i,j) x_j is applied for j in turn, one FMA each; for the transposed
+// backward forms the reference BLAS accumulates each x_i as a dot product in ascending k,
+// which one lane of the quarter-wave does alone. Division by the diagonal on the left side,
+// multiplication by its reciprocal on the right side, as the reference BLAS does.
+// The lanes of a vector live in the same wave and LDS operations of a wave complete in
+// order, so a step reads what the previous step wrote without a workgroup barrier.
 // ---------------------------------------------------------------------------------------
-constexpr int TV = 64;  // vectors per workgroup
+constexpr int TP = 16;           // lanes per vector
+constexpr int TNT = 256;         // threads per workgroup
+constexpr int TV = TNT / TP;     // vectors per workgroup
 
 template <typename T>
-__global__ __launch_bounds__(TV) void trsm_small_kernel(int m, i64 nvec, const T* __restrict__ A,
-                                                        i64 lda, T* B, i64 ldb, int right,
-                                                        int lower_eff, int mtrans, int unit) {
+__global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const T* __restrict__ A,
+                                                         i64 lda, T* B, i64 ldb, int right,
+                                                         int lower_eff, int mtrans, int unit) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* x = reinterpret_cast<T*>(smem_raw);  // x[i * TV + v]
     const int t = threadIdx.x;
+    const int v = t / TP, q = t % TP;
     const i64 v0 = (i64)blockIdx.x * TV;
     // Coalesced staging: left side walks down each column, right side across rows.
-    for (int idx = t; idx < m * TV; idx += TV) {
+    for (int idx = t; idx < m * TV; idx += TNT) {
         int i, vv;
         if (!right) {
             i = idx % m;
@@ -122,50 +129,42 @@ __global__ __launch_bounds__(TV) void trsm_small_kernel(int m, i64 nvec, const T
             vv = idx % TV;
             i = idx / TV;
         }
-        const i64 v = v0 + vv;
+        const i64 gv = v0 + vv;
         T val = T(0);
-        if (v < nvec) val = right ? B[v + (i64)i * ldb] : B[i + v * ldb];
+        if (gv < nvec) val = right ? B[gv + (i64)i * ldb] : B[i + gv * ldb];
         x[i * TV + vv] = val;
     }
     __syncthreads();
-    // M(i,j) = A(i,j) or A(j,i)
     auto M = [&](int i, int j) -> T { return mtrans ? A[j + (i64)i * lda] : A[i + (i64)j * lda]; };
     if (lower_eff) {
         for (int j = 0; j < m; ++j) {
-            T xj = x[j * TV + t];
-            if (!unit) {
-                if (right) xj = xj * (T(1) / M(j, j));
-                else xj = xj / M(j, j);
-                x[j * TV + t] = xj;
-            }
-            for (int i = j + 1; i < m; ++i) x[i * TV + t] = __builtin_fma(-M(i, j), xj, x[i * TV + t]);
+            T xj = x[j * TV + v];
+            if (!unit) xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
+            __builtin_amdgcn_wave_barrier();
+            if (!unit && q == 0) x[j * TV + v] = xj;
+            for (int i = j + 1 + q; i < m; i += TP)
+                x[i * TV + v] = __builtin_fma(-M(i, j), xj, x[i * TV + v]);
+            __builtin_amdgcn_wave_barrier();
         }
     } else if (!mtrans) {
-        // backward, axpy form: x_i receives its updates in descending j
         for (int j = m - 1; j >= 0; --j) {
-            T xj = x[j * TV + t];
-            if (!unit) {
-                if (right) xj = xj * (T(1) / M(j, j));
-                else xj = xj / M(j, j);
-                x[j * TV + t] = xj;
-            }
-            for (int i = 0; i < j; ++i) x[i * TV + t] = __builtin_fma(-M(i, j), xj, x[i * TV + t]);
+            T xj = x[j * TV + v];
+            if (!unit) xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
+            __builtin_amdgcn_wave_barrier();
+            if (!unit && q == 0) x[j * TV + v] = xj;
+            for (int i = q; i < j; i += TP) x[i * TV + v] = __builtin_fma(-M(i, j), xj, x[i * TV + v]);
+            __builtin_amdgcn_wave_barrier();
         }
-    } else {
-        // backward, dot form (the reference BLAS order for transposed-left and
-        // untransposed-right solves): x_i accumulates its terms in ascending k
+    } else if (q == 0) {
         for (int i = m - 1; i >= 0; --i) {
-            T temp = x[i * TV + t];
-            for (int k = i + 1; k < m; ++k) temp = __builtin_fma(-M(i, k), x[k * TV + t], temp);
-            if (!unit) {
-                if (right) temp = temp * (T(1) / M(i, i));
-                else temp = temp / M(i, i);
-            }
-            x[i * TV + t] = temp;
+            T temp = x[i * TV + v];
+            for (int k = i + 1; k < m; ++k) temp = __builtin_fma(-M(i, k), x[k * TV + v], temp);
+            if (!unit) temp = right ? temp * (T(1) / M(i, i)) : temp / M(i, i);
+            x[i * TV + v] = temp;
         }
     }
     __syncthreads();
-    for (int idx = t; idx < m * TV; idx += TV) {
+    for (int idx = t; idx < m * TV; idx += TNT) {
         int i, vv;
         if (!right) {
             i = idx % m;
@@ -174,10 +173,10 @@ __global__ __launch_bounds__(TV) void trsm_small_kernel(int m, i64 nvec, const T
             vv = idx % TV;
             i = idx / TV;
         }
-        const i64 v = v0 + vv;
-        if (v < nvec) {
-            if (right) B[v + (i64)i * ldb] = x[i * TV + vv];
-            else B[i + v * ldb] = x[i * TV + vv];
+        const i64 gv = v0 + vv;
+        if (gv < nvec) {
+            if (right) B[gv + (i64)i * ldb] = x[i * TV + vv];
+            else B[i + gv * ldb] = x[i * TV + vv];
         }
     }
 }
@@ -264,6 +263,19 @@ elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, 
     return ELX_SUCCESS;
 }
 
+elx_status_t launch_row_moves_f32(hipStream_t s, const int2* pairs, int npairs, float* A,
+                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3) {
+    ELX_ARG(npairs <= 256);
+    const i64 total = std::max<i64>(c1 - c0, 0) + std::max<i64>(c3 - c2, 0);
+    if (total <= 0 || npairs <= 0) return ELX_SUCCESS;
+    if (c1 < c0) c1 = c0;
+    if (c3 < c2) c3 = c2;
+    hipLaunchKernelGGL(row_moves_kernel<float>, dim3(cdiv(total, 4)), dim3(256), 0, s, pairs,
+                       npairs, A, lda, r0, c0, c1, c2, c3);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
 elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswaps, i64 size,
                                   int32_t* perm, int inverse) {
     if (size <= 0) return ELX_SUCCESS;
@@ -316,7 +328,7 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
         (void)hipGetLastError();
         attr = true;
     }
-    hipLaunchKernelGGL(trsm_small_kernel<T>, dim3(cdiv(nvec, TV)), dim3(TV), lds, s, (int)order,
+    hipLaunchKernelGGL(trsm_small_kernel<T>, dim3(cdiv(nvec, TV)), dim3(TNT), lds, s, (int)order,
                        nvec, A, lda, B, ldb, right ? 1 : 0, lower_eff ? 1 : 0, mtrans ? 1 : 0,
                        diag == kUnit ? 1 : 0);
     ELX_HIP(hipGetLastError());

@@ -48,6 +48,21 @@ struct PanelArgs {
     T* slot_row;     // [2][G][WMAX] candidate row (columns c..w-1 valid)
 };
 
+// The exchange area lives in uncached device memory (allocated with
+// hipDeviceMallocUncached), so loads and stores go straight to memory and are coherent across
+// the XCDs without L2 write-back / invalidate. Accesses are relaxed agent-scope atomics (the
+// compiler may neither cache nor merge them); ordering of a workgroup's data before its flag
+// comes from waiting for the stores to complete (s_waitcnt) before the flag store.
+template <typename T>
+__device__ __forceinline__ void st_x(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_x(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
+
 template <typename T>
 __device__ __forceinline__ bool better(T v, int l, T bv, int bl) {
     return (v > bv) || (v == bv && l < bl);
@@ -129,27 +144,24 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         }
         block_best(bv, bl, br, sv, sl, sq);
 
-        // Publish the candidate (value with sign, label, row) for this column.
+        // Publish the candidate (value with sign, label, row) for this column, then the flag.
         T* myrow = p.slot_row + ((size_t)(s * G + g)) * WMAX;
         if (br >= 0)
-            for (int j = c + t; j < W; j += NT) myrow[j] = tile[br * LD + j];
+            for (int j = c + t; j < W; j += NT) st_x(&myrow[j], tile[br * LD + j]);
         if (t == 0) {
-            p.slot_val[s * G + g] = (br >= 0) ? tile[br * LD + c] : T(0);
-            p.slot_lbl[s * G + g] = bl;
+            st_x(&p.slot_val[s * G + g], (br >= 0) ? tile[br * LD + c] : T(0));
+            st_x(&p.slot_lbl[s * G + g], bl);
         }
-        __threadfence();
+        stores_complete();
         __syncthreads();
         const int64_t target = p.epoch_base + c + 1;
-        if (t == 0)
-            __hip_atomic_store(&p.flags[g], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == 0) st_x(&p.flags[g], target);
 
         // Wait until every workgroup has published column c.
         long long spins = 0;
         while (true) {
             int ok = 1;
-            for (int q = t; q < G; q += NT)
-                ok &= (__hip_atomic_load(&p.flags[q], __ATOMIC_ACQUIRE,
-                                         __HIP_MEMORY_SCOPE_AGENT) >= target);
+            for (int q = t; q < G; q += NT) ok &= (ld_x(&p.flags[q]) >= target);
             if (__syncthreads_and(ok)) break;
             if (++spins > SPIN_LIMIT) {
                 aborted = true;
@@ -157,15 +169,14 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             }
         }
         if (aborted) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
         // Every workgroup reduces the same G candidates to the same winner.
         T gv = T(-1);
         int gl = INT_MAX, gw = -1;
         for (int q = t; q < G; q += NT) {
-            const int L = p.slot_lbl[s * G + q];
+            const int L = ld_x(&p.slot_lbl[s * G + q]);
             if (L != INT_MAX) {
-                const T v = fabs(p.slot_val[s * G + q]);
+                const T v = fabs(ld_x(&p.slot_val[s * G + q]));
                 if (better(v, L, gv, gl)) {
                     gv = v;
                     gl = L;
@@ -186,7 +197,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         }
         const int piv = gl;
         const T* wrow = p.slot_row + ((size_t)(s * G + gw)) * WMAX;
-        for (int j = c + t; j < W; j += NT) urow[j] = wrow[j];
+        for (int j = c + t; j < W; j += NT) urow[j] = ld_x(&wrow[j]);
         // Swap = relabel: the row at position c moves to position piv ...
         for (int r = t; r < rows; r += NT)
             if (lbl[r] == c) lbl[r] = piv;
@@ -308,6 +319,11 @@ namespace elx {
 elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
                                  int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
     return launch_panel<double>(s, mp, w, A, lda, dests, goff, pairs, info);
+}
+
+elx_status_t launch_lu_panel_f32(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
+                                 int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
+    return launch_panel<float>(s, mp, w, A, lda, dests, goff, pairs, info);
 }
 
 }  // namespace elx

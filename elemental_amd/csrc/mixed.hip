@@ -1,0 +1,446 @@
+// Mixed-precision dense solve: low-precision LU (fp16 / bf16 trailing updates with fp32
+// accumulation, fp32 panels and factor storage) recovered to FP64 backward error by
+// iterative refinement with FP64 residuals.
+//
+// The refinement follows the semantics of El::RefinedSolve
+// (include/El/lapack_like/solve/Refined.hpp:44-119): the residual is formed in the higher
+// precision, the correction comes from the low-precision solve, and the iteration stops at
+// the relative tolerance or when the residual stops decreasing. There is no dense
+// mixed-precision solver in the reference; this path has no reference counterpart beyond
+// those semantics.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "elx_common.hpp"
+#include "mixed.hpp"
+
+namespace elx {
+
+namespace {
+
+// Scratch slots (elx_api.hip uses 0..3)
+enum {
+    kSlotW = 10, kSlotLh, kSlotUh, kSlotLinv, kSlotUinv, kSlotIdx, kSlotVec, kSlotPart,
+    kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP
+};
+
+// Uncached buffers of the persistent triangular solves (never freed before the process
+// ends, see panel_sync_get).
+struct TrsvArea {
+    int64_t* flags = nullptr;
+    int* err = nullptr;
+    float* xa = nullptr;
+    float* xb = nullptr;
+    i64 n = 0;
+    int64_t epoch = 0;
+};
+std::mutex g_tmu;
+std::map<int, TrsvArea> g_trsv;
+
+elx_status_t trsv_area(i64 n, TrsvArea** out) {
+    int dev = 0;
+    ELX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tmu);
+    TrsvArea& a = g_trsv[dev];
+    if (a.n < n) {
+        if (a.flags) ELX_HIP(hipDeviceSynchronize());
+        const i64 nblk = (n + 127) / 128;
+        int64_t* flags = nullptr;
+        float *xa = nullptr, *xb = nullptr;
+        ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&flags), sizeof(int64_t) * nblk,
+                                      hipDeviceMallocUncached));
+        ELX_HIP(hipMemset(flags, 0, sizeof(int64_t) * nblk));
+        int* err = nullptr;
+        ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&err), 64, hipDeviceMallocUncached));
+        ELX_HIP(hipMemset(err, 0, 64));
+        a.err = err;
+        ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&xa), sizeof(float) * n,
+                                      hipDeviceMallocUncached));
+        ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&xb), sizeof(float) * n,
+                                      hipDeviceMallocUncached));
+        ELX_HIP(hipDeviceSynchronize());
+        a.flags = flags;
+        a.xa = xa;
+        a.xb = xb;
+        a.n = n;
+        a.epoch = 0;
+    }
+    *out = &a;
+    return ELX_SUCCESS;
+}
+
+template <typename T>
+elx_status_t get(int slot, size_t count, T** p) {
+    void* v = nullptr;
+    ELX_CHECK(scratch_get(slot, sizeof(T) * std::max<size_t>(count, 1), &v));
+    *p = static_cast<T*>(v);
+    return ELX_SUCCESS;
+}
+
+inline i64 round_up(i64 a, i64 b) { return ((a + b - 1) / b) * b; }
+
+struct Ctx {
+    hipStream_t s;
+    int prec;  // 0 fp16, 1 bf16
+    i64 n, NB;
+    float* W;
+    i64 ldw;
+    uint16_t* Lh;
+    uint16_t* Uh;
+    float* Linv;
+    float* Uinv;
+    int32_t* dests;
+    int32_t* perm;
+    int2* pairs;  // (NB/128) x 256
+    int32_t* info;
+    int* e_dev;
+    unsigned* bits;
+};
+
+// C -= L * U for the column-major fp32 blocks L (m x k at Ls) and U (k x n at Us), with the
+// operands rounded to fp16/bf16 (U scaled by a power of two for fp16).
+elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Ls, const float* Us, float* C,
+                       i64 ld, i64 ldpack) {
+    if (m <= 0 || n <= 0 || k <= 0) return ELX_SUCCESS;
+    const i64 kpad = round_up(k, 32);
+    const int* e = nullptr;
+    if (c.prec == 0) {
+        ELX_HIP(hipMemsetAsync(c.bits, 0, sizeof(unsigned), c.s));
+        ELX_CHECK(launch_scale_exponent(c.s, k, n, Us, ld, c.bits, c.e_dev));
+        e = c.e_dev;
+    }
+    ELX_CHECK(launch_pack(c.s, c.prec, false, m, k, kpad, Ls, ld, 1.0f, nullptr, c.Lh, ldpack));
+    ELX_CHECK(launch_pack(c.s, c.prec, true, n, k, kpad, Us, ld, 1.0f, e, c.Uh, ldpack));
+    return launch_lpgemm_packed(c.s, c.prec, m, n, k, -1.0f, c.Lh, ldpack, c.Uh, ldpack, C, ld,
+                                nullptr, e);
+}
+
+// Two-level blocked LU with partial pivoting of W (fp32): 128-column cooperative panels
+// inside NB-column outer blocks; the outer trailing update is one rank-NB low-precision GEMM.
+elx_status_t getrf_mixed(Ctx& c) {
+    const i64 n = c.n, ld = c.ldw;
+    float* W = c.W;
+    ELX_HIP(hipMemsetAsync(c.info, 0, sizeof(int32_t), c.s));
+    for (i64 K0 = 0; K0 < n; K0 += c.NB) {
+        const i64 nbk = std::min(c.NB, n - K0);
+        const i64 K1 = K0 + nbk;
+        for (i64 j0 = K0; j0 < K1; j0 += 128) {
+            const i64 w = std::min<i64>(128, K1 - j0);
+            int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
+            ELX_CHECK(launch_lu_panel_f32(c.s, n - j0, w, W + j0 + j0 * ld, ld, c.dests + j0, j0,
+                                          pr, c.info));
+            ELX_CHECK(launch_row_moves_f32(c.s, pr, (int)(2 * w), W, ld, j0, K0, j0, j0 + w, K1));
+            const i64 wr = K1 - (j0 + w);
+            if (wr > 0) {
+                ELX_CHECK(launch_trsm_small_f32(c.s, kLeft, kLower, kNormal, kUnit, w, wr,
+                                                W + j0 + j0 * ld, ld, W + j0 + (j0 + w) * ld, ld));
+                ELX_CHECK(lp_update(c, n - (j0 + w), wr, w, W + (j0 + w) + j0 * ld,
+                                    W + j0 + (j0 + w) * ld, W + (j0 + w) + (j0 + w) * ld, ld, 128));
+            }
+        }
+        // the outer block's swaps on the columns outside it, in panel order
+        for (i64 j0 = K0; j0 < K1; j0 += 128) {
+            const i64 w = std::min<i64>(128, K1 - j0);
+            const int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
+            ELX_CHECK(launch_row_moves_f32(c.s, pr, (int)(2 * w), W, ld, j0, 0, K0, K1, n));
+        }
+        const i64 n2 = n - K1;
+        if (n2 <= 0) continue;
+        // U12 = L11^{-1} A12, blocked by 128
+        for (i64 jb = K0; jb < K1; jb += 128) {
+            const i64 wb = std::min<i64>(128, K1 - jb);
+            ELX_CHECK(launch_trsm_small_f32(c.s, kLeft, kLower, kNormal, kUnit, wb, n2,
+                                            W + jb + jb * ld, ld, W + jb + K1 * ld, ld));
+            const i64 below = K1 - (jb + wb);
+            if (below > 0)
+                ELX_CHECK(lp_update(c, below, n2, wb, W + (jb + wb) + jb * ld, W + jb + K1 * ld,
+                                    W + (jb + wb) + K1 * ld, ld, 128));
+        }
+        // A22 -= L21 U12 (rank nbk)
+        ELX_CHECK(lp_update(c, n - K1, n2, nbk, W + K1 + K0 * ld, W + K0 + K1 * ld,
+                            W + K1 + K1 * ld, ld, round_up(c.NB, 32)));
+    }
+    return ELX_SUCCESS;
+}
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, s);
+    }
+    double stop() {
+        (void)hipEventRecord(b, s);
+        (void)hipEventSynchronize(b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return ms * 1e-3;
+    }
+    ~Timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+template <typename T>
+elx_status_t d2h(T* dst, const T* src, size_t count, hipStream_t s) {
+    ELX_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+    ELX_HIP(hipStreamSynchronize(s));
+    return ELX_SUCCESS;
+}
+
+}  // namespace
+
+}  // namespace elx
+
+using namespace elx;
+
+extern "C" {
+
+elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double* dA,
+                             int64_t lda, const double* dB, int64_t ldb, double* dX,
+                             int64_t ldx, int factor_type, int refine, double tol,
+                             int32_t max_iters, elx_ir_stats_t* stats) {
+    ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) &&
+            ldb >= std::max<int64_t>(n, 1) && ldx >= std::max<int64_t>(n, 1));
+    ELX_ARG(factor_type == ELX_FACTOR_FP16 || factor_type == ELX_FACTOR_BF16);
+    ELX_ARG(refine >= ELX_REFINE_NONE && refine <= ELX_REFINE_AUTO);
+    elx_ir_stats_t st;
+    std::memset(&st, 0, sizeof(st));
+    if (n == 0 || nrhs == 0) {
+        st.converged = 1;
+        if (stats) *stats = st;
+        return ELX_SUCCESS;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const double u64 = std::ldexp(1.0, -53);
+    if (!(tol > 0.0)) tol = (double)n * u64;
+    if (max_iters <= 0) max_iters = 30;
+
+    Ctx c;
+    c.s = s;
+    c.prec = (factor_type == ELX_FACTOR_FP16) ? 0 : 1;
+    c.n = n;
+    c.NB = std::min<i64>(1024, round_up(n, 128));
+    c.ldw = n;
+    const i64 nblk = (n + 127) / 128;
+    const i64 nbpad = round_up(c.NB, 32);
+    const int nch = (int)std::max<i64>(1, (n + 1023) / 1024);
+    ELX_CHECK(get(kSlotW, (size_t)n * (size_t)n, &c.W));
+    ELX_CHECK(get(kSlotLh, (size_t)n * (size_t)nbpad, &c.Lh));
+    ELX_CHECK(get(kSlotUh, (size_t)n * (size_t)nbpad, &c.Uh));
+    ELX_CHECK(get(kSlotLinv, (size_t)nblk * 128 * 128, &c.Linv));
+    ELX_CHECK(get(kSlotUinv, (size_t)nblk * 128 * 128, &c.Uinv));
+    int32_t* idx = nullptr;
+    ELX_CHECK(get(kSlotIdx, (size_t)(2 * n + 8 + (c.NB / 128 + 1) * 512), &idx));
+    c.dests = idx;
+    c.perm = idx + n;
+    c.info = idx + 2 * n;
+    c.e_dev = idx + 2 * n + 1;
+    c.bits = reinterpret_cast<unsigned*>(idx + 2 * n + 2);
+    c.pairs = reinterpret_cast<int2*>(idx + 2 * n + 8);
+    double* vec = nullptr;  // r, norms(4), anorm
+    ELX_CHECK(get(kSlotVec, (size_t)(n + 8), &vec));
+    double* r = vec;
+    double* norms = vec + n;   // [0..2]
+    double* anorm_d = vec + n + 4;
+    double* part = nullptr;
+    ELX_CHECK(get(kSlotPart, (size_t)nch * (size_t)n, &part));
+    float* y = nullptr;
+    ELX_CHECK(get(kSlotSmall, (size_t)n, &y));
+    TrsvArea* ta = nullptr;
+    ELX_CHECK(trsv_area(n, &ta));
+    int* err = ta->err;
+    ELX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+
+    // ---- factorization ----
+    Timer tf(s);
+    ELX_HIP(hipMemsetAsync(idx + 2 * n, 0, sizeof(int32_t) * 8, s));
+    ELX_CHECK(launch_convert(s, n, dA, lda, c.W, c.ldw, part, nch));
+    ELX_HIP(hipMemsetAsync(anorm_d, 0, sizeof(double), s));
+    ELX_CHECK(launch_rowabs_max(s, n, nch, part, anorm_d));
+    ELX_CHECK(getrf_mixed(c));
+    ELX_CHECK(launch_swaps_to_perm(s, c.dests, n, n, c.perm, 0));
+    ELX_CHECK(launch_diag_inverse(s, n, c.W, c.ldw, 0, c.Linv));
+    ELX_CHECK(launch_diag_inverse(s, n, c.W, c.ldw, 1, c.Uinv));
+    st.factor_seconds = tf.stop();
+    int32_t finfo = 0;
+    double anorm = 0.0;
+    ELX_CHECK(d2h(&finfo, c.info, 1, s));
+    ELX_CHECK(d2h(&anorm, anorm_d, 1, s));
+    st.factor_info = finfo;
+    if (finfo != 0) {
+        if (stats) *stats = st;
+        return ELX_ERROR_NUMERIC;
+    }
+
+    // ---- refinement, one right-hand side at a time ----
+    const int m_restart = 50;
+    double* V = nullptr;
+    double* Z = nullptr;
+    double* gw = nullptr;   // w, d, h(2 x 64)
+    double* gpart = nullptr;
+    if (refine == ELX_REFINE_GMRES || refine == ELX_REFINE_AUTO) {
+        ELX_CHECK(get(kSlotKrylovV, (size_t)n * (m_restart + 1), &V));
+        ELX_CHECK(get(kSlotKrylovZ, (size_t)n * m_restart, &Z));
+        ELX_CHECK(get(kSlotKrylovW, (size_t)2 * n + 256, &gw));
+        ELX_CHECK(get(kSlotKrylovP, (size_t)((n + 1023) / 1024) * 64 + 64, &gpart));
+    }
+    Timer tr(s);
+    bool all_ok = true;
+    double worst = 0.0;
+    int inner_total = 0;
+    for (i64 col = 0; col < nrhs; ++col) {
+        const double* b = dB + col * ldb;
+        double* x = dX + col * ldx;
+        auto precond = [&](const double* rin, double* xout, int accumulate) -> elx_status_t {
+            ELX_CHECK(launch_gather_to_f32(s, n, c.perm, rin, y));
+            const int64_t t1 = ++ta->epoch;
+            ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Linv, 1, y, ta->xa, ta->flags, t1, err));
+            const int64_t t2 = ++ta->epoch;
+            ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Uinv, 0, ta->xa, ta->xb, ta->flags, t2, err));
+            return launch_update_f64(s, n, ta->xb, xout, accumulate);
+        };
+        // ||v||_2 on the device, read back
+        auto nrm2 = [&](const double* v, double* out) -> elx_status_t {
+            double* h = gw + 2 * n;
+            ELX_CHECK(launch_multidot(s, n, 1, v, n, v, gpart, h));
+            double hh = 0.0;
+            ELX_CHECK(d2h(&hh, h, 1, s));
+            *out = std::sqrt(std::max(hh, 0.0));
+            return ELX_SUCCESS;
+        };
+        // Right-preconditioned flexible GMRES on A d = r (El::FGMRES semantics,
+        // include/El/lapack_like/solve/FGMRES.hpp:37-291): z_j = M^{-1} v_j, w = A z_j,
+        // orthogonalization (classical Gram-Schmidt applied twice), Givens rotations on H,
+        // stop when the rotated residual |t_{j+1}| <= relTol * ||r||. d = Z y on exit.
+        auto fgmres = [&](const double* rr, double* d, double relTol, int* iters) -> elx_status_t {
+            double* w = gw;
+            double* h = gw + 2 * n;
+            double beta = 0.0;
+            ELX_CHECK(nrm2(rr, &beta));
+            *iters = 0;
+            ELX_HIP(hipMemsetAsync(d, 0, sizeof(double) * n, s));
+            if (!(beta > 0.0) || !std::isfinite(beta)) return ELX_SUCCESS;
+            ELX_CHECK(launch_scal_copy(s, n, 1.0 / beta, rr, V));
+            std::vector<double> H((size_t)(m_restart + 1) * m_restart, 0.0), cs(m_restart), sn(m_restart),
+                t(m_restart + 1, 0.0);
+            t[0] = beta;
+            int jend = 0;
+            for (int j = 0; j < m_restart; ++j) {
+                ELX_CHECK(precond(V + (i64)j * n, Z + (i64)j * n, 0));
+                ELX_HIP(hipMemsetAsync(norms, 0, sizeof(double) * 3, s));
+                ELX_CHECK(launch_residual(s, n, dA, lda, Z + (i64)j * n, nullptr, w, part, nch, norms));
+                std::vector<double> hv(j + 1, 0.0), h2(j + 1);
+                for (int pass = 0; pass < 2; ++pass) {
+                    ELX_CHECK(launch_multidot(s, n, j + 1, V, n, w, gpart, h));
+                    ELX_CHECK(d2h(h2.data(), h, (size_t)(j + 1), s));
+                    ELX_CHECK(launch_multiaxpy(s, n, j + 1, -1.0, V, n, h, w));
+                    for (int i = 0; i <= j; ++i) hv[i] += h2[i];
+                }
+                double delta = 0.0;
+                ELX_CHECK(nrm2(w, &delta));
+                if (!std::isfinite(delta)) return ELX_ERROR_NUMERIC;
+                if (j + 1 < m_restart && delta > 0.0)
+                    ELX_CHECK(launch_scal_copy(s, n, 1.0 / delta, w, V + (i64)(j + 1) * n));
+                double* Hj = H.data() + (size_t)j * (m_restart + 1);
+                for (int i = 0; i <= j; ++i) Hj[i] = hv[i];
+                Hj[j + 1] = delta;
+                for (int i = 0; i < j; ++i) {
+                    const double e0 = Hj[i], e1 = Hj[i + 1];
+                    Hj[i] = cs[i] * e0 + sn[i] * e1;
+                    Hj[i + 1] = -sn[i] * e0 + cs[i] * e1;
+                }
+                const double rho = std::hypot(Hj[j], Hj[j + 1]);
+                if (!(rho > 0.0)) return ELX_ERROR_NUMERIC;
+                cs[j] = Hj[j] / rho;
+                sn[j] = Hj[j + 1] / rho;
+                Hj[j] = rho;
+                Hj[j + 1] = 0.0;
+                const double t0 = t[j], t1 = t[j + 1];
+                t[j] = cs[j] * t0 + sn[j] * t1;
+                t[j + 1] = -sn[j] * t0 + cs[j] * t1;
+                jend = j + 1;
+                if (std::fabs(t[j + 1]) <= relTol * beta || delta == 0.0) break;
+            }
+            // y = H^{-1} t (upper triangular), d = Z y
+            std::vector<double> yv(jend);
+            for (int i = jend - 1; i >= 0; --i) {
+                double v = t[i];
+                for (int k = i + 1; k < jend; ++k) v -= H[(size_t)k * (m_restart + 1) + i] * yv[k];
+                yv[i] = v / H[(size_t)i * (m_restart + 1) + i];
+            }
+            ELX_HIP(hipMemcpyAsync(h, yv.data(), sizeof(double) * jend, hipMemcpyHostToDevice, s));
+            ELX_CHECK(launch_multiaxpy(s, n, jend, 1.0, Z, n, h, d));
+            ELX_HIP(hipStreamSynchronize(s));
+            *iters = jend;
+            return ELX_SUCCESS;
+        };
+        // x0 = M^{-1} b
+        ELX_CHECK(precond(b, x, 0));
+        double eta = INFINITY, eta_prev = INFINITY;
+        int it = 0;
+        bool ok = false;
+        bool use_gmres = (refine == ELX_REFINE_GMRES);
+        for (;;) {
+            ELX_HIP(hipMemsetAsync(norms, 0, sizeof(double) * 3, s));
+            ELX_CHECK(launch_residual(s, n, dA, lda, x, b, r, part, nch, norms));
+            double hn[3];
+            ELX_CHECK(d2h(hn, norms, 3, s));
+            const double den = anorm * hn[1] + hn[2];
+            eta = den > 0 ? hn[0] / den : (hn[0] > 0 ? INFINITY : 0.0);
+            if (eta <= tol) {
+                ok = true;
+                break;
+            }
+            if (refine == ELX_REFINE_NONE || it >= max_iters) break;
+            if (!use_gmres) {
+                const bool slow = it > 0 && !(eta < 0.1 * eta_prev);
+                if (slow) {
+                    if (refine == ELX_REFINE_AUTO) use_gmres = true;
+                    else if (!(eta < 0.5 * eta_prev)) break;  // IR stagnated
+                }
+            } else if (it > 0 && !(eta < 0.5 * eta_prev)) {
+                break;  // GMRES-IR stagnated
+            }
+            eta_prev = eta;
+            if (!use_gmres) {
+                ELX_CHECK(precond(r, x, 1));
+            } else {
+                // aim to reach tol in this outer step
+                const double rel = std::min(1e-2, std::max(1e-13, 0.1 * tol / eta));
+                int inner = 0;
+                double* d = gw + n;
+                ELX_CHECK(fgmres(r, d, rel, &inner));
+                inner_total += inner;
+                ELX_CHECK(launch_axpy(s, n, 1.0, d, x));
+            }
+            ++it;
+        }
+        st.outer_iters = std::max(st.outer_iters, it);
+        worst = std::max(worst, eta);
+        all_ok = all_ok && ok;
+    }
+    st.inner_iters = inner_total;
+    st.refine_seconds = tr.stop();
+    int e_h = 0;
+    ELX_CHECK(d2h(&e_h, err, 1, s));
+    st.converged = all_ok ? 1 : 0;
+    st.backward_error = worst;
+    if (stats) *stats = st;
+    if (e_h != 0) return ELX_ERROR_HIP;
+    return all_ok ? ELX_SUCCESS : ELX_ERROR_NUMERIC;
+}
+
+elx_status_t elx_dposv_mixed(void*, int64_t, int64_t, const double*, int64_t, const double*,
+                             int64_t, double*, int64_t, int, int, double, int32_t,
+                             elx_ir_stats_t*) {
+    return ELX_ERROR_NOT_SUPPORTED;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Launchers of the mixed-precision kernels (mixed_kernels.hip) used by the drivers in
+// mixed.hip.
+#pragma once
+#include "elx_common.hpp"
+
+namespace elx {
+
+// W = float(A); rowabs[c][i] = sum over column chunk c of |A(i,j)| (nch chunks)
+elx_status_t launch_convert(hipStream_t s, i64 n, const double* A, i64 lda, float* W, i64 ldw,
+                            double* rowabs, int nch);
+// *out = max_i sum_c rowabs[c][i]   (*out zeroed first)
+elx_status_t launch_rowabs_max(hipStream_t s, i64 n, int nch, const double* rowabs, double* out);
+// *out = max_i |v_i|   (*out zeroed first)
+elx_status_t launch_vec_absmax(hipStream_t s, i64 n, const double* v, double* out);
+// Inverses of the 128x128 diagonal blocks of the unit lower (upper = 0) or upper (upper = 1)
+// factor in W, to Inv (nblk x 128 x 128).
+elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, int upper,
+                                 float* Inv);
+// Persistent triangular solve with the factor in W (fp32); x and flags in uncached memory.
+elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const float* Inv,
+                         int lower, const float* y, float* x, int64_t* flags, int64_t target,
+                         int* err);
+// y(i) = float(r(p[i]))
+elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r,
+                                  float* y);
+// x(i) = [x(i) +] double(d(i))
+elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, double* x, int accumulate);
+// r = b - A x (or r = A x when b is null) with chunk partials in part (nch x n); max |r|,
+// |x|, |b| into norms[0..2] (norms zeroed first). Requires ceil(n/nch) <= 1024.
+elx_status_t launch_residual(hipStream_t s, i64 n, const double* A, i64 lda, const double* x,
+                             const double* b, double* r, double* part, int nch, double* norms);
+
+// h[c] = V(:,c)^T w for c < k (k <= 64); part needs ceil(n/1024)*k doubles
+elx_status_t launch_multidot(hipStream_t s, i64 n, int k, const double* V, i64 ldv,
+                             const double* w, double* part, double* h);
+// w += alpha * V(:,0:k) h
+elx_status_t launch_multiaxpy(hipStream_t s, i64 n, int k, double alpha, const double* V, i64 ldv,
+                              const double* h, double* w);
+// y += alpha x
+elx_status_t launch_axpy(hipStream_t s, i64 n, double alpha, const double* x, double* y);
+// y = alpha x
+elx_status_t launch_scal_copy(hipStream_t s, i64 n, double alpha, const double* x, double* y);
+
+}  // namespace elx

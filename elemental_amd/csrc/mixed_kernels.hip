@@ -1,0 +1,398 @@
+// Kernels of the mixed-precision solvers: precision conversion, diagonal-block inverses,
+// flag-chained persistent triangular solves, and the FP64 residual with its norms.
+#include <algorithm>
+#include <climits>
+
+#include "elx_common.hpp"
+#include "mixed.hpp"
+
+namespace {
+
+constexpr long long SPIN_LIMIT = 1 << 20;
+
+template <typename T>
+__device__ __forceinline__ void st_x(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_x(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// W(i,j) = float(A(i,j)); rowabs[chunk][i] = sum_{j in chunk} |A(i,j)| (chunk = blockIdx.y).
+__global__ __launch_bounds__(256) void convert_kernel(i64 n, const double* __restrict__ A, i64 lda,
+                                                      float* __restrict__ W, i64 ldw,
+                                                      double* __restrict__ rowabs, int cols_per_chunk) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const i64 j0 = (i64)blockIdx.y * cols_per_chunk;
+    const i64 j1 = std::min<i64>(n, j0 + cols_per_chunk);
+    double s = 0.0;
+    for (i64 j = j0; j < j1; ++j) {
+        const double a = A[i + j * lda];
+        W[i + j * ldw] = (float)a;
+        s += fabs(a);
+    }
+    rowabs[(i64)blockIdx.y * n + i] = s;
+}
+
+// Inverse of the 128 x 128 diagonal blocks of a unit-lower (upper == 0) or non-unit upper
+// (upper == 1) triangular factor stored in W. Block b's inverse goes to Inv + b*128*128
+// (column-major, ld 128). One workgroup per block; thread j forms column j.
+__global__ __launch_bounds__(128) void diag_inverse_kernel(i64 n, const float* __restrict__ W,
+                                                           i64 ldw, int upper,
+                                                           float* __restrict__ Inv) {
+    extern __shared__ __align__(16) float sm[];
+    float* T = sm;              // 128 x 128, T[i + k*128]
+    float* X = sm + 128 * 128;  // X[k*128 + j]
+    const i64 b = blockIdx.x;
+    const i64 r0 = b * 128;
+    const int nb = (int)std::min<i64>(128, n - r0);
+    const int j = threadIdx.x;
+    for (int idx = j; idx < 128 * 128; idx += 128) {
+        const int ii = idx & 127, kk = idx >> 7;
+        T[ii + kk * 128] = (ii < nb && kk < nb) ? W[(r0 + ii) + (r0 + kk) * ldw] : 0.0f;
+    }
+    __syncthreads();
+    if (!upper) {
+        for (int i = 0; i < 128; ++i) {
+            float s = (i == j) ? 1.0f : 0.0f;
+            for (int k = 0; k < i; ++k) s = __builtin_fmaf(-T[i + k * 128], X[k * 128 + j], s);
+            X[i * 128 + j] = (i < nb) ? s : 0.0f;
+        }
+    } else {
+        for (int i = 127; i >= 0; --i) {
+            float s = (i == j) ? 1.0f : 0.0f;
+            for (int k = i + 1; k < 128; ++k) s = __builtin_fmaf(-T[i + k * 128], X[k * 128 + j], s);
+            X[i * 128 + j] = (i < nb) ? s / T[i + i * 128] : 0.0f;
+        }
+    }
+    __syncthreads();
+    float* out = Inv + b * 128 * 128;
+    for (int idx = j; idx < 128 * 128; idx += 128) {
+        const int ii = idx & 127, kk = idx >> 7;
+        out[ii + kk * 128] = X[ii * 128 + kk];
+    }
+}
+
+// Persistent flag-chained triangular solve with 128-row blocks, fp32, one right-hand side.
+// lower == 1: L x = y (unit lower, forward); lower == 0: U x = y (upper, backward).
+// Workgroup g handles blocks in solve order g, g+G, ...; block b first accumulates
+// y_b - sum_c T(b,c) x_c as the x_c of earlier blocks are published (flag per block in
+// uncached memory), then multiplies by the precomputed inverse of its diagonal block and
+// publishes x_b. Each block only waits on blocks earlier in solve order, which are handled
+// by co-resident workgroups, so every wait ends; a bounded spin reports failure instead.
+__global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restrict__ W, i64 ldw,
+                                                   const float* __restrict__ Inv, int lower,
+                                                   const float* __restrict__ y, float* x,
+                                                   int64_t* flags, int64_t target, int* err) {
+    __shared__ float xc[128];
+    __shared__ float part[2][128];
+    const int t = threadIdx.x, i = t & 127, h = t >> 7;
+    const i64 nblk = (n + 127) / 128;
+    for (i64 q = blockIdx.x; q < nblk; q += gridDim.x) {
+        const i64 b = lower ? q : (nblk - 1 - q);
+        const i64 r0 = b * 128;
+        const int nb = (int)std::min<i64>(128, n - r0);
+        float acc = (h == 0 && i < nb) ? y[r0 + i] : 0.0f;
+        for (i64 qc = 0; qc < q; ++qc) {
+            const i64 c = lower ? qc : (nblk - 1 - qc);
+            const i64 c0 = c * 128;
+            const int ncb = (int)std::min<i64>(128, n - c0);
+            if (t == 0) {
+                // Bounded wait; after any timeout (err set, uncached) the others stop waiting.
+                long long spins = 0;
+                while (ld_x(&flags[c]) < target) {
+                    if (((++spins) & 1023) == 0 && ld_x(err) != 0) break;
+                    if (spins > SPIN_LIMIT) {
+                        st_x(err, 1);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            if (t < 128) xc[t] = (t < ncb) ? ld_x(&x[c0 + t]) : 0.0f;
+            __syncthreads();
+            if (i < nb) {
+                const float* col = W + (r0 + i) + (c0 + 64 * h) * ldw;
+                const int jend = std::min(64, ncb - 64 * h);
+                for (int jj = 0; jj < jend; ++jj)
+                    acc = __builtin_fmaf(-col[(i64)jj * ldw], xc[64 * h + jj], acc);
+            }
+            __syncthreads();
+        }
+        part[h][i] = acc;
+        __syncthreads();
+        if (t < 128) xc[t] = part[0][t] + part[1][t];
+        __syncthreads();
+        // x_b = Inv_b * v
+        const float* inv = Inv + b * 128 * 128;
+        float s = 0.0f;
+        for (int jj = 64 * h; jj < 64 * h + 64; ++jj) s = __builtin_fmaf(inv[i + jj * 128], xc[jj], s);
+        part[h][i] = s;
+        __syncthreads();
+        if (t < nb) st_x(&x[r0 + t], part[0][t] + part[1][t]);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t == 0) st_x(&flags[b], target);
+    }
+}
+
+// y(i) = float(scale_in * r(p[i])) (p may be null = identity)
+__global__ void gather_to_f32_kernel(i64 n, const int32_t* __restrict__ p, const double* __restrict__ r,
+                                     float* __restrict__ y) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    y[i] = (float)r[p ? p[i] : i];
+}
+
+// x(i) (+)= double(d(i))
+__global__ void update_f64_kernel(i64 n, const float* d, double* __restrict__ x, int accumulate) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double v = (double)__hip_atomic_load(const_cast<float*>(d + i), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    x[i] = accumulate ? x[i] + v : v;
+}
+
+// Residual partials: part[chunk][i] = sum_{j in chunk} A(i,j) x(j)  (FP64, one RHS)
+__global__ __launch_bounds__(256) void gemv_partial_kernel(i64 n, const double* __restrict__ A, i64 lda,
+                                                           const double* __restrict__ x,
+                                                           double* __restrict__ part,
+                                                           int cols_per_chunk) {
+    __shared__ double xs[1024];
+    const i64 j0 = (i64)blockIdx.y * cols_per_chunk;
+    const int nc = (int)std::min<i64>(cols_per_chunk, n - j0);
+    for (int q = threadIdx.x; q < nc; q += 256) xs[q] = x[j0 + q];
+    __syncthreads();
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double* a = A + i + j0 * lda;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = 0;
+    for (; j + 4 <= nc; j += 4) {
+        s0 = __builtin_fma(a[(i64)(j + 0) * lda], xs[j + 0], s0);
+        s1 = __builtin_fma(a[(i64)(j + 1) * lda], xs[j + 1], s1);
+        s2 = __builtin_fma(a[(i64)(j + 2) * lda], xs[j + 2], s2);
+        s3 = __builtin_fma(a[(i64)(j + 3) * lda], xs[j + 3], s3);
+    }
+    for (; j < nc; ++j) s0 = __builtin_fma(a[(i64)j * lda], xs[j], s0);
+    part[(i64)blockIdx.y * n + i] = (s0 + s1) + (s2 + s3);
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// r(i) = b(i) - sum_chunk part; maxima of |r|, |x|, |b| into norms[0..2] (zeroed first).
+__global__ __launch_bounds__(256) void residual_finish_kernel(i64 n, int nch, const double* __restrict__ part,
+                                                              const double* __restrict__ b,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ r, double* norms) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    double rv = 0.0, xv = 0.0, bv = 0.0;
+    if (i < n) {
+        double s = 0.0;
+        for (int c = 0; c < nch; ++c) s += part[(i64)c * n + i];
+        rv = b ? b[i] - s : s;
+        r[i] = rv;
+        rv = fabs(rv);
+        xv = fabs(x[i]);
+        bv = b ? fabs(b[i]) : 0.0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        rv = fmax(rv, __shfl_xor(rv, off));
+        xv = fmax(xv, __shfl_xor(xv, off));
+        bv = fmax(bv, __shfl_xor(bv, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_nonneg(norms + 0, rv);
+        atomic_max_nonneg(norms + 1, xv);
+        atomic_max_nonneg(norms + 2, bv);
+    }
+}
+
+// max_i sum_c rowabs[c][i] -> *out (zeroed first)
+__global__ __launch_bounds__(256) void rowabs_max_kernel(i64 n, int nch, const double* __restrict__ rowabs,
+                                                         double* out) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (i < n)
+        for (int c = 0; c < nch; ++c) v += rowabs[(i64)c * n + i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out, v);
+}
+
+// max_i |v_i| over a column (generic helper), into *out (zeroed first)
+__global__ __launch_bounds__(256) void vec_absmax_kernel(i64 n, const double* __restrict__ v, double* out) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    double m = (i < n) ? fabs(v[i]) : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out, m);
+}
+
+// part[chunk][c] = sum_{i in chunk} V(i,c) w(i), c < k (chunk = 1024 rows = blockIdx.x)
+__global__ __launch_bounds__(256) void multidot_partial_kernel(i64 n, int k, const double* __restrict__ V,
+                                                               i64 ldv, const double* __restrict__ w,
+                                                               double* __restrict__ part) {
+    __shared__ double red[4];
+    const i64 i0 = (i64)blockIdx.x * 1024;
+    const int t = threadIdx.x;
+    for (int c = 0; c < k; ++c) {
+        double s = 0.0;
+        for (int q = t; q < 1024; q += 256) {
+            const i64 i = i0 + q;
+            if (i < n) s = __builtin_fma(V[i + c * ldv], w[i], s);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if ((t & 63) == 0) red[t >> 6] = s;
+        __syncthreads();
+        if (t == 0) part[(i64)blockIdx.x * k + c] = (red[0] + red[1]) + (red[2] + red[3]);
+        __syncthreads();
+    }
+}
+
+// h[c] = sum_chunks part[chunk][c]
+__global__ void multidot_finish_kernel(int nchunk, int k, const double* __restrict__ part, double* h) {
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    double s = 0.0;
+    for (int q = 0; q < nchunk; ++q) s += part[(i64)q * k + c];
+    h[c] = s;
+}
+
+// w(i) += alpha * sum_c V(i,c) h[c]
+__global__ __launch_bounds__(256) void multiaxpy_kernel(i64 n, int k, double alpha, const double* __restrict__ V,
+                                                        i64 ldv, const double* __restrict__ h,
+                                                        double* __restrict__ w) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < k; ++c) s = __builtin_fma(V[i + c * ldv], h[c], s);
+    w[i] = __builtin_fma(alpha, s, w[i]);
+}
+
+// y(i) += alpha * x(i)
+__global__ void daxpy_kernel(i64 n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = __builtin_fma(alpha, x[i], y[i]);
+}
+
+// y(i) = alpha * x(i)
+__global__ void dscal_copy_kernel(i64 n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = alpha * x[i];
+}
+
+}  // namespace
+
+namespace elx {
+
+elx_status_t launch_convert(hipStream_t s, i64 n, const double* A, i64 lda, float* W, i64 ldw,
+                            double* rowabs, int nch) {
+    const int cpc = (int)((n + nch - 1) / nch);
+    dim3 grid(cdiv(n, 256), (unsigned)nch);
+    hipLaunchKernelGGL(convert_kernel, grid, dim3(256), 0, s, n, A, lda, W, ldw, rowabs, cpc);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_rowabs_max(hipStream_t s, i64 n, int nch, const double* rowabs, double* out) {
+    hipLaunchKernelGGL(rowabs_max_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nch, rowabs, out);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_vec_absmax(hipStream_t s, i64 n, const double* v, double* out) {
+    hipLaunchKernelGGL(vec_absmax_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, v, out);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, int upper,
+                                 float* Inv) {
+    const i64 nblk = (n + 127) / 128;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_inverse_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 128 * 128 * 4);
+        (void)hipGetLastError();
+        attr = true;
+    }
+    hipLaunchKernelGGL(diag_inverse_kernel, dim3((unsigned)nblk), dim3(128), 2 * 128 * 128 * 4, s, n,
+                       W, ldw, upper, Inv);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const float* Inv,
+                         int lower, const float* y, float* x, int64_t* flags, int64_t target,
+                         int* err) {
+    const DeviceInfo& di = device_info();
+    const i64 nblk = (n + 127) / 128;
+    const unsigned G = (unsigned)std::min<i64>(nblk, di.num_cus);
+    void* args[] = {&n, &W, &ldw, &Inv, &lower, &y, &x, &flags, &target, &err};
+    ELX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&trsv_kernel), dim3(G),
+                                       dim3(256), args, 0, s));
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r, float* y) {
+    hipLaunchKernelGGL(gather_to_f32_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, p, r, y);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, double* x, int accumulate) {
+    hipLaunchKernelGGL(update_f64_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, x, accumulate);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_multidot(hipStream_t s, i64 n, int k, const double* V, i64 ldv,
+                             const double* w, double* part, double* h) {
+    const unsigned nchunk = cdiv(n, 1024);
+    hipLaunchKernelGGL(multidot_partial_kernel, dim3(nchunk), dim3(256), 0, s, n, k, V, ldv, w, part);
+    ELX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(multidot_finish_kernel, dim3(1), dim3(64), 0, s, (int)nchunk, k, part, h);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_multiaxpy(hipStream_t s, i64 n, int k, double alpha, const double* V, i64 ldv,
+                              const double* h, double* w) {
+    hipLaunchKernelGGL(multiaxpy_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, k, alpha, V, ldv, h, w);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_axpy(hipStream_t s, i64 n, double alpha, const double* x, double* y) {
+    hipLaunchKernelGGL(daxpy_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, alpha, x, y);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_scal_copy(hipStream_t s, i64 n, double alpha, const double* x, double* y) {
+    hipLaunchKernelGGL(dscal_copy_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, alpha, x, y);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_residual(hipStream_t s, i64 n, const double* A, i64 lda, const double* x,
+                             const double* b, double* r, double* part, int nch, double* norms) {
+    const int cpc = (int)((n + nch - 1) / nch);
+    if (cpc > 1024) return ELX_ERROR_INVALID_ARG;
+    dim3 grid(cdiv(n, 256), (unsigned)nch);
+    hipLaunchKernelGGL(gemv_partial_kernel, grid, dim3(256), 0, s, n, A, lda, x, part, cpc);
+    ELX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(residual_finish_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nch, part, b, x,
+                       r, norms);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+}  // namespace elx

@@ -153,9 +153,15 @@ def gesv_mixed(A, B, X, factor="fp16", refine="auto", tol=0.0, max_iters=0, stre
     pX, ldx = _mat(X)
     st = IrStats()
     import ctypes
-    call("elx_dgesv_mixed", _stream(stream), A.shape[0], B.shape[1], pA, lda, pB, ldb, pX, ldx,
-         FACTOR[factor], REFINE[refine], tol, max_iters, ctypes.byref(st))
-    return st.as_dict()
+    status = load().elx_dgesv_mixed(_stream(stream), A.shape[0], B.shape[1], pA, lda, pB, ldb,
+                                     pX, ldx, FACTOR[factor], REFINE[refine], tol, max_iters,
+                                     ctypes.byref(st))
+    if status not in (0, 5):
+        from ._lib import ElxError
+        raise ElxError("elx_dgesv_mixed", status)
+    out = st.as_dict()
+    out["status"] = status
+    return out
 
 
 def posv_mixed(A, B, X, factor="fp32", refine="ir", tol=0.0, max_iters=0, stream=None):

@@ -1,0 +1,110 @@
+"""Mixed-precision LU + FP64 iterative refinement on the GPU.
+
+Bar (north star): every solve reports FP64 normwise backward error <= n * 2^-53, checked
+here independently with the oracle's double-double residual; the factorization alone
+(no refinement) must be a useful preconditioner (backward error well below 1)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def randsvd(n, kappa, seed):
+    """A = U diag(sigma) V^T, U and V Haar (QR of Gaussians with sign-fixed R), sigma
+    geometric from 1 to 1/kappa."""
+    rng = np.random.default_rng(seed)
+    def haar():
+        Q, R = np.linalg.qr(rng.standard_normal((n, n)))
+        return Q * np.sign(np.diag(R))[None, :]
+    U, V = haar(), haar()
+    sigma = kappa ** (-np.arange(n) / max(n - 1, 1))
+    return np.asfortranarray((U * sigma[None, :]) @ V.T)
+
+
+@pytest.mark.parametrize("factor", ["fp16", "bf16"])
+@pytest.mark.parametrize("n", [200, 1000, 2500])
+def test_ir_converges_well_conditioned(dev, orc, factor, n):
+    say("mixed", factor, n)
+    A = randsvd(n, 1e2, n)
+    b = np.random.default_rng(n + 1).standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="auto")
+    x = dev.to_host(dX)
+    eta = orc.backward_error(A, x, b)
+    say(factor, n, st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["converged"] == 1 and st["factor_info"] == 0
+    assert eta <= n * EPS
+    # the device's own backward error agrees with the double-double one
+    assert st["backward_error"] <= 2 * n * EPS
+    assert abs(st["backward_error"] - eta) <= 0.5 * max(eta, st["backward_error"]) + 1e-20
+
+
+@pytest.mark.parametrize("factor", ["fp16", "bf16"])
+def test_factor_alone_is_a_preconditioner(dev, orc, factor):
+    n = 1024
+    A = randsvd(n, 1e2, 7)
+    b = np.random.default_rng(8).standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="none")
+    assert st["status"] in (0, 5)
+    x = dev.to_host(dX)
+    eta = orc.backward_error(A, x, b)
+    u_low = 2.0 ** -11 if factor == "fp16" else 2.0 ** -8
+    say(factor, "one-shot backward error", eta, "u_low", u_low)
+    assert eta < 50 * u_low
+
+
+def test_mixed_multiple_rhs(dev, orc):
+    n = 700
+    A = randsvd(n, 1e2, 3)
+    B = np.random.default_rng(4).standard_normal((n, 3))
+    dA, dB = dev.to_device(A), dev.to_device(B)
+    dX = dev.colmajor_empty(n, 3)
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="ir")
+    X = dev.to_host(dX)
+    assert st["converged"] == 1
+    assert orc.backward_error(A, X, B) <= n * EPS
+
+
+def test_mixed_pivoting_needed(dev, orc):
+    """A matrix that needs row pivoting (tiny leading entries) still converges."""
+    n = 600
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((n, n))
+    A[np.arange(n), np.arange(n)] *= 1e-8
+    A = np.asfortranarray(A)
+    b = rng.standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="ir")
+    x = dev.to_host(dX)
+    assert st["converged"] == 1
+    assert orc.backward_error(A, x, b) <= n * EPS
+
+
+@pytest.mark.parametrize("kappa", [1e4, 1e6])
+@pytest.mark.parametrize("factor", ["fp16", "bf16"])
+def test_gmres_ir_ill_conditioned(dev, orc, kappa, factor):
+    """kappa * u_low >> 1: plain refinement cannot converge; GMRES-IR preconditioned by the
+    low-precision LU reaches FP64 backward error."""
+    n = 1500
+    say("gmres-ir", factor, kappa)
+    A = randsvd(n, kappa, 21)
+    b = np.random.default_rng(22).standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="gmres")
+    x = dev.to_host(dX)
+    eta = orc.backward_error(A, x, b)
+    say(factor, kappa, st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["converged"] == 1, st
+    assert eta <= n * EPS
+    assert st["inner_iters"] > 0
