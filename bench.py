@@ -1,0 +1,280 @@
+#!/usr/bin/env python3
+"""Headline benchmark: FP64-accurate Ax = b solves/s at n = 32768 on one MI355X.
+
+Workload (mixed-precision LU-IR, one right-hand side): A = U diag(sigma) V^T with U, V
+Haar-orthogonal (QR of seeded Gaussians with sign-fixed R), sigma geometric from 1 to
+1/kappa, b ~ N(0,1); FP64 storage in HBM. A step is one complete solve to FP64 normwise
+backward error <= n * 2^-53: fp16 LU (fp16 MFMA trailing updates with fp32 accumulation,
+fp32 panels) and FP64 refinement (IR, switching to GMRES-IR when it stalls), with the
+residual and the error measured on the device in FP64. The plain FP64 GPU LU + solve of the
+same system (a fresh copy of A per solve, as El::LinearSolve copies A) is timed beside it,
+and the reference's CPU path (the oracle's restatement of El::LU + lu::SolveAfter over the
+host BLAS) on a bounded sample.
+
+Prints one JSON line. `value` = mixed-precision solves/s.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {"hbm_GBs": 8000.0, "fp64_TFs": 78.6, "fp32_TFs": 157.3, "fp16_TFs": 2500.0,
+         "bf16_TFs": 2500.0}
+CLASSES = ["lu_panel", "dgemm_f64", "lpgemm", "trsm_small", "row_moves", "trsv", "residual_f64",
+           "pack"]
+
+
+def make_problem(n, kappa, seed):
+    import torch
+    from elemental_amd import device as dev
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+
+    def haar():
+        G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+        Q, R = torch.linalg.qr(G)
+        del G
+        return Q * torch.sign(torch.diagonal(R))[None, :]
+
+    U = haar()
+    V = haar()
+    sig = kappa ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+    A = dev.colmajor_empty(n, n)
+    A.copy_((U * sig[None, :]) @ V.T.contiguous())
+    del U, V
+    b = dev.colmajor_empty(n, 1)
+    b.copy_(torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=g))
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return A, b
+
+
+def backward_error(A, x, b):
+    import torch
+    r = b - A @ x
+    an = A.abs().sum(dim=1).max()
+    return float(r.abs().max() / (an * x.abs().max() + b.abs().max()))
+
+
+def prof_read(lib):
+    out = {}
+    for c, name in enumerate(CLASSES):
+        cnt = ctypes.c_int64()
+        ms = ctypes.c_double()
+        work = ctypes.c_double()
+        lib.elx_profile_query(c, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(work))
+        if cnt.value:
+            out[name] = {"launches": cnt.value, "ms": ms.value, "work": work.value}
+    return out
+
+
+def roofline_of(cls, rec, factor):
+    t = rec["ms"] * 1e-3
+    if cls in ("lpgemm",):
+        peak = PEAKS["fp16_TFs" if factor == "fp16" else "bf16_TFs"]
+        ach = rec["work"] / t / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 5)}
+    if cls in ("dgemm_f64",):
+        peak = PEAKS["fp64_TFs"]
+        ach = rec["work"] / t / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 5)}
+    ach = rec["work"] / t / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAKS["hbm_GBs"], "unit": "GB/s",
+            "frac": round(ach / PEAKS["hbm_GBs"], 5)}
+
+
+def cpu_baseline(n_metric, n_sample):
+    """The oracle (restated El::LU + lu::SolveAfter) over the host BLAS on all allowed cores,
+    one solve at n_sample, scaled to solves/s at n_metric by the n^3 flop ratio."""
+    import numpy as np
+    from oracle import oracle as orc
+    # Pick the fastest host BLAS available on this machine (a short dgemm calibration).
+    import glob
+    cands = ["mkl", "openblas"] + sorted(
+        glob.glob("/usr/local/lib/python3*/dist-packages/scipy.libs/libscipy_openblas*.so"))
+    cores = len(os.sched_getaffinity(0))
+    best, best_t = None, None
+    m = 2048
+    Ac = np.asfortranarray(np.random.default_rng(1).standard_normal((m, m)))
+    for cand in cands:
+        try:
+            name = orc.set_backend(cand)
+        except RuntimeError:
+            continue
+        orc.set_threads(cores)
+        orc.gemm("N", "N", 1.0, Ac, Ac, 0.0, Ac)
+        t0 = time.perf_counter()
+        orc.gemm("N", "N", 1.0, Ac, Ac, 0.0, Ac)
+        t = time.perf_counter() - t0
+        if best_t is None or t < best_t:
+            best, best_t = cand, t
+    if best is None:
+        return None
+    backend = orc.set_backend(best)
+    gemm_gflops = 2 * m ** 3 / best_t / 1e9
+    orc.set_threads(cores)
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    rng = np.random.default_rng(11)
+    A = np.asfortranarray(rng.standard_normal((n_sample, n_sample)))
+    b = np.asfortranarray(rng.standard_normal((n_sample, 1)))
+    t0 = time.perf_counter()
+    F, d, info = orc.lu(A)
+    x = orc.lu_solve_after(F, d, b)
+    t = time.perf_counter() - t0
+    scale = (n_sample / n_metric) ** 3
+    return {"value": round(scale / t, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"oracle El::LU + lu::SolveAfter (nb=128) over {os.path.basename(backend)} "
+                      f"with {cores} threads, one N(0,1) system at n={n_sample} in {t:.2f} s, "
+                      f"scaled by (n_sample/n)^3 to n={n_metric}",
+            "seconds_at_sample": round(t, 3), "info": int(info), "cpu_model": cpu_model,
+            "host_dgemm_gflops": round(gemm_gflops, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--kappa", type=float, default=1e2)
+    ap.add_argument("--factor", default="fp16", choices=["fp16", "bf16"])
+    ap.add_argument("--refine", default="auto", choices=["ir", "gmres", "auto"])
+    ap.add_argument("--seed", type=int, default=20240)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-n", type=int, default=12288)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--no-profile-pass", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
+              "(this build has no multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from elemental_amd import device as dev
+    from elemental_amd._lib import load
+    lib = load()
+    lib.elx_profile_enable.argtypes = [ctypes.c_int]
+    lib.elx_profile_query.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    torch.cuda.set_device(0)
+    n = args.n
+    print(f"[bench] generating n={n} kappa={args.kappa:g}", file=sys.stderr, flush=True)
+    A, b = make_problem(n, args.kappa, args.seed)
+    X = dev.colmajor_empty(n, 1)
+
+    def mixed_solve():
+        return dev.gesv_mixed(A, b, X, factor=args.factor, refine=args.refine)
+
+    for _ in range(args.warmup):
+        st = mixed_solve()
+        print(f"[bench] warmup mixed {st}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        stats.append(mixed_solve())
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    ms_step = 1e3 * t_mixed / args.steps
+    value = args.steps / t_mixed
+    ok = all(s["converged"] == 1 for s in stats)
+    eta_dev = max(s["backward_error"] for s in stats)
+    eta_chk = backward_error(A, X, b)
+    print(f"[bench] mixed {ms_step:.1f} ms/solve, eta(device)={eta_dev:.3e} eta(check)={eta_chk:.3e} "
+          f"{stats[-1]}", file=sys.stderr, flush=True)
+
+    plain = None
+    if not args.skip_plain:
+        Ac = dev.colmajor_empty(n, n)
+        xb = dev.colmajor_empty(n, 1)
+        times = []
+        for rep in range(args.plain_steps + 1):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            Ac.copy_(A)
+            xb.copy_(b)
+            d, info = dev.getrf(Ac)
+            dev.getrs(Ac, d, xb)
+            torch.cuda.synchronize()
+            if rep > 0:
+                times.append(time.perf_counter() - t1)
+        t_plain = sum(times) / len(times)
+        plain = {"solves_per_s": round(1.0 / t_plain, 4), "ms_per_solve": round(1e3 * t_plain, 2),
+                 "backward_error": backward_error(A, xb, b), "info": int(info.item())}
+        del Ac
+        torch.cuda.empty_cache()
+        print(f"[bench] plain fp64 {plain}", file=sys.stderr, flush=True)
+
+    kernels = {}
+    roof = None
+    if not args.no_profile_pass:
+        lib.elx_profile_reset()
+        lib.elx_profile_enable(1)
+        mixed_solve()
+        torch.cuda.synchronize()
+        kernels = prof_read(lib)
+        lib.elx_profile_enable(0)
+        dom = max(kernels, key=lambda k: kernels[k]["ms"])
+        roof = roofline_of(dom, kernels[dom], args.factor)
+        roof["kernel"] = dom
+        roof["traffic"] = None
+        roof["avg_launch_us"] = round(1e3 * kernels[dom]["ms"] / kernels[dom]["launches"], 2)
+        for k, v in kernels.items():
+            v["share"] = round(v["ms"] / sum(x["ms"] for x in kernels.values()), 4)
+        print(f"[bench] kernel classes {json.dumps(kernels)}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.skip_cpu:
+        del A
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(n, args.cpu_n)
+
+    line = {
+        "metric": "FP64-accurate Ax=b solves/s: FP16-LU-IR vs FP64 LU at n=32768, 1 MI355X",
+        "value": round(value, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.factor + "+f64",
+        "data": "synthetic",
+        "config": {"workload": f"LU-IR solve, n={n}, randsvd kappa={args.kappa:g}, 1 RHS, "
+                               f"{args.factor} factor, FP64 refinement ({args.refine})",
+                   "n": n, "kappa": args.kappa, "nrhs": 1, "factor": args.factor,
+                   "refine": args.refine, "seed": args.seed},
+        "converged": ok,
+        "backward_error": eta_dev,
+        "backward_error_check": eta_chk,
+        "tolerance": n * 2.0 ** -53,
+        "refine_stats": {k: stats[-1][k] for k in ("outer_iters", "inner_iters", "factor_seconds",
+                                                    "refine_seconds")},
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": (round(value / plain["solves_per_s"], 3) if plain else None),
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

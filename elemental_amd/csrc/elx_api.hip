@@ -88,6 +88,67 @@ elx_status_t panel_sync_get(int max_groups, size_t slot_bytes, PanelSync** out) 
     return ELX_SUCCESS;
 }
 
+// ---------------------------------------------------------------- kernel-class timing
+namespace {
+struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+    double work;
+};
+std::mutex g_pmu;
+bool g_prof = false;
+std::vector<hipEvent_t> g_evpool;
+std::vector<ProfRec> g_pending;
+ProfRec g_open[kKCount];
+double g_sum_ms[kKCount], g_sum_work[kKCount];
+int64_t g_count[kKCount];
+
+hipEvent_t ev_get() {
+    if (!g_evpool.empty()) {
+        hipEvent_t e = g_evpool.back();
+        g_evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void prof_drain() {
+    (void)hipDeviceSynchronize();
+    for (auto& r : g_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            g_sum_ms[r.cls] += ms;
+            g_sum_work[r.cls] += r.work;
+            g_count[r.cls] += 1;
+        }
+        g_evpool.push_back(r.a);
+        g_evpool.push_back(r.b);
+    }
+    g_pending.clear();
+}
+}  // namespace
+
+bool prof_on() { return g_prof; }
+
+void prof_begin(hipStream_t s, int cls) {
+    std::lock_guard<std::mutex> lk(g_pmu);
+    ProfRec r{cls, ev_get(), nullptr, 0.0};
+    (void)hipEventRecord(r.a, s);
+    g_open[cls] = r;
+}
+
+void prof_end(hipStream_t s, int cls, double work) {
+    std::lock_guard<std::mutex> lk(g_pmu);
+    ProfRec r = g_open[cls];
+    r.b = ev_get();
+    r.work = work;
+    (void)hipEventRecord(r.b, s);
+    g_pending.push_back(r);
+    if (g_pending.size() > 20000) prof_drain();
+}
+
 elx_status_t release_all() {
     std::lock_guard<std::mutex> lk(g_mu);
     (void)hipDeviceSynchronize();
@@ -275,6 +336,32 @@ int elx_device_count(void) {
 }
 
 elx_status_t elx_finalize(void) { return release_all(); }
+
+void elx_profile_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_pmu);
+    if (!on && g_prof) prof_drain();
+    g_prof = (on != 0);
+}
+
+void elx_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(g_pmu);
+    prof_drain();
+    for (int c = 0; c < kKCount; ++c) {
+        g_sum_ms[c] = 0.0;
+        g_sum_work[c] = 0.0;
+        g_count[c] = 0;
+    }
+}
+
+elx_status_t elx_profile_query(int cls, int64_t* launches, double* total_ms, double* work) {
+    ELX_ARG(cls >= 0 && cls < kKCount);
+    std::lock_guard<std::mutex> lk(g_pmu);
+    prof_drain();
+    if (launches) *launches = g_count[cls];
+    if (total_ms) *total_ms = g_sum_ms[cls];
+    if (work) *work = g_sum_work[cls];
+    return ELX_SUCCESS;
+}
 
 elx_status_t elx_dgetrf(void* stream, int64_t m, int64_t n, double* dA, int64_t lda, int64_t nb,
                         int32_t* dDests, int32_t* dInfo) {

@@ -74,6 +74,28 @@ elx_status_t panel_sync_get(int max_groups, size_t slot_bytes, PanelSync** out);
 
 inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 
+// Optional per-kernel-class timing with HIP events on the launching stream (off by default;
+// see elx_profile_enable). `work` is the algorithmic flops or bytes of the launch.
+enum KernelClass {
+    kKPanel = 0, kKDgemm, kKLpgemm, kKTrsm, kKRowMoves, kKTrsv, kKResidual, kKPack, kKOther,
+    kKCount
+};
+bool prof_on();
+void prof_begin(hipStream_t s, int cls);
+void prof_end(hipStream_t s, int cls, double work);
+struct ProfScope {
+    hipStream_t s;
+    int cls;
+    double work;
+    bool on;
+    ProfScope(hipStream_t st, int c, double w) : s(st), cls(c), work(w), on(prof_on()) {
+        if (on) prof_begin(s, cls);
+    }
+    ~ProfScope() {
+        if (on) prof_end(s, cls, work);
+    }
+};
+
 }  // namespace elx
 
 // Kernel launchers shared between translation units (all enqueue on `stream`).

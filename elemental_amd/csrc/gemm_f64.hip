@@ -195,6 +195,7 @@ elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, 
     }
     const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     ELX_ARG(tm * tn < (i64)1 << 30);
+    ProfScope prof_(s, kKDgemm, (tri ? 1.0 : 2.0) * (double)m * (double)n * (double)k);
     const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
     const int itm = (int)tm, itn = (int)tn;
     if (!ta && !tb)

@@ -245,6 +245,7 @@ elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, 
             ldb >= ((k + BK - 1) / BK) * BK);
     const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     ELX_ARG(tm * tn < ((i64)1 << 30));
+    ProfScope prof_(s, kKLpgemm, 2.0 * (double)m * (double)n * (double)k);
     const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
     if (prec == 0)
         hipLaunchKernelGGL(lpgemm_kernel<0>, dim3(blocks), dim3(NT), 0, s, m, n, k, alpha, A, lda,
@@ -263,6 +264,7 @@ elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 
     ELX_ARG(ldh >= Kpad);
     dim3 grid(cdiv(R, 64), cdiv(Kpad, 64));
     ELX_ARG(grid.y <= 65535);
+    ProfScope prof_(s, kKPack, (double)R * (double)Kpad * 6.0);
     if (prec == 0) {
         if (trans)
             hipLaunchKernelGGL((pack_kernel<0, true>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);

@@ -257,6 +257,7 @@ elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, 
     if (total <= 0 || npairs <= 0) return ELX_SUCCESS;
     if (c1 < c0) c1 = c0;
     if (c3 < c2) c3 = c2;
+    ProfScope prof_(s, kKRowMoves, 2.0 * npairs * (double)total * sizeof(double));
     hipLaunchKernelGGL(row_moves_kernel<double>, dim3(cdiv(total, 4)), dim3(256), 0, s, pairs,
                        npairs, A, lda, r0, c0, c1, c2, c3);
     ELX_HIP(hipGetLastError());
@@ -270,6 +271,7 @@ elx_status_t launch_row_moves_f32(hipStream_t s, const int2* pairs, int npairs, 
     if (total <= 0 || npairs <= 0) return ELX_SUCCESS;
     if (c1 < c0) c1 = c0;
     if (c3 < c2) c3 = c2;
+    ProfScope prof_(s, kKRowMoves, 2.0 * npairs * (double)total * sizeof(float));
     hipLaunchKernelGGL(row_moves_kernel<float>, dim3(cdiv(total, 4)), dim3(256), 0, s, pairs,
                        npairs, A, lda, r0, c0, c1, c2, c3);
     ELX_HIP(hipGetLastError());
@@ -320,6 +322,7 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     const bool mtrans = right ? !trans : trans;
     const bool lower_eff = lower != mtrans;
     const size_t lds = sizeof(T) * (size_t)order * TV;
+    ProfScope prof_(s, kKTrsm, (double)order * (double)order * (double)nvec);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_small_kernel<T>),

@@ -11,12 +11,15 @@
 // never goes back to HBM between columns. Rows are never moved during the panel: a row's
 // "label" is its position in the swapped order, and a swap relabels two rows. Each column
 // step is one all-to-all exchange: every workgroup publishes its local pivot candidate
-// (|value|, label, and the candidate row) with an agent-scope release, waits until all
-// workgroups have published for this column, and then every workgroup reduces the same
-// candidates to the same winner (largest magnitude, ties to the smallest label -- the
-// first index idamax would return). At the end each workgroup writes its rows to their
-// final positions, and the list of moved rows (at most 2w) is emitted so the columns
-// outside the panel can be permuted in one gather pass.
+// (value and label in one 16-byte record, and the candidate row in 16-byte chunks) to
+// uncached memory, every workgroup reads all G records, reduces them to the same winner
+// (largest magnitude, ties to the smallest label -- the first index idamax would return),
+// and reads the winner's row. Records and chunks carry a tag derived from the column's
+// sequence number and their own contents, so a reader recognises stale or partly written
+// data and simply reads again: no store-completion waits or separate flags, two memory
+// round trips per column. At the end each workgroup writes its rows to their final
+// positions, and the list of moved rows (at most 2w) is emitted so the columns outside the
+// panel can be permuted in one gather pass.
 #include <algorithm>
 #include <climits>
 
@@ -28,6 +31,7 @@ constexpr int NT = 256;
 constexpr int WMAX = 128;                 // maximum panel width
 constexpr long long SPIN_LIMIT = 1 << 20;  // bounded wait: every wave exits
 constexpr int RPL = 5;                     // rows per lane held in registers (R <= 320)
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct PanelArgs {
@@ -41,27 +45,70 @@ struct PanelArgs {
     int32_t* info;
     int G;
     int R;
-    int64_t* flags;
-    int64_t epoch_base;
-    T* slot_val;     // [2][G] signed pivot candidate value
-    int* slot_lbl;   // [2][G] candidate label (INT_MAX when none)
-    T* slot_row;     // [2][G][WMAX] candidate row (columns c..w-1 valid)
+    uint32_t seq_base;  // sequence number of column 0 of this launch
+    u4* rec;            // [2][G] candidate records
+    u4* rows;           // [2][G][WMAX] candidate row chunks
 };
 
 // The exchange area lives in uncached device memory (allocated with
-// hipDeviceMallocUncached), so loads and stores go straight to memory and are coherent across
-// the XCDs without L2 write-back / invalidate. Accesses are relaxed agent-scope atomics (the
-// compiler may neither cache nor merge them); ordering of a workgroup's data before its flag
-// comes from waiting for the stores to complete (s_waitcnt) before the flag store.
-template <typename T>
-__device__ __forceinline__ void st_x(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// hipDeviceMallocUncached): loads and stores go to memory and are coherent across the XCDs
+// without cache maintenance. Each 16-byte record / chunk is written and read with one
+// volatile 16-byte access and validated by its tag.
+__device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (c + 0x165667B1u) * 0xC2B2AE3Du;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h;
 }
-template <typename T>
-__device__ __forceinline__ T ld_x(const T* p) {
-    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st16(u4* p, u4 v) { *reinterpret_cast<volatile u4*>(p) = v; }
+__device__ __forceinline__ u4 ld16(const u4* p) { return *reinterpret_cast<const volatile u4*>(p); }
+
+// Candidate record: value (as double bits), label, tag.
+__device__ __forceinline__ u4 make_rec(double v, int lbl, uint32_t seq) {
+    const uint64_t bits = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)bits, hi = (uint32_t)(bits >> 32);
+    return u4{lo, hi, (uint32_t)lbl, seq ^ mix32(lo, hi, (uint32_t)lbl)};
 }
-__device__ __forceinline__ void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
+__device__ __forceinline__ bool rec_ok(u4 r, uint32_t seq) { return r[3] == (seq ^ mix32(r[0], r[1], r[2])); }
+__device__ __forceinline__ double rec_val(u4 r) {
+    return __longlong_as_double((long long)(((uint64_t)r[1] << 32) | r[0]));
+}
+
+// Row chunk q: VPC values (one double, or three floats) and a tag.
+template <typename T>
+struct Chunk;
+template <>
+struct Chunk<double> {
+    static constexpr int VPC = 1;
+    __device__ static u4 make(const double* v, int q, uint32_t seq) {
+        const uint64_t bits = (uint64_t)__double_as_longlong(v[0]);
+        const uint32_t lo = (uint32_t)bits, hi = (uint32_t)(bits >> 32);
+        return u4{lo, hi, (uint32_t)q, seq ^ mix32(lo, hi, (uint32_t)q)};
+    }
+    __device__ static bool ok(u4 c, int q, uint32_t seq) {
+        return c[2] == (uint32_t)q && c[3] == (seq ^ mix32(c[0], c[1], (uint32_t)q));
+    }
+    __device__ static void get(u4 c, double* v) {
+        v[0] = __longlong_as_double((long long)(((uint64_t)c[1] << 32) | c[0]));
+    }
+};
+template <>
+struct Chunk<float> {
+    static constexpr int VPC = 3;
+    __device__ static u4 make(const float* v, int q, uint32_t seq) {
+        const uint32_t a = __float_as_uint(v[0]), b = __float_as_uint(v[1]), c = __float_as_uint(v[2]);
+        return u4{a, b, c, seq ^ mix32(a, b, c ^ ((uint32_t)q * 0x27D4EB2Fu))};
+    }
+    __device__ static bool ok(u4 c, int q, uint32_t seq) {
+        return c[3] == (seq ^ mix32(c[0], c[1], c[2] ^ ((uint32_t)q * 0x27D4EB2Fu)));
+    }
+    __device__ static void get(u4 c, float* v) {
+        v[0] = __uint_as_float(c[0]);
+        v[1] = __uint_as_float(c[1]);
+        v[2] = __uint_as_float(c[2]);
+    }
+};
 
 template <typename T>
 __device__ __forceinline__ bool better(T v, int l, T bv, int bl) {
@@ -108,7 +155,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     const int R = p.R, W = p.w, LD = W + 1;
     T* tile = reinterpret_cast<T*>(smem_raw);   // R x LD, row-major
     T* urow = tile + (size_t)R * LD;            // WMAX
-    T* sv = urow + WMAX;                        // NT/64
+    T* lv = urow + WMAX;                        // R: multipliers of the current column
+    T* sv = lv + R;                             // NT/64
     int* lbl = reinterpret_cast<int*>(sv + NT / 64);  // R
     int* sl = lbl + R;                          // NT/64
     int* sq = sl + NT / 64;                     // NT/64
@@ -125,43 +173,74 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     for (int r = t; r < rows; r += NT) lbl[r] = (int)(row0 + r);
     __syncthreads();
 
-    bool aborted = false;
-    for (int c = 0; c < W; ++c) {
-        const int s = c & 1;
-        // Local candidate: largest |a(r,c)| over rows not yet used as pivots.
-        T bv = T(-1);
-        int bl = INT_MAX, br = -1;
-        for (int r = t; r < rows; r += NT) {
-            const int L = lbl[r];
-            if (L >= c) {
-                const T v = fabs(tile[r * LD + c]);
-                if (better(v, L, bv, bl)) {
-                    bv = v;
-                    bl = L;
-                    br = r;
+    typedef Chunk<T> CK;
+    const int q1 = (W + CK::VPC - 1) / CK::VPC;
+    // Publish the candidate of column cc: row chunks (columns cc..w-1), then the record.
+    auto publish = [&](int cc, int cand, int cand_lbl) {
+        const uint32_t seq = p.seq_base + (uint32_t)cc;
+        const int sp = cc & 1;
+        u4* myrow = p.rows + ((size_t)(sp * G + g)) * WMAX;
+        if (cand >= 0)
+            for (int q = cc / CK::VPC + t; q < q1; q += NT) {
+                T v[CK::VPC];
+#pragma unroll
+                for (int e = 0; e < CK::VPC; ++e) {
+                    const int j = q * CK::VPC + e;
+                    v[e] = (j < W) ? tile[cand * LD + j] : T(0);
                 }
+                st16(&myrow[q], CK::make(v, q, seq));
+            }
+        if (t == 0)
+            st16(&p.rec[sp * G + g],
+                 make_rec((cand >= 0) ? (double)tile[cand * LD + cc] : 0.0, cand_lbl, seq));
+    };
+
+    // Prologue: the candidate of column 0.
+    int br = -1, bl = INT_MAX;
+    {
+        T bv = T(-1);
+        for (int r = t; r < rows; r += NT) {
+            const T v = fabs(tile[r * LD + 0]);
+            if (better(v, lbl[r], bv, bl)) {
+                bv = v;
+                bl = lbl[r];
+                br = r;
             }
         }
         block_best(bv, bl, br, sv, sl, sq);
+        publish(0, br, bl);
+    }
 
-        // Publish the candidate (value with sign, label, row) for this column, then the flag.
-        T* myrow = p.slot_row + ((size_t)(s * G + g)) * WMAX;
-        if (br >= 0)
-            for (int j = c + t; j < W; j += NT) st_x(&myrow[j], tile[br * LD + j]);
-        if (t == 0) {
-            st_x(&p.slot_val[s * G + g], (br >= 0) ? tile[br * LD + c] : T(0));
-            st_x(&p.slot_lbl[s * G + g], bl);
-        }
-        stores_complete();
-        __syncthreads();
-        const int64_t target = p.epoch_base + c + 1;
-        if (t == 0) st_x(&p.flags[g], target);
-
-        // Wait until every workgroup has published column c.
-        long long spins = 0;
+    bool aborted = false;
+    long long spins = 0;
+    for (int c = 0; c < W; ++c) {
+        const int s = c & 1;
+        const uint32_t seq = p.seq_base + (uint32_t)c;
+        const int q0 = c / CK::VPC;
+        // (1) every workgroup's record for column c, reduced to the same winner
+        T gv = T(-1);
+        int gl = INT_MAX, gw = -1;
         while (true) {
+            gv = T(-1);
+            gl = INT_MAX;
+            gw = -1;
             int ok = 1;
-            for (int q = t; q < G; q += NT) ok &= (ld_x(&p.flags[q]) >= target);
+            for (int q = t; q < G; q += NT) {
+                const u4 r = ld16(&p.rec[s * G + q]);
+                if (!rec_ok(r, seq)) {
+                    ok = 0;
+                    continue;
+                }
+                const int L = (int)r[2];
+                if (L != INT_MAX) {
+                    const T v = fabs((T)rec_val(r));
+                    if (better(v, L, gv, gl)) {
+                        gv = v;
+                        gl = L;
+                        gw = q;
+                    }
+                }
+            }
             if (__syncthreads_and(ok)) break;
             if (++spins > SPIN_LIMIT) {
                 aborted = true;
@@ -169,73 +248,109 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             }
         }
         if (aborted) break;
-
-        // Every workgroup reduces the same G candidates to the same winner.
-        T gv = T(-1);
-        int gl = INT_MAX, gw = -1;
-        for (int q = t; q < G; q += NT) {
-            const int L = ld_x(&p.slot_lbl[s * G + q]);
-            if (L != INT_MAX) {
-                const T v = fabs(ld_x(&p.slot_val[s * G + q]));
-                if (better(v, L, gv, gl)) {
-                    gv = v;
-                    gl = L;
-                    gw = q;
+        block_best(gv, gl, gw, sv, sl, sq);
+        const bool none = (gw < 0);  // every remaining candidate is NaN (idamax selects none)
+        const int piv = none ? c : gl;
+        // (2) the winner's row, columns c..w-1
+        if (!none) {
+            const u4* wrow = p.rows + ((size_t)(s * G + gw)) * WMAX;
+            while (true) {
+                int ok = 1;
+                for (int q = q0 + t; q < q1; q += NT) {
+                    const u4 ch = ld16(&wrow[q]);
+                    if (!CK::ok(ch, q, seq)) {
+                        ok = 0;
+                        continue;
+                    }
+                    T v[CK::VPC];
+                    CK::get(ch, v);
+#pragma unroll
+                    for (int e = 0; e < CK::VPC; ++e) {
+                        const int j = q * CK::VPC + e;
+                        if (j >= c && j < W) urow[j] = v[e];
+                    }
+                }
+                if (__syncthreads_and(ok)) break;
+                if (++spins > SPIN_LIMIT) {
+                    aborted = true;
+                    break;
                 }
             }
+            if (aborted) break;
+        } else {
+            for (int j = c + t; j < W; j += NT) urow[j] = T(0);
         }
-        block_best(gv, gl, gw, sv, sl, sq);
-        if (gw < 0) {
-            // Every remaining candidate is NaN: no row qualifies (idamax never selects a
-            // NaN). Record the step without a swap and flag the panel.
-            if (g == 0 && t == 0) {
-                p.dests[c] = (int32_t)(p.goff + c);
-                atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
+        // (3) swap = relabel: the row at position c moves to position piv, the winner
+        // becomes row c
+        const bool mine = (!none && gw == g);
+        for (int r = t; r < rows; r += NT) {
+            const int L = lbl[r];
+            if (mine && r == br) lbl[r] = c;
+            else if (!none && L == c) lbl[r] = piv;
+        }
+        if (g == 0 && t == 0) {
+            p.dests[c] = (int32_t)(p.goff + piv);
+            if (none) atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
+        }
+        __syncthreads();
+        const T alpha = none ? T(0) : urow[c];
+        if (alpha == T(0) && !none && g == 0 && t == 0)
+            atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));  // exactly zero pivot
+        // An exactly zero pivot (the reference throws SingularMatrixException here) leaves
+        // the column unscaled and the rest of the panel unchanged.
+        const bool upd = (alpha != T(0));
+        const T inv = upd ? T(1) / alpha : T(0);
+        // (4) multipliers, column c+1, and the candidate of column c+1
+        const int c1 = c + 1;
+        br = -1;
+        bl = INT_MAX;
+        {
+            T bv = T(-1);
+            for (int r = t; r < rows; r += NT) {
+                const int L = lbl[r];
+                if (L > c) {
+                    T l = T(0);
+                    if (upd) {
+                        l = tile[r * LD + c] * inv;
+                        tile[r * LD + c] = l;
+                    }
+                    lv[r] = l;
+                    if (c1 < W) {
+                        T a = tile[r * LD + c1];
+                        if (upd) a = __builtin_fma(-l, urow[c1], a);
+                        tile[r * LD + c1] = a;
+                        const T v = fabs(a);
+                        if (better(v, L, bv, bl)) {
+                            bv = v;
+                            bl = L;
+                            br = r;
+                        }
+                    }
+                }
+            }
+            block_best(bv, bl, br, sv, sl, sq);
+        }
+        if (c1 < W) {
+            // (5) finish the candidate's row (columns c+2..) and publish column c+1
+            if (br >= 0 && upd) {
+                const T l = lv[br];
+                for (int j = c1 + 1 + t; j < W; j += NT)
+                    tile[br * LD + j] = __builtin_fma(-l, urow[j], tile[br * LD + j]);
             }
             __syncthreads();
-            continue;
+            publish(c1, br, bl);
+            // (6) the rest of the rank-1 update, while the exchange is in flight
+            if (upd) {
+                const int lane = t & 63, wv = t >> 6;
+                for (int r = lane; r < rows; r += 64) {
+                    if (lbl[r] <= c || r == br) continue;
+                    const T l = lv[r];
+                    for (int j = c1 + 1 + wv; j < W; j += NT / 64)
+                        tile[r * LD + j] = __builtin_fma(-l, urow[j], tile[r * LD + j]);
+                }
+            }
+            __syncthreads();
         }
-        const int piv = gl;
-        const T* wrow = p.slot_row + ((size_t)(s * G + gw)) * WMAX;
-        for (int j = c + t; j < W; j += NT) urow[j] = ld_x(&wrow[j]);
-        // Swap = relabel: the row at position c moves to position piv ...
-        for (int r = t; r < rows; r += NT)
-            if (lbl[r] == c) lbl[r] = piv;
-        __syncthreads();
-        // ... and the winner becomes row c.
-        if (gw == g && t == 0) lbl[br] = c;
-        if (g == 0 && t == 0) p.dests[c] = (int32_t)(p.goff + piv);
-        __syncthreads();
-
-        const T alpha = urow[c];
-        if (alpha == T(0)) {
-            // Exactly zero pivot: the reference throws SingularMatrixException here.
-            if (g == 0 && t == 0) atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
-            continue;
-        }
-        const T inv = T(1) / alpha;
-        // Multipliers l = a(r,c) * (1/alpha) into registers (every wave needs them) ...
-        const int lane = t & 63, wv = t >> 6;
-        T lq[RPL];
-        bool act[RPL];
-#pragma unroll
-        for (int q = 0; q < RPL; ++q) {
-            const int r = lane + 64 * q;
-            act[q] = (r < rows) && (lbl[r] > c);
-            lq[q] = act[q] ? tile[r * LD + c] * inv : T(0);
-        }
-        __syncthreads();
-        // ... then wave 0 stores them and all waves apply the rank-1 update, wave wv on
-        // columns j = c+1+wv (mod 4).
-#pragma unroll
-        for (int q = 0; q < RPL; ++q) {
-            if (!act[q]) continue;
-            const int r = lane + 64 * q;
-            if (wv == 0) tile[r * LD + c] = lq[q];
-            for (int j = c + 1 + wv; j < W; j += NT / 64)
-                tile[r * LD + j] = __builtin_fma(-lq[q], urow[j], tile[r * LD + j]);
-        }
-        __syncthreads();
     }
 
     if (aborted) {
@@ -257,7 +372,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
 
 template <typename T>
 size_t panel_lds_bytes(int R, int W) {
-    return sizeof(T) * ((size_t)R * (W + 1) + WMAX + NT / 64) + sizeof(int) * ((size_t)R + 2 * (NT / 64) + 1);
+    return sizeof(T) * ((size_t)R * (W + 2) + WMAX + NT / 64) + sizeof(int) * ((size_t)R + 2 * (NT / 64) + 1);
 }
 
 template <typename T>
@@ -280,9 +395,10 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
         return ELX_ERROR_NOT_SUPPORTED;
     }
     elx::PanelSync* ps = nullptr;
-    ELX_CHECK(elx::panel_sync_get(di.num_cus, sizeof(T) * (size_t)2 * di.num_cus * (WMAX + 1) +
-                                                  sizeof(int) * 2 * di.num_cus,
-                                  &ps));
+    elx::ProfScope prof_(s, elx::kKPanel, 2.0 * (double)mp * (double)w * sizeof(T));
+    const size_t rec_bytes = sizeof(u4) * 2 * (size_t)di.num_cus;
+    const size_t row_bytes = sizeof(u4) * 2 * (size_t)di.num_cus * WMAX;
+    ELX_CHECK(elx::panel_sync_get(di.num_cus, rec_bytes + row_bytes, &ps));
     PanelArgs<T> a;
     a.mp = mp;
     a.w = (int)w;
@@ -294,11 +410,9 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     a.info = info;
     a.G = (int)G;
     a.R = (int)R;
-    a.flags = ps->flags;
-    a.epoch_base = (++ps->epoch) * 1024;
-    a.slot_val = reinterpret_cast<T*>(ps->slots);
-    a.slot_row = a.slot_val + 2 * di.num_cus;
-    a.slot_lbl = reinterpret_cast<int*>(a.slot_row + (size_t)2 * di.num_cus * WMAX);
+    a.seq_base = (uint32_t)((++ps->epoch) * 1024);
+    a.rec = reinterpret_cast<u4*>(ps->slots);
+    a.rows = a.rec + 2 * (size_t)di.num_cus;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lu_panel_kernel<T>),

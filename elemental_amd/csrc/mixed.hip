@@ -227,11 +227,11 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
     c.prec = (factor_type == ELX_FACTOR_FP16) ? 0 : 1;
     c.n = n;
     c.NB = std::min<i64>(1024, round_up(n, 128));
-    c.ldw = n;
+    c.ldw = round_up(n, 128);  // whole 128-row blocks and 16-byte row slices for the solves
     const i64 nblk = (n + 127) / 128;
     const i64 nbpad = round_up(c.NB, 32);
     const int nch = (int)std::max<i64>(1, (n + 1023) / 1024);
-    ELX_CHECK(get(kSlotW, (size_t)n * (size_t)n, &c.W));
+    ELX_CHECK(get(kSlotW, (size_t)c.ldw * (size_t)n, &c.W));
     ELX_CHECK(get(kSlotLh, (size_t)n * (size_t)nbpad, &c.Lh));
     ELX_CHECK(get(kSlotUh, (size_t)n * (size_t)nbpad, &c.Uh));
     ELX_CHECK(get(kSlotLinv, (size_t)nblk * 128 * 128, &c.Linv));

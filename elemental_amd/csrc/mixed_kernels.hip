@@ -77,64 +77,128 @@ __global__ __launch_bounds__(128) void diag_inverse_kernel(i64 n, const float* _
 
 // Persistent flag-chained triangular solve with 128-row blocks, fp32, one right-hand side.
 // lower == 1: L x = y (unit lower, forward); lower == 0: U x = y (upper, backward).
-// Workgroup g handles blocks in solve order g, g+G, ...; block b first accumulates
-// y_b - sum_c T(b,c) x_c as the x_c of earlier blocks are published (flag per block in
-// uncached memory), then multiplies by the precomputed inverse of its diagonal block and
-// publishes x_b. Each block only waits on blocks earlier in solve order, which are handled
-// by co-resident workgroups, so every wait ends; a bounded spin reports failure instead.
+// Workgroup g handles blocks in solve order g, g+G, ...; block b accumulates
+// sum_c T(b,c) x_c over the blocks c earlier in solve order as their x_c are published (flag
+// per block in uncached memory), subtracts it from y_b, multiplies by the precomputed inverse
+// of its diagonal block and publishes x_b. Readiness is checked for up to 64 blocks at once
+// and their x blocks are read in one batch; each thread streams 16-byte slices of T (4 rows
+// x 16 columns per block) so the loads of a block are all in flight together. Each block only
+// waits on blocks earlier in solve order, handled by co-resident workgroups, so every wait
+// ends; a bounded spin reports failure instead. Requires ldw % 4 == 0 (16-byte row slices).
 __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restrict__ W, i64 ldw,
                                                    const float* __restrict__ Inv, int lower,
                                                    const float* __restrict__ y, float* x,
                                                    int64_t* flags, int64_t target, int* err) {
-    __shared__ float xc[128];
-    __shared__ float part[2][128];
-    const int t = threadIdx.x, i = t & 127, h = t >> 7;
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    __shared__ float xs[64][128];
+    __shared__ float red[8][128];
+    __shared__ int s_k;
+    const int t = threadIdx.x, rq = 4 * (t & 31), jg = t >> 5;
     const i64 nblk = (n + 127) / 128;
     for (i64 q = blockIdx.x; q < nblk; q += gridDim.x) {
         const i64 b = lower ? q : (nblk - 1 - q);
         const i64 r0 = b * 128;
         const int nb = (int)std::min<i64>(128, n - r0);
-        float acc = (h == 0 && i < nb) ? y[r0 + i] : 0.0f;
-        for (i64 qc = 0; qc < q; ++qc) {
-            const i64 c = lower ? qc : (nblk - 1 - qc);
-            const i64 c0 = c * 128;
-            const int ncb = (int)std::min<i64>(128, n - c0);
-            if (t == 0) {
-                // Bounded wait; after any timeout (err set, uncached) the others stop waiting.
-                long long spins = 0;
-                while (ld_x(&flags[c]) < target) {
-                    if (((++spins) & 1023) == 0 && ld_x(err) != 0) break;
-                    if (spins > SPIN_LIMIT) {
-                        st_x(err, 1);
-                        break;
-                    }
+        const float* inv = Inv + b * 128 * 128;
+        f4v iv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            iv[u] = *reinterpret_cast<const f4v*>(inv + rq + (jg + 8 * u) * 128);
+        f4v acc = {0.f, 0.f, 0.f, 0.f};
+        i64 qc = 0;
+        long long spins = 0;
+        bool bail = false;
+        while (qc < q && !bail) {
+            if (t < 64) {
+                const i64 qq = qc + t;
+                bool ready = false;
+                if (qq < q) {
+                    const i64 c = lower ? qq : (nblk - 1 - qq);
+                    ready = ld_x(&flags[c]) >= target;
+                }
+                const unsigned long long m = __ballot(ready);
+                if (t == 0) s_k = (m == ~0ull) ? 64 : __builtin_ctzll(~m);
+            }
+            __syncthreads();
+            const int k = s_k;
+            __syncthreads();
+            if (k == 0) {
+                ++spins;
+                if ((spins & 255) == 0 && ld_x(err) != 0) bail = true;
+                if (spins > SPIN_LIMIT) {
+                    if (t == 0) st_x(err, 1);
+                    bail = true;
+                }
+                continue;
+            }
+            for (int idx = t; idx < k * 128; idx += 256) {
+                const int kk = idx >> 7, jj = idx & 127;
+                const i64 qq = qc + kk;
+                const i64 c = lower ? qq : (nblk - 1 - qq);
+                const i64 col = c * 128 + jj;
+                xs[kk][jj] = (col < n) ? ld_x(&x[col]) : 0.0f;
+            }
+            __syncthreads();
+            for (int kk = 0; kk < k; ++kk) {
+                const i64 qq = qc + kk;
+                const i64 c0 = (lower ? qq : (nblk - 1 - qq)) * 128;
+                f4v w[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const i64 col = c0 + jg + 8 * u;
+                    w[u] = (col < n) ? *reinterpret_cast<const f4v*>(W + (r0 + rq) + col * ldw)
+                                     : f4v{0.f, 0.f, 0.f, 0.f};
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const float xv = xs[kk][jg + 8 * u];
+                    acc[0] = __builtin_fmaf(w[u][0], xv, acc[0]);
+                    acc[1] = __builtin_fmaf(w[u][1], xv, acc[1]);
+                    acc[2] = __builtin_fmaf(w[u][2], xv, acc[2]);
+                    acc[3] = __builtin_fmaf(w[u][3], xv, acc[3]);
                 }
             }
-            __syncthreads();
-            if (t < 128) xc[t] = (t < ncb) ? ld_x(&x[c0 + t]) : 0.0f;
-            __syncthreads();
-            if (i < nb) {
-                const float* col = W + (r0 + i) + (c0 + 64 * h) * ldw;
-                const int jend = std::min(64, ncb - 64 * h);
-                for (int jj = 0; jj < jend; ++jj)
-                    acc = __builtin_fmaf(-col[(i64)jj * ldw], xc[64 * h + jj], acc);
-            }
+            qc += k;
             __syncthreads();
         }
-        part[h][i] = acc;
+        // v = y_b - sum_c T(b,c) x_c
+        red[jg][rq + 0] = acc[0];
+        red[jg][rq + 1] = acc[1];
+        red[jg][rq + 2] = acc[2];
+        red[jg][rq + 3] = acc[3];
         __syncthreads();
-        if (t < 128) xc[t] = part[0][t] + part[1][t];
+        if (t < 128) {
+            float sacc = 0.f;
+#pragma unroll
+            for (int g2 = 0; g2 < 8; ++g2) sacc += red[g2][t];
+            xs[0][t] = (t < nb) ? (y[r0 + t] - sacc) : 0.0f;
+        }
         __syncthreads();
-        // x_b = Inv_b * v
-        const float* inv = Inv + b * 128 * 128;
-        float s = 0.0f;
-        for (int jj = 64 * h; jj < 64 * h + 64; ++jj) s = __builtin_fmaf(inv[i + jj * 128], xc[jj], s);
-        part[h][i] = s;
+        // x_b = Inv_b v
+        f4v s4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const float vv = xs[0][jg + 8 * u];
+            s4[0] = __builtin_fmaf(iv[u][0], vv, s4[0]);
+            s4[1] = __builtin_fmaf(iv[u][1], vv, s4[1]);
+            s4[2] = __builtin_fmaf(iv[u][2], vv, s4[2]);
+            s4[3] = __builtin_fmaf(iv[u][3], vv, s4[3]);
+        }
+        red[jg][rq + 0] = s4[0];
+        red[jg][rq + 1] = s4[1];
+        red[jg][rq + 2] = s4[2];
+        red[jg][rq + 3] = s4[3];
         __syncthreads();
-        if (t < nb) st_x(&x[r0 + t], part[0][t] + part[1][t]);
+        if (t < nb) {
+            float sx = 0.f;
+#pragma unroll
+            for (int g2 = 0; g2 < 8; ++g2) sx += red[g2][t];
+            st_x(&x[r0 + t], sx);
+        }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (t == 0) st_x(&flags[b], target);
+        __syncthreads();
     }
 }
 
@@ -335,6 +399,7 @@ elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const fl
     const DeviceInfo& di = device_info();
     const i64 nblk = (n + 127) / 128;
     const unsigned G = (unsigned)std::min<i64>(nblk, di.num_cus);
+    ProfScope prof_(s, kKTrsv, 0.5 * (double)n * (double)n * sizeof(float));
     void* args[] = {&n, &W, &ldw, &Inv, &lower, &y, &x, &flags, &target, &err};
     ELX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&trsv_kernel), dim3(G),
                                        dim3(256), args, 0, s));
@@ -386,6 +451,7 @@ elx_status_t launch_residual(hipStream_t s, i64 n, const double* A, i64 lda, con
                              const double* b, double* r, double* part, int nch, double* norms) {
     const int cpc = (int)((n + nch - 1) / nch);
     if (cpc > 1024) return ELX_ERROR_INVALID_ARG;
+    ProfScope prof_(s, kKResidual, (double)n * (double)n * sizeof(double));
     dim3 grid(cdiv(n, 256), (unsigned)nch);
     hipLaunchKernelGGL(gemv_partial_kernel, grid, dim3(256), 0, s, n, A, lda, x, part, cpc);
     ELX_HIP(hipGetLastError());

@@ -43,6 +43,17 @@ const char* elx_version(void);
 int elx_device_count(void);
 elx_status_t elx_finalize(void);
 
+/* Kernel-class timing (no counterpart in the reference; instrumentation for measurement):
+ * when enabled, every launch of the classes below is bracketed by HIP events on its own
+ * stream, and elx_profile_query returns the number of launches, their summed device time
+ * (ms) and summed algorithmic work (flops for GEMM classes, bytes otherwise). Classes:
+ * 0 LU panel, 1 FP64 GEMM, 2 low-precision GEMM, 3 small TRSM, 4 row moves, 5 persistent
+ * TRSV, 6 FP64 residual, 7 operand pack. */
+void elx_profile_enable(int on);
+void elx_profile_reset(void);
+elx_status_t elx_profile_query(int kernel_class, int64_t* launches, double* total_ms,
+                               double* work);
+
 /* ------------------------------------------------------------------------------------
  * Plain FP64 path (the drop-in)
  * ---------------------------------------------------------------------------------- */
