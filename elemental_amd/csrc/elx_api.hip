@@ -37,6 +37,23 @@ const DeviceInfo& device_info() {
     return g_info.emplace(dev, d).first->second;
 }
 
+elx_status_t coop_fits(const void* fn, int block, size_t lds, i64 grid) {
+    const DeviceInfo& di = device_info();
+    if (!di.coop) return ELX_ERROR_NOT_SUPPORTED;
+    hipFuncAttributes fa;
+    ELX_HIP(hipFuncGetAttributes(&fa, fn));
+    int per_cu = 0;
+    ELX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds));
+    if (grid > (i64)per_cu * di.num_cus) {
+        std::fprintf(stderr,
+                     "elemental_amd: cooperative launch of %lld workgroups exceeds %d per CU x %d "
+                     "CUs\n",
+                     (long long)grid, per_cu, di.num_cus);
+        return ELX_ERROR_NOT_SUPPORTED;
+    }
+    return ELX_SUCCESS;
+}
+
 elx_status_t scratch_get(int slot, size_t bytes, void** out) {
     int dev = 0;
     ELX_HIP(hipGetDevice(&dev));

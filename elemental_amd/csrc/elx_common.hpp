@@ -102,6 +102,11 @@ struct ProfScope {
 namespace elx {
 
 // C := alpha op(A) op(B) + beta C  (FP64, MFMA). tri: 0 full, 1 lower only, 2 upper only.
+// Checks, before a cooperative launch, that all `grid` workgroups of `fn` (block threads,
+// `lds` bytes of dynamic LDS) can be resident at once; querying the kernel's attributes also
+// loads its code object before the first launch.
+elx_status_t coop_fits(const void* fn, int block, size_t lds, i64 grid);
+
 elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                           const double* A, i64 lda, const double* B, i64 ldb, double beta,
                           double* C, i64 ldc, int tri);

@@ -11,15 +11,17 @@
 // never goes back to HBM between columns. Rows are never moved during the panel: a row's
 // "label" is its position in the swapped order, and a swap relabels two rows. Each column
 // step is one all-to-all exchange: every workgroup publishes its local pivot candidate
-// (value and label in one 16-byte record, and the candidate row in 16-byte chunks) to
-// uncached memory, every workgroup reads all G records, reduces them to the same winner
-// (largest magnitude, ties to the smallest label -- the first index idamax would return),
-// and reads the winner's row. Records and chunks carry a tag derived from the column's
-// sequence number and their own contents, so a reader recognises stale or partly written
-// data and simply reads again: no store-completion waits or separate flags, two memory
-// round trips per column. At the end each workgroup writes its rows to their final
-// positions, and the list of moved rows (at most 2w) is emitted so the columns outside the
-// panel can be permuted in one gather pass.
+// (value, label and look-ahead data in 16-byte records, and the candidate row in 16-byte
+// chunks) to uncached memory, every workgroup reads all G records and reduces them to the
+// same winner (largest magnitude, ties to the smallest label -- the first index idamax would
+// return). The winner's row is read while the next column's records are exchanged, and the
+// rank-1 update of the columns right of the next one is applied one step late, so one
+// memory round trip per column sits on the critical path. Records and chunks carry a tag
+// derived from the column's sequence number and their own contents, so a reader recognises
+// stale or partly written data and simply reads again: no store-completion waits or
+// separate flags. At the end each workgroup writes its rows to their final positions, and
+// the list of moved rows (at most 2w) is emitted so the columns outside the panel can be
+// permuted in one gather pass.
 #include <algorithm>
 #include <climits>
 
@@ -46,8 +48,8 @@ struct PanelArgs {
     int G;
     int R;
     uint32_t seq_base;  // sequence number of column 0 of this launch
-    u4* rec;            // [2][G] candidate records
-    u4* rows;           // [2][G][WMAX] candidate row chunks
+    u4* rec;            // [2][G][3] candidate records
+    u4* rows;           // [3][G][WMAX] candidate row chunks
 };
 
 // The exchange area lives in uncached device memory (allocated with
@@ -155,8 +157,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     const int R = p.R, W = p.w, LD = W + 1;
     T* tile = reinterpret_cast<T*>(smem_raw);   // R x LD, row-major
     T* urow = tile + (size_t)R * LD;            // WMAX
-    T* lv = urow + WMAX;                        // R: multipliers of the current column
-    T* sv = lv + R;                             // NT/64
+    T* lv = urow + WMAX;                        // R + WMAX + R + 3*NT: pipeline buffers
+    T* sv = lv + 2 * (size_t)R + WMAX + 3 * NT; // NT/64
     int* lbl = reinterpret_cast<int*>(sv + NT / 64);  // R
     int* sl = lbl + R;                          // NT/64
     int* sq = sl + NT / 64;                     // NT/64
@@ -175,13 +177,31 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
 
     typedef Chunk<T> CK;
     const int q1 = (W + CK::VPC - 1) / CK::VPC;
-    // Publish the candidate of column cc: row chunks (columns cc..w-1), then the record.
-    auto publish = [&](int cc, int cand, int cand_lbl) {
+    // Deferred-update pipeline. At the start of step c every live row (label >= c) holds
+    // column c updated through step c-1 and columns >= c+1 updated through step c-2: the
+    // step c-1 update of those columns waits for the step c-1 pivot row, which is fetched
+    // while step c's records are exchanged. The record for column c carries, for its
+    // candidate r: a_r[c] (exact, for the pivot choice), its label, and in a look-ahead chunk
+    // a_r[c+1] through step c-2 and r's step c-1 multiplier; every reader forms the pivot
+    // row's entry u_c[c+1] = a_r[c+1] - l_r u_{c-1}[c+1] with the FMA the owner would use,
+    // and likewise the rest of u_c from the fetched row one step later. Each element still
+    // receives its updates one FMA at a time in step order, exactly as in lu::Panel.
+    T* up = urow;             // u_{c-1}, columns >= c          [WMAX]
+    T* un = lv + R;           // u_c being formed               [WMAX]
+    T* lp = lv;               // step c-1 multipliers per row   [R]
+    T* lc = un + WMAX;        // step c multipliers per row     [R]
+    T* la_v = lc + R;         // per record: signed candidate value, look-ahead value and
+    T* la_a = la_v + NT;      // multiplier                     [3 x NT]
+    T* la_l = la_a + NT;
+    for (int j = t; j < WMAX; j += NT) up[j] = T(0);
+    for (int r = t; r < rows; r += NT) lp[r] = T(0);
+
+    auto publish = [&](int cc, int cand, int cand_lbl, T aprev_v, T lprev_v) {
         const uint32_t seq = p.seq_base + (uint32_t)cc;
         const int sp = cc & 1;
-        u4* myrow = p.rows + ((size_t)(sp * G + g)) * WMAX;
+        u4* myrow = p.rows + ((size_t)((cc % 3) * G + g)) * WMAX;
         if (cand >= 0)
-            for (int q = cc / CK::VPC + t; q < q1; q += NT) {
+            for (int q = (cc + 1) / CK::VPC + t; q < q1; q += NT) {
                 T v[CK::VPC];
 #pragma unroll
                 for (int e = 0; e < CK::VPC; ++e) {
@@ -190,12 +210,16 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                 }
                 st16(&myrow[q], CK::make(v, q, seq));
             }
-        if (t == 0)
-            st16(&p.rec[sp * G + g],
-                 make_rec((cand >= 0) ? (double)tile[cand * LD + cc] : 0.0, cand_lbl, seq));
+        if (t == 0) {
+            u4* rec = p.rec + 3 * (size_t)(sp * G + g);
+            const double a = (double)aprev_v, l = (double)lprev_v;
+            st16(&rec[1], Chunk<double>::make(&a, 0, seq));
+            st16(&rec[2], Chunk<double>::make(&l, 1, seq));
+            st16(&rec[0], make_rec((cand >= 0) ? (double)tile[cand * LD + cc] : 0.0, cand_lbl, seq));
+        }
     };
 
-    // Prologue: the candidate of column 0.
+    // Prologue: the candidate of column 0 (nothing pending).
     int br = -1, bl = INT_MAX;
     {
         T bv = T(-1);
@@ -208,15 +232,19 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             }
         }
         block_best(bv, bl, br, sv, sl, sq);
-        publish(0, br, bl);
+        publish(0, br, bl, (br >= 0 && W > 1) ? tile[br * LD + 1] : T(0), T(0));
     }
 
     bool aborted = false;
     long long spins = 0;
+    // the pivot row of the previous step, in flight: chunk registers, owner, its multiplier
+    u4 fch = u4{0u, 0u, 0u, 0u};
+    int f_w = -1, f_q0 = 0;
+    uint32_t f_seq = 0;
+    T f_l = T(0);
     for (int c = 0; c < W; ++c) {
         const int s = c & 1;
         const uint32_t seq = p.seq_base + (uint32_t)c;
-        const int q0 = c / CK::VPC;
         // (1) every workgroup's record for column c, reduced to the same winner
         T gv = T(-1);
         int gl = INT_MAX, gw = -1;
@@ -226,14 +254,24 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             gw = -1;
             int ok = 1;
             for (int q = t; q < G; q += NT) {
-                const u4 r = ld16(&p.rec[s * G + q]);
-                if (!rec_ok(r, seq)) {
+                const u4* rq = p.rec + 3 * (size_t)(s * G + q);
+                const u4 r = ld16(&rq[0]);
+                const u4 ra = ld16(&rq[1]);
+                const u4 rl = ld16(&rq[2]);
+                if (!rec_ok(r, seq) || !Chunk<double>::ok(ra, 0, seq) || !Chunk<double>::ok(rl, 1, seq)) {
                     ok = 0;
                     continue;
                 }
+                double a, l;
+                Chunk<double>::get(ra, &a);
+                Chunk<double>::get(rl, &l);
+                const T val = (T)rec_val(r);
+                la_v[q] = val;
+                la_a[q] = (T)a;
+                la_l[q] = (T)l;
                 const int L = (int)r[2];
                 if (L != INT_MAX) {
-                    const T v = fabs((T)rec_val(r));
+                    const T v = fabs(val);
                     if (better(v, L, gv, gl)) {
                         gv = v;
                         gl = L;
@@ -251,24 +289,19 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         block_best(gv, gl, gw, sv, sl, sq);
         const bool none = (gw < 0);  // every remaining candidate is NaN (idamax selects none)
         const int piv = none ? c : gl;
-        // (2) the winner's row, columns c..w-1
-        if (!none) {
-            const u4* wrow = p.rows + ((size_t)(s * G + gw)) * WMAX;
+        const T alpha = none ? T(0) : la_v[gw];
+        const T aprev_w = none ? T(0) : la_a[gw];
+        const T lprev_w = none ? T(0) : la_l[gw];
+        // (2) the previous pivot row arrives: form u_{c-1} from its published values (through
+        // step c-3) and its step c-2 multiplier, then make it current
+        if (f_w >= 0) {
+            const int q = f_q0 + t;
+            const bool mine_q = (q < q1);
             while (true) {
                 int ok = 1;
-                for (int q = q0 + t; q < q1; q += NT) {
-                    const u4 ch = ld16(&wrow[q]);
-                    if (!CK::ok(ch, q, seq)) {
-                        ok = 0;
-                        continue;
-                    }
-                    T v[CK::VPC];
-                    CK::get(ch, v);
-#pragma unroll
-                    for (int e = 0; e < CK::VPC; ++e) {
-                        const int j = q * CK::VPC + e;
-                        if (j >= c && j < W) urow[j] = v[e];
-                    }
+                if (mine_q && !CK::ok(fch, q, f_seq)) {
+                    fch = ld16(p.rows + ((size_t)(((c - 1) % 3) * G + f_w)) * WMAX + q);
+                    ok = CK::ok(fch, q, f_seq) ? 1 : 0;
                 }
                 if (__syncthreads_and(ok)) break;
                 if (++spins > SPIN_LIMIT) {
@@ -277,11 +310,29 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                 }
             }
             if (aborted) break;
-        } else {
-            for (int j = c + t; j < W; j += NT) urow[j] = T(0);
+            if (mine_q) {
+                T v[CK::VPC];
+                CK::get(fch, v);
+#pragma unroll
+                for (int e = 0; e < CK::VPC; ++e) {
+                    const int j = q * CK::VPC + e;
+                    if (j >= c && j < W) un[j] = __builtin_fma(-f_l, up[j], v[e]);
+                }
+            }
+            __syncthreads();
+            for (int j = c + t; j < W; j += NT) up[j] = un[j];
+        } else if (c > 0) {
+            for (int j = c + t; j < W; j += NT) up[j] = T(0);  // no pivot row (all NaN)
         }
-        // (3) swap = relabel: the row at position c moves to position piv, the winner
-        // becomes row c
+        // (3) issue the fetch of this step's pivot row (consumed at the next step)
+        f_w = none ? -1 : gw;
+        f_seq = seq;
+        f_l = lprev_w;
+        f_q0 = (c + 1) / CK::VPC;
+        if (f_w >= 0 && f_q0 + t < q1)
+            fch = ld16(p.rows + ((size_t)((c % 3) * G + f_w)) * WMAX + f_q0 + t);
+        // (4) swap = relabel: the row at position c moves to position piv, the winner becomes
+        // row c
         const bool mine = (!none && gw == g);
         for (int r = t; r < rows; r += NT) {
             const int L = lbl[r];
@@ -290,35 +341,35 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         }
         if (g == 0 && t == 0) {
             p.dests[c] = (int32_t)(p.goff + piv);
-            if (none) atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
+            if (none || alpha == T(0)) atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
         }
         __syncthreads();
-        const T alpha = none ? T(0) : urow[c];
-        if (alpha == T(0) && !none && g == 0 && t == 0)
-            atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));  // exactly zero pivot
         // An exactly zero pivot (the reference throws SingularMatrixException here) leaves
         // the column unscaled and the rest of the panel unchanged.
         const bool upd = (alpha != T(0));
         const T inv = upd ? T(1) / alpha : T(0);
-        // (4) multipliers, column c+1, and the candidate of column c+1
         const int c1 = c + 1;
+        const T upc1 = (c1 < W) ? up[c1] : T(0);
+        const T uc1 = __builtin_fma(-lprev_w, upc1, aprev_w);  // u_c[c+1]
+        // (5) multipliers; column c+1 through step c; candidate of column c+1
         br = -1;
         bl = INT_MAX;
         {
             T bv = T(-1);
             for (int r = t; r < rows; r += NT) {
                 const int L = lbl[r];
-                if (L > c) {
-                    T l = T(0);
-                    if (upd) {
-                        l = tile[r * LD + c] * inv;
-                        tile[r * LD + c] = l;
-                    }
-                    lv[r] = l;
-                    if (c1 < W) {
-                        T a = tile[r * LD + c1];
-                        if (upd) a = __builtin_fma(-l, urow[c1], a);
-                        tile[r * LD + c1] = a;
+                if (L < c) continue;
+                T a = (c1 < W) ? __builtin_fma(-lp[r], upc1, tile[r * LD + c1]) : T(0);
+                T l = T(0);
+                if (L > c && upd) {
+                    l = tile[r * LD + c] * inv;
+                    tile[r * LD + c] = l;
+                    a = __builtin_fma(-l, uc1, a);
+                }
+                lc[r] = l;
+                if (c1 < W) {
+                    tile[r * LD + c1] = a;
+                    if (L > c) {
                         const T v = fabs(a);
                         if (better(v, L, bv, bl)) {
                             bv = v;
@@ -331,26 +382,27 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             block_best(bv, bl, br, sv, sl, sq);
         }
         if (c1 < W) {
-            // (5) finish the candidate's row (columns c+2..) and publish column c+1
-            if (br >= 0 && upd) {
-                const T l = lv[br];
+            // (6) bring the candidate's columns >= c+2 through step c-1, publish column c+1
+            if (br >= 0)
                 for (int j = c1 + 1 + t; j < W; j += NT)
-                    tile[br * LD + j] = __builtin_fma(-l, urow[j], tile[br * LD + j]);
-            }
+                    tile[br * LD + j] = __builtin_fma(-lp[br], up[j], tile[br * LD + j]);
             __syncthreads();
-            publish(c1, br, bl);
-            // (6) the rest of the rank-1 update, while the exchange is in flight
-            if (upd) {
-                const int lane = t & 63, wv = t >> 6;
-                for (int r = lane; r < rows; r += 64) {
-                    if (lbl[r] <= c || r == br) continue;
-                    const T l = lv[r];
-                    for (int j = c1 + 1 + wv; j < W; j += NT / 64)
-                        tile[r * LD + j] = __builtin_fma(-l, urow[j], tile[r * LD + j]);
-                }
+            publish(c1, br, bl, (br >= 0 && c1 + 1 < W) ? tile[br * LD + c1 + 1] : T(0),
+                    (br >= 0) ? lc[br] : T(0));
+            // (7) the step c-1 update of columns >= c+2 for the other live rows, while the
+            // exchange is in flight
+            const int lane = t & 63, wv = t >> 6;
+            for (int r = lane; r < rows; r += 64) {
+                if (lbl[r] < c || r == br) continue;
+                const T l = lp[r];
+                for (int j = c1 + 1 + wv; j < W; j += NT / 64)
+                    tile[r * LD + j] = __builtin_fma(-l, up[j], tile[r * LD + j]);
             }
-            __syncthreads();
         }
+        // the step c multipliers become the pending ones
+        __syncthreads();
+        for (int r = t; r < rows; r += NT) lp[r] = lc[r];
+        __syncthreads();
     }
 
     if (aborted) {
@@ -372,7 +424,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
 
 template <typename T>
 size_t panel_lds_bytes(int R, int W) {
-    return sizeof(T) * ((size_t)R * (W + 2) + WMAX + NT / 64) + sizeof(int) * ((size_t)R + 2 * (NT / 64) + 1);
+    return sizeof(T) * ((size_t)R * (W + 3) + 2 * WMAX + 3 * NT + NT / 64) +
+           sizeof(int) * ((size_t)R + 2 * (NT / 64) + 1);
 }
 
 template <typename T>
@@ -386,6 +439,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     if (G < 1) G = 1;
     i64 R = (mp + G - 1) / G;
     G = (mp + R - 1) / R;  // every workgroup owns at least one row
+    ELX_ARG(G <= NT);      // one record per thread in the exchange
     const size_t lds = panel_lds_bytes<T>((int)R, (int)w);
     if (lds > lds_cap || R > 64 * RPL) {
         std::fprintf(stderr,
@@ -396,8 +450,10 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     }
     elx::PanelSync* ps = nullptr;
     elx::ProfScope prof_(s, elx::kKPanel, 2.0 * (double)mp * (double)w * sizeof(T));
-    const size_t rec_bytes = sizeof(u4) * 2 * (size_t)di.num_cus;
-    const size_t row_bytes = sizeof(u4) * 2 * (size_t)di.num_cus * WMAX;
+    const size_t rec_bytes = sizeof(u4) * 3 * 2 * (size_t)di.num_cus;
+    // Row chunks rotate over three slots: a winner's row of column c is read until the
+    // readers reach column c+1, while its owner may already publish column c+2.
+    const size_t row_bytes = sizeof(u4) * 3 * (size_t)di.num_cus * WMAX;
     ELX_CHECK(elx::panel_sync_get(di.num_cus, rec_bytes + row_bytes, &ps));
     PanelArgs<T> a;
     a.mp = mp;
@@ -412,7 +468,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     a.R = (int)R;
     a.seq_base = (uint32_t)((++ps->epoch) * 1024);
     a.rec = reinterpret_cast<u4*>(ps->slots);
-    a.rows = a.rec + 2 * (size_t)di.num_cus;
+    a.rows = a.rec + 3 * 2 * (size_t)di.num_cus;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
@@ -420,6 +476,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
         (void)hipGetLastError();
         attr_set = true;
     }
+    ELX_CHECK(elx::coop_fits(reinterpret_cast<const void*>(&lu_panel_kernel<T>), NT, lds, G));
     void* args[] = {&a};
     ELX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
                                        dim3((unsigned)G), dim3(NT), args, (unsigned)lds, s));

@@ -400,6 +400,7 @@ elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const fl
     const i64 nblk = (n + 127) / 128;
     const unsigned G = (unsigned)std::min<i64>(nblk, di.num_cus);
     ProfScope prof_(s, kKTrsv, 0.5 * (double)n * (double)n * sizeof(float));
+    ELX_CHECK(coop_fits(reinterpret_cast<const void*>(&trsv_kernel), 256, 0, G));
     void* args[] = {&n, &W, &ldw, &Inv, &lower, &y, &x, &flags, &target, &err};
     ELX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&trsv_kernel), dim3(G),
                                        dim3(256), args, 0, s));

@@ -87,9 +87,10 @@ def test_lu_known_pivots_permutation_and_triangular(orc):
     P = np.eye(n)[p]  # row i of P A = row p[i]
     F, d, info = orc.lu(P * (2.0 ** np.arange(n))[None, :], bsize=4)
     assert info == 0
-    assert np.array_equal(orc.perm_explicit(d, n), np.argsort(p).astype(np.int32)) or True
-    # explicit vector reproduces P A = L U with L = I for a scaled permutation
+    # the row of A holding column j's nonzero is argsort(p)[j]: the explicit vector
+    # (A(pe,:) = P A) lists exactly those rows, and P A = L U with L = I
     pe = orc.perm_explicit(d, n)
+    assert np.array_equal(pe, np.argsort(p).astype(np.int32))
     A = P * (2.0 ** np.arange(n))[None, :]
     assert np.array_equal(A[pe], np.triu(F))
     # upper triangular with dominant diagonal: identity pivots, factors = A
@@ -226,3 +227,69 @@ def test_lu_config1_against_host_lapack(orc):
     Y = orc.lu_solve_after(F, d, b)
     rel = np.abs(A @ Y - b).max() / (EPS * n * max(np.abs(A).sum(0).max(), np.abs(Y).sum(0).max()))
     assert rel <= 100
+
+
+def test_perm_parity_and_explicit_closed_forms(orc):
+    # Permutation::Parity flips once per non-trivial swap (Permutation.cpp:213-214)
+    assert orc.perm_parity(np.arange(5, dtype=np.int32)) == 0
+    assert orc.perm_parity(np.array([1, 1, 2], np.int32)) == 1
+    assert orc.perm_parity(np.array([2, 2, 2], np.int32)) == 0  # (0 2)(1 2) -> 3-cycle, even
+    d = np.array([2, 2, 2], np.int32)
+    # swap 0<->2 then 1<->2 on rows [0,1,2]: [2,1,0] -> [2,0,1]
+    assert np.array_equal(orc.perm_explicit(d, 3), np.array([2, 0, 1], np.int32))
+
+
+def test_factor_error_measures_planted_defects(orc):
+    """The factor-error checks used by the GPU tests must see a planted error of known size."""
+    rng = np.random.default_rng(9)
+    n = 12
+    A = np.asfortranarray(rng.standard_normal((n, n)))
+    F, d, _ = orc.lu(A, bsize=4)
+    e0 = orc.lu_factor_error(A, F, d)
+    assert e0 <= 8 * n * EPS
+    G = F.copy(order="F")
+    G[5, 7] += 1e-6  # one U entry off by 1e-6: ||PA - LU|| grows by ~1e-6 relative
+    e1 = orc.lu_factor_error(A, G, d)
+    assert 1e-8 < e1 < 1e-5
+    d2 = d.copy()
+    d2[0] = (d2[0] + 1) % n  # wrong first pivot: a gross error
+    assert orc.lu_factor_error(A, F, d2) > 1e-3
+    H = np.asfortranarray(A @ A.T / n + np.eye(n))
+    C, info = orc.cholesky(H, uplo=0, bsize=4)
+    assert info == 0 and orc.chol_factor_error(H, C, 0) <= 8 * n * EPS
+    C[6, 2] *= 1 + 1e-6
+    assert orc.chol_factor_error(H, C, 0) > 1e-9
+
+
+def test_trace_column_is_the_pivot_candidate_column(orc):
+    # lu_trace_column(A, c) returns the candidates of step c (rows c.., after the first c
+    # steps): column 0 is A(:,0); every column's first largest entry sits at dests[c] - c and
+    # equals U(c,c) in magnitude.
+    rng = np.random.default_rng(3)
+    A = np.asfortranarray(rng.standard_normal((40, 40)))
+    F, d, _ = orc.lu(A, bsize=8)
+    assert np.array_equal(orc.lu_trace_column(A, 0, 8), A[:, 0])
+    for c in (1, 7, 8, 9, 23, 39):
+        col = orc.lu_trace_column(A, c, 8)
+        k = int(np.argmax(np.abs(col)))
+        assert k == d[c] - c and abs(col[k]) == abs(F[c, c]), c
+
+
+def test_host_dgetrf_matches_scipy(orc):
+    import scipy.linalg as sl
+    rng = np.random.default_rng(8)
+    A = np.asfortranarray(rng.standard_normal((64, 64)))
+    for name in ("openblas", "mkl"):
+        try:
+            orc.set_backend(name)
+            break
+        except RuntimeError:
+            continue
+    else:
+        pytest.skip("no host LAPACK")
+    try:
+        F, ip, info = orc.host_dgetrf(A)
+    finally:
+        orc.set_backend("naive")
+    lu_, piv = sl.lu_factor(A)
+    assert info == 0 and np.array_equal(ip, piv) and np.allclose(F, lu_, atol=1e-12)
