@@ -216,6 +216,17 @@ def main():
         t_plain = sum(times) / len(times)
         plain = {"solves_per_s": round(1.0 / t_plain, 4), "ms_per_solve": round(1e3 * t_plain, 2),
                  "backward_error": backward_error(A, xb, b), "info": int(info.item())}
+        if not args.no_profile_pass:
+            # kernel classes of one profiled plain FP64 solve
+            lib.elx_profile_reset()
+            lib.elx_profile_enable(1)
+            Ac.copy_(A)
+            xb.copy_(b)
+            d, info = dev.getrf(Ac)
+            dev.getrs(Ac, d, xb)
+            torch.cuda.synchronize()
+            plain["kernels"] = prof_read(lib)
+            lib.elx_profile_enable(0)
         del Ac
         torch.cuda.empty_cache()
         print(f"[bench] plain fp64 {plain}", file=sys.stderr, flush=True)

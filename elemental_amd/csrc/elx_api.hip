@@ -54,6 +54,23 @@ elx_status_t coop_fits(const void* fn, int block, size_t lds, i64 grid) {
     return ELX_SUCCESS;
 }
 
+elx_status_t launch_resident(const void* fn, i64 grid, int block, void** args, size_t lds,
+                             hipStream_t s) {
+    ELX_CHECK(coop_fits(fn, block, lds, grid));
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3((unsigned)block), args,
+                                              (unsigned)lds, s);
+    if (e == hipErrorIllegalState) {
+        (void)hipGetLastError();
+        e = hipLaunchKernel(fn, dim3((unsigned)grid), dim3((unsigned)block), args, lds, s);
+    }
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "elemental_amd: launch of a resident kernel failed: %s\n",
+                     hipGetErrorString(e));
+        return ELX_ERROR_HIP;
+    }
+    return ELX_SUCCESS;
+}
+
 elx_status_t scratch_get(int slot, size_t bytes, void** out) {
     int dev = 0;
     ELX_HIP(hipGetDevice(&dev));

@@ -107,6 +107,16 @@ namespace elx {
 // loads its code object before the first launch.
 elx_status_t coop_fits(const void* fn, int block, size_t lds, i64 grid);
 
+// Launches a kernel whose workgroups wait on each other (every wait bounded): a cooperative
+// launch, so that all workgroups are resident together. When the runtime cannot give this
+// process a cooperative launch (hipErrorIllegalState, seen while another process on the same
+// GPU holds the cooperative resources) the kernel is launched normally: with the grid no
+// larger than what fits at once (checked by coop_fits) and an otherwise idle GPU, all
+// workgroups are still resident, and if they are not, the bounded waits end the kernel with
+// an error status instead of a hang.
+elx_status_t launch_resident(const void* fn, i64 grid, int block, void** args, size_t lds,
+                             hipStream_t s);
+
 elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                           const double* A, i64 lda, const double* B, i64 ldb, double beta,
                           double* C, i64 ldc, int tri);

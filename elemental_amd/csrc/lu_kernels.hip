@@ -115,10 +115,18 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
                                                          i64 lda, T* B, i64 ldb, int right,
                                                          int lower_eff, int mtrans, int unit) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* x = reinterpret_cast<T*>(smem_raw);  // x[i * TV + v]
+    T* x = reinterpret_cast<T*>(smem_raw);  // x[v * m + i]: a vector's rows are adjacent, so
+                                            // the 16 lanes of a vector hit distinct banks
+    T* Ms = x + (size_t)m * TV;             // the triangle, Ms[i + j*m] = op(A)(i, j)
     const int t = threadIdx.x;
     const int v = t / TP, q = t % TP;
     const i64 v0 = (i64)blockIdx.x * TV;
+    // The triangle is staged in LDS once, so each of the m dependent steps below waits on
+    // LDS rather than on the cache hierarchy.
+    for (int idx = t; idx < m * m; idx += TNT) {
+        const int i = idx % m, j = idx / m;  // walk A's columns: coalesced reads
+        Ms[mtrans ? (j + i * m) : (i + j * m)] = A[i + (i64)j * lda];
+    }
     // Coalesced staging: left side walks down each column, right side across rows.
     for (int idx = t; idx < m * TV; idx += TNT) {
         int i, vv;
@@ -132,35 +140,35 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
         const i64 gv = v0 + vv;
         T val = T(0);
         if (gv < nvec) val = right ? B[gv + (i64)i * ldb] : B[i + gv * ldb];
-        x[i * TV + vv] = val;
+        x[vv * m + i] = val;
     }
     __syncthreads();
-    auto M = [&](int i, int j) -> T { return mtrans ? A[j + (i64)i * lda] : A[i + (i64)j * lda]; };
+    auto M = [&](int i, int j) -> T { return Ms[i + j * m]; };
     if (lower_eff) {
         for (int j = 0; j < m; ++j) {
-            T xj = x[j * TV + v];
+            T xj = x[v * m + j];
             if (!unit) xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
             __builtin_amdgcn_wave_barrier();
-            if (!unit && q == 0) x[j * TV + v] = xj;
+            if (!unit && q == 0) x[v * m + j] = xj;
             for (int i = j + 1 + q; i < m; i += TP)
-                x[i * TV + v] = __builtin_fma(-M(i, j), xj, x[i * TV + v]);
+                x[v * m + i] = __builtin_fma(-M(i, j), xj, x[v * m + i]);
             __builtin_amdgcn_wave_barrier();
         }
     } else if (!mtrans) {
         for (int j = m - 1; j >= 0; --j) {
-            T xj = x[j * TV + v];
+            T xj = x[v * m + j];
             if (!unit) xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
             __builtin_amdgcn_wave_barrier();
-            if (!unit && q == 0) x[j * TV + v] = xj;
-            for (int i = q; i < j; i += TP) x[i * TV + v] = __builtin_fma(-M(i, j), xj, x[i * TV + v]);
+            if (!unit && q == 0) x[v * m + j] = xj;
+            for (int i = q; i < j; i += TP) x[v * m + i] = __builtin_fma(-M(i, j), xj, x[v * m + i]);
             __builtin_amdgcn_wave_barrier();
         }
     } else if (q == 0) {
         for (int i = m - 1; i >= 0; --i) {
-            T temp = x[i * TV + v];
-            for (int k = i + 1; k < m; ++k) temp = __builtin_fma(-M(i, k), x[k * TV + v], temp);
+            T temp = x[v * m + i];
+            for (int k = i + 1; k < m; ++k) temp = __builtin_fma(-M(i, k), x[v * m + k], temp);
             if (!unit) temp = right ? temp * (T(1) / M(i, i)) : temp / M(i, i);
-            x[i * TV + v] = temp;
+            x[v * m + i] = temp;
         }
     }
     __syncthreads();
@@ -175,8 +183,8 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
         }
         const i64 gv = v0 + vv;
         if (gv < nvec) {
-            if (right) B[gv + (i64)i * ldb] = x[i * TV + vv];
-            else B[i + gv * ldb] = x[i * TV + vv];
+            if (right) B[gv + (i64)i * ldb] = x[vv * m + i];
+            else B[i + gv * ldb] = x[vv * m + i];
         }
     }
 }
@@ -321,13 +329,13 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     // Left: M = op(A). Right: M = op(A)^T.
     const bool mtrans = right ? !trans : trans;
     const bool lower_eff = lower != mtrans;
-    const size_t lds = sizeof(T) * (size_t)order * TV;
+    const size_t lds = sizeof(T) * ((size_t)order * TV + (size_t)order * order);
     ProfScope prof_(s, kKTrsm, (double)order * (double)order * (double)nvec);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_small_kernel<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(sizeof(T) * 128 * TV));
+                                    (int)(sizeof(T) * (128 * TV + 128 * 128)));
         (void)hipGetLastError();
         attr = true;
     }

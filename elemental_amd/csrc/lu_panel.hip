@@ -33,6 +33,7 @@ constexpr int NT = 256;
 constexpr int WMAX = 128;                 // maximum panel width
 constexpr long long SPIN_LIMIT = 1 << 20;  // bounded wait: every wave exits
 constexpr int RPL = 5;                     // rows per lane held in registers (R <= 320)
+constexpr int REP = 4;                     // copies of each record (readers spread over them)
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
@@ -48,14 +49,18 @@ struct PanelArgs {
     int G;
     int R;
     uint32_t seq_base;  // sequence number of column 0 of this launch
-    u4* rec;            // [2][G][3] candidate records
+    u4* rec;            // [REP][2][G][2] candidate records
     u4* rows;           // [3][G][WMAX] candidate row chunks
 };
 
 // The exchange area lives in uncached device memory (allocated with
 // hipDeviceMallocUncached): loads and stores go to memory and are coherent across the XCDs
-// without cache maintenance. Each 16-byte record / chunk is written and read with one
-// volatile 16-byte access and validated by its tag.
+// without cache maintenance. Each 16-byte record / chunk is written and read with one 16-byte
+// buffer access at system scope (sc0 sc1) and validated by its tag. Buffer accesses, unlike
+// volatile ones, do not wait for completion one by one: loads issued together overlap, a
+// load issued early (the pivot row) completes while other work proceeds, and stores retire
+// in the background. The buffer descriptors span exactly the area in use, so an access out
+// of range reads zeros (and fails its tag check) instead of touching other memory.
 __device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b, uint32_t c) {
     uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (c + 0x165667B1u) * 0xC2B2AE3Du;
     h ^= h >> 15;
@@ -63,18 +68,27 @@ __device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b, uint32_t c) {
     h ^= h >> 12;
     return h;
 }
-__device__ __forceinline__ void st16(u4* p, u4 v) { *reinterpret_cast<volatile u4*>(p) = v; }
-__device__ __forceinline__ u4 ld16(const u4* p) { return *reinterpret_cast<const volatile u4*>(p); }
-
-// Candidate record: value (as double bits), label, tag.
-__device__ __forceinline__ u4 make_rec(double v, int lbl, uint32_t seq) {
-    const uint64_t bits = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = (uint32_t)bits, hi = (uint32_t)(bits >> 32);
-    return u4{lo, hi, (uint32_t)lbl, seq ^ mix32(lo, hi, (uint32_t)lbl)};
+constexpr int kSysScope = 1 | 16;  // sc0 | sc1
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t area(const u4* base, uint32_t count) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<u4*>(base), 0, (int)(count * 16u), 0x00020000);
 }
-__device__ __forceinline__ bool rec_ok(u4 r, uint32_t seq) { return r[3] == (seq ^ mix32(r[0], r[1], r[2])); }
-__device__ __forceinline__ double rec_val(u4 r) {
-    return __longlong_as_double((long long)(((uint64_t)r[1] << 32) | r[0]));
+__device__ __forceinline__ void st16(const u4* base, uint32_t count, uint32_t i, u4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(v, area(base, count), (int)(i * 16u), 0, kSysScope);
+}
+__device__ __forceinline__ u4 ld16(const u4* base, uint32_t count, uint32_t i) {
+    return __builtin_amdgcn_raw_buffer_load_b128(area(base, count), (int)(i * 16u), 0, kSysScope);
+}
+
+// Candidate record, two 16-byte units: {value (double bits), label, tag} and {look-ahead
+// entry, previous multiplier} (doubles). The tag covers both units, so a reader that sees a
+// current first unit with a stale second one rejects the pair.
+__device__ __forceinline__ uint32_t rec_tag(u4 r0, u4 r1, uint32_t seq) {
+    return seq ^ mix32(r0[0], r0[1], r0[2] ^ mix32(r1[0] ^ r1[2], r1[1], r1[3]));
+}
+__device__ __forceinline__ uint32_t dlo(double v) { return (uint32_t)(uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ uint32_t dhi(double v) { return (uint32_t)((uint64_t)__double_as_longlong(v) >> 32); }
+__device__ __forceinline__ double dbits(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // Row chunk q: VPC values (one double, or three floats) and a tag.
@@ -117,9 +131,9 @@ __device__ __forceinline__ bool better(T v, int l, T bv, int bl) {
     return (v > bv) || (v == bv && l < bl);
 }
 
-// Block-wide argmax of (v, l) with payload q; result broadcast to all threads.
+// Wave-wide argmax of (v, l) with payload q: every lane ends with the wave's best.
 template <typename T>
-__device__ __forceinline__ void block_best(T& v, int& l, int& q, T* sv, int* sl, int* sq) {
+__device__ __forceinline__ void wave_best(T& v, int& l, int& q) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const T ov = __shfl_xor(v, off);
@@ -131,40 +145,46 @@ __device__ __forceinline__ void block_best(T& v, int& l, int& q, T* sv, int* sl,
             q = oq;
         }
     }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        sv[wave] = v;
-        sl[wave] = l;
-        sq[wave] = q;
-    }
-    __syncthreads();
-    v = sv[0];
-    l = sl[0];
-    q = sq[0];
+}
+
+// Best of the NW per-wave results in LDS.
+template <typename T>
+__device__ __forceinline__ void pick(const T* wv, const int* wl, const int* wq, T& v, int& l, int& q) {
+    v = wv[0];
+    l = wl[0];
+    q = wq[0];
 #pragma unroll
     for (int w = 1; w < NT / 64; ++w)
-        if (better(sv[w], sl[w], v, l)) {
-            v = sv[w];
-            l = sl[w];
-            q = sq[w];
+        if (better(wv[w], wl[w], v, l)) {
+            v = wv[w];
+            l = wl[w];
+            q = wq[w];
         }
-    __syncthreads();
 }
 
 template <typename T>
 __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
+    constexpr int NW = NT / 64;
+    constexpr int PW = NW - 1;  // waves that read the exchange; the last wave publishes
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int R = p.R, W = p.w, LD = W + 1;
-    T* tile = reinterpret_cast<T*>(smem_raw);   // R x LD, row-major
-    T* urow = tile + (size_t)R * LD;            // WMAX
-    T* lv = urow + WMAX;                        // R + WMAX + R + 3*NT: pipeline buffers
-    T* sv = lv + 2 * (size_t)R + WMAX + 3 * NT; // NT/64
-    int* lbl = reinterpret_cast<int*>(sv + NT / 64);  // R
-    int* sl = lbl + R;                          // NT/64
-    int* sq = sl + NT / 64;                     // NT/64
-    int* sflag = sq + NT / 64;                  // 1
+    T* tile = reinterpret_cast<T*>(smem_raw);  // R x LD, row-major
+    T* up = tile + (size_t)R * LD;             // WMAX: pivot row of the previous step
+    T* lp = up + WMAX;                         // R: multipliers of the previous step
+    T* lc = lp + R;                            // R: multipliers of this step
+    T* la_v = lc + R;                          // NT per record: candidate value,
+    T* la_a = la_v + NT;                       //   look-ahead entry,
+    T* la_l = la_a + NT;                       //   previous multiplier
+    T* wb_v = la_l + NT;                       // [2][NW] per-wave bests of the record scan
+    T* rb_v = wb_v + 2 * NW;                   // [NW] per-wave bests of the row scan
+    int* lbl = reinterpret_cast<int*>(rb_v + NW);  // R
+    int* wb_l = lbl + R;                       // [2][NW]
+    int* wb_q = wb_l + 2 * NW;                 // [2][NW]
+    int* rb_l = wb_q + 2 * NW;                 // [NW]
+    int* rb_q = rb_l + NW;                     // [NW]
 
     const int g = blockIdx.x, t = threadIdx.x, G = p.G;
+    const int wave = t >> 6, lane = t & 63;
     const i64 row0 = (i64)g * R;
     const int rows = (int)((p.mp - row0) < R ? (p.mp - row0) : R);
 
@@ -172,54 +192,71 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         const int r = idx % rows, j = idx / rows;
         tile[r * LD + j] = p.A[row0 + r + (i64)j * p.lda];
     }
-    for (int r = t; r < rows; r += NT) lbl[r] = (int)(row0 + r);
+    for (int r = t; r < rows; r += NT) {
+        lbl[r] = (int)(row0 + r);
+        lp[r] = T(0);
+    }
+    for (int j = t; j < WMAX; j += NT) up[j] = T(0);
     __syncthreads();
 
     typedef Chunk<T> CK;
     const int q1 = (W + CK::VPC - 1) / CK::VPC;
+    const uint32_t nrec = (uint32_t)REP * 2u * 2u * (uint32_t)G;  // records: [REP][2][G][2]
+    const uint32_t nrows = 3u * (uint32_t)G * WMAX;  // row chunks in use: [3][G][WMAX]
     // Deferred-update pipeline. At the start of step c every live row (label >= c) holds
     // column c updated through step c-1 and columns >= c+1 updated through step c-2: the
     // step c-1 update of those columns waits for the step c-1 pivot row, which is fetched
     // while step c's records are exchanged. The record for column c carries, for its
-    // candidate r: a_r[c] (exact, for the pivot choice), its label, and in a look-ahead chunk
-    // a_r[c+1] through step c-2 and r's step c-1 multiplier; every reader forms the pivot
-    // row's entry u_c[c+1] = a_r[c+1] - l_r u_{c-1}[c+1] with the FMA the owner would use,
-    // and likewise the rest of u_c from the fetched row one step later. Each element still
-    // receives its updates one FMA at a time in step order, exactly as in lu::Panel.
-    T* up = urow;             // u_{c-1}, columns >= c          [WMAX]
-    T* un = lv + R;           // u_c being formed               [WMAX]
-    T* lp = lv;               // step c-1 multipliers per row   [R]
-    T* lc = un + WMAX;        // step c multipliers per row     [R]
-    T* la_v = lc + R;         // per record: signed candidate value, look-ahead value and
-    T* la_a = la_v + NT;      // multiplier                     [3 x NT]
-    T* la_l = la_a + NT;
-    for (int j = t; j < WMAX; j += NT) up[j] = T(0);
-    for (int r = t; r < rows; r += NT) lp[r] = T(0);
+    // candidate r: a_r[c] (exact, for the pivot choice), its label, and in two look-ahead
+    // chunks a_r[c+1] through step c-2 and r's step c-1 multiplier; every reader forms the
+    // pivot row's entry u_c[c+1] = a_r[c+1] - l_r u_{c-1}[c+1] with the FMA the owner would
+    // use, and likewise the rest of u_c from the fetched row one step later. Each element
+    // still receives its updates one FMA at a time in step order, exactly as in lu::Panel.
+    //
+    // Roles: waves 0..PW-1 read the exchange (records, pivot row); the last wave issues every
+    // store to it, so no reader ever waits for store acknowledgements (on CDNA the vector
+    // memory counter covers loads and stores alike); all waves share the LDS work.
 
-    auto publish = [&](int cc, int cand, int cand_lbl, T aprev_v, T lprev_v) {
+    // Publish column cc (called by the last wave): bring the candidate's columns >= cc+1
+    // through step cc-2, store its row chunks, then its records.
+    auto publish = [&](int cc, int cand, int cand_lbl) {
         const uint32_t seq = p.seq_base + (uint32_t)cc;
-        const int sp = cc & 1;
-        u4* myrow = p.rows + ((size_t)((cc % 3) * G + g)) * WMAX;
-        if (cand >= 0)
-            for (int q = (cc + 1) / CK::VPC + t; q < q1; q += NT) {
+        T aprev = T(0);
+        if (cand >= 0) {
+            const T lb = lp[cand];
+            const uint32_t myrow = (uint32_t)((cc % 3) * G + g) * WMAX;
+            for (int q = (cc + 1) / CK::VPC + lane; q < q1; q += 64) {
                 T v[CK::VPC];
 #pragma unroll
                 for (int e = 0; e < CK::VPC; ++e) {
                     const int j = q * CK::VPC + e;
-                    v[e] = (j < W) ? tile[cand * LD + j] : T(0);
+                    if (j > cc && j < W) {
+                        v[e] = __builtin_fma(-lb, up[j], tile[cand * LD + j]);
+                        tile[cand * LD + j] = v[e];
+                        if (j == cc + 1) aprev = v[e];
+                    } else {
+                        v[e] = (j < W) ? tile[cand * LD + j] : T(0);
+                    }
                 }
-                st16(&myrow[q], CK::make(v, q, seq));
+                st16(p.rows, nrows, myrow + q, CK::make(v, q, seq));
             }
-        if (t == 0) {
-            u4* rec = p.rec + 3 * (size_t)(sp * G + g);
-            const double a = (double)aprev_v, l = (double)lprev_v;
-            st16(&rec[1], Chunk<double>::make(&a, 0, seq));
-            st16(&rec[2], Chunk<double>::make(&l, 1, seq));
-            st16(&rec[0], make_rec((cand >= 0) ? (double)tile[cand * LD + cc] : 0.0, cand_lbl, seq));
+        }
+        // lane 0 holds aprev (its first chunk starts at column cc+1); lanes 0..REP-1 store
+        // one copy each of the record
+        aprev = __shfl(aprev, 0);
+        if (lane < REP) {
+            const uint32_t rec = 2u * (uint32_t)(((lane * 2) + (cc & 1)) * G + g);
+            const double a = (double)aprev, l = (cand >= 0) ? (double)lc[cand] : 0.0;
+            const double val = (cand >= 0) ? (double)tile[cand * LD + cc] : 0.0;
+            const u4 r1 = u4{dlo(a), dhi(a), dlo(l), dhi(l)};
+            u4 r0 = u4{dlo(val), dhi(val), (uint32_t)cand_lbl, 0u};
+            r0[3] = rec_tag(r0, r1, seq);
+            st16(p.rec, nrec, rec + 1, r1);
+            st16(p.rec, nrec, rec, r0);
         }
     };
 
-    // Prologue: the candidate of column 0 (nothing pending).
+    // Prologue: the candidate of column 0 (nothing pending: lp = 0, up = 0).
     int br = -1, bl = INT_MAX;
     {
         T bv = T(-1);
@@ -231,12 +268,24 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                 br = r;
             }
         }
-        block_best(bv, bl, br, sv, sl, sq);
-        publish(0, br, bl, (br >= 0 && W > 1) ? tile[br * LD + 1] : T(0), T(0));
+        wave_best(bv, bl, br);
+        if (lane == 0) {
+            rb_v[wave] = bv;
+            rb_l[wave] = bl;
+            rb_q[wave] = br;
+        }
+        __syncthreads();
+        pick(rb_v, rb_l, rb_q, bv, bl, br);
+        if (wave == NW - 1) {
+            // lc is read for the candidate's multiplier: none yet
+            for (int r = lane; r < rows; r += 64) lc[r] = T(0);
+            publish(0, br, bl);
+        }
     }
 
     bool aborted = false;
     long long spins = 0;
+    int iter = 0;
     // the pivot row of the previous step, in flight: chunk registers, owner, its multiplier
     u4 fch = u4{0u, 0u, 0u, 0u};
     int f_w = -1, f_q0 = 0;
@@ -249,116 +298,119 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         T gv = T(-1);
         int gl = INT_MAX, gw = -1;
         while (true) {
-            gv = T(-1);
-            gl = INT_MAX;
-            gw = -1;
+            T bv = T(-1);
+            int bL = INT_MAX, bq = -1;
             int ok = 1;
-            for (int q = t; q < G; q += NT) {
-                const u4* rq = p.rec + 3 * (size_t)(s * G + q);
-                const u4 r = ld16(&rq[0]);
-                const u4 ra = ld16(&rq[1]);
-                const u4 rl = ld16(&rq[2]);
-                if (!rec_ok(r, seq) || !Chunk<double>::ok(ra, 0, seq) || !Chunk<double>::ok(rl, 1, seq)) {
-                    ok = 0;
-                    continue;
-                }
-                double a, l;
-                Chunk<double>::get(ra, &a);
-                Chunk<double>::get(rl, &l);
-                const T val = (T)rec_val(r);
-                la_v[q] = val;
-                la_a[q] = (T)a;
-                la_l[q] = (T)l;
-                const int L = (int)r[2];
-                if (L != INT_MAX) {
-                    const T v = fabs(val);
-                    if (better(v, L, gv, gl)) {
-                        gv = v;
-                        gl = L;
-                        gw = q;
+            if (wave < PW) {
+                for (int q = t; q < G; q += PW * 64) {
+                    const uint32_t rq = 2u * (uint32_t)((((g % REP) * 2) + s) * G + q);
+                    const u4 r0 = ld16(p.rec, nrec, rq);
+                    const u4 r1 = ld16(p.rec, nrec, rq + 1);
+                    if (r0[3] != rec_tag(r0, r1, seq)) {
+                        ok = 0;
+                        continue;
+                    }
+                    const T val = (T)dbits(r0[0], r0[1]);
+                    la_v[q] = val;
+                    la_a[q] = (T)dbits(r1[0], r1[1]);
+                    la_l[q] = (T)dbits(r1[2], r1[3]);
+                    const int L = (int)r0[2];
+                    if (L != INT_MAX) {
+                        const T v = fabs(val);
+                        if (better(v, L, bv, bL)) {
+                            bv = v;
+                            bL = L;
+                            bq = q;
+                        }
                     }
                 }
             }
-            if (__syncthreads_and(ok)) break;
+            wave_best(bv, bL, bq);
+            const int par = iter & 1;
+            ++iter;
+            if (lane == 0) {
+                wb_v[par * NW + wave] = bv;
+                wb_l[par * NW + wave] = bL;
+                wb_q[par * NW + wave] = bq;
+            }
+            if (__syncthreads_and(ok)) {
+                pick(wb_v + par * NW, wb_l + par * NW, wb_q + par * NW, gv, gl, gw);
+                break;
+            }
             if (++spins > SPIN_LIMIT) {
                 aborted = true;
                 break;
             }
         }
         if (aborted) break;
-        block_best(gv, gl, gw, sv, sl, sq);
         const bool none = (gw < 0);  // every remaining candidate is NaN (idamax selects none)
         const int piv = none ? c : gl;
         const T alpha = none ? T(0) : la_v[gw];
         const T aprev_w = none ? T(0) : la_a[gw];
         const T lprev_w = none ? T(0) : la_l[gw];
-        // (2) the previous pivot row arrives: form u_{c-1} from its published values (through
-        // step c-3) and its step c-2 multiplier, then make it current
+        // (2) the previous pivot row (fetched during the last step) becomes u_{c-1}: its
+        // published values (through step c-3) receive the step c-2 update, in place in up
         if (f_w >= 0) {
             const int q = f_q0 + t;
-            const bool mine_q = (q < q1);
+            bool done = !(t < 128 && q < q1);
             while (true) {
-                int ok = 1;
-                if (mine_q && !CK::ok(fch, q, f_seq)) {
-                    fch = ld16(p.rows + ((size_t)(((c - 1) % 3) * G + f_w)) * WMAX + q);
-                    ok = CK::ok(fch, q, f_seq) ? 1 : 0;
+                if (!done) {
+                    if (!CK::ok(fch, q, f_seq))
+                        fch = ld16(p.rows, nrows, (uint32_t)(((c - 1) % 3) * G + f_w) * WMAX + q);
+                    if (CK::ok(fch, q, f_seq)) {
+                        T v[CK::VPC];
+                        CK::get(fch, v);
+#pragma unroll
+                        for (int e = 0; e < CK::VPC; ++e) {
+                            const int j = q * CK::VPC + e;
+                            if (j >= c && j < W) up[j] = __builtin_fma(-f_l, up[j], v[e]);
+                        }
+                        done = true;
+                    }
                 }
-                if (__syncthreads_and(ok)) break;
+                if (__syncthreads_and(done)) break;
                 if (++spins > SPIN_LIMIT) {
                     aborted = true;
                     break;
                 }
             }
             if (aborted) break;
-            if (mine_q) {
-                T v[CK::VPC];
-                CK::get(fch, v);
-#pragma unroll
-                for (int e = 0; e < CK::VPC; ++e) {
-                    const int j = q * CK::VPC + e;
-                    if (j >= c && j < W) un[j] = __builtin_fma(-f_l, up[j], v[e]);
-                }
-            }
+        } else {
+            if (c > 0)
+                for (int j = c + t; j < W; j += NT) up[j] = T(0);  // no pivot row (all NaN)
             __syncthreads();
-            for (int j = c + t; j < W; j += NT) up[j] = un[j];
-        } else if (c > 0) {
-            for (int j = c + t; j < W; j += NT) up[j] = T(0);  // no pivot row (all NaN)
         }
         // (3) issue the fetch of this step's pivot row (consumed at the next step)
         f_w = none ? -1 : gw;
         f_seq = seq;
         f_l = lprev_w;
         f_q0 = (c + 1) / CK::VPC;
-        if (f_w >= 0 && f_q0 + t < q1)
-            fch = ld16(p.rows + ((size_t)((c % 3) * G + f_w)) * WMAX + f_q0 + t);
-        // (4) swap = relabel: the row at position c moves to position piv, the winner becomes
-        // row c
+        if (f_w >= 0 && t < 128 && f_q0 + t < q1)
+            fch = ld16(p.rows, nrows, (uint32_t)((c % 3) * G + f_w) * WMAX + f_q0 + t);
+        // (4) per row: the swap as a relabel (the row at position c moves to position piv,
+        // the winner becomes row c), the multiplier, column c+1 through step c, and the
+        // candidate of column c+1. An exactly zero pivot (the reference throws
+        // SingularMatrixException here) leaves the column unscaled.
         const bool mine = (!none && gw == g);
-        for (int r = t; r < rows; r += NT) {
-            const int L = lbl[r];
-            if (mine && r == br) lbl[r] = c;
-            else if (!none && L == c) lbl[r] = piv;
-        }
-        if (g == 0 && t == 0) {
-            p.dests[c] = (int32_t)(p.goff + piv);
-            if (none || alpha == T(0)) atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
-        }
-        __syncthreads();
-        // An exactly zero pivot (the reference throws SingularMatrixException here) leaves
-        // the column unscaled and the rest of the panel unchanged.
+        const int brp = br;
         const bool upd = (alpha != T(0));
         const T inv = upd ? T(1) / alpha : T(0);
         const int c1 = c + 1;
         const T upc1 = (c1 < W) ? up[c1] : T(0);
         const T uc1 = __builtin_fma(-lprev_w, upc1, aprev_w);  // u_c[c+1]
-        // (5) multipliers; column c+1 through step c; candidate of column c+1
-        br = -1;
-        bl = INT_MAX;
         {
             T bv = T(-1);
+            int bL = INT_MAX, bR = -1;
             for (int r = t; r < rows; r += NT) {
-                const int L = lbl[r];
+                int L = lbl[r];
                 if (L < c) continue;
+                if (mine && r == brp) {
+                    L = c;
+                    lbl[r] = c;
+                } else if (!none && L == c) {
+                    L = piv;
+                    lbl[r] = piv;
+                }
                 T a = (c1 < W) ? __builtin_fma(-lp[r], upc1, tile[r * LD + c1]) : T(0);
                 T l = T(0);
                 if (L > c && upd) {
@@ -371,39 +423,56 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                     tile[r * LD + c1] = a;
                     if (L > c) {
                         const T v = fabs(a);
-                        if (better(v, L, bv, bl)) {
+                        if (better(v, L, bv, bL)) {
                             bv = v;
-                            bl = L;
-                            br = r;
+                            bL = L;
+                            bR = r;
                         }
                     }
                 }
             }
-            block_best(bv, bl, br, sv, sl, sq);
+            if (g == 0 && t == 0) {
+                p.dests[c] = (int32_t)(p.goff + piv);
+                if (none || alpha == T(0)) atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
+            }
+            wave_best(bv, bL, bR);
+            if (lane == 0) {
+                rb_v[wave] = bv;
+                rb_l[wave] = bL;
+                rb_q[wave] = bR;
+            }
+            __syncthreads();
+            pick(rb_v, rb_l, rb_q, bv, bl, br);
         }
         if (c1 < W) {
-            // (6) bring the candidate's columns >= c+2 through step c-1, publish column c+1
-            if (br >= 0)
-                for (int j = c1 + 1 + t; j < W; j += NT)
-                    tile[br * LD + j] = __builtin_fma(-lp[br], up[j], tile[br * LD + j]);
-            __syncthreads();
-            publish(c1, br, bl, (br >= 0 && c1 + 1 < W) ? tile[br * LD + c1 + 1] : T(0),
-                    (br >= 0) ? lc[br] : T(0));
-            // (7) the step c-1 update of columns >= c+2 for the other live rows, while the
-            // exchange is in flight
-            const int lane = t & 63, wv = t >> 6;
-            for (int r = lane; r < rows; r += 64) {
-                if (lbl[r] < c || r == br) continue;
-                const T l = lp[r];
-                for (int j = c1 + 1 + wv; j < W; j += NT / 64)
-                    tile[r * LD + j] = __builtin_fma(-l, up[j], tile[r * LD + j]);
-            }
+            // (5) publish column c+1 (last wave) while the others start (6)
+            if (wave == NW - 1) publish(c1, br, bl);
+            // (6) the step c-1 update of columns >= c+2 for the other live rows
+            const int j0 = c1 + 1;
+            if (j0 < W)
+                for (int r = lane; r < rows; r += 64) {
+                    if (r == br || lbl[r] < c) continue;
+                    const T l = lp[r];
+                    T* row = tile + r * LD;
+                    int j = j0 + wave;
+                    for (; j + 3 * NW < W; j += 4 * NW) {
+                        const T x0 = row[j], x1 = row[j + NW], x2 = row[j + 2 * NW], x3 = row[j + 3 * NW];
+                        const T u0 = up[j], u1 = up[j + NW], u2 = up[j + 2 * NW], u3 = up[j + 3 * NW];
+                        row[j] = __builtin_fma(-l, u0, x0);
+                        row[j + NW] = __builtin_fma(-l, u1, x1);
+                        row[j + 2 * NW] = __builtin_fma(-l, u2, x2);
+                        row[j + 3 * NW] = __builtin_fma(-l, u3, x3);
+                    }
+                    for (; j < W; j += NW) row[j] = __builtin_fma(-l, up[j], row[j]);
+                }
         }
-        // the step c multipliers become the pending ones
-        __syncthreads();
-        for (int r = t; r < rows; r += NT) lp[r] = lc[r];
-        __syncthreads();
+        // the step c multipliers become the pending ones (the next step's exchange barrier
+        // orders every use of the old ones before they are overwritten)
+        T* tmp = lp;
+        lp = lc;
+        lc = tmp;
     }
+    __syncthreads();
 
     if (aborted) {
         if (t == 0) atomicCAS(p.info, 0, -1);
@@ -419,13 +488,12 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         if (L < W) p.pairs[L] = (L != phys) ? int2{L, phys} : int2{-1, -1};
         if (phys < W) p.pairs[W + phys] = (L >= W) ? int2{L, phys} : int2{-1, -1};
     }
-    (void)sflag;
 }
 
 template <typename T>
 size_t panel_lds_bytes(int R, int W) {
-    return sizeof(T) * ((size_t)R * (W + 3) + 2 * WMAX + 3 * NT + NT / 64) +
-           sizeof(int) * ((size_t)R + 2 * (NT / 64) + 1);
+    return sizeof(T) * ((size_t)R * (W + 3) + WMAX + 3 * NT + 3 * (NT / 64)) +
+           sizeof(int) * ((size_t)R + 6 * (NT / 64));
 }
 
 template <typename T>
@@ -450,7 +518,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     }
     elx::PanelSync* ps = nullptr;
     elx::ProfScope prof_(s, elx::kKPanel, 2.0 * (double)mp * (double)w * sizeof(T));
-    const size_t rec_bytes = sizeof(u4) * 3 * 2 * (size_t)di.num_cus;
+    const size_t rec_bytes = sizeof(u4) * REP * 2 * 2 * (size_t)di.num_cus;
     // Row chunks rotate over three slots: a winner's row of column c is read until the
     // readers reach column c+1, while its owner may already publish column c+2.
     const size_t row_bytes = sizeof(u4) * 3 * (size_t)di.num_cus * WMAX;
@@ -468,7 +536,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     a.R = (int)R;
     a.seq_base = (uint32_t)((++ps->epoch) * 1024);
     a.rec = reinterpret_cast<u4*>(ps->slots);
-    a.rows = a.rec + 3 * 2 * (size_t)di.num_cus;
+    a.rows = a.rec + REP * 2 * 2 * (size_t)di.num_cus;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
@@ -476,10 +544,9 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
         (void)hipGetLastError();
         attr_set = true;
     }
-    ELX_CHECK(elx::coop_fits(reinterpret_cast<const void*>(&lu_panel_kernel<T>), NT, lds, G));
     void* args[] = {&a};
-    ELX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
-                                       dim3((unsigned)G), dim3(NT), args, (unsigned)lds, s));
+    ELX_CHECK(elx::launch_resident(reinterpret_cast<const void*>(&lu_panel_kernel<T>), G, NT, args,
+                                   lds, s));
     return ELX_SUCCESS;
 }
 

@@ -24,7 +24,8 @@ namespace {
 // Scratch slots (elx_api.hip uses 0..3)
 enum {
     kSlotW = 10, kSlotLh, kSlotUh, kSlotLinv, kSlotUinv, kSlotIdx, kSlotVec, kSlotPart,
-    kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP
+    kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP,
+    kSlotF64, kSlotF64Idx
 };
 
 // Uncached buffers of the persistent triangular solves (never freed before the process
@@ -147,13 +148,16 @@ elx_status_t getrf_mixed(Ctx& c) {
             const int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
             ELX_CHECK(launch_row_moves_f32(c.s, pr, (int)(2 * w), W, ld, j0, 0, K0, K1, n));
         }
+        // Inverses of the outer block's unit-lower diagonal blocks: final now (later swaps only
+        // move rows below K1), used for U12 here and by the triangular solves afterwards.
+        ELX_CHECK(launch_diag_inverse(c.s, n, W, ld, 0, c.Linv, K0 / 128, (nbk + 127) / 128));
         const i64 n2 = n - K1;
         if (n2 <= 0) continue;
-        // U12 = L11^{-1} A12, blocked by 128
+        // U12 = L11^{-1} A12, blocked by 128, each block by its diagonal inverse
         for (i64 jb = K0; jb < K1; jb += 128) {
             const i64 wb = std::min<i64>(128, K1 - jb);
-            ELX_CHECK(launch_trsm_small_f32(c.s, kLeft, kLower, kNormal, kUnit, wb, n2,
-                                            W + jb + jb * ld, ld, W + jb + K1 * ld, ld));
+            ELX_CHECK(launch_trmm_inv(c.s, wb, n2, c.Linv + (jb / 128) * 128 * 128,
+                                      W + jb + K1 * ld, ld));
             const i64 below = K1 - (jb + wb);
             if (below > 0)
                 ELX_CHECK(lp_update(c, below, n2, wb, W + (jb + wb) + jb * ld, W + jb + K1 * ld,
@@ -266,17 +270,79 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
     ELX_CHECK(launch_rowabs_max(s, n, nch, part, anorm_d));
     ELX_CHECK(getrf_mixed(c));
     ELX_CHECK(launch_swaps_to_perm(s, c.dests, n, n, c.perm, 0));
-    ELX_CHECK(launch_diag_inverse(s, n, c.W, c.ldw, 0, c.Linv));
-    ELX_CHECK(launch_diag_inverse(s, n, c.W, c.ldw, 1, c.Uinv));
+    ELX_CHECK(launch_diag_inverse(s, n, c.W, c.ldw, 1, c.Uinv, 0, 0));
     st.factor_seconds = tf.stop();
     int32_t finfo = 0;
     double anorm = 0.0;
     ELX_CHECK(d2h(&finfo, c.info, 1, s));
     ELX_CHECK(d2h(&anorm, anorm_d, 1, s));
     st.factor_info = finfo;
+
+    // r = b - A x in FP64 and its normwise backward error
+    auto backward = [&](const double* x, const double* b, double* eta) -> elx_status_t {
+        ELX_HIP(hipMemsetAsync(norms, 0, sizeof(double) * 3, s));
+        ELX_CHECK(launch_residual(s, n, dA, lda, x, b, r, part, nch, norms));
+        double hn[3];
+        ELX_CHECK(d2h(hn, norms, 3, s));
+        const double den = anorm * hn[1] + hn[2];
+        *eta = den > 0 ? hn[0] / den : (hn[0] > 0 ? INFINITY : 0.0);
+        return ELX_SUCCESS;
+    };
+    // FP64 fallback (refine = AUTO): El::LU of a copy of A, lu::SolveAfter, then refinement
+    // with the FP64 factors. Taken when the low-precision factorization breaks down or the
+    // refinement stalls.
+    double* F64 = nullptr;
+    int32_t* d64 = nullptr;
+    bool f64_ready = false;
+    auto fp64_solve = [&](const double* b, double* x, double* eta, bool* ok) -> elx_status_t {
+        *ok = false;
+        if (!f64_ready) {
+            ELX_CHECK(get(kSlotF64, (size_t)n * (size_t)n, &F64));
+            ELX_CHECK(get(kSlotF64Idx, (size_t)n + 8, &d64));
+            ELX_HIP(hipMemcpy2DAsync(F64, sizeof(double) * n, dA, sizeof(double) * lda,
+                                     sizeof(double) * n, n, hipMemcpyDeviceToDevice, s));
+            ELX_CHECK(elx_dgetrf(s, n, n, F64, n, 128, d64, d64 + n));
+            int32_t info64 = 0;
+            ELX_CHECK(d2h(&info64, d64 + n, 1, s));
+            if (info64 != 0) return ELX_SUCCESS;  // exactly singular in FP64 as well
+            f64_ready = true;
+        }
+        ELX_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        ELX_CHECK(elx_dgetrs(s, 0, n, 1, F64, n, d64, x, n));
+        for (int k = 0; k <= 5; ++k) {
+            ELX_CHECK(backward(x, b, eta));
+            if (*eta <= tol) {
+                *ok = true;
+                break;
+            }
+            if (k == 5) break;
+            ELX_CHECK(elx_dgetrs(s, 0, n, 1, F64, n, d64, r, n));
+            ELX_CHECK(launch_axpy(s, n, 1.0, r, x));
+        }
+        return ELX_SUCCESS;
+    };
     if (finfo != 0) {
+        // an exactly zero pivot in the low-precision factors
+        if (refine != ELX_REFINE_AUTO) {
+            if (stats) *stats = st;
+            return ELX_ERROR_NUMERIC;
+        }
+        Timer tr(s);
+        bool all_ok = true;
+        double worst = 0.0;
+        for (i64 col = 0; col < nrhs; ++col) {
+            double eta = INFINITY;
+            bool ok = false;
+            ELX_CHECK(fp64_solve(dB + col * ldb, dX + col * ldx, &eta, &ok));
+            worst = std::max(worst, eta);
+            all_ok = all_ok && ok;
+        }
+        st.fell_back = 1;
+        st.refine_seconds = tr.stop();
+        st.converged = all_ok ? 1 : 0;
+        st.backward_error = worst;
         if (stats) *stats = st;
-        return ELX_ERROR_NUMERIC;
+        return all_ok ? ELX_SUCCESS : ELX_ERROR_NUMERIC;
     }
 
     // ---- refinement, one right-hand side at a time ----
@@ -388,12 +454,7 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
         bool ok = false;
         bool use_gmres = (refine == ELX_REFINE_GMRES);
         for (;;) {
-            ELX_HIP(hipMemsetAsync(norms, 0, sizeof(double) * 3, s));
-            ELX_CHECK(launch_residual(s, n, dA, lda, x, b, r, part, nch, norms));
-            double hn[3];
-            ELX_CHECK(d2h(hn, norms, 3, s));
-            const double den = anorm * hn[1] + hn[2];
-            eta = den > 0 ? hn[0] / den : (hn[0] > 0 ? INFINITY : 0.0);
+            ELX_CHECK(backward(x, b, &eta));
             if (eta <= tol) {
                 ok = true;
                 break;
@@ -423,6 +484,10 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
             ++it;
         }
         st.outer_iters = std::max(st.outer_iters, it);
+        if (!ok && refine == ELX_REFINE_AUTO) {
+            ELX_CHECK(fp64_solve(b, x, &eta, &ok));
+            st.fell_back = 1;
+        }
         worst = std::max(worst, eta);
         all_ok = all_ok && ok;
     }

@@ -14,8 +14,11 @@ elx_status_t launch_rowabs_max(hipStream_t s, i64 n, int nch, const double* rowa
 elx_status_t launch_vec_absmax(hipStream_t s, i64 n, const double* v, double* out);
 // Inverses of the 128x128 diagonal blocks of the unit lower (upper = 0) or upper (upper = 1)
 // factor in W, to Inv (nblk x 128 x 128).
+// Blocks b0 .. b0+count-1 (count <= 0: through the last block).
 elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, int upper,
-                                 float* Inv);
+                                 float* Inv, i64 b0, i64 count);
+// B := Inv * B for an nb-row block (nb <= 128) of ncols columns, in place.
+elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb);
 // Persistent triangular solve with the factor in W (fp32); x and flags in uncached memory.
 elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const float* Inv,
                          int lower, const float* y, float* x, int64_t* flags, int64_t target,

@@ -41,11 +41,11 @@ __global__ __launch_bounds__(256) void convert_kernel(i64 n, const double* __res
 // (column-major, ld 128). One workgroup per block; thread j forms column j.
 __global__ __launch_bounds__(128) void diag_inverse_kernel(i64 n, const float* __restrict__ W,
                                                            i64 ldw, int upper,
-                                                           float* __restrict__ Inv) {
+                                                           float* __restrict__ Inv, i64 b0) {
     extern __shared__ __align__(16) float sm[];
     float* T = sm;              // 128 x 128, T[i + k*128]
     float* X = sm + 128 * 128;  // X[k*128 + j]
-    const i64 b = blockIdx.x;
+    const i64 b = b0 + blockIdx.x;
     const i64 r0 = b * 128;
     const int nb = (int)std::min<i64>(128, n - r0);
     const int j = threadIdx.x;
@@ -72,6 +72,57 @@ __global__ __launch_bounds__(128) void diag_inverse_kernel(i64 n, const float* _
     for (int idx = j; idx < 128 * 128; idx += 128) {
         const int ii = idx & 127, kk = idx >> 7;
         out[ii + kk * 128] = X[ii * 128 + kk];
+    }
+}
+
+// B := Inv * B for one 128-row block: Inv is the 128 x 128 inverse of a diagonal block (column-
+// major, rows and columns >= nb zero), B the nb x ncols block at B (column-major, ldb). This
+// is the triangular solve L11^{-1} A12 of the blocked LU as a product with the precomputed
+// inverse: every column's result depends only on that column, so a workgroup reads its
+// 64 columns into LDS, multiplies, and writes them back in place. Thread (ig, jg) forms an
+// 8 x 4 block of the result.
+__global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const float* __restrict__ Inv,
+                                                       float* B, i64 ldb) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    extern __shared__ __align__(16) float sm[];
+    float* Is = sm;               // 128 x 128, Is[i + k*128]
+    float* Bs = sm + 128 * 128;   // 128 x 64, Bs[k*64 + j]
+    const int t = threadIdx.x;
+    const i64 c0 = (i64)blockIdx.x * 64;
+    const int nc = (int)std::min<i64>(64, ncols - c0);
+    for (int idx = t; idx < 128 * 128 / 4; idx += 256)
+        reinterpret_cast<f4v*>(Is)[idx] = reinterpret_cast<const f4v*>(Inv)[idx];
+    for (int idx = t; idx < 128 * 64; idx += 256) {
+        const int k = idx & 127, j = idx >> 7;
+        Bs[k * 64 + j] = (k < nb && j < nc) ? B[k + (c0 + j) * ldb] : 0.0f;
+    }
+    __syncthreads();
+    const int ig = t & 15, jg = t >> 4;  // rows 8*ig.., columns 4*jg..
+    float acc[8][4];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < 128; ++k) {
+        const f4v i0 = *reinterpret_cast<const f4v*>(Is + k * 128 + 8 * ig);
+        const f4v i1 = *reinterpret_cast<const f4v*>(Is + k * 128 + 8 * ig + 4);
+        const f4v bv = *reinterpret_cast<const f4v*>(Bs + k * 64 + 4 * jg);
+        const float iv[8] = {i0[0], i0[1], i0[2], i0[3], i1[0], i1[1], i1[2], i1[3]};
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fmaf(iv[a], bv[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int j = 4 * jg + b;
+        if (j >= nc) continue;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const int i = 8 * ig + a;
+            if (i < nb) B[i + (c0 + j) * ldb] = acc[a][b];
+        }
     }
 }
 
@@ -378,8 +429,10 @@ elx_status_t launch_vec_absmax(hipStream_t s, i64 n, const double* v, double* ou
 }
 
 elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, int upper,
-                                 float* Inv) {
-    const i64 nblk = (n + 127) / 128;
+                                 float* Inv, i64 b0, i64 count) {
+    const i64 nblk = (count > 0) ? count : (n + 127) / 128 - b0;
+    ELX_ARG(b0 >= 0 && b0 + nblk <= (n + 127) / 128);
+    if (nblk <= 0) return ELX_SUCCESS;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_inverse_kernel),
@@ -388,7 +441,25 @@ elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, 
         attr = true;
     }
     hipLaunchKernelGGL(diag_inverse_kernel, dim3((unsigned)nblk), dim3(128), 2 * 128 * 128 * 4, s, n,
-                       W, ldw, upper, Inv);
+                       W, ldw, upper, Inv, b0);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb) {
+    ELX_ARG(nb >= 0 && nb <= 128 && ncols >= 0 && ldb >= nb);
+    if (nb == 0 || ncols == 0) return ELX_SUCCESS;
+    ProfScope prof_(s, kKTrsm, (double)nb * (double)nb * (double)ncols);
+    const size_t lds = (128 * 128 + 128 * 64) * sizeof(float);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trmm_inv_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+        attr = true;
+    }
+    hipLaunchKernelGGL(trmm_inv_kernel, dim3((unsigned)cdiv(ncols, 64)), dim3(256), lds, s, (int)nb,
+                       ncols, Inv, B, ldb);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
@@ -400,10 +471,8 @@ elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const fl
     const i64 nblk = (n + 127) / 128;
     const unsigned G = (unsigned)std::min<i64>(nblk, di.num_cus);
     ProfScope prof_(s, kKTrsv, 0.5 * (double)n * (double)n * sizeof(float));
-    ELX_CHECK(coop_fits(reinterpret_cast<const void*>(&trsv_kernel), 256, 0, G));
     void* args[] = {&n, &W, &ldw, &Inv, &lower, &y, &x, &flags, &target, &err};
-    ELX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&trsv_kernel), dim3(G),
-                                       dim3(256), args, 0, s));
+    ELX_CHECK(launch_resident(reinterpret_cast<const void*>(&trsv_kernel), G, 256, args, 0, s));
     return ELX_SUCCESS;
 }
 

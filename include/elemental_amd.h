@@ -130,7 +130,9 @@ typedef enum {
     ELX_REFINE_NONE = 0,
     ELX_REFINE_IR = 1,      /* classical iterative refinement */
     ELX_REFINE_GMRES = 2,   /* GMRES-IR: each correction by GMRES preconditioned with the LU */
-    ELX_REFINE_AUTO = 3     /* IR, switching to GMRES-IR when IR stagnates */
+    ELX_REFINE_AUTO = 3     /* IR, switching to GMRES-IR when IR stagnates, and to the FP64
+                               LU (elx_dgetrf of a copy of A) when GMRES-IR stalls or the
+                               low-precision factorization meets an exactly zero pivot */
 } elx_refine_t;
 
 typedef struct {

@@ -49,7 +49,7 @@ static void check(bool ok, const char* what, double v) {
     if (!ok) ++failures;
 }
 
-int main() {
+static int run() {
     const Int m = 300, numRHS = 100;
     // ---- LU with partial pivoting + SolveAfter (tests/lapack_like/LU.cpp:20-53) ----
     {
@@ -141,4 +141,14 @@ int main() {
     }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
+}
+
+int main() {
+    std::setvbuf(stdout, nullptr, _IONBF, 0);
+    try {
+        return run();
+    } catch (std::exception& e) {
+        std::printf("uncaught exception after %d failure(s): %s\n", failures, e.what());
+        return 2;
+    }
 }

@@ -90,21 +90,61 @@ def test_mixed_pivoting_needed(dev, orc):
     assert orc.backward_error(A, x, b) <= n * EPS
 
 
+def test_gmres_ir_ill_conditioned(dev, orc):
+    """kappa * u16 = 5: plain refinement converges slowly or not at all; GMRES-IR
+    preconditioned by the fp16 LU reaches FP64 backward error without the fallback."""
+    n = 1500
+    say("gmres-ir fp16 1e4")
+    A = randsvd(n, 1e4, 21)
+    b = np.random.default_rng(22).standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="gmres")
+    x = dev.to_host(dX)
+    eta = orc.backward_error(A, x, b)
+    say(st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["converged"] == 1 and st["fell_back"] == 0, st
+    assert eta <= n * EPS
+    assert st["inner_iters"] > 0
+
+
 @pytest.mark.parametrize("kappa", [1e4, 1e6])
 @pytest.mark.parametrize("factor", ["fp16", "bf16"])
-def test_gmres_ir_ill_conditioned(dev, orc, kappa, factor):
-    """kappa * u_low >> 1: plain refinement cannot converge; GMRES-IR preconditioned by the
-    low-precision LU reaches FP64 backward error."""
+def test_auto_always_reaches_fp64_accuracy(dev, orc, kappa, factor):
+    """AUTO goes IR -> GMRES-IR -> FP64 LU: whatever kappa * u_low is, the answer meets the
+    FP64 backward-error bound (through the fallback when the low-precision LU is too
+    inaccurate to precondition)."""
     n = 1500
-    say("gmres-ir", factor, kappa)
+    say("auto", factor, kappa)
     A = randsvd(n, kappa, 21)
     b = np.random.default_rng(22).standard_normal((n, 1))
     dA, dB = dev.to_device(A), dev.to_device(b)
     dX = dev.colmajor_empty(n, 1)
-    st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="gmres")
+    st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="auto")
     x = dev.to_host(dX)
     eta = orc.backward_error(A, x, b)
     say(factor, kappa, st, "oracle eta/(n u) =", eta / (n * EPS))
     assert st["converged"] == 1, st
     assert eta <= n * EPS
-    assert st["inner_iters"] > 0
+    if factor == "fp16" and kappa == 1e4:
+        assert st["fell_back"] == 0
+
+
+def test_fallback_when_low_precision_factor_breaks_down(dev, orc):
+    """A column of magnitude 1e-50 is exactly zero in fp32: the low-precision LU reports a
+    zero pivot, AUTO solves with the FP64 LU instead; IR/GMRES modes report the failure."""
+    n = 400
+    rng = np.random.default_rng(31)
+    A = rng.standard_normal((n, n))
+    A[:, 5] *= 1e-50
+    A = np.asfortranarray(A)
+    b = rng.standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto")
+    say("breakdown", st)
+    assert st["factor_info"] != 0 and st["fell_back"] == 1 and st["converged"] == 1
+    x = dev.to_host(dX)
+    assert orc.backward_error(A, x, b) <= n * EPS
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="ir")
+    assert st["factor_info"] != 0 and st["converged"] == 0 and st["status"] == 5
