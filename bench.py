@@ -155,7 +155,7 @@ def main():
     ap.add_argument("--refine", default="auto", choices=["ir", "gmres", "auto"])
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-n", type=int, default=12288)
+    ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--no-profile-pass", action="store_true")
