@@ -77,7 +77,8 @@ inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 // Optional per-kernel-class timing with HIP events on the launching stream (off by default;
 // see elx_profile_enable). `work` is the algorithmic flops or bytes of the launch.
 enum KernelClass {
-    kKPanel = 0, kKDgemm, kKLpgemm, kKTrsm, kKRowMoves, kKTrsv, kKResidual, kKPack, kKOther,
+    kKPanel = 0, kKDgemm, kKLpgemm, kKTrsm, kKRowMoves, kKTrsv, kKResidual, kKPack, kKBatched,
+    kKBatchedSolve, kKOther,
     kKCount
 };
 bool prof_on();

@@ -178,4 +178,53 @@ def posv_mixed(A, B, X, factor="fp32", refine="ir", tol=0.0, max_iters=0, stream
 __all__ = [
     "load", "getrf", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
     "permute_rows", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
+    "Batch", "potrf_vbatched", "potrs_vbatched",
 ]
+
+
+class Batch:
+    """A batch of column-major matrices (one torch tensor each) for the vbatched entry
+    points: host arrays of sizes and leading dimensions, a device array of pointers."""
+
+    def __init__(self, mats, dtype):
+        self.mats = list(mats)
+        self.dtype = dtype
+        ptrs, ns, lds = [], [], []
+        for t in self.mats:
+            p, ld = _mat(t, dtype)
+            ptrs.append(p)
+            ns.append(t.shape[0])
+            lds.append(ld)
+        self.n = np.ascontiguousarray(ns, dtype=np.int32)
+        self.ld = np.ascontiguousarray(lds, dtype=np.int32)
+        self.cols = np.ascontiguousarray([t.shape[1] for t in self.mats], dtype=np.int32)
+        self.dptrs = torch.tensor(ptrs, dtype=torch.int64, device="cuda") if ptrs else \
+            torch.zeros(1, dtype=torch.int64, device="cuda")
+
+    def __len__(self):
+        return len(self.mats)
+
+
+def _np_ptr(a):
+    import ctypes
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def potrf_vbatched(batch, uplo=LOWER, stream=None):
+    """Cholesky of every matrix of the batch in place; returns the device int32 info array."""
+    info = torch.zeros(max(len(batch), 1), dtype=torch.int32, device="cuda")
+    name = "elx_dpotrf_vbatched" if batch.dtype == torch.float64 else "elx_spotrf_vbatched"
+    call(name, _stream(stream), uplo, len(batch), _np_ptr(batch.n), ctypes_void(batch.dptrs.data_ptr()),
+         _np_ptr(batch.ld), ctypes_void(info.data_ptr()))
+    return info[: len(batch)]
+
+
+def potrs_vbatched(factors, rhs, uplo=LOWER, stream=None):
+    """L L^T X = B for every (factor, right-hand sides) pair; B is overwritten with X."""
+    if len(factors) != len(rhs):
+        raise ValueError("factor and right-hand-side batches differ in length")
+    name = "elx_dpotrs_vbatched" if factors.dtype == torch.float64 else "elx_spotrs_vbatched"
+    call(name, _stream(stream), uplo, len(factors), _np_ptr(factors.n), _np_ptr(rhs.cols),
+         ctypes_void(factors.dptrs.data_ptr()), _np_ptr(factors.ld),
+         ctypes_void(rhs.dptrs.data_ptr()), _np_ptr(rhs.ld))
+

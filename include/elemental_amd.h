@@ -48,7 +48,8 @@ elx_status_t elx_finalize(void);
  * stream, and elx_profile_query returns the number of launches, their summed device time
  * (ms) and summed algorithmic work (flops for GEMM classes, bytes otherwise). Classes:
  * 0 LU panel, 1 FP64 GEMM, 2 low-precision GEMM, 3 small TRSM, 4 row moves, 5 persistent
- * TRSV, 6 FP64 residual, 7 operand pack. */
+ * TRSV, 6 FP64 residual, 7 operand pack, 8 batched Cholesky factor, 9 batched Cholesky
+ * solve. */
 void elx_profile_enable(int on);
 void elx_profile_reset(void);
 elx_status_t elx_profile_query(int kernel_class, int64_t* launches, double* total_ms,
@@ -162,6 +163,28 @@ elx_status_t elx_dposv_mixed(void* stream, int64_t n, int64_t nrhs, const double
                              int64_t lda, const double* dB, int64_t ldb, double* dX,
                              int64_t ldx, int factor_type, int refine, double tol,
                              int32_t max_iters, elx_ir_stats_t* stats);
+
+/* ------------------------------------------------------------------------------------
+ * Batched variable-size Cholesky (lower). No counterpart in the reference, which factors one
+ * Matrix at a time: each matrix i gets exactly what El::Cholesky(LOWER, A_i)
+ * (src/lapack_like/factor/Cholesky.cpp:26-38, LowerVariant3.hpp:15-68) and
+ * cholesky::SolveAfter(LOWER, NORMAL, L_i, B_i) (Cholesky/SolveAfter.hpp:15-43) compute, up to
+ * rounding (32-column blocks instead of Blocksize()). n, nrhs, lda, ldb are host arrays of
+ * `count` entries (0 <= n_i <= 512); dA_array / dB_array are device arrays of device
+ * pointers to column-major matrices; dinfo (device, count entries) receives 0 or j+1 for the
+ * first non-positive pivot of matrix i (NonHPDMatrixException in the reference). The
+ * matrices are scheduled largest first over persistent workgroups. Asynchronous on stream.
+ * ---------------------------------------------------------------------------------- */
+elx_status_t elx_dpotrf_vbatched(void* stream, int uplo, int64_t count, const int32_t* n,
+                                 double* const* dA_array, const int32_t* lda, int32_t* dinfo);
+elx_status_t elx_spotrf_vbatched(void* stream, int uplo, int64_t count, const int32_t* n,
+                                 float* const* dA_array, const int32_t* lda, int32_t* dinfo);
+elx_status_t elx_dpotrs_vbatched(void* stream, int uplo, int64_t count, const int32_t* n,
+                                 const int32_t* nrhs, const double* const* dA_array,
+                                 const int32_t* lda, double* const* dB_array, const int32_t* ldb);
+elx_status_t elx_spotrs_vbatched(void* stream, int uplo, int64_t count, const int32_t* n,
+                                 const int32_t* nrhs, const float* const* dA_array,
+                                 const int32_t* lda, float* const* dB_array, const int32_t* ldb);
 
 #ifdef __cplusplus
 }

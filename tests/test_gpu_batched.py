@@ -1,0 +1,107 @@
+"""Batched variable-size Cholesky (lower) on the GPU against the oracle.
+
+Every matrix of the batch must get what El::Cholesky(LOWER, A_i) and
+cholesky::SolveAfter(LOWER, NORMAL, L_i, B_i) give one matrix at a time
+(src/lapack_like/factor/Cholesky/LowerVariant3.hpp:15-68, Cholesky/SolveAfter.hpp:15-43):
+factor within 8 n u of A in the normwise sense (u of the arithmetic type), the same
+non-HPD column (info = j+1) for a matrix that is not positive definite, and FP64 solves to
+backward error <= n u. The batch mixes every block-boundary size around 16 and 32 with
+random sizes, in an order unrelated to size."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+EPS64 = 2.0 ** -53
+EPS32 = 2.0 ** -24
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def spd(n, rng):
+    G = rng.standard_normal((n, n))
+    return np.asfortranarray(G @ G.T / max(n, 1) + np.eye(n))
+
+
+def sizes(rng):
+    edge = [1, 2, 15, 16, 17, 31, 32, 33, 47, 63, 64, 65, 96, 127, 128, 129, 255, 256, 257,
+            383, 511, 512]
+    rand = list(rng.integers(16, 513, size=26))
+    s = edge + rand
+    rng.shuffle(s)
+    return [int(v) for v in s]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_cholesky_factor_and_solve(dev, orc, dtype):
+    rng = np.random.default_rng(41)
+    ns = sizes(rng)
+    say("batched cholesky", dtype, len(ns), "matrices, n in", min(ns), max(ns))
+    As = [spd(n, rng) for n in ns]
+    # one matrix that is not positive definite at global column 40 (0-based): its diagonal
+    # there is made very negative
+    bad = ns.index(max(n for n in ns if n > 64))
+    As[bad][40, 40] = -1e3
+    mats = [dev.to_device(A, dtype=dtype) for A in As]
+    batch = dev.Batch(mats, dtype)
+    info = dev.potrf_vbatched(batch).cpu().numpy()
+    torch.cuda.synchronize()
+    u = EPS64 if dtype == torch.float64 else EPS32
+    for i, (A, n) in enumerate(zip(As, ns)):
+        F = dev.to_host(mats[i]).astype(np.float64)
+        Fo, info_o = orc.cholesky(A, uplo=0)
+        assert info[i] == info_o, (i, n, info[i], info_o)
+        if info_o:
+            assert info_o == 41
+            continue
+        err = orc.chol_factor_error(A, np.asfortranarray(F), 0)
+        assert err <= 8 * n * u, (i, n, err)
+        if dtype == torch.float64:
+            # same algorithm up to the blocking: entrywise close to the oracle's factor
+            L, Lo = np.tril(F), np.tril(Fo)
+            assert np.abs(L - Lo).max() <= 64 * n * u * np.abs(Lo).max(), (i, n)
+    # solves, for the HPD matrices: 1..8 right-hand sides, one with 11 (two column chunks)
+    keep = [i for i in range(len(ns)) if info[i] == 0]
+    nr = [int(rng.integers(1, 9)) for _ in keep]
+    nr[0] = 11
+    Bs = [rng.standard_normal((ns[i], r)) for i, r in zip(keep, nr)]
+    fac = dev.Batch([mats[i] for i in keep], dtype)
+    rhs = dev.Batch([dev.to_device(B, dtype=dtype) for B in Bs], dtype)
+    dev.potrs_vbatched(fac, rhs)
+    torch.cuda.synchronize()
+    for j, i in enumerate(keep):
+        X = dev.to_host(rhs.mats[j]).astype(np.float64)
+        eta = orc.backward_error(As[i], np.asfortranarray(X), np.asfortranarray(Bs[j]))
+        n = ns[i]
+        # n u, with room for the two roundings (two divisions) of the smallest orders
+        bound = max(n, 4) * u if dtype == torch.float64 else 4 * max(n, 4) * u
+        assert eta <= bound, (i, n, nr[j], eta / (n * u))
+
+
+def test_batched_cholesky_known_answers(dev, golden):
+    """Exact integer Cholesky factors (every floating-point step exact) come back exactly,
+    in a batch with other sizes around them."""
+    rng = np.random.default_rng(5)
+    mats, expect = [], []
+    for idx in range(int(golden["chol_count"])):
+        mats.append(dev.to_device(golden[f"chol{idx}_A"]))
+        expect.append(golden[f"chol{idx}_G"])
+        A = spd(int(rng.integers(20, 90)), rng)
+        mats.append(dev.to_device(A))
+        expect.append(None)
+    batch = dev.Batch(mats, torch.float64)
+    info = dev.potrf_vbatched(batch).cpu().numpy()
+    assert (info == 0).all()
+    for t, G in zip(mats, expect):
+        if G is not None:
+            assert np.array_equal(np.tril(dev.to_host(t)), G)
+
+
+def test_batched_empty_and_zero_sizes(dev):
+    batch = dev.Batch([], torch.float64)
+    dev.potrf_vbatched(batch)
+    mats = [dev.colmajor_empty(0, 0), dev.to_device(np.eye(3) * 4.0)]
+    info = dev.potrf_vbatched(dev.Batch(mats, torch.float64)).cpu().numpy()
+    assert (info == 0).all() and np.array_equal(dev.to_host(mats[1]), np.eye(3) * 2.0)
