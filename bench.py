@@ -144,6 +144,82 @@ def cpu_baseline(n_metric, n_sample):
             "host_dgemm_gflops": round(gemm_gflops, 1)}
 
 
+def batched_cholesky(args, lib):
+    """20,000 HPD matrices, n_i uniform in [16, 512], A_i = G_i G_i^T / n_i + I, 1..8 right-hand
+    sides each: timed batched factorization (and solve) in FP64 and FP32."""
+    import numpy as np
+    import torch
+    from elemental_amd import device as dev
+    rng = np.random.default_rng(args.seed + 4)
+    count = args.batch_count
+    ns = rng.integers(16, 513, size=count)
+    nrhs = rng.integers(1, 9, size=count)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 4)
+    out = {"count": int(count), "n_range": [16, 512], "nrhs_range": [1, 8],
+           "factor_flops": float(np.sum(ns.astype(np.float64) ** 3) / 3.0)}
+    # matrices grouped by order, one (k, n, n) tensor per order (symmetric, so the row-major
+    # storage is the column-major matrix)
+    orders = {}
+    for i, n in enumerate(ns):
+        orders.setdefault(int(n), []).append(i)
+    for dtype, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+        store, rhs_store, mats, rhs = {}, {}, [None] * count, [None] * count
+        for n, idx in orders.items():
+            G = torch.randn(len(idx), n, n, dtype=torch.float64, device="cuda", generator=g)
+            A = torch.bmm(G, G.transpose(1, 2).contiguous()) / n
+            A += torch.eye(n, dtype=torch.float64, device="cuda")
+            store[n] = A.to(dtype)
+            del G, A
+            for k, i in enumerate(idx):
+                mats[i] = store[n][k]
+                r = int(nrhs[i])
+                rhs[i] = dev.colmajor_empty(n, r, dtype=dtype)
+                rhs[i].copy_(torch.randn(n, r, dtype=torch.float64, device="cuda", generator=g))
+        backup = {n: t.clone() for n, t in store.items()}
+        rhs_backup = [r.clone() for r in rhs]
+        fac = dev.Batch([m.t() for m in mats], dtype)   # .t(): column-major view, same data
+        rb = dev.Batch(rhs, dtype)
+        times, stimes = [], []
+        for rep in range(args.batch_steps + 1):
+            for n in store:
+                store[n].copy_(backup[n])
+            for r, r0 in zip(rhs, rhs_backup):
+                r.copy_(r0)
+            torch.cuda.synchronize()
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
+            info = dev.potrf_vbatched(fac)
+            e1.record()
+            dev.potrs_vbatched(fac, rb)
+            e2.record()
+            torch.cuda.synchronize()
+            if rep > 0:
+                times.append(e0.elapsed_time(e1))
+                stimes.append(e1.elapsed_time(e2))
+        bad = int((info != 0).sum().item())
+        # backward error of a sample of solves, in FP64 on the device
+        worst = 0.0
+        for i in list(range(0, count, max(1, count // 64))):
+            A0 = backup[int(ns[i])][orders[int(ns[i])].index(i)].to(torch.float64)
+            x = rhs[i].to(torch.float64)
+            b = rhs_backup[i].to(torch.float64)
+            r = b - A0 @ x
+            eta = float(r.abs().max() / (A0.abs().sum(1).max() * x.abs().max() + b.abs().max()))
+            worst = max(worst, eta / (int(ns[i]) * (2.0 ** -53 if tag == "f64" else 2.0 ** -24)))
+        ms = sum(times) / len(times)
+        tf = out["factor_flops"] / (ms * 1e-3) / 1e12
+        peak = PEAKS["fp64_TFs"] if tag == "f64" else PEAKS["fp32_TFs"]
+        out[tag] = {"factor_ms": round(ms, 3), "factor_tflops": round(tf, 3),
+                    "frac_of_peak": round(tf / peak, 4), "peak_tflops": peak,
+                    "solve_ms": round(sum(stimes) / len(stimes), 3), "non_hpd": bad,
+                    "solve_backward_error_max_over_n_u": worst}
+        print(f"[bench] batched cholesky {tag}: {out[tag]}", file=sys.stderr, flush=True)
+        del store, backup, mats, rhs, rhs_backup, fac, rb
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +235,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--no-profile-pass", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=20000)
+    ap.add_argument("--batch-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -249,6 +328,10 @@ def main():
             v["share"] = round(v["ms"] / sum(x["ms"] for x in kernels.values()), 4)
         print(f"[bench] kernel classes {json.dumps(kernels)}", file=sys.stderr, flush=True)
 
+    batched = None
+    if not args.skip_batched:
+        batched = batched_cholesky(args, lib)
+
     cpu = None
     if not args.skip_cpu:
         del A
@@ -283,6 +366,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batched_cholesky": batched,
     }
     print(json.dumps(line), flush=True)
 

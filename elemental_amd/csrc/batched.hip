@@ -4,10 +4,11 @@
 // src/lapack_like/factor/Cholesky/LowerVariant3.hpp:15-68) with 32-column blocks, in the
 // left-looking order: a block column first receives the updates of every earlier block column
 // (one GEMM with K = the number of earlier columns, on MFMA), then its 32 x 32 diagonal block
-// is factored unblocked (alpha = sqrt(a_jj), a_jj <= 0 -> not HPD, a21 *= 1/alpha, dsyr with
-// the update skipped for a zero x_l, as LowerVariant3Unblocked and the reference BLAS dsyr do)
-// and the rows below are solved against it (X L11^T = A21 in the column order of the
-// reference BLAS dtrsm: x_k *= 1/l_kk, then b_j -= l_jk x_k). The solve follows
+// is factored unblocked (alpha = sqrt(a_jj), a_jj <= 0 -> not HPD, a21 *= 1/alpha, then the
+// dsyr rank-1 update, as LowerVariant3Unblocked does)
+// and the rows below are solved against it (X L11^T = A21, here as the MFMA product of A21
+// with the transposed inverse of L11, which differs from the reference's substitution only
+// in rounding). The solve follows
 // cholesky::SolveAfter (src/lapack_like/factor/Cholesky/SolveAfter.hpp:15-43): L y = b, then
 // L^T x = y, blocked by 32 rows.
 //
@@ -68,65 +69,105 @@ __device__ __forceinline__ float fsqrt<float>(float x) {
     return __builtin_sqrtf(x);
 }
 
+// Broadcast of lane `src`'s value (src uniform) through the scalar unit.
+__device__ __forceinline__ double bcast(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// S[kk * SM + r] = A(k0 + r, c0 + kk) for r < m, kk < kc, zero for kc <= kk < kcp: threads
+// walk the (row, column) pairs column by column (coalesced), 16 loads in flight per thread.
+template <typename T>
+__device__ __forceinline__ void stage_cols(const T* A, int lda, int k0, int m, int c0, int kc,
+                                           int kcp, int SM, T* S) {
+    const int total = m * kcp;
+    for (int base = threadIdx.x; base < total; base += 16 * BNT) {
+        T v[16];
+        int off[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = base + u * BNT;
+            const int kk = idx / m, r = idx - kk * m;
+            off[u] = (idx < total) ? kk * SM + r : -1;
+            v[u] = (idx < total && kk < kc) ? A[(k0 + r) + (size_t)(c0 + kk) * lda] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (off[u] >= 0) S[off[u]] = v[u];
+    }
+}
+
 // One workgroup: the lower Cholesky factor of one matrix, in place.
 template <typename T>
-__device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P) {
+__device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* Minv) {
     typedef typename Mma<T>::v4 v4;
+    constexpr int MT = BNMAX / 16 / 4;  // row tiles per wave
+    constexpr int PCAP = BNMAX * BLD;   // LDS values of P, reused as the staging area
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int lr = lane & 15, lk = lane >> 4;
+    T* S = P;
+    __shared__ int s_fail;
     for (int k0 = 0; k0 < n; k0 += BNB) {
         const int kb = min(BNB, n - k0), m = n - k0;
         const int nrt = (m + 15) >> 4;
-        // (1) C = A(k0:n, k0:k0+kb) - L(k0:n, 0:k0) L(k0:k0+kb, 0:k0)^T into P (row-major, BLD)
-        for (int rt0 = wave; rt0 < nrt; rt0 += 8) {
-            const int rt1 = rt0 + 4;
-            const bool has1 = rt1 < nrt;
-            v4 acc[2][2];
+        const int SM = nrt * 16;
+        // the widest chunk (a multiple of 32 columns) that fits next to nothing else in LDS
+        const int kcw = min(256, (PCAP / SM) & ~(BNB - 1));
+        // (1) C = A(k0:n, k0:k0+kb) - L(k0:n, 0:k0) L(k0:k0+kb, 0:k0)^T: the block column and
+        // then chunks of L to its left pass through LDS; wave w accumulates row tiles w, w+4, ...
+        // (all of them at once, in MFMA registers). Rows >= m of a staged tile and columns
+        // >= kb of the block only reach accumulator entries that are never used.
+        stage_cols(A, lda, k0, m, k0, kb, BNB, SM, S);
+        __syncthreads();
+        v4 acc[MT][2];
 #pragma unroll
-            for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < MT; ++u)
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
+            for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = (u ? rt1 : rt0) * 16 + Mma<T>::drow(lane, r);
-                        const int col = ct * 16 + lr;
-                        acc[u][ct][r] = (row < m && col < kb && (u == 0 || has1))
-                                            ? A[(k0 + row) + (size_t)(k0 + col) * lda]
-                                            : T(0);
+                for (int r = 0; r < 4; ++r) {
+                    const int row = (wave + 4 * u) * 16 + Mma<T>::drow(lane, r);
+                    const int col = ct * 16 + lr;
+                    acc[u][ct][r] = (row < m) ? S[col * SM + row] : T(0);
+                }
+        for (int kc = 0; kc < k0; kc += kcw) {
+            const int w = min(kcw, k0 - kc);
+            __syncthreads();
+            stage_cols(A, lda, k0, m, kc, w, w, SM, S);
+            __syncthreads();
+            for (int kk = 0; kk < w; kk += 4) {
+                const T* Sk = S + (kk + lk) * SM;
+                const T b0 = Sk[lr], b1 = Sk[16 + lr];
+#pragma unroll
+                for (int u = 0; u < MT; ++u) {
+                    const int rt = wave + 4 * u;
+                    if (rt < nrt) {
+                        const T a = -Sk[rt * 16 + lr];
+                        acc[u][0] = Mma<T>::mma(a, b0, acc[u][0]);
+                        acc[u][1] = Mma<T>::mma(a, b1, acc[u][1]);
                     }
-            const int ra0 = rt0 * 16 + lr, ra1 = rt1 * 16 + lr;
-            const bool va0 = ra0 < m, va1 = has1 && ra1 < m;
-            const bool vb0 = lr < kb, vb1 = 16 + lr < kb;
-            const T* pa0 = A + (k0 + (va0 ? ra0 : 0));
-            const T* pa1 = A + (k0 + (va1 ? ra1 : 0));
-            const T* pb0 = A + (k0 + (vb0 ? lr : 0));
-            const T* pb1 = A + (k0 + (vb1 ? 16 + lr : 0));
-#pragma unroll 2
-            for (int k = 0; k < k0; k += 4) {
-                const size_t off = (size_t)(k + lk) * lda;
-                const T a0 = va0 ? -pa0[off] : T(0);
-                const T a1 = va1 ? -pa1[off] : T(0);
-                const T b0 = vb0 ? pb0[off] : T(0);
-                const T b1 = vb1 ? pb1[off] : T(0);
-                acc[0][0] = Mma<T>::mma(a0, b0, acc[0][0]);
-                acc[0][1] = Mma<T>::mma(a0, b1, acc[0][1]);
-                acc[1][0] = Mma<T>::mma(a1, b0, acc[1][0]);
-                acc[1][1] = Mma<T>::mma(a1, b1, acc[1][1]);
+                }
             }
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = (u ? rt1 : rt0) * 16 + Mma<T>::drow(lane, r);
-                        const int col = ct * 16 + lr;
-                        if (row < m && col < BNB && (u == 0 || has1)) P[row * BLD + col] = acc[u][ct][r];
-                    }
         }
         __syncthreads();
-        // (2) the diagonal block, unblocked, by wave 0: lane i holds row i
-        __shared__ int s_fail;
+#pragma unroll
+        for (int u = 0; u < MT; ++u)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = (wave + 4 * u) * 16 + Mma<T>::drow(lane, r);
+                    const int col = ct * 16 + lr;
+                    if (row < m) P[row * BLD + col] = acc[u][ct][r];
+                }
+        __syncthreads();
+        // (2) the diagonal block, unblocked, by wave 0: lane i holds row i. Each step
+        // broadcasts the pivot and the scaled column through the scalar unit; the dsyr update
+        // is a select, not a branch.
         if (wave == 0) {
             T rw[BNB];
 #pragma unroll
@@ -135,34 +176,54 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P) {
 #pragma unroll
             for (int j = 0; j < BNB; ++j) {
                 if (j < kb && fail < 0) {
-                    const T ajj = __shfl(rw[j], j);
+                    const T ajj = bcast(rw[j], j);
                     if (ajj <= T(0)) {
                         fail = j;
                     } else {
                         const T alpha = fsqrt(ajj);
                         const T inv = T(1) / alpha;
-                        if (lane == j) rw[j] = alpha;
-                        else if (lane > j) rw[j] = rw[j] * inv;
+                        rw[j] = (lane == j) ? alpha : ((lane > j) ? rw[j] * inv : rw[j]);
+                        // dsyr on the trailing lower triangle. Lanes < l update upper-triangle
+                        // entries that are never read, so no lane mask is needed; the update is
+                        // applied for a zero x_l too (the reference BLAS skips it), which can
+                        // only change the sign of a zero entry.
                         const T xi = rw[j];
 #pragma unroll
-                        for (int l = j + 1; l < BNB; ++l) {
-                            const T xl = __shfl(rw[j], l);
-                            if (lane >= l && xl != T(0)) rw[l] = __builtin_fma(xi, -xl, rw[l]);
-                        }
+                        for (int l = j + 1; l < BNB; ++l) rw[l] = __builtin_fma(xi, -bcast(rw[j], l), rw[l]);
                     }
                 }
             }
             if (lane < kb)
 #pragma unroll
-                for (int c = 0; c < BNB; ++c) P[lane * BLD + c] = rw[c];
+                for (int c = 0; c < BNB; ++c) {
+                    P[lane * BLD + c] = rw[c];
+                    Ls[lane * BLD + c] = rw[c];
+                }
             if (lane == 0) s_fail = fail;
+            // inv(L11) when rows below exist (then kb == 32): lane c forms column c,
+            // y = L11^{-1} e_c in the column order (y_k /= l_kk, y_i -= l_ik y_k), and stores
+            // it as row c of Minv, so Minv(k, j) = inv(L11)(j, k)
+            __builtin_amdgcn_wave_barrier();
+            if (fail < 0 && m > BNB && lane < BNB) {
+                T y[BNB];
+#pragma unroll
+                for (int i = 0; i < BNB; ++i) y[i] = (i == lane) ? T(1) : T(0);
+#pragma unroll
+                for (int k = 0; k < BNB; ++k) {
+                    y[k] = y[k] / Ls[k * BLD + k];
+#pragma unroll
+                    for (int i = k + 1; i < BNB; ++i) y[i] = __builtin_fma(-Ls[i * BLD + k], y[k], y[i]);
+                }
+#pragma unroll
+                for (int k = 0; k < BNB; ++k) Minv[lane * BLD + k] = y[k];
+            }
         }
         __syncthreads();
         const int fail = s_fail;
         if (fail >= 0) {
-            // NonHPDMatrixException at global column k0+fail. The block column as the blocked reference holds it when it throws: the diagonal
-            // block's lower triangle after the finished steps, the rows below updated by the
-            // earlier block columns
+            // NonHPDMatrixException at global column k0+fail. The block column as the blocked
+            // reference holds it when it throws: the diagonal block's lower triangle after the
+            // finished steps, the rows below updated by the earlier block columns
             if (t == 0) *info = k0 + fail + 1;
             for (int idx = t; idx < m * kb; idx += BNT) {
                 const int i = idx % m, c = idx / m;
@@ -171,27 +232,27 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P) {
             __syncthreads();
             return;
         }
-        // (3) rows below: X L11^T = A21, one row per thread, row in registers
-        for (int row = kb + t; row < m; row += BNT) {
-            T x[BNB];
+        // (3) rows below (present only when kb == 32): X = A21 L11^{-T} as one MFMA product
+        // with the inverse, Minv(k, j) = inv(L11)(j, k); results go straight to A
+        for (int rt = 2 + wave; rt < nrt; rt += 4) {
+            v4 acc2[2];
 #pragma unroll
-            for (int c = 0; c < BNB; ++c) x[c] = (c < kb) ? P[row * BLD + c] : T(0);
+            for (int ct = 0; ct < 2; ++ct) acc2[ct] = v4{T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int k = 0; k < BNB; ++k) {
-                if (k < kb) {
-                    x[k] = x[k] * (T(1) / P[k * BLD + k]);
-#pragma unroll
-                    for (int j = k + 1; j < BNB; ++j) {
-                        if (j < kb) {
-                            const T ljk = P[j * BLD + k];
-                            if (ljk != T(0)) x[j] = __builtin_fma(-ljk, x[k], x[j]);
-                        }
-                    }
-                }
+            for (int kk = 0; kk < BNB; kk += 4) {
+                const T a = P[(rt * 16 + lr) * BLD + kk + lk];
+                const T b0 = Minv[(kk + lk) * BLD + lr];
+                const T b1 = Minv[(kk + lk) * BLD + 16 + lr];
+                acc2[0] = Mma<T>::mma(a, b0, acc2[0]);
+                acc2[1] = Mma<T>::mma(a, b1, acc2[1]);
             }
 #pragma unroll
-            for (int c = 0; c < BNB; ++c)
-                if (c < kb) A[(k0 + row) + (size_t)(k0 + c) * lda] = x[c];
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rt * 16 + Mma<T>::drow(lane, r);
+                    if (row < m) A[(k0 + row) + (size_t)(k0 + ct * 16 + lr) * lda] = acc2[ct][r];
+                }
         }
         // the diagonal block's lower triangle
         for (int idx = t; idx < kb * kb; idx += BNT) {
@@ -209,6 +270,8 @@ __global__ __launch_bounds__(BNT) void potrf_vbatched_kernel(const BatchItem* __
                                                              int* queue) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* P = reinterpret_cast<T*>(smem_raw);  // BNMAX x BLD
+    T* Ls = P + BNMAX * BLD;                // BNB x BLD: L11
+    T* Minv = Ls + BNB * BLD;               // BNB x BLD: inv(L11)^T
     __shared__ int s_i;
     for (;;) {
         if (threadIdx.x == 0) s_i = atomicAdd(queue, 1);
@@ -218,7 +281,7 @@ __global__ __launch_bounds__(BNT) void potrf_vbatched_kernel(const BatchItem* __
         if (i >= count) break;
         const BatchItem it = items[i];
         if (threadIdx.x == 0) info[it.index] = 0;
-        if (it.n > 0) potrf_one<T>(it.n, As[it.index], it.lda, info + it.index, P);
+        if (it.n > 0) potrf_one<T>(it.n, As[it.index], it.lda, info + it.index, P, Ls, Minv);
     }
 }
 
@@ -327,58 +390,82 @@ __global__ void zero_ints(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Per-device batch area: the schedule (sorted items) and the queue counters.
-struct BatchArea {
-    BatchItem* items = nullptr;
-    int* queue = nullptr;
+// Per-device schedules (sorted items) for the factor and the solve, kept between calls: a call
+// with the same sizes and leading dimensions as the previous one of its kind reuses the
+// device schedule without sorting or copying. The host staging buffer is pinned, and its
+// previous copy is waited for before it is refilled.
+struct Schedule {
+    std::vector<int32_t> key;
+    BatchItem* dev = nullptr;
+    BatchItem* host = nullptr;  // pinned
     size_t cap = 0;
+    int valid = 0;
+    hipEvent_t copied = nullptr;
+};
+struct BatchArea {
+    Schedule sched[2];  // 0: factor, 1: solve
+    int* queue = nullptr;
 };
 std::mutex g_bmu;
 std::map<int, BatchArea> g_batch;
 
-elx_status_t batch_area(size_t count, BatchArea** out) {
+elx_status_t batch_area(BatchArea** out) {
     int dev = 0;
     ELX_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_bmu);
     BatchArea& a = g_batch[dev];
-    if (a.cap < count) {
-        if (a.items) {
-            ELX_HIP(hipDeviceSynchronize());
-            ELX_HIP(hipFree(a.items));
-            a.items = nullptr;
-        }
-        const size_t cap = std::max<size_t>(count, 1024);
-        ELX_HIP(hipMalloc(&a.items, sizeof(BatchItem) * cap));
-        a.cap = cap;
-    }
     if (!a.queue) ELX_HIP(hipMalloc(&a.queue, sizeof(int) * 64));
     *out = &a;
     return ELX_SUCCESS;
 }
 
-// Host schedule: valid items in decreasing order of n (ties by index), uploaded to the device.
-elx_status_t upload_schedule(hipStream_t s, int64_t count, const int32_t* n, const int32_t* lda,
-                             const int32_t* nrhs, const int32_t* ldb, BatchArea* a, int* valid) {
-    std::vector<BatchItem> items;
-    items.reserve((size_t)count);
+// Valid items in decreasing order of n (ties by index), on the device.
+elx_status_t schedule(hipStream_t s, Schedule& sc, int64_t count, const int32_t* n,
+                      const int32_t* lda, const int32_t* nrhs, const int32_t* ldb) {
+    std::vector<int32_t> key;
+    key.reserve((size_t)count * 4);
     for (int64_t i = 0; i < count; ++i) {
-        BatchItem it;
-        std::memset(&it, 0, sizeof(it));
+        ELX_ARG(n[i] >= 0 && n[i] <= BNMAX && lda[i] >= std::max(n[i], 1));
+        ELX_ARG(!nrhs || (nrhs[i] >= 0 && ldb[i] >= std::max(n[i], 1)));
+        key.push_back(n[i]);
+        key.push_back(lda[i]);
+        key.push_back(nrhs ? nrhs[i] : 0);
+        key.push_back(ldb ? ldb[i] : 0);
+    }
+    if (sc.dev && key == sc.key) return ELX_SUCCESS;
+    if (sc.copied) ELX_HIP(hipEventSynchronize(sc.copied));
+    if (sc.cap < (size_t)count) {
+        if (sc.dev) {
+            ELX_HIP(hipDeviceSynchronize());
+            ELX_HIP(hipFree(sc.dev));
+            ELX_HIP(hipHostFree(sc.host));
+            sc.dev = nullptr;
+            sc.host = nullptr;
+        }
+        const size_t cap = std::max<size_t>((size_t)count, 1024);
+        ELX_HIP(hipMalloc(&sc.dev, sizeof(BatchItem) * cap));
+        ELX_HIP(hipHostMalloc(reinterpret_cast<void**>(&sc.host), sizeof(BatchItem) * cap, 0));
+        sc.cap = cap;
+    }
+    if (!sc.copied) ELX_HIP(hipEventCreateWithFlags(&sc.copied, hipEventDisableTiming));
+    std::vector<int32_t> order((size_t)count);
+    for (int64_t i = 0; i < count; ++i) order[(size_t)i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return n[x] > n[y]; });
+    for (int64_t k = 0; k < count; ++k) {
+        const int32_t i = order[(size_t)k];
+        BatchItem& it = sc.host[k];
         it.n = n[i];
         it.lda = lda[i];
-        it.index = (int32_t)i;
+        it.index = i;
         it.nrhs = nrhs ? nrhs[i] : 0;
         it.ldb = ldb ? ldb[i] : 0;
-        ELX_ARG(it.n >= 0 && it.n <= BNMAX && it.lda >= std::max(it.n, 1));
-        ELX_ARG(!nrhs || (it.nrhs >= 0 && it.ldb >= std::max(it.n, 1)));
-        items.push_back(it);
+        it.pad = 0;
     }
-    std::stable_sort(items.begin(), items.end(),
-                     [](const BatchItem& x, const BatchItem& y) { return x.n > y.n; });
-    ELX_HIP(hipMemcpyAsync(a->items, items.data(), sizeof(BatchItem) * items.size(),
+    ELX_HIP(hipMemcpyAsync(sc.dev, sc.host, sizeof(BatchItem) * (size_t)count,
                            hipMemcpyHostToDevice, s));
-    ELX_HIP(hipStreamSynchronize(s));  // the host vector is released on return
-    *valid = (int)items.size();
+    ELX_HIP(hipEventRecord(sc.copied, s));
+    sc.key.swap(key);
+    sc.valid = (int)count;
     return ELX_SUCCESS;
 }
 
@@ -390,12 +477,12 @@ elx_status_t potrf_vbatched(hipStream_t s, int uplo, int64_t count, const int32_
     if (count == 0) return ELX_SUCCESS;
     ELX_ARG(n && lda && dA && dinfo);
     BatchArea* a = nullptr;
-    ELX_CHECK(batch_area((size_t)count, &a));
-    int valid = 0;
-    ELX_CHECK(upload_schedule(s, count, n, lda, nullptr, nullptr, a, &valid));
+    ELX_CHECK(batch_area(&a));
+    ELX_CHECK(schedule(s, a->sched[0], count, n, lda, nullptr, nullptr));
+    const int valid = a->sched[0].valid;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, a->queue, 1);
     ELX_HIP(hipGetLastError());
-    const size_t lds = sizeof(T) * (size_t)BNMAX * BLD;
+    const size_t lds = sizeof(T) * ((size_t)BNMAX * BLD + 2 * BNB * BLD);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&potrf_vbatched_kernel<T>),
@@ -411,8 +498,8 @@ elx_status_t potrf_vbatched(hipStream_t s, int uplo, int64_t count, const int32_
     double flops = 0.0;
     for (int64_t i = 0; i < count; ++i) flops += (double)n[i] * n[i] * n[i] / 3.0;
     elx::ProfScope prof_(s, elx::kKBatched, flops);
-    hipLaunchKernelGGL(potrf_vbatched_kernel<T>, dim3(G), dim3(BNT), lds, s, a->items, valid, dA,
-                       dinfo, a->queue);
+    hipLaunchKernelGGL(potrf_vbatched_kernel<T>, dim3(G), dim3(BNT), lds, s, a->sched[0].dev, valid,
+                       dA, dinfo, a->queue);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
@@ -426,9 +513,9 @@ elx_status_t potrs_vbatched(hipStream_t s, int uplo, int64_t count, const int32_
     if (count == 0) return ELX_SUCCESS;
     ELX_ARG(n && nrhs && lda && ldb && dA && dB);
     BatchArea* a = nullptr;
-    ELX_CHECK(batch_area((size_t)count, &a));
-    int valid = 0;
-    ELX_CHECK(upload_schedule(s, count, n, lda, nrhs, ldb, a, &valid));
+    ELX_CHECK(batch_area(&a));
+    ELX_CHECK(schedule(s, a->sched[1], count, n, lda, nrhs, ldb));
+    const int valid = a->sched[1].valid;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, a->queue + 1, 1);
     ELX_HIP(hipGetLastError());
     int per_cu = 0;
@@ -439,8 +526,8 @@ elx_status_t potrs_vbatched(hipStream_t s, int uplo, int64_t count, const int32_
     double bytes = 0.0;
     for (int64_t i = 0; i < count; ++i) bytes += (double)n[i] * n[i] * sizeof(T) + 2.0 * n[i] * nrhs[i] * sizeof(T);
     elx::ProfScope prof_(s, elx::kKBatchedSolve, bytes);
-    hipLaunchKernelGGL(potrs_vbatched_kernel<T>, dim3(G), dim3(BNT), 0, s, a->items, valid, dA, dB,
-                       a->queue + 1);
+    hipLaunchKernelGGL(potrs_vbatched_kernel<T>, dim3(G), dim3(BNT), 0, s, a->sched[1].dev, valid, dA,
+                       dB, a->queue + 1);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
