@@ -115,8 +115,10 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
         const int kb = min(BNB, n - k0), m = n - k0;
         const int nrt = (m + 15) >> 4;
         const int SM = nrt * 16;
-        // the widest chunk (a multiple of 32 columns) that fits next to nothing else in LDS
-        const int kcw = min(256, (PCAP / SM) & ~(BNB - 1));
+        // chunk width: at most 32 staged values per thread (held in registers while the
+        // previous chunk is multiplied), a multiple of 4 columns
+        const int kcw = min(256, max(4, ((32 * BNT) / SM) & ~3));
+        const int cq = BNT / m, cr = BNT - cq * m;  // BNT = cq*m + cr: the walk over (row, column)
         // (1) C = A(k0:n, k0:k0+kb) - L(k0:n, 0:k0) L(k0:k0+kb, 0:k0)^T: the block column and
         // then chunks of L to its left pass through LDS; wave w accumulates row tiles w, w+4, ...
         // (all of them at once, in MFMA registers). Rows >= m of a staged tile and columns
@@ -134,11 +136,48 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
                     const int col = ct * 16 + lr;
                     acc[u][ct][r] = (row < m) ? S[col * SM + row] : T(0);
                 }
-        for (int kc = 0; kc < k0; kc += kcw) {
-            const int w = min(kcw, k0 - kc);
+        // chunk pipeline: the loads of chunk i+1 are in flight while chunk i is multiplied
+        T pv[32];
+        auto load_chunk = [&](int kc, int w) {
+            const int total = m * w;
+            int kk = t / m, r = t - kk * m;
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + u * BNT;
+                pv[u] = (idx < total) ? A[(k0 + r) + (size_t)(kc + kk) * lda] : T(0);
+                r += cr;
+                kk += cq;
+                if (r >= m) {
+                    r -= m;
+                    ++kk;
+                }
+            }
+        };
+        auto store_chunk = [&](int w) {
+            const int total = m * w;
+            int kk = t / m, r = t - kk * m;
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + u * BNT;
+                if (idx < total) S[kk * SM + r] = pv[u];
+                r += cr;
+                kk += cq;
+                if (r >= m) {
+                    r -= m;
+                    ++kk;
+                }
+            }
+        };
+        int wnext = min(kcw, k0);
+        if (k0 > 0) load_chunk(0, wnext);
+        for (int kc = 0; kc < k0;) {
+            const int w = wnext;
             __syncthreads();
-            stage_cols(A, lda, k0, m, kc, w, w, SM, S);
+            store_chunk(w);
             __syncthreads();
+            const int kcn = kc + w;
+            wnext = min(kcw, k0 - kcn);
+            if (kcn < k0) load_chunk(kcn, wnext);
             for (int kk = 0; kk < w; kk += 4) {
                 const T* Sk = S + (kk + lk) * SM;
                 const T b0 = Sk[lr], b1 = Sk[16 + lr];
@@ -152,6 +191,7 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
                     }
                 }
             }
+            kc = kcn;
         }
         __syncthreads();
 #pragma unroll
@@ -200,26 +240,33 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
                     Ls[lane * BLD + c] = rw[c];
                 }
             if (lane == 0) s_fail = fail;
-            // inv(L11) when rows below exist (then kb == 32): lane c forms column c,
-            // y = L11^{-1} e_c in the column order (y_k /= l_kk, y_i -= l_ik y_k), and stores
-            // it as row c of Minv, so Minv(k, j) = inv(L11)(j, k)
-            __builtin_amdgcn_wave_barrier();
-            if (fail < 0 && m > BNB && lane < BNB) {
-                T y[BNB];
-#pragma unroll
-                for (int i = 0; i < BNB; ++i) y[i] = (i == lane) ? T(1) : T(0);
-#pragma unroll
-                for (int k = 0; k < BNB; ++k) {
-                    y[k] = y[k] / Ls[k * BLD + k];
-#pragma unroll
-                    for (int i = k + 1; i < BNB; ++i) y[i] = __builtin_fma(-Ls[i * BLD + k], y[k], y[i]);
-                }
-#pragma unroll
-                for (int k = 0; k < BNB; ++k) Minv[lane * BLD + k] = y[k];
-            }
         }
         __syncthreads();
         const int fail = s_fail;
+        if (fail < 0 && m > BNB) {
+            // inv(L11) (rows below exist, so kb == 32) by all threads: 8 lanes per column c,
+            // lane g of the group holds rows g, g+8, g+16, g+24 of y = L11^{-1} e_c, formed in
+            // the column order (y_k /= l_kk, then y_i -= l_ik y_k); stored as row c of Minv, so
+            // Minv(k, j) = inv(L11)(j, k)
+            const int c = t >> 3, g = t & 7;
+            T y[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y[q] = (g + 8 * q == c) ? T(1) : T(0);
+#pragma unroll
+            for (int k = 0; k < BNB; ++k) {
+                const int qk = k >> 3, gk = k & 7;
+                if (g == gk) y[qk] = y[qk] / Ls[k * BLD + k];
+                const T yk = __shfl(y[qk], (lane & ~7) | gk);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = g + 8 * q;
+                    if (i > k) y[q] = __builtin_fma(-Ls[i * BLD + k], yk, y[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Minv[c * BLD + g + 8 * q] = y[q];
+            __syncthreads();
+        }
         if (fail >= 0) {
             // NonHPDMatrixException at global column k0+fail. The block column as the blocked
             // reference holds it when it throws: the diagonal block's lower triangle after the
