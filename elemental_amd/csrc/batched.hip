@@ -178,6 +178,7 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
             const int kcn = kc + w;
             wnext = min(kcw, k0 - kcn);
             if (kcn < k0) load_chunk(kcn, wnext);
+#pragma unroll 4
             for (int kk = 0; kk < w; kk += 4) {
                 const T* Sk = S + (kk + lk) * SM;
                 const T b0 = Sk[lr], b1 = Sk[16 + lr];
