@@ -139,20 +139,31 @@ __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float a
     // Lane holds D(i = n, j = m) with n = n0+wn+ni*16+4*kq+r, m = m0+wm+mi*16+r16.
     const int ea = ea_dev ? *ea_dev : 0, eb = eb_dev ? *eb_dev : 0;
     const float a = alpha * __builtin_ldexpf(1.0f, -(ea + eb));
+    // C += a * acc, 32 values per lane at a time: all 32 loads are issued before any store,
+    // so they are in flight together (a load may not move above a store to C that could
+    // alias it, which would make each read-modify-write wait out a full memory round trip).
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
+    for (int ni = 0; ni < 4; ++ni) {
+        float cv[4][8];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const i64 n = n0 + wn + ni * 16 + 4 * kq + r;
-            if (n >= N) continue;
 #pragma unroll
             for (int mi = 0; mi < 8; ++mi) {
                 const i64 m = m0 + wm + mi * 16 + r16;
-                if (m >= M) continue;
-                float* c = C + m + n * ldc;
-                *c = __builtin_fmaf(a, acc[ni][mi][r], *c);
+                cv[r][mi] = (n < N && m < M) ? C[m + n * ldc] : 0.0f;
             }
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const i64 n = n0 + wn + ni * 16 + 4 * kq + r;
+#pragma unroll
+            for (int mi = 0; mi < 8; ++mi) {
+                const i64 m = m0 + wm + mi * 16 + r16;
+                if (n < N && m < M) C[m + n * ldc] = __builtin_fmaf(a, acc[ni][mi][r], cv[r][mi]);
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
