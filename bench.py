@@ -55,10 +55,11 @@ def make_problem(n, kappa, seed):
 
 
 def backward_error(A, x, b):
-    import torch
+    """max over columns of ||b - A x||inf / (||A||inf ||x||inf + ||b||inf), in FP64."""
     r = b - A @ x
     an = A.abs().sum(dim=1).max()
-    return float(r.abs().max() / (an * x.abs().max() + b.abs().max()))
+    eta = r.abs().amax(dim=0) / (an * x.abs().amax(dim=0) + b.abs().amax(dim=0))
+    return float(eta.max())
 
 
 def prof_read(lib):
@@ -142,6 +143,60 @@ def cpu_baseline(n_metric, n_sample):
                       f"scaled by (n_sample/n)^3 to n={n_metric}",
             "seconds_at_sample": round(t, 3), "info": int(info), "cpu_model": cpu_model,
             "host_dgemm_gflops": round(gemm_gflops, 1)}
+
+
+def hpd_solve(args):
+    """HPD system n = 16384 with 16 right-hand sides: A = G G^T / n + I (FP64), B ~ N(0,1).
+    Plain FP64 El::HPDSolve path (copy A, Cholesky, solve) against the low-precision
+    Cholesky + FP64 refinement (fp16 factor, AUTO), both to backward error <= n u."""
+    import torch
+    from elemental_amd import device as dev
+    n, nrhs = args.hpd_n, 16
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 2)
+    G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    A = dev.colmajor_empty(n, n)
+    A.copy_(G @ G.T.contiguous() / n + torch.eye(n, dtype=torch.float64, device="cuda"))
+    del G
+    B = dev.colmajor_empty(n, nrhs)
+    B.copy_(torch.randn(n, nrhs, dtype=torch.float64, device="cuda", generator=g))
+    torch.cuda.empty_cache()
+    out = {"n": n, "nrhs": nrhs, "matrix": "G G^T/n + I", "flops_factor": n ** 3 / 3.0}
+    Ac = dev.colmajor_empty(n, n)
+    Xc = dev.colmajor_empty(n, nrhs)
+    times = []
+    for rep in range(args.hpd_steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Ac.copy_(A)
+        Xc.copy_(B)
+        dev.posv(Ac, Xc)
+        torch.cuda.synchronize()
+        if rep:
+            times.append(time.perf_counter() - t0)
+    out["plain_fp64_ms"] = round(1e3 * sum(times) / len(times), 2)
+    out["plain_fp64_backward_error"] = backward_error(A, Xc, B)
+    del Ac
+    torch.cuda.empty_cache()
+    X = dev.colmajor_empty(n, nrhs)
+    times, st = [], None
+    for rep in range(args.hpd_steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = dev.posv_mixed(A, B, X, factor=args.factor, refine="auto")
+        torch.cuda.synchronize()
+        if rep:
+            times.append(time.perf_counter() - t0)
+    out["mixed_ms"] = round(1e3 * sum(times) / len(times), 2)
+    out["mixed_backward_error"] = backward_error(A, X, B)
+    out["mixed_stats"] = {k: st[k] for k in ("converged", "outer_iters", "inner_iters", "fell_back",
+                                              "factor_seconds", "refine_seconds")}
+    out["speedup"] = round(out["plain_fp64_ms"] / out["mixed_ms"], 3)
+    out["tolerance"] = n * 2.0 ** -53
+    print(f"[bench] hpd solve {out}", file=sys.stderr, flush=True)
+    del A, B, X, Xc
+    torch.cuda.empty_cache()
+    return out
 
 
 def batched_cholesky(args, lib):
@@ -236,6 +291,9 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--no-profile-pass", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-hpd", action="store_true")
+    ap.add_argument("--hpd-n", type=int, default=16384)
+    ap.add_argument("--hpd-steps", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=20000)
     ap.add_argument("--batch-steps", type=int, default=2)
     args = ap.parse_args()
@@ -328,6 +386,10 @@ def main():
             v["share"] = round(v["ms"] / sum(x["ms"] for x in kernels.values()), 4)
         print(f"[bench] kernel classes {json.dumps(kernels)}", file=sys.stderr, flush=True)
 
+    hpd = None
+    if not args.skip_hpd:
+        hpd = hpd_solve(args)
+
     batched = None
     if not args.skip_batched:
         batched = batched_cholesky(args, lib)
@@ -366,6 +428,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "hpd_solve": hpd,
         "batched_cholesky": batched,
     }
     print(json.dumps(line), flush=True)

@@ -174,5 +174,7 @@ elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, 
 // Unblocked Cholesky of the diagonal block (n <= 128) in one workgroup.
 elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,
                               int32_t* info);
+elx_status_t launch_potf2_f32(hipStream_t s, int uplo, i64 n, float* A, i64 lda, i64 goff,
+                              int32_t* info);
 
 }  // namespace elx

@@ -375,4 +375,15 @@ elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda
     return ELX_SUCCESS;
 }
 
+elx_status_t launch_potf2_f32(hipStream_t s, int uplo, i64 n, float* A, i64 lda, i64 goff,
+                              int32_t* info) {
+    if (n <= 0) return ELX_SUCCESS;
+    ELX_ARG(n <= 128);
+    const size_t lds = sizeof(float) * (size_t)n * (n + 1);
+    hipLaunchKernelGGL(potf2_kernel<float>, dim3(1), dim3(256), lds, s, (int)n, A, lda,
+                       uplo == kUpper ? 1 : 0, goff, info);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
 }  // namespace elx

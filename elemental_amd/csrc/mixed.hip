@@ -25,7 +25,7 @@ namespace {
 enum {
     kSlotW = 10, kSlotLh, kSlotUh, kSlotLinv, kSlotUinv, kSlotIdx, kSlotVec, kSlotPart,
     kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP,
-    kSlotF64, kSlotF64Idx
+    kSlotF64, kSlotF64Idx, kSlotSym, kSlotDiag
 };
 
 // Uncached buffers of the persistent triangular solves (never freed before the process
@@ -170,6 +170,36 @@ elx_status_t getrf_mixed(Ctx& c) {
     return ELX_SUCCESS;
 }
 
+// Blocked lower Cholesky of W (fp32): 128-column diagonal blocks unblocked in FP32
+// (cholesky::LowerVariant3Unblocked), the block column below by an FP32 triangular solve, the
+// trailing matrix by one low-precision rank-128 GEMM (the packed L21 serves as both operands,
+// both scaled by the same power of two). The full square is updated; only the lower triangle
+// is read afterwards.
+elx_status_t potrf_mixed(Ctx& c) {
+    const i64 n = c.n, ld = c.ldw;
+    float* W = c.W;
+    ELX_HIP(hipMemsetAsync(c.info, 0, sizeof(int32_t), c.s));
+    for (i64 k0 = 0; k0 < n; k0 += 128) {
+        const i64 kb = std::min<i64>(128, n - k0);
+        ELX_CHECK(launch_potf2_f32(c.s, kLower, kb, W + k0 + k0 * ld, ld, k0, c.info));
+        const i64 n2 = n - k0 - kb;
+        if (n2 <= 0) break;
+        float* L21 = W + (k0 + kb) + k0 * ld;
+        ELX_CHECK(launch_trsm_small_f32(c.s, kRight, kLower, kAdjoint, kNonUnit, n2, kb,
+                                        W + k0 + k0 * ld, ld, L21, ld));
+        const int* e = nullptr;
+        if (c.prec == 0) {
+            ELX_HIP(hipMemsetAsync(c.bits, 0, sizeof(unsigned), c.s));
+            ELX_CHECK(launch_scale_exponent(c.s, n2, kb, L21, ld, c.bits, c.e_dev));
+            e = c.e_dev;
+        }
+        ELX_CHECK(launch_pack(c.s, c.prec, false, n2, kb, 128, L21, ld, 1.0f, e, c.Lh, 128));
+        ELX_CHECK(launch_lpgemm_packed(c.s, c.prec, n2, n2, kb, -1.0f, c.Lh, 128, c.Lh, 128,
+                                       W + (k0 + kb) + (k0 + kb) * ld, ld, e, e));
+    }
+    return ELX_SUCCESS;
+}
+
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
@@ -206,10 +236,13 @@ using namespace elx;
 
 extern "C" {
 
-elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double* dA,
-                             int64_t lda, const double* dB, int64_t ldb, double* dX,
-                             int64_t ldx, int factor_type, int refine, double tol,
-                             int32_t max_iters, elx_ir_stats_t* stats) {
+// The mixed-precision solve shared by the LU (kind 0) and Cholesky (kind 1) forms: both
+// leave W as a unit-lower / upper pair with a row permutation, which the preconditioner
+// applies; dA is the full FP64 matrix for the residuals.
+static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs, const double* dA,
+                                int64_t lda, const double* dB, int64_t ldb, double* dX,
+                                int64_t ldx, int factor_type, int refine, double tol,
+                                int32_t max_iters, elx_ir_stats_t* stats) {
     ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) &&
             ldb >= std::max<int64_t>(n, 1) && ldx >= std::max<int64_t>(n, 1));
     ELX_ARG(factor_type == ELX_FACTOR_FP16 || factor_type == ELX_FACTOR_BF16);
@@ -268,8 +301,17 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
     ELX_CHECK(launch_convert(s, n, dA, lda, c.W, c.ldw, part, nch));
     ELX_HIP(hipMemsetAsync(anorm_d, 0, sizeof(double), s));
     ELX_CHECK(launch_rowabs_max(s, n, nch, part, anorm_d));
-    ELX_CHECK(getrf_mixed(c));
-    ELX_CHECK(launch_swaps_to_perm(s, c.dests, n, n, c.perm, 0));
+    if (kind == 0) {
+        ELX_CHECK(getrf_mixed(c));
+        ELX_CHECK(launch_swaps_to_perm(s, c.dests, n, n, c.perm, 0));
+    } else {
+        ELX_CHECK(potrf_mixed(c));
+        float* dg = nullptr;
+        ELX_CHECK(get(kSlotDiag, (size_t)n, &dg));
+        ELX_CHECK(launch_chol_to_lu(s, n, c.W, c.ldw, dg));
+        ELX_CHECK(launch_iota(s, n, c.perm));
+        ELX_CHECK(launch_diag_inverse(s, n, c.W, c.ldw, 0, c.Linv, 0, 0));
+    }
     ELX_CHECK(launch_diag_inverse(s, n, c.W, c.ldw, 1, c.Uinv, 0, 0));
     st.factor_seconds = tf.stop();
     int32_t finfo = 0;
@@ -301,14 +343,20 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
             ELX_CHECK(get(kSlotF64Idx, (size_t)n + 8, &d64));
             ELX_HIP(hipMemcpy2DAsync(F64, sizeof(double) * n, dA, sizeof(double) * lda,
                                      sizeof(double) * n, n, hipMemcpyDeviceToDevice, s));
-            ELX_CHECK(elx_dgetrf(s, n, n, F64, n, 128, d64, d64 + n));
+            ELX_HIP(hipMemsetAsync(d64 + n, 0, sizeof(int32_t), s));
+            if (kind == 0) ELX_CHECK(elx_dgetrf(s, n, n, F64, n, 128, d64, d64 + n));
+            else ELX_CHECK(elx_dpotrf(s, kLower, n, F64, n, 128, d64 + n));
             int32_t info64 = 0;
             ELX_CHECK(d2h(&info64, d64 + n, 1, s));
             if (info64 != 0) return ELX_SUCCESS;  // exactly singular in FP64 as well
             f64_ready = true;
         }
+        auto f64_apply = [&](double* v) -> elx_status_t {
+            if (kind == 0) return elx_dgetrs(s, 0, n, 1, F64, n, d64, v, n);
+            return elx_dpotrs(s, kLower, 0, n, 1, F64, n, v, n);
+        };
         ELX_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        ELX_CHECK(elx_dgetrs(s, 0, n, 1, F64, n, d64, x, n));
+        ELX_CHECK(f64_apply(x));
         for (int k = 0; k <= 5; ++k) {
             ELX_CHECK(backward(x, b, eta));
             if (*eta <= tol) {
@@ -316,7 +364,7 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
                 break;
             }
             if (k == 5) break;
-            ELX_CHECK(elx_dgetrs(s, 0, n, 1, F64, n, d64, r, n));
+            ELX_CHECK(f64_apply(r));
             ELX_CHECK(launch_axpy(s, n, 1.0, r, x));
         }
         return ELX_SUCCESS;
@@ -502,10 +550,28 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
     return all_ok ? ELX_SUCCESS : ELX_ERROR_NUMERIC;
 }
 
-elx_status_t elx_dposv_mixed(void*, int64_t, int64_t, const double*, int64_t, const double*,
-                             int64_t, double*, int64_t, int, int, double, int32_t,
-                             elx_ir_stats_t*) {
-    return ELX_ERROR_NOT_SUPPORTED;
+elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double* dA,
+                             int64_t lda, const double* dB, int64_t ldb, double* dX,
+                             int64_t ldx, int factor_type, int refine, double tol,
+                             int32_t max_iters, elx_ir_stats_t* stats) {
+    return mixed_solve(0, stream, n, nrhs, dA, lda, dB, ldb, dX, ldx, factor_type, refine, tol,
+                       max_iters, stats);
+}
+
+elx_status_t elx_dposv_mixed(void* stream, int uplo, int64_t n, int64_t nrhs, const double* dA,
+                             int64_t lda, const double* dB, int64_t ldb, double* dX,
+                             int64_t ldx, int factor_type, int refine, double tol,
+                             int32_t max_iters, elx_ir_stats_t* stats) {
+    ELX_ARG(uplo == kLower || uplo == kUpper);
+    ELX_ARG(n >= 0 && lda >= std::max<int64_t>(n, 1));
+    if (n == 0) return mixed_solve(1, stream, n, nrhs, dA, lda, dB, ldb, dX, ldx, factor_type,
+                                   refine, tol, max_iters, stats);
+    // the full symmetric matrix from the stored triangle (the residuals read all of it)
+    double* S = nullptr;
+    ELX_CHECK(get(kSlotSym, (size_t)n * (size_t)n, &S));
+    ELX_CHECK(launch_symmetrize(static_cast<hipStream_t>(stream), n, dA, lda, uplo == kUpper, S));
+    return mixed_solve(1, stream, n, nrhs, S, n, dB, ldb, dX, ldx, factor_type, refine, tol,
+                       max_iters, stats);
 }
 
 }  // extern "C"

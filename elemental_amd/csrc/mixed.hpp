@@ -17,6 +17,13 @@ elx_status_t launch_vec_absmax(hipStream_t s, i64 n, const double* v, double* ou
 // Blocks b0 .. b0+count-1 (count <= 0: through the last block).
 elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, int upper,
                                  float* Inv, i64 b0, i64 count);
+// S (n x n, ld n) = the full symmetric matrix stored in the lower (upper = 0) or upper
+// triangle of A.
+elx_status_t launch_symmetrize(hipStream_t s, i64 n, const double* A, i64 lda, int upper, double* S);
+// Cholesky factor L in the lower triangle of W -> W holds L D^{-1} (strict lower) and D L^T
+// (upper); d receives diag(L).
+elx_status_t launch_chol_to_lu(hipStream_t s, i64 n, float* W, i64 ldw, float* d);
+elx_status_t launch_iota(hipStream_t s, i64 n, int32_t* p);
 // B := Inv * B for an nb-row block (nb <= 128) of ncols columns, in place.
 elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb);
 // Persistent triangular solve with the factor in W (fp32); x and flags in uncached memory.

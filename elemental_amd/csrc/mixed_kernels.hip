@@ -126,6 +126,76 @@ __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const 
     }
 }
 
+// S = the full symmetric matrix whose lower (upper == 0) or upper triangle is stored in A
+// (column-major, lda); S has leading dimension n. 32 x 32 tiles through LDS, so the
+// mirrored (transposed) half is read and written coalesced.
+__global__ __launch_bounds__(256) void symmetrize_kernel(i64 n, const double* __restrict__ A, i64 lda,
+                                                         int upper, double* __restrict__ S) {
+    __shared__ double tl[32][33];
+    const i64 bi = blockIdx.x, bj = blockIdx.y;  // output tile (rows bi, columns bj)
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    // the output tile lies in the stored triangle, or mirrors tile (bj, bi)
+    const bool direct = upper ? (bi <= bj) : (bi >= bj);
+    const i64 ri = (direct ? bi : bj) * 32, ci = (direct ? bj : bi) * 32;
+    for (int q = ty; q < 32; q += 8) {
+        const i64 r = ri + tx, c = ci + q;
+        tl[q][tx] = (r < n && c < n) ? A[r + c * lda] : 0.0;
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        const i64 r = bi * 32 + tx, c = bj * 32 + q;
+        if (r >= n || c >= n) continue;
+        // element (r, c): stored directly when it is in the triangle, else as (c, r)
+        const bool in_tri = upper ? (r <= c) : (r >= c);
+        double v;
+        if (direct) v = in_tri ? tl[q][tx] : tl[tx][q];  // diagonal tile: mirror inside it
+        else v = tl[tx][q];
+        S[r + c * n] = v;
+    }
+}
+
+// The Cholesky factor L (lower triangle of W, ldw) into the unit-lower / upper pair of the
+// LU-based triangular solves: L_u = L D^{-1} (strict lower), U = D L^T (upper, diagonal
+// d_i^2), D = diag(L) given in d. 32 x 32 tiles of the lower triangle, tile (bi, bj) also
+// writing its mirror (bj, bi).
+__global__ __launch_bounds__(256) void chol_to_lu_kernel(i64 n, float* W, i64 ldw, const float* __restrict__ d,
+                                                         int ntiles) {
+    __shared__ float tl[32][33];
+    // lower tile index -> (bi, bj), bi >= bj
+    int bi = (int)((__builtin_sqrtf(8.0f * (float)blockIdx.x + 1.0f) - 1.0f) * 0.5f);
+    while ((bi + 1) * (bi + 2) / 2 <= (int)blockIdx.x) ++bi;
+    while (bi * (bi + 1) / 2 > (int)blockIdx.x) --bi;
+    const int bj = (int)blockIdx.x - bi * (bi + 1) / 2;
+    (void)ntiles;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int q = ty; q < 32; q += 8) {
+        const i64 r = (i64)bi * 32 + tx, c = (i64)bj * 32 + q;
+        tl[q][tx] = (r < n && c < n) ? W[r + c * ldw] : 0.0f;  // tl[col][row]
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        // lower part: (r, c) = (bi*32+tx, bj*32+q)
+        const i64 r = (i64)bi * 32 + tx, c = (i64)bj * 32 + q;
+        if (r < n && c < n) {
+            if (r > c) W[r + c * ldw] = tl[q][tx] / d[c];
+            else if (r == c) W[r + c * ldw] = d[r] * d[r];
+        }
+        // upper part: (r2, c2) = (bj*32+tx, bi*32+q) = d_r2 * L(c2, r2)
+        const i64 r2 = (i64)bj * 32 + tx, c2 = (i64)bi * 32 + q;
+        if (r2 < n && c2 < n && r2 < c2) W[r2 + c2 * ldw] = d[r2] * tl[tx][q];
+    }
+}
+
+__global__ void diag_extract_kernel(i64 n, const float* __restrict__ W, i64 ldw, float* d) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = W[i + i * ldw];
+}
+
+__global__ void iota_kernel(i64 n, int32_t* p) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
 // Persistent flag-chained triangular solve with 128-row blocks, fp32, one right-hand side.
 // lower == 1: L x = y (unit lower, forward); lower == 0: U x = y (upper, backward).
 // Workgroup g handles blocks in solve order g, g+G, ...; block b accumulates
@@ -442,6 +512,33 @@ elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, 
     }
     hipLaunchKernelGGL(diag_inverse_kernel, dim3((unsigned)nblk), dim3(128), 2 * 128 * 128 * 4, s, n,
                        W, ldw, upper, Inv, b0);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_symmetrize(hipStream_t s, i64 n, const double* A, i64 lda, int upper, double* S) {
+    if (n <= 0) return ELX_SUCCESS;
+    const unsigned T = (unsigned)cdiv(n, 32);
+    hipLaunchKernelGGL(symmetrize_kernel, dim3(T, T), dim3(256), 0, s, n, A, lda, upper, S);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_chol_to_lu(hipStream_t s, i64 n, float* W, i64 ldw, float* d) {
+    if (n <= 0) return ELX_SUCCESS;
+    hipLaunchKernelGGL(diag_extract_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, W, ldw, d);
+    ELX_HIP(hipGetLastError());
+    const i64 T = (n + 31) / 32;
+    const i64 ntiles = T * (T + 1) / 2;
+    ELX_ARG(ntiles < (1ll << 31));
+    hipLaunchKernelGGL(chol_to_lu_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, n, W, ldw, d, (int)ntiles);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_iota(hipStream_t s, i64 n, int32_t* p) {
+    if (n <= 0) return ELX_SUCCESS;
+    hipLaunchKernelGGL(iota_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, p);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }

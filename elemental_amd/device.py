@@ -164,15 +164,24 @@ def gesv_mixed(A, B, X, factor="fp16", refine="auto", tol=0.0, max_iters=0, stre
     return out
 
 
-def posv_mixed(A, B, X, factor="fp32", refine="ir", tol=0.0, max_iters=0, stream=None):
+def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_iters=0,
+               stream=None):
+    """Low-precision Cholesky + FP64 refinement of an HPD A (its `uplo` triangle); returns the
+    stats dict (synchronous), with "status" 0 or 5 (tolerance not reached)."""
     pA, lda = _mat(A)
     pB, ldb = _mat(B)
     pX, ldx = _mat(X)
     st = IrStats()
     import ctypes
-    call("elx_dposv_mixed", _stream(stream), A.shape[0], B.shape[1], pA, lda, pB, ldb, pX, ldx,
-         FACTOR[factor], REFINE[refine], tol, max_iters, ctypes.byref(st))
-    return st.as_dict()
+    status = load().elx_dposv_mixed(_stream(stream), uplo, A.shape[0], B.shape[1], pA, lda, pB,
+                                     ldb, pX, ldx, FACTOR[factor], REFINE[refine], tol, max_iters,
+                                     ctypes.byref(st))
+    if status not in (0, 5):
+        from ._lib import ElxError
+        raise ElxError("elx_dposv_mixed", status)
+    out = st.as_dict()
+    out["status"] = status
+    return out
 
 
 __all__ = [

@@ -158,8 +158,12 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
                              int64_t ldx, int factor_type, int refine, double tol,
                              int32_t max_iters, elx_ir_stats_t* stats);
 
-/* Same for symmetric positive definite A: low-precision Cholesky + FP64 refinement. */
-elx_status_t elx_dposv_mixed(void* stream, int64_t n, int64_t nrhs, const double* dA,
+/* Same for Hermitian positive definite A, whose `uplo` triangle is read (the other is never
+ * touched; El::HPDSolve semantics, src/lapack_like/solve/HPD.cpp:47-57): Cholesky in FP32
+ * diagonal blocks with FP16 / BF16 MFMA trailing updates (factor_type), refinement in FP64 as
+ * above, FP64 Cholesky (elx_dpotrf / elx_dpotrs) as the AUTO fallback. factor_info > 0 reports
+ * the first non-positive pivot of the low-precision factorization. */
+elx_status_t elx_dposv_mixed(void* stream, int uplo, int64_t n, int64_t nrhs, const double* dA,
                              int64_t lda, const double* dB, int64_t ldb, double* dX,
                              int64_t ldx, int factor_type, int refine, double tol,
                              int32_t max_iters, elx_ir_stats_t* stats);

@@ -148,3 +148,55 @@ def test_fallback_when_low_precision_factor_breaks_down(dev, orc):
     assert orc.backward_error(A, x, b) <= n * EPS
     st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="ir")
     assert st["factor_info"] != 0 and st["converged"] == 0 and st["status"] == 5
+
+
+def spd_kappa(n, kappa, seed):
+    """A = Q diag(sigma) Q^T, Q Haar, sigma geometric from 1 to 1/kappa (HPD, condition kappa)."""
+    rng = np.random.default_rng(seed)
+    Q, R = np.linalg.qr(rng.standard_normal((n, n)))
+    Q = Q * np.sign(np.diag(R))[None, :]
+    sigma = kappa ** (-np.arange(n) / max(n - 1, 1))
+    A = (Q * sigma[None, :]) @ Q.T
+    return np.asfortranarray((A + A.T) / 2)
+
+
+@pytest.mark.parametrize("uplo", [0, 1])
+@pytest.mark.parametrize("factor", ["fp16", "bf16"])
+@pytest.mark.parametrize("n", [200, 1000, 2300])
+def test_hpd_mixed_solve(dev, orc, uplo, factor, n):
+    """El::HPDSolve semantics (only the uplo triangle is read: the other one holds NaNs here)
+    to FP64 backward error <= n u, from a low-precision Cholesky + FP64 refinement."""
+    say("posv mixed", factor, n, "uplo", uplo)
+    A = spd_kappa(n, 1e2, n + 7)
+    b = np.random.default_rng(n + 8).standard_normal((n, 2))
+    Ast = A.copy(order="F")
+    if uplo == 0:
+        Ast[np.triu_indices(n, 1)] = np.nan
+    else:
+        Ast[np.tril_indices(n, -1)] = np.nan
+    dA, dB = dev.to_device(Ast), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 2)
+    st = dev.posv_mixed(dA, dB, dX, uplo=uplo, factor=factor, refine="auto")
+    X = dev.to_host(dX)
+    eta = orc.backward_error(A, X, b)
+    say(st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["converged"] == 1 and st["factor_info"] == 0
+    assert eta <= n * EPS
+    if factor == "fp16":
+        assert st["fell_back"] == 0
+    assert np.array_equal(dev.to_host(dA), Ast, equal_nan=True)  # A is not modified
+
+
+def test_hpd_mixed_not_positive_definite(dev):
+    """A symmetric indefinite matrix: the low-precision Cholesky reports the column, the FP64
+    fallback fails the same way, and the call says so instead of returning an answer."""
+    n = 300
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((n, n))
+    A = np.asfortranarray(G @ G.T / n + np.eye(n))
+    A[150, 150] = -50.0
+    b = rng.standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.posv_mixed(dA, dB, dX, factor="fp16", refine="auto")
+    assert st["factor_info"] > 0 and st["converged"] == 0 and st["status"] == 5
