@@ -66,6 +66,8 @@ _SIGS = {
                         c_dbl, c_i32, ctypes.POINTER(IrStats)],
     "elx_dposv_mixed": [c_vp, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int,
                         c_int, c_dbl, c_i32, ctypes.POINTER(IrStats)],
+    "elx_dgesv_mixed_batched": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                c_int, c_int, c_dbl, c_i32, c_vp],
     "elx_dpotrf_vbatched": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp],
     "elx_spotrf_vbatched": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp],
     "elx_dpotrs_vbatched": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],

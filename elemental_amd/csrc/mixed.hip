@@ -558,6 +558,33 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
                        max_iters, stats);
 }
 
+elx_status_t elx_dgesv_mixed_batched(void* stream, int64_t count, const int32_t* n,
+                                     const int32_t* nrhs, const double* const* A_ptrs,
+                                     const int32_t* lda, const double* const* B_ptrs,
+                                     const int32_t* ldb, double* const* X_ptrs,
+                                     const int32_t* ldx, int factor_type, int refine, double tol,
+                                     int32_t max_iters, elx_ir_stats_t* stats) {
+    ELX_ARG(count >= 0);
+    if (count == 0) return ELX_SUCCESS;
+    ELX_ARG(n && nrhs && A_ptrs && lda && B_ptrs && ldb && X_ptrs && ldx);
+    // largest first: the workspaces grow once, to the largest order
+    std::vector<int64_t> order((size_t)count);
+    for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return n[a] > n[b]; });
+    elx_status_t worst = ELX_SUCCESS;
+    for (int64_t i : order) {
+        elx_ir_stats_t st;
+        std::memset(&st, 0, sizeof(st));
+        const elx_status_t r = mixed_solve(0, stream, n[i], nrhs[i], A_ptrs[i], lda[i], B_ptrs[i],
+                                           ldb[i], X_ptrs[i], ldx[i], factor_type, refine, tol,
+                                           max_iters, &st);
+        if (stats) stats[i] = st;
+        if (r != ELX_SUCCESS && r != ELX_ERROR_NUMERIC) return r;
+        if (r == ELX_ERROR_NUMERIC) worst = r;
+    }
+    return worst;
+}
+
 elx_status_t elx_dposv_mixed(void* stream, int uplo, int64_t n, int64_t nrhs, const double* dA,
                              int64_t lda, const double* dB, int64_t ldb, double* dX,
                              int64_t ldx, int factor_type, int refine, double tol,

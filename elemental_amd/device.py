@@ -187,7 +187,7 @@ def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_i
 __all__ = [
     "load", "getrf", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
     "permute_rows", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
-    "Batch", "potrf_vbatched", "potrs_vbatched",
+    "Batch", "potrf_vbatched", "potrs_vbatched", "gesv_mixed_batched",
 ]
 
 
@@ -236,4 +236,38 @@ def potrs_vbatched(factors, rhs, uplo=LOWER, stream=None):
     call(name, _stream(stream), uplo, len(factors), _np_ptr(factors.n), _np_ptr(rhs.cols),
          ctypes_void(factors.dptrs.data_ptr()), _np_ptr(factors.ld),
          ctypes_void(rhs.dptrs.data_ptr()), _np_ptr(rhs.ld))
+
+
+def gesv_mixed_batched(As, Bs, Xs, factor="fp16", refine="auto", tol=0.0, max_iters=0,
+                       stream=None):
+    """elx_dgesv_mixed over lists of column-major FP64 tensors; returns (status, list of stats
+    dicts)."""
+    import ctypes
+    cnt = len(As)
+    if not (len(Bs) == len(Xs) == cnt):
+        raise ValueError("batch lengths differ")
+    pa, la, pb, lb, px, lx, ns, nr = [], [], [], [], [], [], [], []
+    for A, B, X in zip(As, Bs, Xs):
+        p, l = _mat(A)
+        pa.append(p)
+        la.append(l)
+        p, l = _mat(B)
+        pb.append(p)
+        lb.append(l)
+        p, l = _mat(X)
+        px.append(p)
+        lx.append(l)
+        ns.append(A.shape[0])
+        nr.append(B.shape[1])
+    arr64 = lambda v: np.ascontiguousarray(v, dtype=np.uint64)
+    arr32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)
+    keep = [arr32(ns), arr32(nr), arr64(pa), arr32(la), arr64(pb), arr32(lb), arr64(px), arr32(lx)]
+    stats = (IrStats * max(cnt, 1))()
+    status = load().elx_dgesv_mixed_batched(_stream(stream), cnt, *[_np_ptr(a) for a in keep],
+                                             FACTOR[factor], REFINE[refine], tol, max_iters,
+                                             ctypes.cast(stats, ctypes.c_void_p))
+    if status not in (0, 5):
+        from ._lib import ElxError
+        raise ElxError("elx_dgesv_mixed_batched", status)
+    return status, [stats[i].as_dict() for i in range(cnt)]
 

@@ -158,6 +158,19 @@ elx_status_t elx_dgesv_mixed(void* stream, int64_t n, int64_t nrhs, const double
                              int64_t ldx, int factor_type, int refine, double tol,
                              int32_t max_iters, elx_ir_stats_t* stats);
 
+/* elx_dgesv_mixed for `count` independent systems: A_i X_i = B_i (n_i x n_i, n_i x nrhs_i).
+ * The arrays are host arrays of `count` entries; A_ptrs / B_ptrs / X_ptrs hold device pointers.
+ * Each system gets its own stats entry (stats may be NULL) and its own FP64 fallback under
+ * ELX_REFINE_AUTO. The systems are solved one after another on the stream, largest first.
+ * Returns ELX_ERROR_NUMERIC if any system missed the tolerance. No counterpart in the
+ * reference (a caller would loop over El::LinearSolve). */
+elx_status_t elx_dgesv_mixed_batched(void* stream, int64_t count, const int32_t* n,
+                                     const int32_t* nrhs, const double* const* A_ptrs,
+                                     const int32_t* lda, const double* const* B_ptrs,
+                                     const int32_t* ldb, double* const* X_ptrs,
+                                     const int32_t* ldx, int factor_type, int refine, double tol,
+                                     int32_t max_iters, elx_ir_stats_t* stats);
+
 /* Same for Hermitian positive definite A, whose `uplo` triangle is read (the other is never
  * touched; El::HPDSolve semantics, src/lapack_like/solve/HPD.cpp:47-57): Cholesky in FP32
  * diagonal blocks with FP16 / BF16 MFMA trailing updates (factor_type), refinement in FP64 as

@@ -200,3 +200,29 @@ def test_hpd_mixed_not_positive_definite(dev):
     dX = dev.colmajor_empty(n, 1)
     st = dev.posv_mixed(dA, dB, dX, factor="fp16", refine="auto")
     assert st["factor_info"] > 0 and st["converged"] == 0 and st["status"] == 5
+
+
+def test_batched_mixed_lu_ir(dev, orc):
+    """The batched LU-IR configuration in small: nonsymmetric randsvd systems of log-uniform
+    order and condition (1e1 .. 1e8), 4 right-hand sides each, fp16 factors with AUTO
+    refinement: every system meets the FP64 backward-error bound, the ill-conditioned ones
+    through the FP64 fallback."""
+    rng = np.random.default_rng(77)
+    cnt = 14
+    ns = np.exp(rng.uniform(np.log(64), np.log(700), cnt)).astype(int)
+    kappas = 10.0 ** rng.uniform(1, 8, cnt)
+    kappas[0], kappas[1] = 1e8, 10.0
+    As = [randsvd(int(n), float(k), 100 + i) for i, (n, k) in enumerate(zip(ns, kappas))]
+    Bs = [rng.standard_normal((int(n), 4)) for n in ns]
+    dA = [dev.to_device(A) for A in As]
+    dB = [dev.to_device(B) for B in Bs]
+    dX = [dev.colmajor_empty(int(n), 4) for n in ns]
+    say("batched lu-ir", list(ns), [f"{k:.0e}" for k in kappas])
+    status, stats = dev.gesv_mixed_batched(dA, dB, dX, factor="fp16", refine="auto")
+    assert status == 0
+    for i in range(cnt):
+        X = dev.to_host(dX[i])
+        eta = orc.backward_error(As[i], X, Bs[i])
+        say(i, ns[i], f"{kappas[i]:.1e}", stats[i]["fell_back"], eta / (ns[i] * EPS))
+        assert stats[i]["converged"] == 1 and eta <= ns[i] * EPS, (i, ns[i], kappas[i], stats[i])
+    assert stats[0]["fell_back"] == 1 and stats[1]["fell_back"] == 0
