@@ -200,7 +200,8 @@ def hpd_solve(args):
 
 
 def batched_lu_ir(args):
-    """2,000 nonsymmetric systems, n log-uniform in [64, 2048] (rounded to a multiple of 64),
+    """Nonsymmetric systems (2,000 in the full configuration; --lu-batch-count, 200 by default
+    to keep the run short), n log-uniform in [64, 2048] (rounded to a multiple of 64),
     kappa log-uniform in [1e1, 1e8], A = U diag(sigma) V^T (U, V Haar from batched QR, sigma
     geometric), 4 right-hand sides each: fp16 LU + FP64 GMRES-IR with the FP64 LU fallback,
     every system to backward error <= n u."""
@@ -348,7 +349,7 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-hpd", action="store_true")
     ap.add_argument("--skip-lu-batch", action="store_true")
-    ap.add_argument("--lu-batch-count", type=int, default=2000)
+    ap.add_argument("--lu-batch-count", type=int, default=200)
     ap.add_argument("--hpd-n", type=int, default=16384)
     ap.add_argument("--hpd-steps", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=20000)
