@@ -303,7 +303,18 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(launch_rowabs_max(s, n, nch, part, anorm_d));
     if (kind == 0) {
         ELX_CHECK(getrf_mixed(c));
-        ELX_CHECK(launch_swaps_to_perm(s, c.dests, n, n, c.perm, 0));
+        // the explicit permutation (Permutation::ExplicitVector): the swaps replayed on the host,
+        // which is quicker than one device thread doing n dependent steps
+        std::vector<int32_t> dh((size_t)n), ph((size_t)n);
+        ELX_CHECK(d2h(dh.data(), c.dests, (size_t)n, s));
+        for (i64 i = 0; i < n; ++i) ph[(size_t)i] = (int32_t)i;
+        for (i64 i = 0; i < n; ++i) {
+            const int32_t d = dh[(size_t)i];
+            if (d >= 0 && d < n && d != i) std::swap(ph[(size_t)i], ph[(size_t)d]);
+        }
+        ELX_HIP(hipMemcpyAsync(c.perm, ph.data(), sizeof(int32_t) * (size_t)n,
+                               hipMemcpyHostToDevice, s));
+        ELX_HIP(hipStreamSynchronize(s));
     } else {
         ELX_CHECK(potrf_mixed(c));
         float* dg = nullptr;

@@ -229,6 +229,20 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
         i64 qc = 0;
         long long spins = 0;
         bool bail = false;
+        // the tile of the next block in solve order is loaded ahead, while its flag is awaited
+        f4v pw[16];
+        i64 pf = -1;
+        auto prefetch = [&](i64 qq) {
+            const i64 c0 = (lower ? qq : (nblk - 1 - qq)) * 128;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const i64 col = c0 + jg + 8 * u;
+                pw[u] = (col < n) ? *reinterpret_cast<const f4v*>(W + (r0 + rq) + col * ldw)
+                                  : f4v{0.f, 0.f, 0.f, 0.f};
+            }
+            pf = qq;
+        };
+        if (q > 0) prefetch(0);
         while (qc < q && !bail) {
             if (t < 64) {
                 const i64 qq = qc + t;
@@ -264,11 +278,16 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
                 const i64 qq = qc + kk;
                 const i64 c0 = (lower ? qq : (nblk - 1 - qq)) * 128;
                 f4v w[16];
+                if (qq == pf) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const i64 col = c0 + jg + 8 * u;
-                    w[u] = (col < n) ? *reinterpret_cast<const f4v*>(W + (r0 + rq) + col * ldw)
-                                     : f4v{0.f, 0.f, 0.f, 0.f};
+                    for (int u = 0; u < 16; ++u) w[u] = pw[u];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        const i64 col = c0 + jg + 8 * u;
+                        w[u] = (col < n) ? *reinterpret_cast<const f4v*>(W + (r0 + rq) + col * ldw)
+                                         : f4v{0.f, 0.f, 0.f, 0.f};
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 16; ++u) {
@@ -280,6 +299,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
                 }
             }
             qc += k;
+            if (qc < q) prefetch(qc);
             __syncthreads();
         }
         // v = y_b - sum_c T(b,c) x_c
