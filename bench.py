@@ -29,11 +29,19 @@ CLASSES = ["lu_panel", "dgemm_f64", "lpgemm", "trsm_small", "row_moves", "trsv",
            "pack"]
 
 
-def make_problem(n, kappa, seed):
+def make_problem(n, kappa, seed, matrix="randsvd"):
     import torch
     from elemental_amd import device as dev
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
+    if matrix == "gaussian":
+        # N(0,1) entries: the same kernels and traffic as randsvd without the QR
+        A = dev.colmajor_empty(n, n)
+        A.copy_(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g).T)
+        b = dev.colmajor_empty(n, 1)
+        b.copy_(torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=g))
+        torch.cuda.synchronize()
+        return A, b
 
     def haar():
         G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
@@ -89,6 +97,68 @@ def roofline_of(cls, rec, factor):
     ach = rec["work"] / t / 1e9
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAKS["hbm_GBs"], "unit": "GB/s",
             "frac": round(ach / PEAKS["hbm_GBs"], 5)}
+
+
+# kernel class -> (kernel-name prefixes whose dispatches are the class's launches, prefixes of
+# helper kernels launched inside the same class launch) in a rocprofv3 counter CSV
+CLASS_KERNELS = {
+    "lu_panel": (("void (anonymous namespace)::lu_panel_kernel<float>",), ()),
+    "lpgemm": (("void (anonymous namespace)::lpgemm_kernel",), ()),
+    "row_moves": (("void (anonymous namespace)::row_moves_kernel<float>",), ()),
+    "trsm_small": (("void (anonymous namespace)::trsm_small_kernel<float>",
+                    "(anonymous namespace)::trmm_inv_kernel"), ()),
+    "trsv": (("(anonymous namespace)::trsv_kernel",), ()),
+    "residual_f64": (("(anonymous namespace)::gemv_partial_kernel",),
+                     ("(anonymous namespace)::residual_finish_kernel",)),
+    "pack": (("void (anonymous namespace)::pack_kernel",), ()),
+}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "lu_ir_n32768_hbm_traffic_pmc.md")
+
+
+def counters_summary(fetch_csv, write_csv):
+    """Markdown summary of two rocprofv3 --pmc CSVs (FETCH_SIZE, WRITE_SIZE) over a
+    `bench.py --counters` run: per kernel class, KiB per dispatch as read, then HBM bytes per
+    dispatch after scaling each counter by what it reads for the run's closing 1 GiB copy
+    (the last dispatch, a device-to-device hipMemcpy)."""
+    import csv
+    def load(path):
+        with open(path, newline="") as f:
+            return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(f)]
+    fetch, write = load(fetch_csv), load(write_csv)
+    gib = float(1 << 30)
+    f_scale = gib / (fetch[-1][1] * 1024.0)
+    w_scale = gib / (write[-1][1] * 1024.0)
+    out = ["| kernel class | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | "
+           "HBM bytes/dispatch (scaled) |", "|---|---|---|---|---|"]
+    machine = []
+    for cls, (main, helpers) in CLASS_KERNELS.items():
+        launches = sum(1 for k, _ in fetch if k.startswith(main))
+        if not launches or launches != sum(1 for k, _ in write if k.startswith(main)):
+            continue
+        fm = sum(v for k, v in fetch if k.startswith(main + helpers)) / launches
+        wm = sum(v for k, v in write if k.startswith(main + helpers)) / launches
+        tb = 1024.0 * (fm * f_scale + wm * w_scale)
+        out.append(f"| {cls} | {launches} | {fm:.1f} | {wm:.1f} | {tb:.4g} |")
+        machine.append(f"traffic_bytes_per_launch[{cls}] = {int(round(tb))}")
+    out.append("")
+    out.append(f"Calibration (1 GiB copy): FETCH_SIZE read {fetch[-1][1]:.1f} KiB -> scale "
+               f"{f_scale:.4f}; WRITE_SIZE read {write[-1][1]:.1f} KiB -> scale {w_scale:.4f}.")
+    out.append("")
+    out.extend(machine)
+    return "\n".join(out)
+
+
+def traffic_from_profile(cls):
+    """HBM bytes per launch of `cls` from the committed counter summary (n = 32768, fp16
+    factor), or None when the summary or the class is absent."""
+    import re
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(cls) + r"\] = (\d+)$", text, re.M)
+    return int(m.group(1)) if m else None
 
 
 def cpu_baseline(n_metric, n_sample):
@@ -348,6 +418,11 @@ def main():
     ap.add_argument("--no-profile-pass", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-hpd", action="store_true")
+    ap.add_argument("--matrix", default="randsvd", choices=["randsvd", "gaussian"])
+    ap.add_argument("--counters", action="store_true",
+                    help="one solve and a 1 GiB copy, for hardware-counter runs; no JSON bench line")
+    ap.add_argument("--counters-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the markdown summary of two rocprofv3 --pmc CSVs and exit")
     ap.add_argument("--skip-lu-batch", action="store_true")
     ap.add_argument("--lu-batch-count", type=int, default=200)
     ap.add_argument("--hpd-n", type=int, default=16384)
@@ -355,6 +430,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=20000)
     ap.add_argument("--batch-steps", type=int, default=2)
     args = ap.parse_args()
+    if args.counters_summary:
+        print(counters_summary(*args.counters_summary))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
               "(this build has no multi-GPU path)", file=sys.stderr)
@@ -369,7 +447,18 @@ def main():
     torch.cuda.set_device(0)
     n = args.n
     print(f"[bench] generating n={n} kappa={args.kappa:g}", file=sys.stderr, flush=True)
-    A, b = make_problem(n, args.kappa, args.seed)
+    if args.counters:
+        # one solve plus a 1 GiB copy (the counters' calibration) for rocprofv3 --pmc runs
+        A, b = make_problem(n, args.kappa, args.seed, "gaussian")
+        X = dev.colmajor_empty(n, 1)
+        st = dev.gesv_mixed(A, b, X, factor=args.factor, refine=args.refine)
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"counters_run": True, "solve": st, "copy_bytes": src.numel() * 8}), flush=True)
+        return
+    A, b = make_problem(n, args.kappa, args.seed, args.matrix)
     X = dev.colmajor_empty(n, 1)
 
     def mixed_solve():
@@ -438,7 +527,8 @@ def main():
         dom = max(kernels, key=lambda k: kernels[k]["ms"])
         roof = roofline_of(dom, kernels[dom], args.factor)
         roof["kernel"] = dom
-        roof["traffic"] = None
+        roof["traffic"] = (traffic_from_profile(dom)
+                           if (n, args.factor) == (32768, "fp16") else None)
         roof["avg_launch_us"] = round(1e3 * kernels[dom]["ms"] / kernels[dom]["launches"], 2)
         for k, v in kernels.items():
             v["share"] = round(v["ms"] / sum(x["ms"] for x in kernels.values()), 4)
