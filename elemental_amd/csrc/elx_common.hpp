@@ -139,10 +139,14 @@ elx_status_t dtrsm(hipStream_t s, int side, int uplo, int orient, int diag, i64 
 elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
                                  int32_t* dests, i64 goff, int2* pairs, int32_t* info);
 
-elx_status_t launch_lu_panel_f32(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
-                                 int32_t* dests, i64 goff, int2* pairs, int32_t* info);
-elx_status_t launch_row_moves_f32(hipStream_t s, const int2* pairs, int npairs, float* A,
-                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3);
+// The same panel for an fp32 matrix stored row by row: element (r, j) at A[r*lda + j].
+elx_status_t launch_lu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
+                                          int32_t* dests, i64 goff, int2* pairs, int32_t* info);
+// The row-move list applied to an fp32 matrix stored row by row (row i contiguous at
+// A + i*lda): columns [c0,c1) and [c2,c3) of each moved row, starting at row r0.
+elx_status_t launch_row_moves_f32_rowmajor(hipStream_t s, const int2* pairs, int npairs,
+                                           float* A, i64 lda, i64 r0, i64 c0, i64 c1, i64 c2,
+                                           i64 c3);
 
 // Apply a row-move list (pairs[i] = {dst, src}, panel-relative, -1 = none) to columns
 // [c0,c1) and [c2,c3) of A starting at row `r0`.

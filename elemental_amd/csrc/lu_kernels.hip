@@ -43,6 +43,37 @@ __global__ __launch_bounds__(256) void row_moves_kernel(const int2* __restrict__
     }
 }
 
+// The same moves for an fp32 matrix stored row by row (row i contiguous at A + i*lda), as the
+// mixed-precision factorization keeps its working copy: each move is a contiguous segment
+// copy. A workgroup owns a 64-column slice of every moved row; wave w reads the slice of moves
+// 32w .. 32w+31 (a float per lane, 256 contiguous bytes per move), and all writes follow one
+// barrier, so overlapping sources and destinations are safe.
+__global__ __launch_bounds__(512) void row_moves_rowmajor_kernel(const int2* __restrict__ pairs,
+                                                                 int npairs, float* A, i64 lda,
+                                                                 i64 r0, i64 c0, i64 c1, i64 c2,
+                                                                 i64 c3) {
+    __shared__ int2 pr[256];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t < 256) pr[t] = (t < npairs) ? pairs[t] : int2{-1, -1};
+    __syncthreads();
+    const i64 s1 = (c1 - c0 + 63) / 64;  // slices of the first range
+    const i64 sl = blockIdx.x;
+    const i64 col = (sl < s1) ? (c0 + sl * 64 + lane) : (c2 + (sl - s1) * 64 + lane);
+    const bool in = (sl < s1) ? (col < c1) : (col < c3);
+    float v[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        const int2 p = pr[wave * 32 + q];
+        v[q] = (in && p.x >= 0) ? A[(r0 + p.y) * lda + col] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        const int2 p = pr[wave * 32 + q];
+        if (in && p.x >= 0) A[(r0 + p.x) * lda + col] = v[q];
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Explicit permutation of a swap sequence (Permutation::ExplicitVector,
 // src/lapack_like/perm/Permutation.cpp:731-745). The swaps are sequential by nature; one
@@ -272,15 +303,17 @@ elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, 
     return ELX_SUCCESS;
 }
 
-elx_status_t launch_row_moves_f32(hipStream_t s, const int2* pairs, int npairs, float* A,
-                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3) {
+elx_status_t launch_row_moves_f32_rowmajor(hipStream_t s, const int2* pairs, int npairs,
+                                           float* A, i64 lda, i64 r0, i64 c0, i64 c1, i64 c2,
+                                           i64 c3) {
     ELX_ARG(npairs <= 256);
-    const i64 total = std::max<i64>(c1 - c0, 0) + std::max<i64>(c3 - c2, 0);
-    if (total <= 0 || npairs <= 0) return ELX_SUCCESS;
     if (c1 < c0) c1 = c0;
     if (c3 < c2) c3 = c2;
+    const i64 total = (c1 - c0) + (c3 - c2);
+    if (total <= 0 || npairs <= 0) return ELX_SUCCESS;
+    const i64 slices = cdiv(c1 - c0, 64) + cdiv(c3 - c2, 64);
     ProfScope prof_(s, kKRowMoves, 2.0 * npairs * (double)total * sizeof(float));
-    hipLaunchKernelGGL(row_moves_kernel<float>, dim3(cdiv(total, 4)), dim3(256), 0, s, pairs,
+    hipLaunchKernelGGL(row_moves_rowmajor_kernel, dim3((unsigned)slices), dim3(512), 0, s, pairs,
                        npairs, A, lda, r0, c0, c1, c2, c3);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;

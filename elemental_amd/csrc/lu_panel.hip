@@ -42,6 +42,7 @@ struct PanelArgs {
     int w;
     T* A;
     i64 lda;
+    int rowmajor;  // element (r, j) at A[r*lda + j] (1) or A[r + j*lda] (0)
     int32_t* dests;
     i64 goff;
     int2* pairs;
@@ -188,9 +189,16 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     const i64 row0 = (i64)g * R;
     const int rows = (int)((p.mp - row0) < R ? (p.mp - row0) : R);
 
-    for (int idx = t; idx < rows * W; idx += NT) {
-        const int r = idx % rows, j = idx / rows;
-        tile[r * LD + j] = p.A[row0 + r + (i64)j * p.lda];
+    if (p.rowmajor) {
+        for (int idx = t; idx < rows * W; idx += NT) {
+            const int j = idx % W, r = idx / W;
+            tile[r * LD + j] = p.A[(row0 + r) * p.lda + j];
+        }
+    } else {
+        for (int idx = t; idx < rows * W; idx += NT) {
+            const int r = idx % rows, j = idx / rows;
+            tile[r * LD + j] = p.A[row0 + r + (i64)j * p.lda];
+        }
     }
     for (int r = t; r < rows; r += NT) {
         lbl[r] = (int)(row0 + r);
@@ -479,9 +487,16 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         return;
     }
     // Rows to their final positions; the moved-row list for the other columns.
-    for (int idx = t; idx < rows * W; idx += NT) {
-        const int r = idx % rows, j = idx / rows;
-        p.A[(i64)lbl[r] + (i64)j * p.lda] = tile[r * LD + j];
+    if (p.rowmajor) {
+        for (int idx = t; idx < rows * W; idx += NT) {
+            const int j = idx % W, r = idx / W;
+            p.A[(i64)lbl[r] * p.lda + j] = tile[r * LD + j];
+        }
+    } else {
+        for (int idx = t; idx < rows * W; idx += NT) {
+            const int r = idx % rows, j = idx / rows;
+            p.A[(i64)lbl[r] + (i64)j * p.lda] = tile[r * LD + j];
+        }
     }
     for (int r = t; r < rows; r += NT) {
         const int phys = (int)(row0 + r), L = lbl[r];
@@ -497,8 +512,8 @@ size_t panel_lds_bytes(int R, int W) {
 }
 
 template <typename T>
-elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* dests, i64 goff,
-                          int2* pairs, int32_t* info) {
+elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowmajor,
+                          int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
     ELX_ARG(w >= 1 && w <= WMAX && mp >= w);
     const elx::DeviceInfo& di = elx::device_info();
     const size_t lds_cap = (size_t)di.max_lds_per_block;
@@ -528,6 +543,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int32_t* 
     a.w = (int)w;
     a.A = A;
     a.lda = lda;
+    a.rowmajor = rowmajor;
     a.dests = dests;
     a.goff = goff;
     a.pairs = pairs;
@@ -556,12 +572,12 @@ namespace elx {
 
 elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
                                  int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
-    return launch_panel<double>(s, mp, w, A, lda, dests, goff, pairs, info);
+    return launch_panel<double>(s, mp, w, A, lda, 0, dests, goff, pairs, info);
 }
 
-elx_status_t launch_lu_panel_f32(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
-                                 int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
-    return launch_panel<float>(s, mp, w, A, lda, dests, goff, pairs, info);
+elx_status_t launch_lu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
+                                          int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
+    return launch_panel<float>(s, mp, w, A, lda, 1, dests, goff, pairs, info);
 }
 
 }  // namespace elx

@@ -101,29 +101,35 @@ struct Ctx {
     unsigned* bits;
 };
 
-// C -= L * U for the column-major fp32 blocks L (m x k at Ls) and U (k x n at Us), with the
-// operands rounded to fp16/bf16 (U scaled by a power of two for fp16).
-elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Ls, const float* Us, float* C,
+// C -= L * U for blocks of the fp32 working matrix, which holds A row by row (A(i, j) at
+// W[j + i*ld]): L (m x k) at Lt with L(i, kk) at Lt[kk + i*ld], U (k x n) at Ut with U(kk, j) at
+// Ut[j + kk*ld], C (m x n) at Ct with C(i, j) at Ct[j + i*ld]. Stored this way C is C^T in
+// column-major order, so the update runs as C^T -= U^T L^T with the operands rounded to
+// fp16/bf16 (U scaled by a power of two for fp16).
+elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float* Ut, float* Ct,
                        i64 ld, i64 ldpack) {
     if (m <= 0 || n <= 0 || k <= 0) return ELX_SUCCESS;
     const i64 kpad = round_up(k, 32);
     const int* e = nullptr;
     if (c.prec == 0) {
         ELX_HIP(hipMemsetAsync(c.bits, 0, sizeof(unsigned), c.s));
-        ELX_CHECK(launch_scale_exponent(c.s, k, n, Us, ld, c.bits, c.e_dev));
+        ELX_CHECK(launch_scale_exponent(c.s, n, k, Ut, ld, c.bits, c.e_dev));
         e = c.e_dev;
     }
-    ELX_CHECK(launch_pack(c.s, c.prec, false, m, k, kpad, Ls, ld, 1.0f, nullptr, c.Lh, ldpack));
-    ELX_CHECK(launch_pack(c.s, c.prec, true, n, k, kpad, Us, ld, 1.0f, e, c.Uh, ldpack));
-    return launch_lpgemm_packed(c.s, c.prec, m, n, k, -1.0f, c.Lh, ldpack, c.Uh, ldpack, C, ld,
-                                nullptr, e);
+    ELX_CHECK(launch_pack(c.s, c.prec, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Lh, ldpack));
+    ELX_CHECK(launch_pack(c.s, c.prec, false, n, k, kpad, Ut, ld, 1.0f, e, c.Uh, ldpack));
+    return launch_lpgemm_packed(c.s, c.prec, n, m, k, -1.0f, c.Uh, ldpack, c.Lh, ldpack, Ct, ld,
+                                e, nullptr);
 }
 
-// Two-level blocked LU with partial pivoting of W (fp32): 128-column cooperative panels
-// inside NB-column outer blocks; the outer trailing update is one rank-NB low-precision GEMM.
+// Two-level blocked LU with partial pivoting of W (fp32, A stored row by row, so a row
+// interchange moves contiguous memory): 128-column cooperative panels inside NB-column outer
+// blocks; the outer trailing update is one rank-NB low-precision GEMM.
 elx_status_t getrf_mixed(Ctx& c) {
     const i64 n = c.n, ld = c.ldw;
     float* W = c.W;
+    // A(i, j) of the working matrix
+    auto at = [&](i64 i, i64 j) { return W + j + i * ld; };
     ELX_HIP(hipMemsetAsync(c.info, 0, sizeof(int32_t), c.s));
     for (i64 K0 = 0; K0 < n; K0 += c.NB) {
         const i64 nbk = std::min(c.NB, n - K0);
@@ -131,22 +137,25 @@ elx_status_t getrf_mixed(Ctx& c) {
         for (i64 j0 = K0; j0 < K1; j0 += 128) {
             const i64 w = std::min<i64>(128, K1 - j0);
             int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
-            ELX_CHECK(launch_lu_panel_f32(c.s, n - j0, w, W + j0 + j0 * ld, ld, c.dests + j0, j0,
-                                          pr, c.info));
-            ELX_CHECK(launch_row_moves_f32(c.s, pr, (int)(2 * w), W, ld, j0, K0, j0, j0 + w, K1));
+            ELX_CHECK(launch_lu_panel_f32_rowmajor(c.s, n - j0, w, at(j0, j0), ld, c.dests + j0,
+                                                   j0, pr, c.info));
+            ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, K0, j0,
+                                                    j0 + w, K1));
             const i64 wr = K1 - (j0 + w);
             if (wr > 0) {
-                ELX_CHECK(launch_trsm_small_f32(c.s, kLeft, kLower, kNormal, kUnit, w, wr,
-                                                W + j0 + j0 * ld, ld, W + j0 + (j0 + w) * ld, ld));
-                ELX_CHECK(lp_update(c, n - (j0 + w), wr, w, W + (j0 + w) + j0 * ld,
-                                    W + j0 + (j0 + w) * ld, W + (j0 + w) + (j0 + w) * ld, ld, 128));
+                // U12 = L11^{-1} A12, as U12^T L11^T = A12^T on the stored transposes
+                ELX_CHECK(launch_trsm_small_f32(c.s, kRight, kUpper, kNormal, kUnit, wr, w,
+                                                at(j0, j0), ld, at(j0, j0 + w), ld));
+                ELX_CHECK(lp_update(c, n - (j0 + w), wr, w, at(j0 + w, j0), at(j0, j0 + w),
+                                    at(j0 + w, j0 + w), ld, 128));
             }
         }
         // the outer block's swaps on the columns outside it, in panel order
         for (i64 j0 = K0; j0 < K1; j0 += 128) {
             const i64 w = std::min<i64>(128, K1 - j0);
             const int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
-            ELX_CHECK(launch_row_moves_f32(c.s, pr, (int)(2 * w), W, ld, j0, 0, K0, K1, n));
+            ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, 0, K0, K1,
+                                                    n));
         }
         // Inverses of the outer block's unit-lower diagonal blocks: final now (later swaps only
         // move rows below K1), used for U12 here and by the triangular solves afterwards.
@@ -156,16 +165,16 @@ elx_status_t getrf_mixed(Ctx& c) {
         // U12 = L11^{-1} A12, blocked by 128, each block by its diagonal inverse
         for (i64 jb = K0; jb < K1; jb += 128) {
             const i64 wb = std::min<i64>(128, K1 - jb);
-            ELX_CHECK(launch_trmm_inv(c.s, wb, n2, c.Linv + (jb / 128) * 128 * 128,
-                                      W + jb + K1 * ld, ld));
+            ELX_CHECK(launch_trmm_inv(c.s, wb, n2, c.Linv + (jb / 128) * 128 * 128, at(jb, K1),
+                                      ld));
             const i64 below = K1 - (jb + wb);
             if (below > 0)
-                ELX_CHECK(lp_update(c, below, n2, wb, W + (jb + wb) + jb * ld, W + jb + K1 * ld,
-                                    W + (jb + wb) + K1 * ld, ld, 128));
+                ELX_CHECK(lp_update(c, below, n2, wb, at(jb + wb, jb), at(jb, K1),
+                                    at(jb + wb, K1), ld, 128));
         }
         // A22 -= L21 U12 (rank nbk)
-        ELX_CHECK(lp_update(c, n - K1, n2, nbk, W + K1 + K0 * ld, W + K0 + K1 * ld,
-                            W + K1 + K1 * ld, ld, round_up(c.NB, 32)));
+        ELX_CHECK(lp_update(c, n - K1, n2, nbk, at(K1, K0), at(K0, K1), at(K1, K1), ld,
+                            round_up(c.NB, 32)));
     }
     return ELX_SUCCESS;
 }

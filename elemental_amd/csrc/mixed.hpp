@@ -5,7 +5,8 @@
 
 namespace elx {
 
-// W = float(A); rowabs[c][i] = sum over column chunk c of |A(i,j)| (nch chunks)
+// W = float(A) stored row by row (A(i,j) at W[j + i*ldw], columns n..ldw-1 zeroed);
+// rowabs[c][i] = sum over column chunk c of |A(i,j)| (nch chunks)
 elx_status_t launch_convert(hipStream_t s, i64 n, const double* A, i64 lda, float* W, i64 ldw,
                             double* rowabs, int nch);
 // *out = max_i sum_c rowabs[c][i]   (*out zeroed first)
@@ -13,20 +14,22 @@ elx_status_t launch_rowabs_max(hipStream_t s, i64 n, int nch, const double* rowa
 // *out = max_i |v_i|   (*out zeroed first)
 elx_status_t launch_vec_absmax(hipStream_t s, i64 n, const double* v, double* out);
 // Inverses of the 128x128 diagonal blocks of the unit lower (upper = 0) or upper (upper = 1)
-// factor in W, to Inv (nblk x 128 x 128).
+// factor stored row by row in W, to Inv (nblk x 128 x 128, column-major blocks).
 // Blocks b0 .. b0+count-1 (count <= 0: through the last block).
 elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, int upper,
                                  float* Inv, i64 b0, i64 count);
 // S (n x n, ld n) = the full symmetric matrix stored in the lower (upper = 0) or upper
 // triangle of A.
 elx_status_t launch_symmetrize(hipStream_t s, i64 n, const double* A, i64 lda, int upper, double* S);
-// Cholesky factor L in the lower triangle of W -> W holds L D^{-1} (strict lower) and D L^T
-// (upper); d receives diag(L).
+// Cholesky factor L in the lower triangle of W (column-major) -> W holds the LU pair
+// L D^{-1} (strict lower) and D L^T (upper) stored row by row; d receives diag(L).
 elx_status_t launch_chol_to_lu(hipStream_t s, i64 n, float* W, i64 ldw, float* d);
 elx_status_t launch_iota(hipStream_t s, i64 n, int32_t* p);
-// B := Inv * B for an nb-row block (nb <= 128) of ncols columns, in place.
+// X := Inv * X for an nb-row block X (nb <= 128) of ncols columns stored row by row
+// (X(k, j) at B[j + k*ldb]), in place.
 elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb);
-// Persistent triangular solve with the factor in W (fp32); x and flags in uncached memory.
+// Persistent triangular solve with the factor stored row by row in W (fp32, ldw % 4 == 0,
+// padding columns zero); x and flags in uncached memory.
 elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const float* Inv,
                          int lower, const float* y, float* x, int64_t* flags, int64_t target,
                          int* err);

@@ -19,52 +19,76 @@ __device__ __forceinline__ T ld_x(const T* p) {
     return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// W(i,j) = float(A(i,j)); rowabs[chunk][i] = sum_{j in chunk} |A(i,j)| (chunk = blockIdx.y).
+// The working copy W = A^T in fp32: W[j + i*ldw] = float(A(i,j)), i.e. A stored row by row, so
+// that the factorization's row interchanges move contiguous memory; the padding W[j + i*ldw],
+// n <= j < ldw, is zeroed. rowabs[chunk][i] = sum_{j in chunk} |A(i,j)|, summed in column
+// order (chunk = blockIdx.y). A workgroup owns 64 rows and walks its chunk's columns 64 at a
+// time through an LDS tile, so the column-major reads and the transposed writes are both
+// coalesced.
 __global__ __launch_bounds__(256) void convert_kernel(i64 n, const double* __restrict__ A, i64 lda,
                                                       float* __restrict__ W, i64 ldw,
                                                       double* __restrict__ rowabs, int cols_per_chunk) {
-    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double tl[64][65];  // tl[column][row]
+    const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
+    const i64 i0 = (i64)blockIdx.x * 64;
     const i64 j0 = (i64)blockIdx.y * cols_per_chunk;
     const i64 j1 = std::min<i64>(n, j0 + cols_per_chunk);
-    double s = 0.0;
-    for (i64 j = j0; j < j1; ++j) {
-        const double a = A[i + j * lda];
-        W[i + j * ldw] = (float)a;
-        s += fabs(a);
+    double sum = 0.0;  // row i0 + t, for t < 64
+    for (i64 jb = j0; jb < j1; jb += 64) {
+        const int nc = (int)std::min<i64>(64, j1 - jb);
+        for (int q = ty; q < 64; q += 4) {
+            const i64 i = i0 + tx;
+            tl[q][tx] = (i < n && q < nc) ? A[i + (jb + q) * lda] : 0.0;
+        }
+        __syncthreads();
+        if (t < 64)
+            for (int q = 0; q < nc; ++q) sum += fabs(tl[q][t]);
+        for (int r = ty; r < 64; r += 4) {
+            const i64 i = i0 + r;
+            if (i < n && tx < nc) W[(jb + tx) + i * ldw] = (float)tl[tx][r];
+        }
+        __syncthreads();
     }
-    rowabs[(i64)blockIdx.y * n + i] = s;
+    if (t < 64 && i0 + t < n) rowabs[(i64)blockIdx.y * n + i0 + t] = sum;
+    const int pad = (int)(ldw - n);
+    if (blockIdx.y == gridDim.y - 1 && pad > 0)
+        for (int idx = t; idx < 64 * pad; idx += 256) {
+            const i64 i = i0 + idx / pad;
+            if (i < n) W[(n + idx % pad) + i * ldw] = 0.0f;
+        }
 }
 
 // Inverse of the 128 x 128 diagonal blocks of a unit-lower (upper == 0) or non-unit upper
-// (upper == 1) triangular factor stored in W. Block b's inverse goes to Inv + b*128*128
-// (column-major, ld 128). One workgroup per block; thread j forms column j.
+// (upper == 1) triangular factor stored row by row in W (element (i, k) at W[k + i*ldw]).
+// Block b's inverse goes to Inv + b*128*128 (column-major, ld 128). One workgroup per block;
+// thread j forms column j.
 __global__ __launch_bounds__(128) void diag_inverse_kernel(i64 n, const float* __restrict__ W,
                                                            i64 ldw, int upper,
                                                            float* __restrict__ Inv, i64 b0) {
     extern __shared__ __align__(16) float sm[];
-    float* T = sm;              // 128 x 128, T[i + k*128]
-    float* X = sm + 128 * 128;  // X[k*128 + j]
+    constexpr int TS = 129;     // T's column stride: the transposing stores hit distinct banks
+    float* T = sm;              // 128 x 128, T[i + k*TS]
+    float* X = sm + 128 * TS;   // X[k*128 + j]
     const i64 b = b0 + blockIdx.x;
     const i64 r0 = b * 128;
     const int nb = (int)std::min<i64>(128, n - r0);
     const int j = threadIdx.x;
     for (int idx = j; idx < 128 * 128; idx += 128) {
-        const int ii = idx & 127, kk = idx >> 7;
-        T[ii + kk * 128] = (ii < nb && kk < nb) ? W[(r0 + ii) + (r0 + kk) * ldw] : 0.0f;
+        const int kk = idx & 127, ii = idx >> 7;
+        T[ii + kk * TS] = (ii < nb && kk < nb) ? W[(r0 + kk) + (r0 + ii) * ldw] : 0.0f;
     }
     __syncthreads();
     if (!upper) {
         for (int i = 0; i < 128; ++i) {
             float s = (i == j) ? 1.0f : 0.0f;
-            for (int k = 0; k < i; ++k) s = __builtin_fmaf(-T[i + k * 128], X[k * 128 + j], s);
+            for (int k = 0; k < i; ++k) s = __builtin_fmaf(-T[i + k * TS], X[k * 128 + j], s);
             X[i * 128 + j] = (i < nb) ? s : 0.0f;
         }
     } else {
         for (int i = 127; i >= 0; --i) {
             float s = (i == j) ? 1.0f : 0.0f;
-            for (int k = i + 1; k < 128; ++k) s = __builtin_fmaf(-T[i + k * 128], X[k * 128 + j], s);
-            X[i * 128 + j] = (i < nb) ? s / T[i + i * 128] : 0.0f;
+            for (int k = i + 1; k < 128; ++k) s = __builtin_fmaf(-T[i + k * TS], X[k * 128 + j], s);
+            X[i * 128 + j] = (i < nb) ? s / T[i + i * TS] : 0.0f;
         }
     }
     __syncthreads();
@@ -75,12 +99,13 @@ __global__ __launch_bounds__(128) void diag_inverse_kernel(i64 n, const float* _
     }
 }
 
-// B := Inv * B for one 128-row block: Inv is the 128 x 128 inverse of a diagonal block (column-
-// major, rows and columns >= nb zero), B the nb x ncols block at B (column-major, ldb). This
-// is the triangular solve L11^{-1} A12 of the blocked LU as a product with the precomputed
-// inverse: every column's result depends only on that column, so a workgroup reads its
-// 64 columns into LDS, multiplies, and writes them back in place. Thread (ig, jg) forms an
-// 8 x 4 block of the result.
+// X := Inv * X for one 128-row block X (nb x ncols) of a matrix stored row by row: X(k, j) at
+// B[j + k*ldb]. Inv is the 128 x 128 inverse of a diagonal block (column-major, rows and
+// columns >= nb zero). This is the triangular solve L11^{-1} A12 of the blocked LU as a product
+// with the precomputed inverse: every column's result depends only on that column, so a
+// workgroup reads its 64 columns (contiguous in every row) into LDS, multiplies, and writes
+// them back in place. Thread (jg, ig) forms an 8 x 4 block of the result: rows 8*ig..,
+// columns 4*jg.., so the 16 lanes of a row group write 256 contiguous bytes.
 __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const float* __restrict__ Inv,
                                                        float* B, i64 ldb) {
     typedef float f4v __attribute__((ext_vector_type(4)));
@@ -93,11 +118,11 @@ __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const 
     for (int idx = t; idx < 128 * 128 / 4; idx += 256)
         reinterpret_cast<f4v*>(Is)[idx] = reinterpret_cast<const f4v*>(Inv)[idx];
     for (int idx = t; idx < 128 * 64; idx += 256) {
-        const int k = idx & 127, j = idx >> 7;
-        Bs[k * 64 + j] = (k < nb && j < nc) ? B[k + (c0 + j) * ldb] : 0.0f;
+        const int j = idx & 63, k = idx >> 6;
+        Bs[k * 64 + j] = (k < nb && j < nc) ? B[(c0 + j) + k * ldb] : 0.0f;
     }
     __syncthreads();
-    const int ig = t & 15, jg = t >> 4;  // rows 8*ig.., columns 4*jg..
+    const int jg = t & 15, ig = t >> 4;  // columns 4*jg.., rows 8*ig..
     float acc[8][4];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
@@ -115,13 +140,13 @@ __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const 
             for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fmaf(iv[a], bv[b], acc[a][b]);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int j = 4 * jg + b;
-        if (j >= nc) continue;
+    for (int a = 0; a < 8; ++a) {
+        const int i = 8 * ig + a;
+        if (i >= nb) continue;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const int i = 8 * ig + a;
-            if (i < nb) B[i + (c0 + j) * ldb] = acc[a][b];
+        for (int b = 0; b < 4; ++b) {
+            const int j = 4 * jg + b;
+            if (j < nc) B[(c0 + j) + i * ldb] = acc[a][b];
         }
     }
 }
@@ -154,10 +179,12 @@ __global__ __launch_bounds__(256) void symmetrize_kernel(i64 n, const double* __
     }
 }
 
-// The Cholesky factor L (lower triangle of W, ldw) into the unit-lower / upper pair of the
-// LU-based triangular solves: L_u = L D^{-1} (strict lower), U = D L^T (upper, diagonal
-// d_i^2), D = diag(L) given in d. 32 x 32 tiles of the lower triangle, tile (bi, bj) also
-// writing its mirror (bj, bi).
+// The Cholesky factor L (lower triangle of W, column-major, ldw) into the unit-lower / upper
+// pair of the LU-based triangular solves, stored row by row like the LU factors (element
+// (i, j) of a factor at W[j + i*ldw]): L_u = L D^{-1} (strict lower), U = D L^T (upper,
+// diagonal d_i^2), D = diag(L) given in d. L(r, c), r > c, sits at W[r + c*ldw], which is where
+// U(c, r) = d_c L(r, c) belongs; L_u(r, c) = L(r, c) / d_c goes to the mirrored W[c + r*ldw].
+// 32 x 32 tiles of the lower triangle, tile (bi, bj) also writing its mirror (bj, bi).
 __global__ __launch_bounds__(256) void chol_to_lu_kernel(i64 n, float* W, i64 ldw, const float* __restrict__ d,
                                                          int ntiles) {
     __shared__ float tl[32][33];
@@ -177,12 +204,12 @@ __global__ __launch_bounds__(256) void chol_to_lu_kernel(i64 n, float* W, i64 ld
         // lower part: (r, c) = (bi*32+tx, bj*32+q)
         const i64 r = (i64)bi * 32 + tx, c = (i64)bj * 32 + q;
         if (r < n && c < n) {
-            if (r > c) W[r + c * ldw] = tl[q][tx] / d[c];
+            if (r > c) W[r + c * ldw] = d[c] * tl[q][tx];
             else if (r == c) W[r + c * ldw] = d[r] * d[r];
         }
-        // upper part: (r2, c2) = (bj*32+tx, bi*32+q) = d_r2 * L(c2, r2)
+        // mirrored part: (r2, c2) = (bj*32+tx, bi*32+q), r2 < c2, = L(c2, r2) / d_r2
         const i64 r2 = (i64)bj * 32 + tx, c2 = (i64)bi * 32 + q;
-        if (r2 < n && c2 < n && r2 < c2) W[r2 + c2 * ldw] = d[r2] * tl[tx][q];
+        if (r2 < n && c2 < n && r2 < c2) W[r2 + c2 * ldw] = tl[tx][q] / d[r2];
     }
 }
 
@@ -196,16 +223,19 @@ __global__ void iota_kernel(i64 n, int32_t* p) {
     if (i < n) p[i] = (int32_t)i;
 }
 
-// Persistent flag-chained triangular solve with 128-row blocks, fp32, one right-hand side.
+// Persistent flag-chained triangular solve with 128-row blocks, fp32, one right-hand side, for
+// factors stored row by row (element (i, j) at W[j + i*ldw]).
 // lower == 1: L x = y (unit lower, forward); lower == 0: U x = y (upper, backward).
 // Workgroup g handles blocks in solve order g, g+G, ...; block b accumulates
 // sum_c T(b,c) x_c over the blocks c earlier in solve order as their x_c are published (flag
 // per block in uncached memory), subtracts it from y_b, multiplies by the precomputed inverse
 // of its diagonal block and publishes x_b. Readiness is checked for up to 64 blocks at once
-// and their x blocks are read in one batch; each thread streams 16-byte slices of T (4 rows
-// x 16 columns per block) so the loads of a block are all in flight together. Each block only
-// waits on blocks earlier in solve order, handled by co-resident workgroups, so every wait
-// ends; a bounded spin reports failure instead. Requires ldw % 4 == 0 (16-byte row slices).
+// and their x blocks are read in one batch; each thread streams 16-byte slices of T (16 rows
+// x 4 columns per block; the 32 lanes of a row read 512 contiguous bytes) so the loads of a
+// block are all in flight together, and the tile of the next block in solve order is loaded
+// while its flag is awaited. Each block only waits on blocks earlier in solve order, handled
+// by co-resident workgroups, so every wait ends; a bounded spin reports failure instead.
+// Requires ldw % 4 == 0 and the padding columns n..ldw-1 of every row zero.
 __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restrict__ W, i64 ldw,
                                                    const float* __restrict__ Inv, int lower,
                                                    const float* __restrict__ y, float* x,
@@ -213,8 +243,10 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
     typedef float f4v __attribute__((ext_vector_type(4)));
     __shared__ float xs[64][128];
     __shared__ float red[8][128];
+    __shared__ float part[128][33];
     __shared__ int s_k;
-    const int t = threadIdx.x, rq = 4 * (t & 31), jg = t >> 5;
+    const int t = threadIdx.x, rq = 4 * (t & 31), jg = t >> 5;  // (rq, jg): the Inv product
+    const int cq = 4 * (t & 31), rg = t >> 5;                   // (cq, rg): the tile product
     const i64 nblk = (n + 127) / 128;
     for (i64 q = blockIdx.x; q < nblk; q += gridDim.x) {
         const i64 b = lower ? q : (nblk - 1 - q);
@@ -225,24 +257,28 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
 #pragma unroll
         for (int u = 0; u < 16; ++u)
             iv[u] = *reinterpret_cast<const f4v*>(inv + rq + (jg + 8 * u) * 128);
-        f4v acc = {0.f, 0.f, 0.f, 0.f};
+        // acc[u]: this thread's share (columns cq..cq+3 of every tile) of row rg + 8u
+        float acc[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[u] = 0.f;
         i64 qc = 0;
         long long spins = 0;
         bool bail = false;
-        // the tile of the next block in solve order is loaded ahead, while its flag is awaited
         f4v pw[16];
         i64 pf = -1;
-        auto prefetch = [&](i64 qq) {
-            const i64 c0 = (lower ? qq : (nblk - 1 - qq)) * 128;
+        auto load_tile = [&](i64 qq, f4v (&w)[16]) {
+            const i64 col = (lower ? qq : (nblk - 1 - qq)) * 128 + cq;
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
-                const i64 col = c0 + jg + 8 * u;
-                pw[u] = (col < n) ? *reinterpret_cast<const f4v*>(W + (r0 + rq) + col * ldw)
-                                  : f4v{0.f, 0.f, 0.f, 0.f};
+                const i64 row = r0 + rg + 8 * u;
+                w[u] = (row < n && col < n) ? *reinterpret_cast<const f4v*>(W + col + row * ldw)
+                                            : f4v{0.f, 0.f, 0.f, 0.f};
             }
-            pf = qq;
         };
-        if (q > 0) prefetch(0);
+        if (q > 0) {
+            load_tile(0, pw);
+            pf = 0;
+        }
         while (qc < q && !bail) {
             if (t < 64) {
                 const i64 qq = qc + t;
@@ -276,42 +312,37 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
             __syncthreads();
             for (int kk = 0; kk < k; ++kk) {
                 const i64 qq = qc + kk;
-                const i64 c0 = (lower ? qq : (nblk - 1 - qq)) * 128;
                 f4v w[16];
                 if (qq == pf) {
 #pragma unroll
                     for (int u = 0; u < 16; ++u) w[u] = pw[u];
                 } else {
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        const i64 col = c0 + jg + 8 * u;
-                        w[u] = (col < n) ? *reinterpret_cast<const f4v*>(W + (r0 + rq) + col * ldw)
-                                         : f4v{0.f, 0.f, 0.f, 0.f};
-                    }
+                    load_tile(qq, w);
                 }
+                const f4v xv = *reinterpret_cast<const f4v*>(&xs[kk][cq]);
 #pragma unroll
                 for (int u = 0; u < 16; ++u) {
-                    const float xv = xs[kk][jg + 8 * u];
-                    acc[0] = __builtin_fmaf(w[u][0], xv, acc[0]);
-                    acc[1] = __builtin_fmaf(w[u][1], xv, acc[1]);
-                    acc[2] = __builtin_fmaf(w[u][2], xv, acc[2]);
-                    acc[3] = __builtin_fmaf(w[u][3], xv, acc[3]);
+                    acc[u] = __builtin_fmaf(w[u][0], xv[0], acc[u]);
+                    acc[u] = __builtin_fmaf(w[u][1], xv[1], acc[u]);
+                    acc[u] = __builtin_fmaf(w[u][2], xv[2], acc[u]);
+                    acc[u] = __builtin_fmaf(w[u][3], xv[3], acc[u]);
                 }
             }
             qc += k;
-            if (qc < q) prefetch(qc);
+            if (qc < q) {
+                load_tile(qc, pw);
+                pf = qc;
+            }
             __syncthreads();
         }
-        // v = y_b - sum_c T(b,c) x_c
-        red[jg][rq + 0] = acc[0];
-        red[jg][rq + 1] = acc[1];
-        red[jg][rq + 2] = acc[2];
-        red[jg][rq + 3] = acc[3];
+        // v = y_b - sum_c T(b,c) x_c: the 32 column shares of each row summed in a fixed order
+#pragma unroll
+        for (int u = 0; u < 16; ++u) part[rg + 8 * u][t & 31] = acc[u];
         __syncthreads();
         if (t < 128) {
             float sacc = 0.f;
 #pragma unroll
-            for (int g2 = 0; g2 < 8; ++g2) sacc += red[g2][t];
+            for (int g2 = 0; g2 < 32; ++g2) sacc += part[t][g2];
             xs[0][t] = (t < nb) ? (y[r0 + t] - sacc) : 0.0f;
         }
         __syncthreads();
@@ -500,7 +531,8 @@ namespace elx {
 elx_status_t launch_convert(hipStream_t s, i64 n, const double* A, i64 lda, float* W, i64 ldw,
                             double* rowabs, int nch) {
     const int cpc = (int)((n + nch - 1) / nch);
-    dim3 grid(cdiv(n, 256), (unsigned)nch);
+    ELX_ARG(ldw >= n && ldw - n < 1024);
+    dim3 grid(cdiv(n, 64), (unsigned)nch);
     hipLaunchKernelGGL(convert_kernel, grid, dim3(256), 0, s, n, A, lda, W, ldw, rowabs, cpc);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
@@ -523,15 +555,16 @@ elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, 
     const i64 nblk = (count > 0) ? count : (n + 127) / 128 - b0;
     ELX_ARG(b0 >= 0 && b0 + nblk <= (n + 127) / 128);
     if (nblk <= 0) return ELX_SUCCESS;
+    const size_t lds = (128 * 129 + 128 * 128) * sizeof(float);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_inverse_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 128 * 128 * 4);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         (void)hipGetLastError();
         attr = true;
     }
-    hipLaunchKernelGGL(diag_inverse_kernel, dim3((unsigned)nblk), dim3(128), 2 * 128 * 128 * 4, s, n,
-                       W, ldw, upper, Inv, b0);
+    hipLaunchKernelGGL(diag_inverse_kernel, dim3((unsigned)nblk), dim3(128), lds, s, n, W, ldw,
+                       upper, Inv, b0);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
