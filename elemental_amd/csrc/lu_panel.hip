@@ -132,19 +132,66 @@ __device__ __forceinline__ bool better(T v, int l, T bv, int bl) {
     return (v > bv) || (v == bv && l < bl);
 }
 
-// Wave-wide argmax of (v, l) with payload q: every lane ends with the wave's best.
+// Wave-wide argmax of (v, l) with payload q (q >= 0 marks a candidate; v >= 0, l < 2^23 - 1,
+// q < 512): every lane ends with the wave's best. The candidate is packed into an order key
+// -- value bits (non-negative floats order like their bit patterns), then 2^23-1-l, then q --
+// reduced by DPP butterflies inside each 16-lane row and readlanes across the four rows.
+struct OKey {
+    uint32_t v1, v0, l;  // value bits (high word zero for float), label key | q
+};
+__device__ __forceinline__ bool okey_gt(OKey a, OKey b) {
+    const uint64_t av = ((uint64_t)a.v1 << 32) | a.v0, bv = ((uint64_t)b.v1 << 32) | b.v0;
+    return av > bv || (av == bv && a.l > b.l);
+}
+template <int CTRL, bool WIDE>
+__device__ __forceinline__ OKey okey_step(OKey a) {
+    OKey o;
+    o.v1 = WIDE ? (uint32_t)__builtin_amdgcn_update_dpp(0, (int)a.v1, CTRL, 0xF, 0xF, false) : 0u;
+    o.v0 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)a.v0, CTRL, 0xF, 0xF, false);
+    o.l = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)a.l, CTRL, 0xF, 0xF, false);
+    return okey_gt(o, a) ? o : a;
+}
+template <bool WIDE>
+__device__ __forceinline__ OKey okey_lane(OKey a, int lane) {
+    OKey o;
+    o.v1 = WIDE ? (uint32_t)__builtin_amdgcn_readlane((int)a.v1, lane) : 0u;
+    o.v0 = (uint32_t)__builtin_amdgcn_readlane((int)a.v0, lane);
+    o.l = (uint32_t)__builtin_amdgcn_readlane((int)a.l, lane);
+    return o;
+}
 template <typename T>
 __device__ __forceinline__ void wave_best(T& v, int& l, int& q) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(v, off);
-        const int ol = __shfl_xor(l, off);
-        const int oq = __shfl_xor(q, off);
-        if (better(ov, ol, v, l)) {
-            v = ov;
-            l = ol;
-            q = oq;
+    constexpr bool WIDE = sizeof(T) == 8;
+    OKey k{0u, 0u, 0u};
+    if (q >= 0) {
+        if constexpr (WIDE) {
+            const uint64_t bits = (uint64_t)__double_as_longlong((double)v);
+            k.v1 = (uint32_t)(bits >> 32);
+            k.v0 = (uint32_t)bits;
+        } else {
+            k.v0 = __float_as_uint((float)v);
         }
+        k.l = ((0x7FFFFFu - (uint32_t)l) << 9) | (uint32_t)q;
+    }
+    k = okey_step<0xB1, WIDE>(k);   // quad_perm [1,0,3,2]
+    k = okey_step<0x4E, WIDE>(k);   // quad_perm [2,3,0,1]
+    k = okey_step<0x141, WIDE>(k);  // row_half_mirror
+    k = okey_step<0x140, WIDE>(k);  // row_mirror
+    const OKey r0 = okey_lane<WIDE>(k, 0), r1 = okey_lane<WIDE>(k, 16);
+    const OKey r2 = okey_lane<WIDE>(k, 32), r3 = okey_lane<WIDE>(k, 48);
+    const OKey b01 = okey_gt(r1, r0) ? r1 : r0, b23 = okey_gt(r3, r2) ? r3 : r2;
+    k = okey_gt(b23, b01) ? b23 : b01;
+    if (k.l == 0u) {
+        v = T(-1);
+        l = INT_MAX;
+        q = -1;
+    } else {
+        if constexpr (WIDE)
+            v = (T)__longlong_as_double((long long)(((uint64_t)k.v1 << 32) | k.v0));
+        else
+            v = (T)__uint_as_float(k.v0);
+        l = (int)(0x7FFFFFu - (k.l >> 9));
+        q = (int)(k.l & 511u);
     }
 }
 
@@ -183,6 +230,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     int* wb_q = wb_l + 2 * NW;                 // [2][NW]
     int* rb_l = wb_q + 2 * NW;                 // [NW]
     int* rb_q = rb_l + NW;                     // [NW]
+    int* wb_ok = rb_q + NW;                    // [2][NW] per-wave votes
 
     const int g = blockIdx.x, t = threadIdx.x, G = p.G;
     const int wave = t >> 6, lane = t & 63;
@@ -225,11 +273,39 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     // store to it, so no reader ever waits for store acknowledgements (on CDNA the vector
     // memory counter covers loads and stores alike); all waves share the LDS work.
 
+    // Block-wide AND with a single barrier: each wave's vote goes to its slot of the current
+    // parity. Every vote ends in a barrier, so no wave is ever more than one vote ahead of
+    // another and the two parities never collide (the per-wave bests of the record scan use
+    // the same parity).
+    auto vote_all = [&](bool pred, int par) -> bool {
+        const bool w = __all(pred ? 1 : 0);
+        if (lane == 0) wb_ok[par * NW + wave] = w ? 1 : 0;
+        __syncthreads();
+        bool all = true;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) all = all && (wb_ok[par * NW + i] != 0);
+        return all;
+    };
+
     // Publish column cc (called by the last wave): bring the candidate's columns >= cc+1
     // through step cc-2, store its row chunks, then its records.
     auto publish = [&](int cc, int cand, int cand_lbl) {
         const uint32_t seq = p.seq_base + (uint32_t)cc;
-        T aprev = T(0);
+        // The record first (every workgroup waits for it); the row chunks are only read by
+        // the winner's readers one step later, and carry their own tags.
+        const T aprev = (cand >= 0 && cc + 1 < W)
+                            ? __builtin_fma(-lp[cand], up[cc + 1], tile[cand * LD + cc + 1])
+                            : T(0);
+        if (lane < REP) {
+            const uint32_t rec = 2u * (uint32_t)(((lane * 2) + (cc & 1)) * G + g);
+            const double a = (double)aprev, l = (cand >= 0) ? (double)lc[cand] : 0.0;
+            const double val = (cand >= 0) ? (double)tile[cand * LD + cc] : 0.0;
+            const u4 r1 = u4{dlo(a), dhi(a), dlo(l), dhi(l)};
+            u4 r0 = u4{dlo(val), dhi(val), (uint32_t)cand_lbl, 0u};
+            r0[3] = rec_tag(r0, r1, seq);
+            st16(p.rec, nrec, rec + 1, r1);
+            st16(p.rec, nrec, rec, r0);
+        }
         if (cand >= 0) {
             const T lb = lp[cand];
             const uint32_t myrow = (uint32_t)((cc % 3) * G + g) * WMAX;
@@ -241,26 +317,12 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                     if (j > cc && j < W) {
                         v[e] = __builtin_fma(-lb, up[j], tile[cand * LD + j]);
                         tile[cand * LD + j] = v[e];
-                        if (j == cc + 1) aprev = v[e];
                     } else {
                         v[e] = (j < W) ? tile[cand * LD + j] : T(0);
                     }
                 }
                 st16(p.rows, nrows, myrow + q, CK::make(v, q, seq));
             }
-        }
-        // lane 0 holds aprev (its first chunk starts at column cc+1); lanes 0..REP-1 store
-        // one copy each of the record
-        aprev = __shfl(aprev, 0);
-        if (lane < REP) {
-            const uint32_t rec = 2u * (uint32_t)(((lane * 2) + (cc & 1)) * G + g);
-            const double a = (double)aprev, l = (cand >= 0) ? (double)lc[cand] : 0.0;
-            const double val = (cand >= 0) ? (double)tile[cand * LD + cc] : 0.0;
-            const u4 r1 = u4{dlo(a), dhi(a), dlo(l), dhi(l)};
-            u4 r0 = u4{dlo(val), dhi(val), (uint32_t)cand_lbl, 0u};
-            r0[3] = rec_tag(r0, r1, seq);
-            st16(p.rec, nrec, rec + 1, r1);
-            st16(p.rec, nrec, rec, r0);
         }
     };
 
@@ -341,7 +403,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                 wb_l[par * NW + wave] = bL;
                 wb_q[par * NW + wave] = bq;
             }
-            if (__syncthreads_and(ok)) {
+            if (vote_all(ok != 0, par)) {
                 pick(wb_v + par * NW, wb_l + par * NW, wb_q + par * NW, gv, gl, gw);
                 break;
             }
@@ -376,7 +438,9 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                         done = true;
                     }
                 }
-                if (__syncthreads_and(done)) break;
+                const int par = iter & 1;
+                ++iter;
+                if (vote_all(done, par)) break;
                 if (++spins > SPIN_LIMIT) {
                     aborted = true;
                     break;
@@ -508,7 +572,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
 template <typename T>
 size_t panel_lds_bytes(int R, int W) {
     return sizeof(T) * ((size_t)R * (W + 3) + WMAX + 3 * NT + 3 * (NT / 64)) +
-           sizeof(int) * ((size_t)R + 6 * (NT / 64));
+           sizeof(int) * ((size_t)R + 8 * (NT / 64));
 }
 
 template <typename T>
@@ -517,12 +581,15 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowma
     ELX_ARG(w >= 1 && w <= WMAX && mp >= w);
     const elx::DeviceInfo& di = elx::device_info();
     const size_t lds_cap = (size_t)di.max_lds_per_block;
-    const i64 RMIN = 16;
+    // A step costs about the same for 16 to 128 rows per workgroup, while the exchange gets
+    // slower with more workgroups: short panels use fewer, fuller workgroups.
+    const i64 RMIN = 128;
     i64 G = std::min<i64>(di.num_cus, (mp + RMIN - 1) / RMIN);
     if (G < 1) G = 1;
     i64 R = (mp + G - 1) / G;
     G = (mp + R - 1) / R;  // every workgroup owns at least one row
     ELX_ARG(G <= NT);      // one record per thread in the exchange
+    ELX_ARG(mp < (1 << 23) - 1);
     const size_t lds = panel_lds_bytes<T>((int)R, (int)w);
     if (lds > lds_cap || R > 64 * RPL) {
         std::fprintf(stderr,
