@@ -26,6 +26,7 @@
 #include <climits>
 
 #include "elx_common.hpp"
+#include "exchange.hpp"
 
 namespace {
 
@@ -34,7 +35,7 @@ constexpr int WMAX = 128;                 // maximum panel width
 constexpr long long SPIN_LIMIT = 1 << 20;  // bounded wait: every wave exits
 constexpr int RPL = 5;                     // rows per lane held in registers (R <= 320)
 constexpr int REP = 4;                     // copies of each record (readers spread over them)
-typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+using elx::xchg::u4;
 
 template <typename T>
 struct PanelArgs {
@@ -54,31 +55,11 @@ struct PanelArgs {
     u4* rows;           // [3][G][WMAX] candidate row chunks
 };
 
-// The exchange area lives in uncached device memory (allocated with
-// hipDeviceMallocUncached): loads and stores go to memory and are coherent across the XCDs
-// without cache maintenance. Each 16-byte record / chunk is written and read with one 16-byte
-// buffer access at system scope (sc0 sc1) and validated by its tag. Buffer accesses, unlike
-// volatile ones, do not wait for completion one by one: loads issued together overlap, a
-// load issued early (the pivot row) completes while other work proceeds, and stores retire
-// in the background. The buffer descriptors span exactly the area in use, so an access out
-// of range reads zeros (and fails its tag check) instead of touching other memory.
-__device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (c + 0x165667B1u) * 0xC2B2AE3Du;
-    h ^= h >> 15;
-    h *= 0x2C1B3C6Du;
-    h ^= h >> 12;
-    return h;
-}
-constexpr int kSysScope = 1 | 16;  // sc0 | sc1
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t area(const u4* base, uint32_t count) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<u4*>(base), 0, (int)(count * 16u), 0x00020000);
-}
-__device__ __forceinline__ void st16(const u4* base, uint32_t count, uint32_t i, u4 v) {
-    __builtin_amdgcn_raw_buffer_store_b128(v, area(base, count), (int)(i * 16u), 0, kSysScope);
-}
-__device__ __forceinline__ u4 ld16(const u4* base, uint32_t count, uint32_t i) {
-    return __builtin_amdgcn_raw_buffer_load_b128(area(base, count), (int)(i * 16u), 0, kSysScope);
-}
+// The exchange area is uncached device memory accessed through the tagged 16-byte units of
+// exchange.hpp.
+using elx::xchg::ld16;
+using elx::xchg::mix32;
+using elx::xchg::st16;
 
 // Candidate record, two 16-byte units: {value (double bits), label, tag} and {look-ahead
 // entry, previous multiplier} (doubles). The tag covers both units, so a reader that sees a
@@ -364,34 +345,48 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     for (int c = 0; c < W; ++c) {
         const int s = c & 1;
         const uint32_t seq = p.seq_base + (uint32_t)c;
-        // (1) every workgroup's record for column c, reduced to the same winner
+        // (1) every workgroup's record for column c, reduced to the same winner. Each polling
+        // wave re-reads only its missing records until all of them carry the column's tag,
+        // without waiting for the other waves; one vote then combines the waves.
         T gv = T(-1);
         int gl = INT_MAX, gw = -1;
-        while (true) {
+        {
             T bv = T(-1);
             int bL = INT_MAX, bq = -1;
-            int ok = 1;
+            bool wave_aborted = false;
             if (wave < PW) {
-                for (int q = t; q < G; q += PW * 64) {
-                    const uint32_t rq = 2u * (uint32_t)((((g % REP) * 2) + s) * G + q);
-                    const u4 r0 = ld16(p.rec, nrec, rq);
-                    const u4 r1 = ld16(p.rec, nrec, rq + 1);
-                    if (r0[3] != rec_tag(r0, r1, seq)) {
-                        ok = 0;
-                        continue;
-                    }
-                    const T val = (T)dbits(r0[0], r0[1]);
-                    la_v[q] = val;
-                    la_a[q] = (T)dbits(r1[0], r1[1]);
-                    la_l[q] = (T)dbits(r1[2], r1[3]);
-                    const int L = (int)r0[2];
-                    if (L != INT_MAX) {
-                        const T v = fabs(val);
-                        if (better(v, L, bv, bL)) {
-                            bv = v;
-                            bL = L;
-                            bq = q;
+                constexpr int STRIDE = PW * 64;
+                bool got0 = !(t < G), got1 = !(t + STRIDE < G);
+                while (true) {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const bool got = k ? got1 : got0;
+                        if (got) continue;
+                        const int q = t + k * STRIDE;
+                        const uint32_t rq = 2u * (uint32_t)((((g % REP) * 2) + s) * G + q);
+                        const u4 r0 = ld16(p.rec, nrec, rq);
+                        const u4 r1 = ld16(p.rec, nrec, rq + 1);
+                        if (r0[3] != rec_tag(r0, r1, seq)) continue;
+                        if (k) got1 = true;
+                        else got0 = true;
+                        const T val = (T)dbits(r0[0], r0[1]);
+                        la_v[q] = val;
+                        la_a[q] = (T)dbits(r1[0], r1[1]);
+                        la_l[q] = (T)dbits(r1[2], r1[3]);
+                        const int L = (int)r0[2];
+                        if (L != INT_MAX) {
+                            const T v = fabs(val);
+                            if (better(v, L, bv, bL)) {
+                                bv = v;
+                                bL = L;
+                                bq = q;
+                            }
                         }
+                    }
+                    if (__all((got0 && got1) ? 1 : 0)) break;
+                    if (++spins > SPIN_LIMIT) {
+                        wave_aborted = true;
+                        break;
                     }
                 }
             }
@@ -403,14 +398,10 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                 wb_l[par * NW + wave] = bL;
                 wb_q[par * NW + wave] = bq;
             }
-            if (vote_all(ok != 0, par)) {
+            if (vote_all(!wave_aborted, par))
                 pick(wb_v + par * NW, wb_l + par * NW, wb_q + par * NW, gv, gl, gw);
-                break;
-            }
-            if (++spins > SPIN_LIMIT) {
+            else
                 aborted = true;
-                break;
-            }
         }
         if (aborted) break;
         const bool none = (gw < 0);  // every remaining candidate is NaN (idamax selects none)

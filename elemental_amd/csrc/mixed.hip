@@ -31,12 +31,12 @@ enum {
 // Uncached buffers of the persistent triangular solves (never freed before the process
 // ends, see panel_sync_get).
 struct TrsvArea {
-    int64_t* flags = nullptr;
+    void* xchunks = nullptr;  // 64 x 16 bytes per 128-row block
     int* err = nullptr;
     float* xa = nullptr;
     float* xb = nullptr;
     i64 n = 0;
-    int64_t epoch = 0;
+    int64_t epoch = 0;  // last epoch used; grows for the life of the process
 };
 std::mutex g_tmu;
 std::map<int, TrsvArea> g_trsv;
@@ -47,13 +47,13 @@ elx_status_t trsv_area(i64 n, TrsvArea** out) {
     std::lock_guard<std::mutex> lk(g_tmu);
     TrsvArea& a = g_trsv[dev];
     if (a.n < n) {
-        if (a.flags) ELX_HIP(hipDeviceSynchronize());
+        // (a smaller earlier area is kept: uncached memory a kernel has read is never freed)
+        if (a.xchunks) ELX_HIP(hipDeviceSynchronize());
         const i64 nblk = (n + 127) / 128;
-        int64_t* flags = nullptr;
+        void* xchunks = nullptr;
         float *xa = nullptr, *xb = nullptr;
-        ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&flags), sizeof(int64_t) * nblk,
-                                      hipDeviceMallocUncached));
-        ELX_HIP(hipMemset(flags, 0, sizeof(int64_t) * nblk));
+        ELX_HIP(hipExtMallocWithFlags(&xchunks, (size_t)nblk * 64 * 16, hipDeviceMallocUncached));
+        ELX_HIP(hipMemset(xchunks, 0, (size_t)nblk * 64 * 16));
         int* err = nullptr;
         ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&err), 64, hipDeviceMallocUncached));
         ELX_HIP(hipMemset(err, 0, 64));
@@ -63,11 +63,10 @@ elx_status_t trsv_area(i64 n, TrsvArea** out) {
         ELX_HIP(hipExtMallocWithFlags(reinterpret_cast<void**>(&xb), sizeof(float) * n,
                                       hipDeviceMallocUncached));
         ELX_HIP(hipDeviceSynchronize());
-        a.flags = flags;
+        a.xchunks = xchunks;
         a.xa = xa;
         a.xb = xb;
         a.n = n;
-        a.epoch = 0;
     }
     *out = &a;
     return ELX_SUCCESS;
@@ -434,10 +433,11 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
         double* x = dX + col * ldx;
         auto precond = [&](const double* rin, double* xout, int accumulate) -> elx_status_t {
             ELX_CHECK(launch_gather_to_f32(s, n, c.perm, rin, y));
-            const int64_t t1 = ++ta->epoch;
-            ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Linv, 1, y, ta->xa, ta->flags, t1, err));
-            const int64_t t2 = ++ta->epoch;
-            ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Uinv, 0, ta->xa, ta->xb, ta->flags, t2, err));
+            const uint32_t t1 = (uint32_t)++ta->epoch;
+            ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Linv, 1, y, ta->xa, ta->xchunks, t1, err));
+            const uint32_t t2 = (uint32_t)++ta->epoch;
+            ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Uinv, 0, ta->xa, ta->xb, ta->xchunks, t2,
+                                  err));
             return launch_update_f64(s, n, ta->xb, xout, accumulate);
         };
         // ||v||_2 on the device, read back

@@ -29,9 +29,11 @@ elx_status_t launch_iota(hipStream_t s, i64 n, int32_t* p);
 // (X(k, j) at B[j + k*ldb]), in place.
 elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb);
 // Persistent triangular solve with the factor stored row by row in W (fp32, ldw % 4 == 0,
-// padding columns zero); x and flags in uncached memory.
+// padding columns zero).
+// xchunks: 64 x 16 bytes per 128-row block, uncached; epoch distinguishes this launch's
+// chunks from every earlier one (it must differ from all epochs used before on the area).
 elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const float* Inv,
-                         int lower, const float* y, float* x, int64_t* flags, int64_t target,
+                         int lower, const float* y, float* x, void* xchunks, uint32_t epoch,
                          int* err);
 // y(i) = float(r(p[i]))
 elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r,

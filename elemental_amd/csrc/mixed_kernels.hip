@@ -4,6 +4,7 @@
 #include <climits>
 
 #include "elx_common.hpp"
+#include "exchange.hpp"
 #include "mixed.hpp"
 
 namespace {
@@ -223,31 +224,44 @@ __global__ void iota_kernel(i64 n, int32_t* p) {
     if (i < n) p[i] = (int32_t)i;
 }
 
-// Persistent flag-chained triangular solve with 128-row blocks, fp32, one right-hand side, for
+// Chunk q of solution block b: {x[2q], x[2q+1], epoch, tag}. The epoch word rejects chunks of
+// earlier launches (and the zeroed area) exactly; the tag over the values, (b, q) and the epoch
+// rejects a partly written chunk.
+__device__ __forceinline__ uint32_t xblock_tag(uint32_t a, uint32_t b, uint32_t id, uint32_t epoch) {
+    return epoch ^ elx::xchg::mix32(a, b, id ^ 0x5BD1E995u);
+}
+
+// Persistent chained triangular solve with 128-row blocks, fp32, one right-hand side, for
 // factors stored row by row (element (i, j) at W[j + i*ldw]).
 // lower == 1: L x = y (unit lower, forward); lower == 0: U x = y (upper, backward).
 // Workgroup g handles blocks in solve order g, g+G, ...; block b accumulates
-// sum_c T(b,c) x_c over the blocks c earlier in solve order as their x_c are published (flag
-// per block in uncached memory), subtracts it from y_b, multiplies by the precomputed inverse
-// of its diagonal block and publishes x_b. Readiness is checked for up to 64 blocks at once
-// and their x blocks are read in one batch; each thread streams 16-byte slices of T (16 rows
-// x 4 columns per block; the 32 lanes of a row read 512 contiguous bytes) so the loads of a
-// block are all in flight together, and the tile of the next block in solve order is loaded
-// while its flag is awaited. Each block only waits on blocks earlier in solve order, handled
-// by co-resident workgroups, so every wait ends; a bounded spin reports failure instead.
-// Requires ldw % 4 == 0 and the padding columns n..ldw-1 of every row zero.
+// sum_c T(b,c) x_c over the blocks c earlier in solve order as their x_c are published,
+// subtracts it from y_b, multiplies by the precomputed inverse of its diagonal block and
+// publishes x_b: to x (read by later kernels) and as 64 tagged 16-byte chunks in uncached
+// memory (xc), which the other workgroups poll directly -- a block is complete when all its
+// tags match the launch's epoch, so no store-completion wait or flag sits on the chain. Each
+// wave checks one block per poll (up to four at once); each thread streams 16-byte slices of
+// T (16 rows x 4 columns per block; the 32 lanes of a row read 512 contiguous bytes), and the
+// tile of the next block in solve order is loaded while it is awaited. Each block only waits
+// on blocks earlier in solve order, handled by co-resident workgroups, so every wait ends; a
+// bounded spin reports failure instead. Requires ldw % 4 == 0 and the padding columns
+// n..ldw-1 of every row zero.
 __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restrict__ W, i64 ldw,
                                                    const float* __restrict__ Inv, int lower,
                                                    const float* __restrict__ y, float* x,
-                                                   int64_t* flags, int64_t target, int* err) {
+                                                   elx::xchg::u4* xc, uint32_t epoch, int* err) {
+    using elx::xchg::u4;
     typedef float f4v __attribute__((ext_vector_type(4)));
-    __shared__ float xs[64][128];
+    __shared__ float xs[4][128];
     __shared__ float red[8][128];
     __shared__ float part[128][33];
-    __shared__ int s_k;
+    __shared__ int s_rdy[2][4];
     const int t = threadIdx.x, rq = 4 * (t & 31), jg = t >> 5;  // (rq, jg): the Inv product
     const int cq = 4 * (t & 31), rg = t >> 5;                   // (cq, rg): the tile product
+    const int wave = t >> 6, lane = t & 63;
     const i64 nblk = (n + 127) / 128;
+    const uint32_t nchunks = (uint32_t)(nblk * 64);
+    int par = 0;
     for (i64 q = blockIdx.x; q < nblk; q += gridDim.x) {
         const i64 b = lower ? q : (nblk - 1 - q);
         const i64 r0 = b * 128;
@@ -280,44 +294,41 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
             pf = 0;
         }
         while (qc < q && !bail) {
-            if (t < 64) {
-                const i64 qq = qc + t;
-                bool ready = false;
-                if (qq < q) {
-                    const i64 c = lower ? qq : (nblk - 1 - qq);
-                    ready = ld_x(&flags[c]) >= target;
-                }
-                const unsigned long long m = __ballot(ready);
-                if (t == 0) s_k = (m == ~0ull) ? 64 : __builtin_ctzll(~m);
+            // wave w checks block qc + w: lane l reads chunk l (x[2l], x[2l+1])
+            const i64 qq = qc + wave;
+            bool ready = false;
+            if (qq < q) {
+                const i64 cblk = lower ? qq : (nblk - 1 - qq);
+                const uint32_t id = (uint32_t)(cblk * 64 + lane);
+                const u4 ch = elx::xchg::ld16(xc, nchunks, id);
+                ready = (ch[2] == epoch && ch[3] == xblock_tag(ch[0], ch[1], id, epoch));
+                xs[wave][2 * lane] = __uint_as_float(ch[0]);
+                xs[wave][2 * lane + 1] = __uint_as_float(ch[1]);
             }
+            const bool wready = __all(ready ? 1 : 0);
+            if (lane == 0) s_rdy[par][wave] = wready ? 1 : 0;
             __syncthreads();
-            const int k = s_k;
-            __syncthreads();
+            int k = 0;
+            while (k < 4 && qc + k < q && s_rdy[par][k]) ++k;
+            par ^= 1;  // a wave is at most one poll ahead of another: the parities never collide
             if (k == 0) {
                 ++spins;
-                if ((spins & 255) == 0 && ld_x(err) != 0) bail = true;
+                if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                    bail = true;
                 if (spins > SPIN_LIMIT) {
-                    if (t == 0) st_x(err, 1);
+                    if (t == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     bail = true;
                 }
                 continue;
             }
-            for (int idx = t; idx < k * 128; idx += 256) {
-                const int kk = idx >> 7, jj = idx & 127;
-                const i64 qq = qc + kk;
-                const i64 c = lower ? qq : (nblk - 1 - qq);
-                const i64 col = c * 128 + jj;
-                xs[kk][jj] = (col < n) ? ld_x(&x[col]) : 0.0f;
-            }
-            __syncthreads();
             for (int kk = 0; kk < k; ++kk) {
-                const i64 qq = qc + kk;
+                const i64 qk = qc + kk;
                 f4v w[16];
-                if (qq == pf) {
+                if (qk == pf) {
 #pragma unroll
                     for (int u = 0; u < 16; ++u) w[u] = pw[u];
                 } else {
-                    load_tile(qq, w);
+                    load_tile(qk, w);
                 }
                 const f4v xv = *reinterpret_cast<const f4v*>(&xs[kk][cq]);
 #pragma unroll
@@ -333,7 +344,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
                 load_tile(qc, pw);
                 pf = qc;
             }
-            __syncthreads();
+            __syncthreads();  // every use of xs before the next poll overwrites it
         }
         // v = y_b - sum_c T(b,c) x_c: the 32 column shares of each row summed in a fixed order
 #pragma unroll
@@ -361,15 +372,21 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
         red[jg][rq + 2] = s4[2];
         red[jg][rq + 3] = s4[3];
         __syncthreads();
-        if (t < nb) {
-            float sx = 0.f;
+        if (t < 64) {
+            float s0 = 0.f, s1 = 0.f;
 #pragma unroll
-            for (int g2 = 0; g2 < 8; ++g2) sx += red[g2][t];
-            st_x(&x[r0 + t], sx);
+            for (int g2 = 0; g2 < 8; ++g2) {
+                s0 += red[g2][2 * t];
+                s1 += red[g2][2 * t + 1];
+            }
+            if (2 * t >= nb) s0 = 0.f;
+            if (2 * t + 1 >= nb) s1 = 0.f;
+            if (2 * t < nb) x[r0 + 2 * t] = s0;
+            if (2 * t + 1 < nb) x[r0 + 2 * t + 1] = s1;
+            const uint32_t id = (uint32_t)(b * 64 + t);
+            const uint32_t a0 = __float_as_uint(s0), a1 = __float_as_uint(s1);
+            elx::xchg::st16(xc, nchunks, id, u4{a0, a1, epoch, xblock_tag(a0, a1, id, epoch)});
         }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (t == 0) st_x(&flags[b], target);
         __syncthreads();
     }
 }
@@ -615,13 +632,15 @@ elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv,
 }
 
 elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const float* Inv,
-                         int lower, const float* y, float* x, int64_t* flags, int64_t target,
+                         int lower, const float* y, float* x, void* xchunks, uint32_t epoch,
                          int* err) {
     const DeviceInfo& di = device_info();
     const i64 nblk = (n + 127) / 128;
+    ELX_ARG(ldw % 4 == 0 && nblk * 64 < ((i64)1 << 26));
     const unsigned G = (unsigned)std::min<i64>(nblk, di.num_cus);
     ProfScope prof_(s, kKTrsv, 0.5 * (double)n * (double)n * sizeof(float));
-    void* args[] = {&n, &W, &ldw, &Inv, &lower, &y, &x, &flags, &target, &err};
+    elx::xchg::u4* xc = static_cast<elx::xchg::u4*>(xchunks);
+    void* args[] = {&n, &W, &ldw, &Inv, &lower, &y, &x, &xc, &epoch, &err};
     ELX_CHECK(launch_resident(reinterpret_cast<const void*>(&trsv_kernel), G, 256, args, 0, s));
     return ELX_SUCCESS;
 }
