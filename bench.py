@@ -85,7 +85,7 @@ def prof_read(lib):
 def roofline_of(cls, rec, factor):
     t = rec["ms"] * 1e-3
     if cls in ("lpgemm",):
-        peak = PEAKS["fp16_TFs" if factor == "fp16" else "bf16_TFs"]
+        peak = PEAKS["bf16_TFs" if factor == "bf16" else "fp16_TFs"]  # fp32: fp16 MFMA products
         ach = rec["work"] / t / 1e12
         return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 5)}
@@ -408,7 +408,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--kappa", type=float, default=1e2)
-    ap.add_argument("--factor", default="fp16", choices=["fp16", "bf16"])
+    ap.add_argument("--factor", default="fp16", choices=["fp16", "bf16", "fp32"])
     ap.add_argument("--refine", default="auto", choices=["ir", "gmres", "auto"])
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--plain-steps", type=int, default=1)

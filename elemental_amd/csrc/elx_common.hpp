@@ -167,10 +167,11 @@ elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i
 elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, float alpha,
                                   const uint16_t* A, i64 lda, const uint16_t* B, i64 ldb,
                                   float* C, i64 ldc, const int* ea_dev, const int* eb_dev);
-// H(r,k) = cvt(sign * 2^e * S(r,k)), S(r,k) = S[r + k*lds] (trans false) or S[k + r*lds].
+// H(r,k) = cvt(sign * 2^e * S(r,k)), S(r,k) = S[r + k*lds] (trans false) or S[k + r*lds];
+// lo (fp16 only): the low part cvt((v - float(cvt(v))) * 2^11) of v = sign * 2^e * S(r,k).
 elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
                          const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
-                         i64 ldh);
+                         i64 ldh, bool lo = false);
 // e (device) with max|S| * 2^e < 2^15; bits points at a zeroed unsigned.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
                                    unsigned* bits, int* e_dev);

@@ -168,15 +168,25 @@ __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float a
 
 // ---------------------------------------------------------------------------------------
 // Pack: H(r, k) = cvt(sign * 2^e * S(r, k)) for r < R, k < K; zeros for K <= k < Kpad.
+// LO (fp16 only): the low part instead, cvt((v - float(cvt(v))) * 2^11) with v = sign * 2^e *
+// S(r, k); the difference is exact in fp32, so hi + 2^-11 lo carries v to about 22 bits and a
+// product of two such pairs (hi*hi + hi*lo + lo*hi) is FP32-accurate.
 // S(r, k) = S[r + k*lds] (column-major, r fastest: TRANS = false) or S[k + r*lds]
 // (TRANS = true). The column-major case goes through a 64x64 LDS transpose so both the reads
 // and the K-major writes are coalesced.
 // ---------------------------------------------------------------------------------------
-template <int PREC, bool TRANS>
+template <int PREC, bool TRANS, bool LO>
+__device__ __forceinline__ uint16_t pack_one(float v) {
+    typedef typename HalfT<PREC>::T HT;
+    HT h = (HT)v;
+    if constexpr (LO) h = (HT)((v - (float)h) * 2048.0f);
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+template <int PREC, bool TRANS, bool LO>
 __global__ __launch_bounds__(256) void pack_kernel(i64 R, i64 K, i64 Kpad, const float* __restrict__ S,
                                                    i64 lds, float sign, const int* e_dev,
                                                    uint16_t* __restrict__ H, i64 ldh) {
-    typedef typename HalfT<PREC>::T HT;
     __shared__ float tile[64][65];
     const float sc = sign * __builtin_ldexpf(1.0f, e_dev ? *e_dev : 0);
     const i64 r0 = (i64)blockIdx.x * 64, k0 = (i64)blockIdx.y * 64;
@@ -192,10 +202,7 @@ __global__ __launch_bounds__(256) void pack_kernel(i64 R, i64 K, i64 Kpad, const
         for (int idx = t; idx < 64 * 64; idx += 256) {
             const int kk = idx & 63, rr = idx >> 6;
             const i64 r = r0 + rr, k = k0 + kk;
-            if (r < R && k < Kpad) {
-                const HT h = (HT)(sc * tile[kk][rr]);
-                H[r * ldh + k] = __builtin_bit_cast(uint16_t, h);
-            }
+            if (r < R && k < Kpad) H[r * ldh + k] = pack_one<PREC, TRANS, LO>(sc * tile[kk][rr]);
         }
     } else {
         for (int idx = t; idx < 64 * 64; idx += 256) {
@@ -203,8 +210,7 @@ __global__ __launch_bounds__(256) void pack_kernel(i64 R, i64 K, i64 Kpad, const
             const i64 r = r0 + rr, k = k0 + kk;
             if (r < R && k < Kpad) {
                 const float v = (k < K) ? S[k + r * lds] : 0.0f;
-                const HT h = (HT)(sc * v);
-                H[r * ldh + k] = __builtin_bit_cast(uint16_t, h);
+                H[r * ldh + k] = pack_one<PREC, TRANS, LO>(sc * v);
             }
         }
     }
@@ -270,23 +276,28 @@ elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, 
 
 elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
                          const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
-                         i64 ldh) {
+                         i64 ldh, bool lo) {
     if (R <= 0 || Kpad <= 0) return ELX_SUCCESS;
-    ELX_ARG(ldh >= Kpad);
+    ELX_ARG(ldh >= Kpad && (prec == 0 || !lo));
     dim3 grid(cdiv(R, 64), cdiv(Kpad, 64));
     ELX_ARG(grid.y <= 65535);
     ProfScope prof_(s, kKPack, (double)R * (double)Kpad * 6.0);
+#define ELX_PACK(P, TR, LO_)                                                                   \
+    hipLaunchKernelGGL((pack_kernel<P, TR, LO_>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, \
+                       e_dev, H, ldh)
     if (prec == 0) {
-        if (trans)
-            hipLaunchKernelGGL((pack_kernel<0, true>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
-        else
-            hipLaunchKernelGGL((pack_kernel<0, false>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
+        if (lo) {
+            if (trans) ELX_PACK(0, true, true);
+            else ELX_PACK(0, false, true);
+        } else {
+            if (trans) ELX_PACK(0, true, false);
+            else ELX_PACK(0, false, false);
+        }
     } else {
-        if (trans)
-            hipLaunchKernelGGL((pack_kernel<1, true>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
-        else
-            hipLaunchKernelGGL((pack_kernel<1, false>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, e_dev, H, ldh);
+        if (trans) ELX_PACK(1, true, false);
+        else ELX_PACK(1, false, false);
     }
+#undef ELX_PACK
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }

@@ -25,7 +25,7 @@ namespace {
 enum {
     kSlotW = 10, kSlotLh, kSlotUh, kSlotLinv, kSlotUinv, kSlotIdx, kSlotVec, kSlotPart,
     kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP,
-    kSlotF64, kSlotF64Idx, kSlotSym, kSlotDiag
+    kSlotF64, kSlotF64Idx, kSlotSym, kSlotDiag, kSlotLl, kSlotUl
 };
 
 // Uncached buffers of the persistent triangular solves (never freed before the process
@@ -84,12 +84,14 @@ inline i64 round_up(i64 a, i64 b) { return ((a + b - 1) / b) * b; }
 
 struct Ctx {
     hipStream_t s;
-    int prec;  // 0 fp16, 1 bf16
+    int prec;  // 0 fp16, 1 bf16, 2 FP32-accurate (fp16 high and low parts, three products)
     i64 n, NB;
     float* W;
     i64 ldw;
     uint16_t* Lh;
     uint16_t* Uh;
+    uint16_t* Ll;  // low parts (prec 2)
+    uint16_t* Ul;
     float* Linv;
     float* Uinv;
     int32_t* dests;
@@ -104,21 +106,33 @@ struct Ctx {
 // W[j + i*ld]): L (m x k) at Lt with L(i, kk) at Lt[kk + i*ld], U (k x n) at Ut with U(kk, j) at
 // Ut[j + kk*ld], C (m x n) at Ct with C(i, j) at Ct[j + i*ld]. Stored this way C is C^T in
 // column-major order, so the update runs as C^T -= U^T L^T with the operands rounded to
-// fp16/bf16 (U scaled by a power of two for fp16).
+// fp16/bf16 (U scaled by a power of two for fp16). FP32-accurate form: both operands split
+// into fp16 high and low parts, C^T -= Uh^T Lh^T + 2^-11 (Uh^T Ll^T + Ul^T Lh^T), the small
+// products first.
 elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float* Ut, float* Ct,
                        i64 ld, i64 ldpack) {
     if (m <= 0 || n <= 0 || k <= 0) return ELX_SUCCESS;
     const i64 kpad = round_up(k, 32);
+    const bool split = (c.prec == 2);
+    const int hp = split ? 0 : c.prec;  // element type of the MFMA operands
     const int* e = nullptr;
-    if (c.prec == 0) {
+    if (hp == 0) {
         ELX_HIP(hipMemsetAsync(c.bits, 0, sizeof(unsigned), c.s));
         ELX_CHECK(launch_scale_exponent(c.s, n, k, Ut, ld, c.bits, c.e_dev));
         e = c.e_dev;
     }
-    ELX_CHECK(launch_pack(c.s, c.prec, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Lh, ldpack));
-    ELX_CHECK(launch_pack(c.s, c.prec, false, n, k, kpad, Ut, ld, 1.0f, e, c.Uh, ldpack));
-    return launch_lpgemm_packed(c.s, c.prec, n, m, k, -1.0f, c.Uh, ldpack, c.Lh, ldpack, Ct, ld,
-                                e, nullptr);
+    ELX_CHECK(launch_pack(c.s, hp, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Lh, ldpack));
+    ELX_CHECK(launch_pack(c.s, hp, false, n, k, kpad, Ut, ld, 1.0f, e, c.Uh, ldpack));
+    if (split) {
+        ELX_CHECK(launch_pack(c.s, 0, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Ll, ldpack, true));
+        ELX_CHECK(launch_pack(c.s, 0, false, n, k, kpad, Ut, ld, 1.0f, e, c.Ul, ldpack, true));
+        ELX_CHECK(launch_lpgemm_packed(c.s, 0, n, m, k, -1.0f / 2048.0f, c.Uh, ldpack, c.Ll,
+                                       ldpack, Ct, ld, e, nullptr));
+        ELX_CHECK(launch_lpgemm_packed(c.s, 0, n, m, k, -1.0f / 2048.0f, c.Ul, ldpack, c.Lh,
+                                       ldpack, Ct, ld, e, nullptr));
+    }
+    return launch_lpgemm_packed(c.s, hp, n, m, k, -1.0f, c.Uh, ldpack, c.Lh, ldpack, Ct, ld, e,
+                                nullptr);
 }
 
 // Two-level blocked LU with partial pivoting of W (fp32, A stored row by row, so a row
@@ -196,13 +210,23 @@ elx_status_t potrf_mixed(Ctx& c) {
         ELX_CHECK(launch_trsm_small_f32(c.s, kRight, kLower, kAdjoint, kNonUnit, n2, kb,
                                         W + k0 + k0 * ld, ld, L21, ld));
         const int* e = nullptr;
-        if (c.prec == 0) {
+        const bool split = (c.prec == 2);
+        const int hp = split ? 0 : c.prec;
+        if (hp == 0) {
             ELX_HIP(hipMemsetAsync(c.bits, 0, sizeof(unsigned), c.s));
             ELX_CHECK(launch_scale_exponent(c.s, n2, kb, L21, ld, c.bits, c.e_dev));
             e = c.e_dev;
         }
-        ELX_CHECK(launch_pack(c.s, c.prec, false, n2, kb, 128, L21, ld, 1.0f, e, c.Lh, 128));
-        ELX_CHECK(launch_lpgemm_packed(c.s, c.prec, n2, n2, kb, -1.0f, c.Lh, 128, c.Lh, 128,
+        ELX_CHECK(launch_pack(c.s, hp, false, n2, kb, 128, L21, ld, 1.0f, e, c.Lh, 128));
+        if (split) {
+            float* C22 = W + (k0 + kb) + (k0 + kb) * ld;
+            ELX_CHECK(launch_pack(c.s, 0, false, n2, kb, 128, L21, ld, 1.0f, e, c.Ll, 128, true));
+            ELX_CHECK(launch_lpgemm_packed(c.s, 0, n2, n2, kb, -1.0f / 2048.0f, c.Lh, 128, c.Ll,
+                                           128, C22, ld, e, e));
+            ELX_CHECK(launch_lpgemm_packed(c.s, 0, n2, n2, kb, -1.0f / 2048.0f, c.Ll, 128, c.Lh,
+                                           128, C22, ld, e, e));
+        }
+        ELX_CHECK(launch_lpgemm_packed(c.s, hp, n2, n2, kb, -1.0f, c.Lh, 128, c.Lh, 128,
                                        W + (k0 + kb) + (k0 + kb) * ld, ld, e, e));
     }
     return ELX_SUCCESS;
@@ -253,7 +277,8 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
                                 int32_t max_iters, elx_ir_stats_t* stats) {
     ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) &&
             ldb >= std::max<int64_t>(n, 1) && ldx >= std::max<int64_t>(n, 1));
-    ELX_ARG(factor_type == ELX_FACTOR_FP16 || factor_type == ELX_FACTOR_BF16);
+    ELX_ARG(factor_type == ELX_FACTOR_FP16 || factor_type == ELX_FACTOR_BF16 ||
+            factor_type == ELX_FACTOR_FP32);
     ELX_ARG(refine >= ELX_REFINE_NONE && refine <= ELX_REFINE_AUTO);
     elx_ir_stats_t st;
     std::memset(&st, 0, sizeof(st));
@@ -269,7 +294,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
 
     Ctx c;
     c.s = s;
-    c.prec = (factor_type == ELX_FACTOR_FP16) ? 0 : 1;
+    c.prec = (factor_type == ELX_FACTOR_FP16) ? 0 : (factor_type == ELX_FACTOR_BF16) ? 1 : 2;
     c.n = n;
     c.NB = std::min<i64>(1024, round_up(n, 128));
     c.ldw = round_up(n, 128);  // whole 128-row blocks and 16-byte row slices for the solves
@@ -279,6 +304,11 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(get(kSlotW, (size_t)c.ldw * (size_t)n, &c.W));
     ELX_CHECK(get(kSlotLh, (size_t)n * (size_t)nbpad, &c.Lh));
     ELX_CHECK(get(kSlotUh, (size_t)n * (size_t)nbpad, &c.Uh));
+    c.Ll = c.Ul = nullptr;
+    if (c.prec == 2) {
+        ELX_CHECK(get(kSlotLl, (size_t)n * (size_t)nbpad, &c.Ll));
+        ELX_CHECK(get(kSlotUl, (size_t)n * (size_t)nbpad, &c.Ul));
+    }
     ELX_CHECK(get(kSlotLinv, (size_t)nblk * 128 * 128, &c.Linv));
     ELX_CHECK(get(kSlotUinv, (size_t)nblk * 128 * 128, &c.Uinv));
     int32_t* idx = nullptr;
@@ -428,6 +458,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     bool all_ok = true;
     double worst = 0.0;
     int inner_total = 0;
+    bool escalate = false;
     for (i64 col = 0; col < nrhs; ++col) {
         const double* b = dB + col * ldb;
         double* x = dX + col * ldx;
@@ -453,7 +484,12 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
         // include/El/lapack_like/solve/FGMRES.hpp:37-291): z_j = M^{-1} v_j, w = A z_j,
         // orthogonalization (classical Gram-Schmidt applied twice), Givens rotations on H,
         // stop when the rotated residual |t_{j+1}| <= relTol * ||r||. d = Z y on exit.
-        auto fgmres = [&](const double* rr, double* d, double relTol, int* iters) -> elx_status_t {
+        // *slow (AUTO with FP16/BF16 factors): the residual reduction of the first 10 steps
+        // projects more than 80 steps for relTol, more than the FP32-accurate refactorization
+        // costs (about 120 steps at n = 32768): the cycle stops and the caller escalates.
+        auto fgmres = [&](const double* rr, double* d, double relTol, int* iters,
+                          bool* slow) -> elx_status_t {
+            *slow = false;
             double* w = gw;
             double* h = gw + 2 * n;
             double beta = 0.0;
@@ -501,6 +537,20 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
                 t[j + 1] = -sn[j] * t0 + cs[j] * t1;
                 jend = j + 1;
                 if (std::fabs(t[j + 1]) <= relTol * beta || delta == 0.0) break;
+                // AUTO: a cycle that has not halved the residual in 20 steps is not worth
+                // finishing; the caller sees the stall and escalates
+                if (refine == ELX_REFINE_AUTO && jend >= 20 && std::fabs(t[j + 1]) > 0.5 * beta)
+                    break;
+                if (refine == ELX_REFINE_AUTO && c.prec != 2 && jend == 10) {
+                    const double red = std::fabs(t[j + 1]) / beta;
+                    const double need = (red > 0.0 && red < 1.0)
+                                            ? 10.0 * std::log(relTol) / std::log(red)
+                                            : INFINITY;
+                    if (need > 80.0) {
+                        *slow = true;
+                        break;
+                    }
+                }
             }
             // y = H^{-1} t (upper triangular), d = Z y
             std::vector<double> yv(jend);
@@ -544,15 +594,25 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
                 // aim to reach tol in this outer step
                 const double rel = std::min(1e-2, std::max(1e-13, 0.1 * tol / eta));
                 int inner = 0;
+                bool slow = false;
                 double* d = gw + n;
-                ELX_CHECK(fgmres(r, d, rel, &inner));
+                ELX_CHECK(fgmres(r, d, rel, &inner, &slow));
                 inner_total += inner;
+                if (slow) {
+                    ++it;
+                    break;  // escalate (not ok)
+                }
                 ELX_CHECK(launch_axpy(s, n, 1.0, d, x));
             }
             ++it;
         }
         st.outer_iters = std::max(st.outer_iters, it);
         if (!ok && refine == ELX_REFINE_AUTO) {
+            if (c.prec != 2) {
+                // too inaccurate a preconditioner for this matrix: refactor FP32-accurately
+                escalate = true;
+                break;
+            }
             ELX_CHECK(fp64_solve(b, x, &eta, &ok));
             st.fell_back = 1;
         }
@@ -561,6 +621,18 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     }
     st.inner_iters = inner_total;
     st.refine_seconds = tr.stop();
+    if (escalate) {
+        elx_ir_stats_t s2;
+        const elx_status_t r2 = mixed_solve(kind, stream, n, nrhs, dA, lda, dB, ldb, dX, ldx,
+                                            ELX_FACTOR_FP32, refine, tol, max_iters, &s2);
+        s2.factor_seconds += st.factor_seconds;
+        s2.refine_seconds += st.refine_seconds;
+        s2.inner_iters += st.inner_iters;
+        s2.outer_iters += st.outer_iters;
+        s2.fell_back = s2.fell_back ? 1 : 2;
+        if (stats) *stats = s2;
+        return r2;
+    }
     int e_h = 0;
     ELX_CHECK(d2h(&e_h, err, 1, s));
     st.converged = all_ok ? 1 : 0;

@@ -122,7 +122,8 @@ elx_status_t elx_dpermute_rows(void* stream, int inverse, const int32_t* dDests,
  * ---------------------------------------------------------------------------------- */
 typedef enum {
     ELX_FACTOR_FP64 = 0,
-    ELX_FACTOR_FP32 = 1,
+    ELX_FACTOR_FP32 = 1,  /* FP32-accurate trailing updates: each operand split into fp16 high
+                             and low parts, three fp16 MFMA products (hi*hi + hi*lo + lo*hi) */
     ELX_FACTOR_FP16 = 2,
     ELX_FACTOR_BF16 = 3
 } elx_factor_type_t;
@@ -131,16 +132,19 @@ typedef enum {
     ELX_REFINE_NONE = 0,
     ELX_REFINE_IR = 1,      /* classical iterative refinement */
     ELX_REFINE_GMRES = 2,   /* GMRES-IR: each correction by GMRES preconditioned with the LU */
-    ELX_REFINE_AUTO = 3     /* IR, switching to GMRES-IR when IR stagnates, and to the FP64
-                               LU (elx_dgetrf of a copy of A) when GMRES-IR stalls or the
-                               low-precision factorization meets an exactly zero pivot */
+    ELX_REFINE_AUTO = 3     /* IR, switching to GMRES-IR when IR stagnates; when GMRES-IR
+                               stalls, an FP16/BF16 factorization is redone FP32-accurately
+                               (ELX_FACTOR_FP32) and refined again, and the FP64 LU
+                               (elx_dgetrf of a copy of A) is the last resort; an exactly zero
+                               pivot in the low-precision factors goes to the FP64 LU */
 } elx_refine_t;
 
 typedef struct {
     int32_t converged;        /* 1 when every column met the tolerance */
     int32_t outer_iters;      /* refinement steps (residual evaluations after the first) */
     int32_t inner_iters;      /* total GMRES iterations */
-    int32_t fell_back;        /* 1 when the FP64 LU fallback was used */
+    int32_t fell_back;        /* 1: the FP64 LU fallback was used; 2: refactored FP32-accurately
+                                 (ELX_REFINE_AUTO) and refined to tol */
     int32_t factor_info;      /* info of the low-precision factorization */
     int32_t near_tie_pivots;  /* unused by the mixed path (0) */
     double backward_error;    /* max over columns of ||b-Ax||inf/(||A||inf||x||inf+||b||inf) */
