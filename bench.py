@@ -104,7 +104,7 @@ def roofline_of(cls, rec, factor):
 CLASS_KERNELS = {
     "lu_panel": (("void (anonymous namespace)::lu_panel_kernel<float>",), ()),
     "lpgemm": (("void (anonymous namespace)::lpgemm_kernel",), ()),
-    "row_moves": (("void (anonymous namespace)::row_moves_kernel<float>",), ()),
+    "row_moves": (("(anonymous namespace)::row_moves_rowmajor_kernel",), ()),
     "trsm_small": (("void (anonymous namespace)::trsm_small_kernel<float>",
                     "(anonymous namespace)::trmm_inv_kernel"), ()),
     "trsv": (("(anonymous namespace)::trsv_kernel",), ()),
