@@ -172,9 +172,9 @@ elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, 
 elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
                          const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
                          i64 ldh, bool lo = false);
-// e (device) with max|S| * 2^e < 2^15; bits points at a zeroed unsigned.
+// e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
-                                   unsigned* bits, int* e_dev);
+                                   unsigned* ctl, int* e_dev);
 
 // Unblocked Cholesky of the diagonal block (n <= 128) in one workgroup.
 elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,

@@ -216,30 +216,8 @@ __global__ __launch_bounds__(256) void pack_kernel(i64 R, i64 K, i64 Kpad, const
     }
 }
 
-// max |S(i,j)| over an m x n column-major block, accumulated into *out (as uint bits of a
-// non-negative float; *out must start at 0).
-__global__ __launch_bounds__(256) void absmax_kernel(i64 m, i64 n, const float* __restrict__ S,
-                                                     i64 lds, unsigned* out) {
-    float mx = 0.0f;
-    const i64 total = m * n;
-    for (i64 idx = (i64)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (i64)gridDim.x * 256) {
-        const i64 i = idx % m, j = idx / m;
-        mx = fmaxf(mx, fabsf(S[i + j * lds]));
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    __shared__ float w[4];
-    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        mx = fmaxf(fmaxf(w[0], w[1]), fmaxf(w[2], w[3]));
-        atomicMax(out, __float_as_uint(mx));
-    }
-}
-
 // e such that max * 2^e < 2^15 (fp16 headroom for K-term fp32 accumulation), clamped.
-__global__ void scale_exp_kernel(const unsigned* mx_bits, int* e_out) {
-    const float mx = __uint_as_float(*mx_bits);
+__device__ __forceinline__ int scale_exp_of(float mx) {
     int e = 0;
     if (mx > 0.0f && isfinite(mx)) {
         int p;
@@ -247,7 +225,38 @@ __global__ void scale_exp_kernel(const unsigned* mx_bits, int* e_out) {
         e = 15 - p;
         e = max(-100, min(100, e));
     }
-    *e_out = e;
+    return e;
+}
+
+// e_out = scale_exp_of(max |S(i,j)|) over an m x n column-major block. Workgroups take whole
+// columns (rows walked by the 256 threads, no index division) and fold their maxima into
+// ctl[0] (as the bits of a non-negative float); the last workgroup to finish (ctl[1] counts)
+// turns the maximum into the exponent. ctl[0..1] must start at zero.
+__global__ __launch_bounds__(256) void absmax_exp_kernel(i64 m, i64 n, const float* __restrict__ S,
+                                                         i64 lds, unsigned* ctl, int* e_out) {
+    float mx = 0.0f;
+    for (i64 j = blockIdx.x; j < n; j += gridDim.x) {
+        const float* col = S + j * lds;
+        for (i64 i = threadIdx.x; i < m; i += 256) mx = fmaxf(mx, fabsf(col[i]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    __shared__ float w[4];
+    __shared__ bool last;
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = fmaxf(fmaxf(w[0], w[1]), fmaxf(w[2], w[3]));
+        atomicMax(ctl, __float_as_uint(mx));
+        __threadfence();
+        last = (atomicAdd(ctl + 1, 1u) == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        const unsigned bits = __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *e_out = scale_exp_of(__uint_as_float(bits));
+    }
 }
 
 }  // namespace
@@ -302,15 +311,12 @@ elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 
     return ELX_SUCCESS;
 }
 
-// Exponent e (device int) with max|S| * 2^e < 2^15; bits must point at a zeroed unsigned.
+// Exponent e (device int) with max|S| * 2^e < 2^15; ctl must point at two zeroed unsigneds.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
-                                   unsigned* bits, int* e_dev) {
-    if (m > 0 && n > 0) {
-        const unsigned blocks = (unsigned)std::min<i64>((m * n + 255) / 256, 2048);
-        hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(256), 0, s, m, n, S, lds, bits);
-        ELX_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(scale_exp_kernel, dim3(1), dim3(1), 0, s, bits, e_dev);
+                                   unsigned* ctl, int* e_dev) {
+    const unsigned blocks = (unsigned)std::max<i64>(1, std::min<i64>(n, 2048));
+    hipLaunchKernelGGL(absmax_exp_kernel, dim3(blocks), dim3(256), 0, s, std::max<i64>(m, 0),
+                       std::max<i64>(n, 0), S, lds, ctl, e_dev);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
