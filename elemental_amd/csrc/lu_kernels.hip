@@ -175,24 +175,58 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
     }
     __syncthreads();
     auto M = [&](int i, int j) -> T { return Ms[i + j * m]; };
-    if (lower_eff) {
-        for (int j = 0; j < m; ++j) {
-            T xj = x[v * m + j];
-            if (!unit) xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
-            __builtin_amdgcn_wave_barrier();
-            if (!unit && q == 0) x[v * m + j] = xj;
-            for (int i = j + 1 + q; i < m; i += TP)
-                x[v * m + i] = __builtin_fma(-M(i, j), xj, x[v * m + i]);
-            __builtin_amdgcn_wave_barrier();
+    // Column-oriented substitutions keep each vector in the registers of its 16 lanes (lane q
+    // holds entries q, q+16, ...); x_j reaches the other lanes by one shuffle per step. Every
+    // entry still receives its updates one FMA at a time in step order.
+    constexpr int KR = 128 / TP;  // register slots per lane (order <= 128)
+    if (lower_eff || !mtrans) {
+        T xr[KR];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int i = q + TP * k;
+            xr[k] = (i < m) ? x[v * m + i] : T(0);
         }
-    } else if (!mtrans) {
-        for (int j = m - 1; j >= 0; --j) {
-            T xj = x[v * m + j];
-            if (!unit) xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
-            __builtin_amdgcn_wave_barrier();
-            if (!unit && q == 0) x[v * m + j] = xj;
-            for (int i = q; i < j; i += TP) x[v * m + i] = __builtin_fma(-M(i, j), xj, x[v * m + i]);
-            __builtin_amdgcn_wave_barrier();
+        if (lower_eff) {
+#pragma unroll
+            for (int kb = 0; kb < KR; ++kb) {
+                for (int jj = 0; jj < TP; ++jj) {
+                    const int j = kb * TP + jj;
+                    if (j >= m) break;
+                    T xj = __shfl(xr[kb], jj, TP);
+                    if (!unit) {
+                        xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
+                        if (q == jj) xr[kb] = xj;
+                    }
+#pragma unroll
+                    for (int k = kb; k < KR; ++k) {
+                        const int i = q + TP * k;
+                        if (i > j && i < m) xr[k] = __builtin_fma(-M(i, j), xj, xr[k]);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kb = KR - 1; kb >= 0; --kb) {
+                for (int jj = TP - 1; jj >= 0; --jj) {
+                    const int j = kb * TP + jj;
+                    if (j >= m) continue;
+                    T xj = __shfl(xr[kb], jj, TP);
+                    if (!unit) {
+                        xj = right ? xj * (T(1) / M(j, j)) : xj / M(j, j);
+                        if (q == jj) xr[kb] = xj;
+                    }
+#pragma unroll
+                    for (int k = 0; k <= kb; ++k) {
+                        const int i = q + TP * k;
+                        if (i < j) xr[k] = __builtin_fma(-M(i, j), xj, xr[k]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int i = q + TP * k;
+            if (i < m) x[v * m + i] = xr[k];
         }
     } else if (q == 0) {
         for (int i = m - 1; i >= 0; --i) {
