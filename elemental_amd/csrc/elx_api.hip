@@ -189,6 +189,7 @@ elx_status_t release_all() {
     for (auto& kv : g_scratch)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     g_scratch.clear();
+    library_gemm_release();
     // the panel exchange areas (uncached memory) stay allocated until the process ends
     return ELX_SUCCESS;
 }

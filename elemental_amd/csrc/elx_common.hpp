@@ -172,6 +172,16 @@ elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, 
 elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
                          const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
                          i64 ldh, bool lo = false);
+// C(m x n, column-major) += alpha * A B^T with A (m x k) and B (n x k) fp16 (prec 0) or
+// bf16 (prec 1) packs (row r at A + r*lda), FP32 C, as a rocBLAS gemm_ex call; alpha_dev and
+// beta_dev are device floats.
+elx_status_t library_gemm_lowp(hipStream_t s, int prec, i64 m, i64 n, i64 k, const float* alpha_dev,
+                               const uint16_t* A, i64 lda, const uint16_t* B, i64 ldb,
+                               const float* beta_dev, float* C, i64 ldc);
+// out[0] = -2^-(mul * e), out[1] = out[0] * 2^-11, out[2] = 1 on the device (e = 0 when
+// e_dev is null): the alphas of the library GEMM for the packed, power-of-two scaled operands.
+elx_status_t launch_gemm_alphas(hipStream_t s, const int* e_dev, int mul, float* out);
+void library_gemm_release();
 // e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
                                    unsigned* ctl, int* e_dev);

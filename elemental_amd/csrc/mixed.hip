@@ -100,6 +100,7 @@ struct Ctx {
     int32_t* info;
     int* e_dev;
     unsigned* bits;
+    float* alphas;  // device alphas of the library GEMM (3 floats)
 };
 
 // C -= L * U for blocks of the fp32 working matrix, which holds A row by row (A(i, j) at
@@ -126,6 +127,20 @@ elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float
     if (split) {
         ELX_CHECK(launch_pack(c.s, 0, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Ll, ldpack, true));
         ELX_CHECK(launch_pack(c.s, 0, false, n, k, kpad, Ut, ld, 1.0f, e, c.Ul, ldpack, true));
+    }
+    if (k >= 512) {
+        // the rank-NB update: a plain library GEMM (rocBLAS), same packs and scaling
+        ELX_CHECK(launch_gemm_alphas(c.s, e, 1, c.alphas));
+        if (split) {
+            ELX_CHECK(library_gemm_lowp(c.s, 0, n, m, kpad, c.alphas + 1, c.Uh, ldpack, c.Ll,
+                                        ldpack, c.alphas + 2, Ct, ld));
+            ELX_CHECK(library_gemm_lowp(c.s, 0, n, m, kpad, c.alphas + 1, c.Ul, ldpack, c.Lh,
+                                        ldpack, c.alphas + 2, Ct, ld));
+        }
+        return library_gemm_lowp(c.s, hp, n, m, kpad, c.alphas, c.Uh, ldpack, c.Lh, ldpack,
+                                 c.alphas + 2, Ct, ld);
+    }
+    if (split) {
         ELX_CHECK(launch_lpgemm_packed(c.s, 0, n, m, k, -1.0f / 2048.0f, c.Uh, ldpack, c.Ll,
                                        ldpack, Ct, ld, e, nullptr));
         ELX_CHECK(launch_lpgemm_packed(c.s, 0, n, m, k, -1.0f / 2048.0f, c.Ul, ldpack, c.Lh,
@@ -318,6 +333,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     c.info = idx + 2 * n;
     c.e_dev = idx + 2 * n + 1;
     c.bits = reinterpret_cast<unsigned*>(idx + 2 * n + 2);
+    c.alphas = reinterpret_cast<float*>(idx + 2 * n + 4);
     c.pairs = reinterpret_cast<int2*>(idx + 2 * n + 8);
     double* vec = nullptr;  // r, norms(4), anorm
     ELX_CHECK(get(kSlotVec, (size_t)(n + 8), &vec));
