@@ -181,6 +181,13 @@ elx_status_t library_gemm_lowp(hipStream_t s, int prec, i64 m, i64 n, i64 k, con
 // out[0] = -2^-(mul * e), out[1] = out[0] * 2^-11, out[2] = 1 on the device (e = 0 when
 // e_dev is null): the alphas of the library GEMM for the packed, power-of-two scaled operands.
 elx_status_t launch_gemm_alphas(hipStream_t s, const int* e_dev, int mul, float* out);
+// FP64 C = alpha op(A) op(B) + beta C (rocblas_dgemm) and the symmetric rank-k update of one
+// triangle of C (rocblas_dsyrk: trans false -> A A^T, true -> A^T A), host alpha and beta.
+elx_status_t library_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
+                           const double* A, i64 lda, const double* B, i64 ldb, double beta,
+                           double* C, i64 ldc);
+elx_status_t library_dsyrk(hipStream_t s, bool upper, bool trans, i64 n, i64 k, double alpha,
+                           const double* A, i64 lda, double beta, double* C, i64 ldc);
 void library_gemm_release();
 // e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,

@@ -193,6 +193,16 @@ elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, 
         ELX_HIP(hipGetLastError());
         return ELX_SUCCESS;
     }
+    // Large products are plain library GEMMs: rocBLAS runs the FP64 rank-128 trailing updates
+    // of the LU at 46-47 TFLOP/s against 34-35 for dgemm_kernel (m = n = 8192..32640); the
+    // symmetric rank-k update of the Cholesky (A == B, one operand transposed, one triangle
+    // of C) goes to dsyrk.
+    if ((double)m * (double)n * (double)k >= (double)(1 << 24)) {
+        ProfScope prof_(s, kKDgemm, (tri ? 1.0 : 2.0) * (double)m * (double)n * (double)k);
+        if (tri == 0) return library_dgemm(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (A == B && lda == ldb && m == n && ta != tb && (tri == 1) == tb)
+            return library_dsyrk(s, tri == 2, ta, n, k, alpha, A, lda, beta, C, ldc);
+    }
     const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     ELX_ARG(tm * tn < (i64)1 << 30);
     ProfScope prof_(s, kKDgemm, (tri ? 1.0 : 2.0) * (double)m * (double)n * (double)k);
