@@ -61,42 +61,111 @@ __global__ __launch_bounds__(256) void convert_kernel(i64 n, const double* __res
 
 // Inverse of the 128 x 128 diagonal blocks of a unit-lower (upper == 0) or non-unit upper
 // (upper == 1) triangular factor stored row by row in W (element (i, k) at W[k + i*ldw]).
-// Block b's inverse goes to Inv + b*128*128 (column-major, ld 128). One workgroup per block;
-// thread j forms column j.
-__global__ __launch_bounds__(128) void diag_inverse_kernel(i64 n, const float* __restrict__ W,
+// Block b's inverse goes to Inv + b*128*128 (column-major, ld 128; rows and columns >= nb
+// zero). One workgroup per block, blocked by 32: the block is loaded as a lower triangle L
+// (U^T for the upper factor, whose inverse is then the transpose) stored by columns, each
+// wave inverts one 32 x 32 diagonal block by right-looking substitution (lane j holds column
+// j in registers; step k reads column k of L as 16-byte vectors and updates every later
+// entry), and the off-diagonal blocks follow level by level (distance d from the diagonal, all
+// blocks of a level at once): X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj).
+__global__ __launch_bounds__(256) void diag_inverse_kernel(i64 n, const float* __restrict__ W,
                                                            i64 ldw, int upper,
                                                            float* __restrict__ Inv, i64 b0) {
     extern __shared__ __align__(16) float sm[];
-    constexpr int TS = 129;     // T's column stride: the transposing stores hit distinct banks
-    float* T = sm;              // 128 x 128, T[i + k*TS]
-    float* X = sm + 128 * TS;   // X[k*128 + j]
+    constexpr int LS = 132;       // column stride of L (16-byte aligned columns)
+    constexpr int XS = 129;       // row stride of X
+    float* L = sm;                // L(i, k) at L[k*LS + i]
+    float* X = sm + 128 * LS;     // X(i, k) at X[i*XS + k]
+    float* S = X + 128 * XS;      // [3][32][33] block sums of one level
     const i64 b = b0 + blockIdx.x;
     const i64 r0 = b * 128;
     const int nb = (int)std::min<i64>(128, n - r0);
-    const int j = threadIdx.x;
-    for (int idx = j; idx < 128 * 128; idx += 128) {
-        const int kk = idx & 127, ii = idx >> 7;
-        T[ii + kk * TS] = (ii < nb && kk < nb) ? W[(r0 + kk) + (r0 + ii) * ldw] : 0.0f;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < 128 * 128; idx += 256) {
+        int i, k;
+        if (!upper) {
+            k = idx & 127;  // W[(r0+k) + (r0+i)*ldw]: contiguous in k
+            i = idx >> 7;
+        } else {
+            i = idx & 127;  // W[(r0+i) + (r0+k)*ldw]: contiguous in i
+            k = idx >> 7;
+        }
+        float v = 0.0f;
+        if (i < nb && k < nb) {
+            if (i == k) v = upper ? W[(r0 + i) + (r0 + i) * ldw] : 1.0f;
+            else if (k < i) v = upper ? W[(r0 + i) + (r0 + k) * ldw] : W[(r0 + k) + (r0 + i) * ldw];
+        } else if (i == k) {
+            v = 1.0f;  // identity beyond nb
+        }
+        L[k * LS + i] = v;
+        X[i * XS + k] = 0.0f;
     }
     __syncthreads();
-    if (!upper) {
-        for (int i = 0; i < 128; ++i) {
-            float s = (i == j) ? 1.0f : 0.0f;
-            for (int k = 0; k < i; ++k) s = __builtin_fmaf(-T[i + k * TS], X[k * 128 + j], s);
-            X[i * 128 + j] = (i < nb) ? s : 0.0f;
-        }
-    } else {
-        for (int i = 127; i >= 0; --i) {
-            float s = (i == j) ? 1.0f : 0.0f;
-            for (int k = i + 1; k < 128; ++k) s = __builtin_fmaf(-T[i + k * TS], X[k * 128 + j], s);
-            X[i * 128 + j] = (i < nb) ? s / T[i + i * TS] : 0.0f;
+    {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        const int w = t >> 6, j = t & 63, base = 32 * w;
+        if (j < 32) {
+            float x[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) x[i] = (i == j) ? 1.0f : 0.0f;
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const float* col = L + (base + k) * LS + base;  // L(base.., base+k)
+                float cv[32];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const f4v v4 = *reinterpret_cast<const f4v*>(col + 4 * q);
+                    cv[4 * q] = v4[0];
+                    cv[4 * q + 1] = v4[1];
+                    cv[4 * q + 2] = v4[2];
+                    cv[4 * q + 3] = v4[3];
+                }
+                const float xk = x[k] / cv[k];
+                x[k] = xk;
+#pragma unroll
+                for (int i = k + 1; i < 32; ++i) x[i] = __builtin_fmaf(-cv[i], xk, x[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 32; ++i) X[(base + i) * XS + base + j] = x[i];
         }
     }
     __syncthreads();
+    const int r = t >> 3, c0 = 4 * (t & 7);  // this thread's 4 entries of each 32 x 32 block
+    for (int d = 1; d < 4; ++d) {
+        const int nbl = 4 - d;  // blocks (bj + d, bj), bj = 0 .. nbl-1
+        for (int bj = 0; bj < nbl; ++bj) {
+            const int bi = bj + d;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+            for (int kk = 32 * bj; kk < 32 * bi; ++kk) {
+                const float l = L[kk * LS + 32 * bi + r];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = __builtin_fmaf(l, X[kk * XS + 32 * bj + c0 + e], acc[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) S[(bj * 32 + r) * 33 + c0 + e] = acc[e];
+        }
+        __syncthreads();
+        for (int bj = 0; bj < nbl; ++bj) {
+            const int bi = bj + d;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+            for (int m = 0; m < 32; ++m) {
+                const float xv = X[(32 * bi + r) * XS + 32 * bi + m];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = __builtin_fmaf(xv, S[(bj * 32 + m) * 33 + c0 + e], acc[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) X[(32 * bi + r) * XS + 32 * bj + c0 + e] = -acc[e];
+        }
+        __syncthreads();
+    }
     float* out = Inv + b * 128 * 128;
-    for (int idx = j; idx < 128 * 128; idx += 128) {
-        const int ii = idx & 127, kk = idx >> 7;
-        out[ii + kk * 128] = X[ii * 128 + kk];
+    for (int idx = t; idx < 128 * 128; idx += 256) {
+        const int i = idx & 127, k = idx >> 7;
+        float v = 0.0f;
+        if (i < nb && k < nb) v = upper ? X[k * XS + i] : X[i * XS + k];
+        out[i + k * 128] = v;
     }
 }
 
@@ -572,7 +641,7 @@ elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, 
     const i64 nblk = (count > 0) ? count : (n + 127) / 128 - b0;
     ELX_ARG(b0 >= 0 && b0 + nblk <= (n + 127) / 128);
     if (nblk <= 0) return ELX_SUCCESS;
-    const size_t lds = (128 * 129 + 128 * 128) * sizeof(float);
+    const size_t lds = (128 * 132 + 128 * 129 + 3 * 32 * 33) * sizeof(float);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_inverse_kernel),
@@ -580,7 +649,7 @@ elx_status_t launch_diag_inverse(hipStream_t s, i64 n, const float* W, i64 ldw, 
         (void)hipGetLastError();
         attr = true;
     }
-    hipLaunchKernelGGL(diag_inverse_kernel, dim3((unsigned)nblk), dim3(128), lds, s, n, W, ldw,
+    hipLaunchKernelGGL(diag_inverse_kernel, dim3((unsigned)nblk), dim3(256), lds, s, n, W, ldw,
                        upper, Inv, b0);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
