@@ -255,67 +255,139 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
 }
 
 // ---------------------------------------------------------------------------------------
+// v of lane `src` (uniform) for every lane
+__device__ __forceinline__ float bcast_lane(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double bcast_lane(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Unblocked Cholesky of an n x n (n <= 128) diagonal block in one workgroup, in LDS.
 // Lower: cholesky::LowerVariant3Unblocked (src/lapack_like/factor/Cholesky/LowerVariant3.hpp:
 // 15-41): alpha = sqrt(a_jj) (a_jj <= 0 -> NonHPD), a21 *= 1/alpha, dsyr('L', -1, a21).
 // Upper: UpperVariant3Unblocked (UpperVariant3.hpp:15-46) on the row instead of the column.
 // The dsyr update is a(i,l) = fma(x_i, -x_l, a(i,l)), skipped when x_l == 0, as in the
 // reference BLAS.
+//
+// Schedule: every entry receives exactly those FMAs in column order, so the result is the
+// unblocked algorithm's bit for bit, but the work is blocked by 32 columns: (1) all threads
+// bring a 32-column block column up to date with the columns to its left (one sequential FMA
+// chain per entry), (2) wave 0 factors the block column in registers (lane q holds rows q and
+// q+64 of it; pivots and multipliers reach the other lanes by readlane), so only a few
+// workgroup barriers are needed per 32 columns instead of two per column. The upper case is
+// factored as its transpose; fma(x, -y, a) and fma(y, -x, a) are the same number, so only the
+// skip rule differs (the row's own entry for the upper case). On a non-positive pivot the
+// block columns to the right receive the updates of the finished columns, which leaves the
+// block as the unblocked algorithm does when it stops.
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void potf2_kernel(int n, T* A, i64 lda, int upper, i64 goff,
                                                     int32_t* info) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* S = reinterpret_cast<T*>(smem_raw);  // column-major, leading dimension n+1
-    __shared__ int fail;
-    const int LD = n + 1, t = threadIdx.x;
+    T* S = reinterpret_cast<T*>(smem_raw);  // lower triangle L(r, c) at S[r + c*LD]
+    __shared__ int s_fail;
+    const int LD = n + 1, t = threadIdx.x, wave = t >> 6, lane = t & 63;
     for (int idx = t; idx < n * n; idx += 256) {
         const int i = idx % n, j = idx / n;
-        S[i + j * LD] = A[i + (i64)j * lda];
+        // (r, c) = (i, j) of the lower triangle, read from (j, i) of the upper one
+        if (i >= j) S[i + j * LD] = upper ? A[j + (i64)i * lda] : A[i + (i64)j * lda];
     }
-    if (t == 0) fail = 0;
+    if (t == 0) s_fail = -1;
     __syncthreads();
-    int j = 0;
-    for (; j < n; ++j) {
-        T alpha = S[j + j * LD];
-        if (alpha <= T(0)) {
-            if (t == 0) fail = 1;
+    // (1) entries (r, c) of block column J, c in [c0, c1), r >= c: FMAs of columns [0, jmax)
+    auto left_update = [&](int c0, int c1, int jmax) {
+        const int h = n - c0, w = c1 - c0;
+        for (int idx = t; idx < h * w; idx += 256) {
+            const int r = c0 + idx % h, c = c0 + idx / h;
+            if (r < c) continue;
+            T acc = S[r + c * LD];
+#pragma unroll 4
+            for (int j = 0; j < jmax; ++j) {
+                const T x = S[r + j * LD], y = S[c + j * LD];
+                if ((upper ? x : y) != T(0)) acc = __builtin_fma(x, -y, acc);
+            }
+            S[r + c * LD] = acc;
+        }
+    };
+    int fail = -1;
+    for (int c0 = 0; c0 < n; c0 += 32) {
+        const int c1 = min(n, c0 + 32);
+        if (c0 > 0) {
+            left_update(c0, c1, c0);
+            __syncthreads();
+        }
+        // (2) the block column in wave 0's registers: rows c0 + q and c0 + 64 + q of lane q
+        if (wave == 0) {
+            T rw[2][32];
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int r = c0 + 64 * sl + lane;
+#pragma unroll
+                for (int cc = 0; cc < 32; ++cc)
+                    rw[sl][cc] = (r < n && c0 + cc < c1 && r >= c0 + cc) ? S[r + (c0 + cc) * LD] : T(0);
+            }
+            int f = -1;
+#pragma unroll
+            for (int jj = 0; jj < 32; ++jj) {
+                const int cj = c0 + jj;
+                if (cj < c1 && f < 0) {
+                    T alpha = bcast_lane(rw[0][jj], jj);
+                    if (alpha <= T(0)) {
+                        f = cj;
+                    } else {
+                        alpha = sqrt(alpha);
+                        const T inv = T(1) / alpha;
+#pragma unroll
+                        for (int sl = 0; sl < 2; ++sl) {
+                            const int r = c0 + 64 * sl + lane;
+                            if (r == cj) rw[sl][jj] = alpha;
+                            else if (r > cj) rw[sl][jj] = inv * rw[sl][jj];
+                        }
+#pragma unroll
+                        for (int cc = jj + 1; cc < 32; ++cc) {
+                            if (c0 + cc < c1) {
+                                const T y = bcast_lane(rw[0][jj], cc);  // L(c0+cc, cj)
+#pragma unroll
+                                for (int sl = 0; sl < 2; ++sl) {
+                                    const int r = c0 + 64 * sl + lane;
+                                    const T x = rw[sl][jj];
+                                    if (r >= c0 + cc && (upper ? x : y) != T(0))
+                                        rw[sl][cc] = __builtin_fma(x, -y, rw[sl][cc]);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int r = c0 + 64 * sl + lane;
+#pragma unroll
+                for (int cc = 0; cc < 32; ++cc)
+                    if (r < n && c0 + cc < c1 && r >= c0 + cc) S[r + (c0 + cc) * LD] = rw[sl][cc];
+            }
+            if (lane == 0) s_fail = f;
+        }
+        __syncthreads();
+        fail = s_fail;
+        if (fail >= 0) {
+            // the columns to the right, as the unblocked algorithm leaves them at the stop
+            for (int d0 = c1; d0 < n; d0 += 32) left_update(d0, min(n, d0 + 32), fail);
+            __syncthreads();
             break;
         }
-        alpha = sqrt(alpha);
-        const T inv = T(1) / alpha;
-        __syncthreads();
-        if (t == 0) S[j + j * LD] = alpha;
-        const int h = n - (j + 1);
-        // scale the column (lower) or row (upper) of the block
-        for (int q = t; q < h; q += 256) {
-            if (!upper) S[(j + 1 + q) + j * LD] = inv * S[(j + 1 + q) + j * LD];
-            else S[j + (j + 1 + q) * LD] = inv * S[j + (j + 1 + q) * LD];
-        }
-        __syncthreads();
-        // rank-1 update of the trailing triangle: column l, rows i
-        for (int idx = t; idx < h * h; idx += 256) {
-            const int il = idx % h, ll = idx / h;
-            const int i = j + 1 + il, l = j + 1 + ll;
-            if (!upper) {
-                if (i < l) continue;
-                const T xl = S[l + j * LD];
-                if (xl == T(0)) continue;
-                S[i + l * LD] = __builtin_fma(S[i + j * LD], -xl, S[i + l * LD]);
-            } else {
-                if (i > l) continue;
-                const T xl = S[j + l * LD];
-                if (xl == T(0)) continue;
-                S[i + l * LD] = __builtin_fma(S[j + i * LD], -xl, S[i + l * LD]);
-            }
-        }
-        __syncthreads();
     }
-    if (fail && t == 0) atomicCAS(info, 0, (int32_t)(goff + j + 1));
-    __syncthreads();
+    if (fail >= 0 && t == 0) atomicCAS(info, 0, (int32_t)(goff + fail + 1));
     for (int idx = t; idx < n * n; idx += 256) {
-        const int i = idx % n, jj = idx / n;
-        if (!upper ? (i >= jj) : (i <= jj)) A[i + (i64)jj * lda] = S[i + jj * LD];
+        const int i = idx % n, j = idx / n;
+        if (i >= j) {
+            if (upper) A[j + (i64)i * lda] = S[i + j * LD];
+            else A[i + (i64)j * lda] = S[i + j * LD];
+        }
     }
 }
 
