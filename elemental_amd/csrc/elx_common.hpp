@@ -188,6 +188,9 @@ elx_status_t library_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k,
                            double* C, i64 ldc);
 elx_status_t library_dsyrk(hipStream_t s, bool upper, bool trans, i64 n, i64 k, double alpha,
                            const double* A, i64 lda, double beta, double* C, i64 ldc);
+// FP32 B := op(A)^{-1} B from the left (rocblas_strsm), op = transpose when trans
+elx_status_t library_strsm(hipStream_t s, bool upper, bool trans, bool unit, i64 m, i64 n,
+                           const float* A, i64 lda, float* B, i64 ldb);
 void library_gemm_release();
 // e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,

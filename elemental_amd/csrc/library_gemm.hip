@@ -127,6 +127,29 @@ elx_status_t library_dsyrk(hipStream_t s, bool upper, bool trans, i64 n, i64 k, 
     return ELX_SUCCESS;
 }
 
+elx_status_t library_strsm(hipStream_t s, bool upper, bool trans, bool unit, i64 m, i64 n,
+                           const float* A, i64 lda, float* B, i64 ldb) {
+    if (m <= 0 || n <= 0) return ELX_SUCCESS;
+    ELX_ARG(m < INT_MAX && n < INT_MAX && lda < INT_MAX && ldb < INT_MAX);
+    rocblas_handle h = nullptr;
+    ELX_CHECK(rb_handle(&h));
+    std::lock_guard<std::mutex> lk(g_rb_mu);
+    if (rocblas_set_stream(h, s) != rocblas_status_success ||
+        rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success)
+        return ELX_ERROR_HIP;
+    const float one = 1.0f;
+    const rocblas_status st = rocblas_strsm(
+        h, rocblas_side_left, upper ? rocblas_fill_upper : rocblas_fill_lower,
+        trans ? rocblas_operation_transpose : rocblas_operation_none,
+        unit ? rocblas_diagonal_unit : rocblas_diagonal_non_unit, (rocblas_int)m, (rocblas_int)n,
+        &one, A, (rocblas_int)lda, B, (rocblas_int)ldb);
+    if (st != rocblas_status_success) {
+        std::fprintf(stderr, "elemental_amd: rocblas_strsm failed (%d)\n", (int)st);
+        return ELX_ERROR_HIP;
+    }
+    return ELX_SUCCESS;
+}
+
 void library_gemm_release() {
     std::lock_guard<std::mutex> lk(g_rb_mu);
     for (auto& kv : g_rb) {

@@ -25,7 +25,8 @@ namespace {
 enum {
     kSlotW = 10, kSlotLh, kSlotUh, kSlotLinv, kSlotUinv, kSlotIdx, kSlotVec, kSlotPart,
     kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP,
-    kSlotF64, kSlotF64Idx, kSlotSym, kSlotDiag, kSlotLl, kSlotUl
+    kSlotF64, kSlotF64Idx, kSlotSym, kSlotDiag, kSlotLl, kSlotUl, kSlotBlockY, kSlotBlockR,
+    kSlotBlockN, kSlotBlockQ
 };
 
 // Uncached buffers of the persistent triangular solves (never freed before the process
@@ -475,7 +476,83 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     double worst = 0.0;
     int inner_total = 0;
     bool escalate = false;
+    // Several right-hand sides (refine NONE, IR, AUTO): block refinement first. A step is one
+    // row gather, two FP32 triangular solves over the active columns (rocBLAS strsm on the
+    // row-stored factors: W read column-major with leading dimension ldw is L^T in its strict
+    // upper triangle and U^T in its lower one, so L y = P b is a transposed unit-upper solve and
+    // U x = y a transposed lower one) and one FP64 residual product R = B - A X for the group.
+    // A column leaves the block when its backward error reaches tol, or when it stops falling
+    // tenfold per step or runs out of steps: those columns start again in the per-column loop
+    // below (GMRES, escalation, FP64 fallback).
+    std::vector<char> done((size_t)nrhs, 0);
+    if (nrhs > 1 && refine != ELX_REFINE_GMRES) {
+        const i64 G = std::min<i64>(nrhs, 64);
+        float* Yb = nullptr;
+        double* Rb = nullptr;
+        double* cn = nullptr;
+        int32_t* qd = nullptr;
+        ELX_CHECK(get(kSlotBlockY, (size_t)n * (size_t)G, &Yb));
+        ELX_CHECK(get(kSlotBlockR, (size_t)n * (size_t)G, &Rb));
+        ELX_CHECK(get(kSlotBlockN, (size_t)(3 * G), &cn));
+        ELX_CHECK(get(kSlotBlockQ, (size_t)G, &qd));
+        // X(:, q) [+]= M^{-1} Src(:, q) for nq columns
+        auto block_precond = [&](const double* Src, i64 lds, const int32_t* q, i64 nq, double* X0,
+                                 int accumulate) -> elx_status_t {
+            ELX_CHECK(launch_gather_rows_f32(s, n, nq, c.perm, q, Src, lds, Yb, n));
+            ELX_CHECK(library_strsm(s, true, true, true, n, nq, c.W, c.ldw, Yb, n));
+            ELX_CHECK(library_strsm(s, false, true, false, n, nq, c.W, c.ldw, Yb, n));
+            return launch_update_f64_2d(s, n, nq, q, Yb, n, X0, ldx, accumulate);
+        };
+        std::vector<double> hn((size_t)(3 * G)), eta_prev((size_t)G);
+        std::vector<char> active((size_t)G);
+        std::vector<int32_t> q;
+        for (i64 c0 = 0; c0 < nrhs; c0 += G) {
+            const i64 g = std::min<i64>(G, nrhs - c0);
+            const double* B0 = dB + c0 * ldb;
+            double* X0 = dX + c0 * ldx;
+            ELX_CHECK(block_precond(B0, ldb, nullptr, g, X0, 0));
+            std::fill(active.begin(), active.end(), 1);
+            for (int it = 0;; ++it) {
+                ELX_HIP(hipMemcpy2DAsync(Rb, sizeof(double) * n, B0, sizeof(double) * ldb,
+                                         sizeof(double) * n, g, hipMemcpyDeviceToDevice, s));
+                ELX_CHECK(launch_dgemm(s, false, false, n, g, n, -1.0, dA, lda, X0, ldx, 1.0, Rb, n,
+                                       0));
+                ELX_CHECK(launch_col_absmax(s, n, g, Rb, n, cn));
+                ELX_CHECK(launch_col_absmax(s, n, g, X0, ldx, cn + G));
+                ELX_CHECK(launch_col_absmax(s, n, g, B0, ldb, cn + 2 * G));
+                ELX_CHECK(d2h(hn.data(), cn, (size_t)(3 * G), s));
+                q.clear();
+                for (i64 k = 0; k < g; ++k) {
+                    if (!active[(size_t)k]) continue;
+                    const double den = anorm * hn[(size_t)(G + k)] + hn[(size_t)(2 * G + k)];
+                    const double rn = hn[(size_t)k];
+                    const double e = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
+                    const bool conv = e <= tol;
+                    if (conv || refine == ELX_REFINE_NONE) {
+                        // NONE: one solve, reported as it stands
+                        done[(size_t)(c0 + k)] = 1;
+                        active[(size_t)k] = 0;
+                        worst = std::max(worst, e);
+                        all_ok = all_ok && conv;
+                        st.outer_iters = std::max(st.outer_iters, it);
+                        continue;
+                    }
+                    if (it >= max_iters || (it > 0 && !(e < 0.1 * eta_prev[(size_t)k]))) {
+                        active[(size_t)k] = 0;  // to the per-column loop
+                        continue;
+                    }
+                    eta_prev[(size_t)k] = e;
+                    q.push_back((int32_t)k);
+                }
+                if (q.empty()) break;
+                ELX_HIP(hipMemcpyAsync(qd, q.data(), sizeof(int32_t) * q.size(),
+                                       hipMemcpyHostToDevice, s));
+                ELX_CHECK(block_precond(Rb, n, qd, (i64)q.size(), X0, 1));
+            }
+        }
+    }
     for (i64 col = 0; col < nrhs; ++col) {
+        if (done[(size_t)col]) continue;
         const double* b = dB + col * ldb;
         double* x = dX + col * ldx;
         auto precond = [&](const double* rin, double* xout, int accumulate) -> elx_status_t {

@@ -38,6 +38,13 @@ elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const fl
 // y(i) = float(r(p[i]))
 elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r,
                                   float* y);
+// Block forms for several right-hand sides, q a column list (null = identity):
+// Y(i,k) = float(R(p[i],q[k])); X(i,q[k]) = [X +] double(Y(i,k)); out[k] = max_i |M(i,k)|
+elx_status_t launch_gather_rows_f32(hipStream_t s, i64 n, i64 nr, const int32_t* p,
+                                   const int32_t* q, const double* R, i64 ldr, float* Y, i64 ldy);
+elx_status_t launch_update_f64_2d(hipStream_t s, i64 n, i64 nr, const int32_t* q, const float* Y,
+                                  i64 ldy, double* X, i64 ldx, int accumulate);
+elx_status_t launch_col_absmax(hipStream_t s, i64 n, i64 nr, const double* M, i64 ldm, double* out);
 // x(i) = [x(i) +] double(d(i))
 elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, double* x, int accumulate);
 // r = b - A x (or r = A x when b is null) with chunk partials in part (nch x n); max |r|,

@@ -469,6 +469,40 @@ __global__ void gather_to_f32_kernel(i64 n, const int32_t* __restrict__ p, const
 }
 
 // x(i) (+)= double(d(i))
+// Y(i, k) = float(R(p[i], q[k])) for the block refinement (p, q null = identity)
+__global__ void gather_rows_f32_kernel(i64 n, const int32_t* __restrict__ p,
+                                       const int32_t* __restrict__ q, const double* __restrict__ R,
+                                       i64 ldr, float* __restrict__ Y, i64 ldy) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const i64 k = blockIdx.y;
+    if (i >= n) return;
+    const i64 kc = q ? q[k] : k;
+    Y[i + k * ldy] = (float)R[(p ? p[i] : i) + kc * ldr];
+}
+// X(i, q[k]) = [X(i, q[k]) +] double(Y(i, k))
+__global__ void update_f64_2d_kernel(i64 n, const int32_t* __restrict__ q,
+                                     const float* __restrict__ Y, i64 ldy, double* __restrict__ X,
+                                     i64 ldx, int accumulate) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const i64 k = blockIdx.y;
+    if (i >= n) return;
+    const i64 kc = q ? q[k] : k;
+    const double v = (double)Y[i + k * ldy];
+    X[i + kc * ldx] = accumulate ? X[i + kc * ldx] + v : v;
+}
+// out[k] = max_i |M(i, k)|, one workgroup per column
+__global__ __launch_bounds__(256) void col_absmax_kernel(i64 n, const double* __restrict__ M, i64 ldm,
+                                                         double* __restrict__ out) {
+    const i64 k = blockIdx.x;
+    double mx = 0.0;
+    for (i64 i = threadIdx.x; i < n; i += 256) mx = fmax(mx, fabs(M[i + k * ldm]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
+}
 __global__ void update_f64_kernel(i64 n, const float* d, double* __restrict__ x, int accumulate) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -716,6 +750,33 @@ elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const fl
 
 elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r, float* y) {
     hipLaunchKernelGGL(gather_to_f32_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, p, r, y);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_gather_rows_f32(hipStream_t s, i64 n, i64 nr, const int32_t* p,
+                                   const int32_t* q, const double* R, i64 ldr, float* Y, i64 ldy) {
+    if (n <= 0 || nr <= 0) return ELX_SUCCESS;
+    ELX_ARG(nr < 65536);
+    hipLaunchKernelGGL(gather_rows_f32_kernel, dim3(cdiv(n, 256), (unsigned)nr), dim3(256), 0, s, n,
+                       p, q, R, ldr, Y, ldy);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_update_f64_2d(hipStream_t s, i64 n, i64 nr, const int32_t* q, const float* Y,
+                                  i64 ldy, double* X, i64 ldx, int accumulate) {
+    if (n <= 0 || nr <= 0) return ELX_SUCCESS;
+    ELX_ARG(nr < 65536);
+    hipLaunchKernelGGL(update_f64_2d_kernel, dim3(cdiv(n, 256), (unsigned)nr), dim3(256), 0, s, n, q,
+                       Y, ldy, X, ldx, accumulate);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_col_absmax(hipStream_t s, i64 n, i64 nr, const double* M, i64 ldm, double* out) {
+    if (nr <= 0) return ELX_SUCCESS;
+    hipLaunchKernelGGL(col_absmax_kernel, dim3((unsigned)nr), dim3(256), 0, s, n, M, ldm, out);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }

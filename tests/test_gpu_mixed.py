@@ -205,6 +205,59 @@ def test_hpd_mixed_solve(dev, orc, uplo, factor, n):
     assert np.array_equal(dev.to_host(dA), Ast, equal_nan=True)  # A is not modified
 
 
+@pytest.mark.parametrize("kind", ["gesv", "posv"])
+@pytest.mark.parametrize("factor", ["fp16", "fp32"])
+def test_block_refinement_many_rhs(dev, orc, kind, factor):
+    """Many right-hand sides go through the block refinement (groups of 64 columns, rocBLAS
+    strsm on the low-precision factors, one FP64 residual product per step): 70 columns, one
+    of them zero, one scaled by 2^100 and one by 2^-100; every column meets the FP64
+    backward-error bound on its own and the zero column gives x = 0."""
+    n = 900
+    say("block refinement", kind, factor, n)
+    A = randsvd(n, 1e3, 21) if kind == "gesv" else spd_kappa(n, 1e3, 21)
+    B = np.random.default_rng(22).standard_normal((n, 70))
+    B[:, 5] = 0.0
+    B[:, 6] *= 2.0 ** 100
+    B[:, 66] *= 2.0 ** -100
+    dA, dB = dev.to_device(A), dev.to_device(B)
+    dX = dev.colmajor_empty(n, 70)
+    if kind == "gesv":
+        st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="ir")
+    else:
+        st = dev.posv_mixed(dA, dB, dX, factor=factor, refine="ir")
+    X = dev.to_host(dX)
+    say(st)
+    assert st["converged"] == 1 and st["fell_back"] == 0
+    assert np.all(X[:, 5] == 0.0)
+    worst = 0.0
+    for k in range(70):
+        if k == 5:
+            continue
+        eta = orc.backward_error(A, X[:, k:k + 1], B[:, k:k + 1])
+        worst = max(worst, eta)
+        assert eta <= n * EPS, (k, eta / (n * EPS))
+    assert st["backward_error"] <= 2 * n * EPS
+    say("worst oracle eta/(n u) =", worst / (n * EPS))
+
+
+@pytest.mark.parametrize("factor", ["fp16", "bf16"])
+def test_block_one_shot_solve(dev, orc, factor):
+    """refine = none with several right-hand sides: one block solve with the low-precision
+    factors, each column a useful approximation, reported without refinement."""
+    n = 1024
+    A = randsvd(n, 1e2, 7)
+    B = np.random.default_rng(9).standard_normal((n, 5))
+    dA, dB = dev.to_device(A), dev.to_device(B)
+    dX = dev.colmajor_empty(n, 5)
+    st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="none")
+    assert st["status"] in (0, 5) and st["outer_iters"] == 0
+    X = dev.to_host(dX)
+    u_low = {"fp16": 2.0 ** -11, "bf16": 2.0 ** -8}[factor]
+    for k in range(5):
+        eta = orc.backward_error(A, X[:, k:k + 1], B[:, k:k + 1])
+        assert eta < 50 * u_low, (k, eta)
+
+
 def test_hpd_mixed_not_positive_definite(dev):
     """A symmetric indefinite matrix: the low-precision Cholesky reports the column, the FP64
     fallback fails the same way, and the call says so instead of returning an answer."""
