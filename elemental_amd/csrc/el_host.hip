@@ -431,6 +431,21 @@ void HPDSolve(UpperOrLower uplo, Orientation orientation, const Matrix<double>& 
     dB.to_host(B);
 }
 
+// El::RuizEquil (src/lapack_like/equilibrate/RuizEquil.cpp:23-60): A becomes
+// diag(dRow)^{-1} A diag(dCol)^{-1}; dRow, dCol are resized to m x 1 and n x 1.
+void RuizEquil(Matrix<double>& A, Matrix<double>& dRow, Matrix<double>& dCol, bool progress) {
+    (void)progress;
+    const Int m = A.Height(), n = A.Width();
+    dRow.Resize(m, 1);
+    dCol.Resize(n, 1);
+    DevMatrix<double> dA(A), dR(dRow), dC(dCol);
+    check(elx_druiz_equil(nullptr, m, n, dA.ptr(), dA.ld, dR.ptr(), dC.ptr()), "RuizEquil");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dA.to_host(A);
+    dR.to_host(dRow);
+    dC.to_host(dCol);
+}
+
 }  // namespace El
 
 // ====================================================================== C interface

@@ -61,6 +61,8 @@ struct Scratch {
     size_t bytes = 0;
 };
 elx_status_t scratch_get(int slot, size_t bytes, void** out);
+// scratch slot of the Ruiz equilibration's work vectors
+constexpr int kSlotRuizWork = 64;
 
 // Per-device persistent state of the cooperative panel kernel's exchange area.
 struct PanelSync {

@@ -26,7 +26,7 @@ enum {
     kSlotW = 10, kSlotLh, kSlotUh, kSlotLinv, kSlotUinv, kSlotIdx, kSlotVec, kSlotPart,
     kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP,
     kSlotF64, kSlotF64Idx, kSlotSym, kSlotDiag, kSlotLl, kSlotUl, kSlotBlockY, kSlotBlockR,
-    kSlotBlockN, kSlotBlockQ
+    kSlotBlockN, kSlotBlockQ, kSlotEquilA, kSlotEquilV, kSlotEquilW
 };
 
 // Uncached buffers of the persistent triangular solves (never freed before the process
@@ -289,8 +289,11 @@ extern "C" {
 // applies; dA is the full FP64 matrix for the residuals.
 static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs, const double* dA,
                                 int64_t lda, const double* dB, int64_t ldb, double* dX,
-                                int64_t ldx, int factor_type, int refine, double tol,
+                                int64_t ldx, int factor_flags, int refine, double tol,
                                 int32_t max_iters, elx_ir_stats_t* stats) {
+    const bool equil = (factor_flags & ELX_FACTOR_EQUILIBRATE) != 0;
+    const int factor_type = factor_flags & ~ELX_FACTOR_EQUILIBRATE;
+    if (equil && kind != 0) return ELX_ERROR_NOT_SUPPORTED;
     ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) &&
             ldb >= std::max<int64_t>(n, 1) && ldx >= std::max<int64_t>(n, 1));
     ELX_ARG(factor_type == ELX_FACTOR_FP16 || factor_type == ELX_FACTOR_BF16 ||
@@ -356,6 +359,26 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(launch_convert(s, n, dA, lda, c.W, c.ldw, part, nch));
     ELX_HIP(hipMemsetAsync(anorm_d, 0, sizeof(double), s));
     ELX_CHECK(launch_rowabs_max(s, n, nch, part, anorm_d));
+    // Equilibrated: W from diag(dRow)^-1 A diag(dCol)^-1 (an FP64 copy); the preconditioner
+    // scales by sr = 1/dRow before and sc = 1/dCol after the triangular solves.
+    double* sr = nullptr;
+    double* sc = nullptr;
+    if (equil) {
+        double* Aeq = nullptr;
+        double* ev = nullptr;
+        double* ew = nullptr;
+        ELX_CHECK(get(kSlotEquilA, (size_t)n * (size_t)n, &Aeq));
+        ELX_CHECK(get(kSlotEquilV, (size_t)(4 * n), &ev));
+        ELX_CHECK(get(kSlotEquilW, ruiz_work_doubles(n, n), &ew));
+        ELX_HIP(hipMemcpy2DAsync(Aeq, sizeof(double) * n, dA, sizeof(double) * lda,
+                                 sizeof(double) * n, n, hipMemcpyDeviceToDevice, s));
+        ELX_CHECK(launch_ruiz_equil(s, n, n, Aeq, n, ev, ev + n, ew));
+        sr = ev + 2 * n;
+        sc = ev + 3 * n;
+        ELX_CHECK(launch_reciprocal(s, n, ev, sr));
+        ELX_CHECK(launch_reciprocal(s, n, ev + n, sc));
+        ELX_CHECK(launch_convert(s, n, Aeq, n, c.W, c.ldw, part, nch));
+    }
     if (kind == 0) {
         ELX_CHECK(getrf_mixed(c));
         // the explicit permutation (Permutation::ExplicitVector): the swaps replayed on the host,
@@ -498,10 +521,10 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
         // X(:, q) [+]= M^{-1} Src(:, q) for nq columns
         auto block_precond = [&](const double* Src, i64 lds, const int32_t* q, i64 nq, double* X0,
                                  int accumulate) -> elx_status_t {
-            ELX_CHECK(launch_gather_rows_f32(s, n, nq, c.perm, q, Src, lds, Yb, n));
+            ELX_CHECK(launch_gather_rows_f32(s, n, nq, c.perm, q, Src, lds, sr, Yb, n));
             ELX_CHECK(library_strsm(s, true, true, true, n, nq, c.W, c.ldw, Yb, n));
             ELX_CHECK(library_strsm(s, false, true, false, n, nq, c.W, c.ldw, Yb, n));
-            return launch_update_f64_2d(s, n, nq, q, Yb, n, X0, ldx, accumulate);
+            return launch_update_f64_2d(s, n, nq, q, Yb, n, sc, X0, ldx, accumulate);
         };
         std::vector<double> hn((size_t)(3 * G)), eta_prev((size_t)G);
         std::vector<char> active((size_t)G);
@@ -556,13 +579,13 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
         const double* b = dB + col * ldb;
         double* x = dX + col * ldx;
         auto precond = [&](const double* rin, double* xout, int accumulate) -> elx_status_t {
-            ELX_CHECK(launch_gather_to_f32(s, n, c.perm, rin, y));
+            ELX_CHECK(launch_gather_to_f32(s, n, c.perm, rin, sr, y));
             const uint32_t t1 = (uint32_t)++ta->epoch;
             ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Linv, 1, y, ta->xa, ta->xchunks, t1, err));
             const uint32_t t2 = (uint32_t)++ta->epoch;
             ELX_CHECK(launch_trsv(s, n, c.W, c.ldw, c.Uinv, 0, ta->xa, ta->xb, ta->xchunks, t2,
                                   err));
-            return launch_update_f64(s, n, ta->xb, xout, accumulate);
+            return launch_update_f64(s, n, ta->xb, sc, xout, accumulate);
         };
         // ||v||_2 on the device, read back
         auto nrm2 = [&](const double* v, double* out) -> elx_status_t {
@@ -717,7 +740,8 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     if (escalate) {
         elx_ir_stats_t s2;
         const elx_status_t r2 = mixed_solve(kind, stream, n, nrhs, dA, lda, dB, ldb, dX, ldx,
-                                            ELX_FACTOR_FP32, refine, tol, max_iters, &s2);
+                                            ELX_FACTOR_FP32 | (factor_flags & ELX_FACTOR_EQUILIBRATE),
+                                            refine, tol, max_iters, &s2);
         s2.factor_seconds += st.factor_seconds;
         s2.refine_seconds += st.refine_seconds;
         s2.inner_iters += st.inner_iters;
@@ -776,6 +800,7 @@ elx_status_t elx_dposv_mixed(void* stream, int uplo, int64_t n, int64_t nrhs, co
                              int32_t max_iters, elx_ir_stats_t* stats) {
     ELX_ARG(uplo == kLower || uplo == kUpper);
     ELX_ARG(n >= 0 && lda >= std::max<int64_t>(n, 1));
+    if (factor_type & ELX_FACTOR_EQUILIBRATE) return ELX_ERROR_NOT_SUPPORTED;
     if (n == 0) return mixed_solve(1, stream, n, nrhs, dA, lda, dB, ldb, dX, ldx, factor_type,
                                    refine, tol, max_iters, stats);
     // the full symmetric matrix from the stored triangle (the residuals read all of it)

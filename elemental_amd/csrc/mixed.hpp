@@ -35,18 +35,28 @@ elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv,
 elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const float* Inv,
                          int lower, const float* y, float* x, void* xchunks, uint32_t epoch,
                          int* err);
-// y(i) = float(r(p[i]))
+// y(i) = float(r(p[i]) * sr(p[i]))   (sr null: no scaling)
 elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r,
-                                  float* y);
+                                  const double* sr, float* y);
 // Block forms for several right-hand sides, q a column list (null = identity):
-// Y(i,k) = float(R(p[i],q[k])); X(i,q[k]) = [X +] double(Y(i,k)); out[k] = max_i |M(i,k)|
+// Y(i,k) = float(R(p[i],q[k]) * sr(p[i])); X(i,q[k]) = [X +] double(Y(i,k)) * sc(i);
+// out[k] = max_i |M(i,k)|
 elx_status_t launch_gather_rows_f32(hipStream_t s, i64 n, i64 nr, const int32_t* p,
-                                   const int32_t* q, const double* R, i64 ldr, float* Y, i64 ldy);
+                                   const int32_t* q, const double* R, i64 ldr, const double* sr,
+                                   float* Y, i64 ldy);
 elx_status_t launch_update_f64_2d(hipStream_t s, i64 n, i64 nr, const int32_t* q, const float* Y,
-                                  i64 ldy, double* X, i64 ldx, int accumulate);
+                                  i64 ldy, const double* sc, double* X, i64 ldx, int accumulate);
 elx_status_t launch_col_absmax(hipStream_t s, i64 n, i64 nr, const double* M, i64 ldm, double* out);
-// x(i) = [x(i) +] double(d(i))
-elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, double* x, int accumulate);
+// x(i) = [x(i) +] double(d(i)) * sc(i)   (sc null: no scaling)
+elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, const double* sc, double* x,
+                               int accumulate);
+// Ruiz equilibration (El::RuizEquil) of the m x n column-major A in place; dRow, dCol get
+// the accumulated scales; work holds ruiz_work_doubles(m, n) doubles.
+elx_status_t launch_ruiz_equil(hipStream_t s, i64 m, i64 n, double* A, i64 lda, double* dRow,
+                               double* dCol, double* work);
+size_t ruiz_work_doubles(i64 m, i64 n);
+// out(i) = 1 / in(i)
+elx_status_t launch_reciprocal(hipStream_t s, i64 n, const double* in, double* out);
 // r = b - A x (or r = A x when b is null) with chunk partials in part (nch x n); max |r|,
 // |x|, |b| into norms[0..2] (norms zeroed first). Requires ceil(n/nch) <= 1024.
 elx_status_t launch_residual(hipStream_t s, i64 n, const double* A, i64 lda, const double* x,

@@ -462,32 +462,39 @@ __global__ __launch_bounds__(256) void trsv_kernel(i64 n, const float* __restric
 
 // y(i) = float(scale_in * r(p[i])) (p may be null = identity)
 __global__ void gather_to_f32_kernel(i64 n, const int32_t* __restrict__ p, const double* __restrict__ r,
-                                     float* __restrict__ y) {
+                                     const double* __restrict__ sr, float* __restrict__ y) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    y[i] = (float)r[p ? p[i] : i];
+    const i64 k = p ? p[i] : i;
+    y[i] = (float)(sr ? r[k] * sr[k] : r[k]);
 }
 
 // x(i) (+)= double(d(i))
-// Y(i, k) = float(R(p[i], q[k])) for the block refinement (p, q null = identity)
+// Y(i, k) = float(R(p[i], q[k]) * sr[p[i]]) for the block refinement (p, q, sr null =
+// identity, identity, no scaling)
 __global__ void gather_rows_f32_kernel(i64 n, const int32_t* __restrict__ p,
                                        const int32_t* __restrict__ q, const double* __restrict__ R,
-                                       i64 ldr, float* __restrict__ Y, i64 ldy) {
+                                       i64 ldr, const double* __restrict__ sr, float* __restrict__ Y,
+                                       i64 ldy) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
     const i64 k = blockIdx.y;
     if (i >= n) return;
     const i64 kc = q ? q[k] : k;
-    Y[i + k * ldy] = (float)R[(p ? p[i] : i) + kc * ldr];
+    const i64 r = p ? p[i] : i;
+    const double v = R[r + kc * ldr];
+    Y[i + k * ldy] = (float)(sr ? v * sr[r] : v);
 }
-// X(i, q[k]) = [X(i, q[k]) +] double(Y(i, k))
+// X(i, q[k]) = [X(i, q[k]) +] double(Y(i, k)) * sc[i]   (sc null: no scaling)
 __global__ void update_f64_2d_kernel(i64 n, const int32_t* __restrict__ q,
-                                     const float* __restrict__ Y, i64 ldy, double* __restrict__ X,
+                                     const float* __restrict__ Y, i64 ldy,
+                                     const double* __restrict__ sc, double* __restrict__ X,
                                      i64 ldx, int accumulate) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
     const i64 k = blockIdx.y;
     if (i >= n) return;
     const i64 kc = q ? q[k] : k;
-    const double v = (double)Y[i + k * ldy];
+    double v = (double)Y[i + k * ldy];
+    if (sc) v *= sc[i];
     X[i + kc * ldx] = accumulate ? X[i + kc * ldx] + v : v;
 }
 // out[k] = max_i |M(i, k)|, one workgroup per column
@@ -503,11 +510,13 @@ __global__ __launch_bounds__(256) void col_absmax_kernel(i64 n, const double* __
     __syncthreads();
     if (threadIdx.x == 0) out[k] = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
 }
-__global__ void update_f64_kernel(i64 n, const float* d, double* __restrict__ x, int accumulate) {
+__global__ void update_f64_kernel(i64 n, const float* d, const double* __restrict__ sc,
+                                  double* __restrict__ x, int accumulate) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const double v = (double)__hip_atomic_load(const_cast<float*>(d + i), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+    double v = (double)__hip_atomic_load(const_cast<float*>(d + i), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    if (sc) v *= sc[i];
     x[i] = accumulate ? x[i] + v : v;
 }
 
@@ -748,28 +757,30 @@ elx_status_t launch_trsv(hipStream_t s, i64 n, const float* W, i64 ldw, const fl
     return ELX_SUCCESS;
 }
 
-elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r, float* y) {
-    hipLaunchKernelGGL(gather_to_f32_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, p, r, y);
+elx_status_t launch_gather_to_f32(hipStream_t s, i64 n, const int32_t* p, const double* r,
+                                  const double* sr, float* y) {
+    hipLaunchKernelGGL(gather_to_f32_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, p, r, sr, y);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
 
 elx_status_t launch_gather_rows_f32(hipStream_t s, i64 n, i64 nr, const int32_t* p,
-                                   const int32_t* q, const double* R, i64 ldr, float* Y, i64 ldy) {
+                                   const int32_t* q, const double* R, i64 ldr, const double* sr,
+                                   float* Y, i64 ldy) {
     if (n <= 0 || nr <= 0) return ELX_SUCCESS;
     ELX_ARG(nr < 65536);
     hipLaunchKernelGGL(gather_rows_f32_kernel, dim3(cdiv(n, 256), (unsigned)nr), dim3(256), 0, s, n,
-                       p, q, R, ldr, Y, ldy);
+                       p, q, R, ldr, sr, Y, ldy);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
 
 elx_status_t launch_update_f64_2d(hipStream_t s, i64 n, i64 nr, const int32_t* q, const float* Y,
-                                  i64 ldy, double* X, i64 ldx, int accumulate) {
+                                  i64 ldy, const double* sc, double* X, i64 ldx, int accumulate) {
     if (n <= 0 || nr <= 0) return ELX_SUCCESS;
     ELX_ARG(nr < 65536);
     hipLaunchKernelGGL(update_f64_2d_kernel, dim3(cdiv(n, 256), (unsigned)nr), dim3(256), 0, s, n, q,
-                       Y, ldy, X, ldx, accumulate);
+                       Y, ldy, sc, X, ldx, accumulate);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
@@ -781,8 +792,10 @@ elx_status_t launch_col_absmax(hipStream_t s, i64 n, i64 nr, const double* M, i6
     return ELX_SUCCESS;
 }
 
-elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, double* x, int accumulate) {
-    hipLaunchKernelGGL(update_f64_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, x, accumulate);
+elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, const double* sc, double* x,
+                              int accumulate) {
+    hipLaunchKernelGGL(update_f64_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, sc, x,
+                       accumulate);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }

@@ -139,6 +139,17 @@ def syrk(uplo, orient, alpha, A, beta, C, stream=None):
     return C
 
 
+def ruiz_equil(A, stream=None):
+    """El::RuizEquil in place on a column-major FP64 device matrix; returns (dRow, dCol)."""
+    import torch
+    pA, lda = _mat(A)
+    m, n = A.shape
+    dRow = torch.empty(m, dtype=torch.float64, device=A.device)
+    dCol = torch.empty(n, dtype=torch.float64, device=A.device)
+    call("elx_druiz_equil", _stream(stream), m, n, pA, lda, dRow.data_ptr(), dCol.data_ptr())
+    return dRow, dCol
+
+
 def permute_rows(dests, A, inverse=False, stream=None):
     pA, lda = _mat(A)
     call("elx_dpermute_rows", _stream(stream), 1 if inverse else 0, dests.data_ptr(),
@@ -146,16 +157,18 @@ def permute_rows(dests, A, inverse=False, stream=None):
     return A
 
 
-def gesv_mixed(A, B, X, factor="fp16", refine="auto", tol=0.0, max_iters=0, stream=None):
-    """Low-precision LU + FP64 refinement; returns the stats dict (synchronous)."""
+def gesv_mixed(A, B, X, factor="fp16", refine="auto", tol=0.0, max_iters=0, stream=None,
+               equilibrate=False):
+    """Low-precision LU + FP64 refinement; returns the stats dict (synchronous). With
+    equilibrate, the LU factors the Ruiz-equilibrated matrix (ELX_FACTOR_EQUILIBRATE)."""
     pA, lda = _mat(A)
     pB, ldb = _mat(B)
     pX, ldx = _mat(X)
     st = IrStats()
     import ctypes
     status = load().elx_dgesv_mixed(_stream(stream), A.shape[0], B.shape[1], pA, lda, pB, ldb,
-                                     pX, ldx, FACTOR[factor], REFINE[refine], tol, max_iters,
-                                     ctypes.byref(st))
+                                     pX, ldx, FACTOR[factor] | (0x100 if equilibrate else 0),
+                                     REFINE[refine], tol, max_iters, ctypes.byref(st))
     if status not in (0, 5):
         from ._lib import ElxError
         raise ElxError("elx_dgesv_mixed", status)
@@ -186,7 +199,7 @@ def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_i
 
 __all__ = [
     "load", "getrf", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
-    "permute_rows", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
+    "permute_rows", "ruiz_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
     "Batch", "potrf_vbatched", "potrs_vbatched", "gesv_mixed_batched",
 ]
 

@@ -174,6 +174,12 @@ void LinearSolve(const Matrix<F>& A, Matrix<F>& B);
 template <typename F>
 void HPDSolve(UpperOrLower uplo, Orientation orientation, const Matrix<F>& A, Matrix<F>& B);
 
+// ---- equilibration ----
+// A := diag(dRow)^{-1} A diag(dCol)^{-1}, four Ruiz sweeps
+// (include/El/lapack_like/equilibrate.hpp:17-21). Defined for double.
+void RuizEquil(Matrix<double>& A, Matrix<double>& dRow, Matrix<double>& dCol,
+               bool progress = false);
+
 }  // namespace El
 
 #endif  // EL_AMD_HPP

@@ -114,6 +114,14 @@ elx_status_t elx_dsyrk(void* stream, int uplo, int orient, int64_t n, int64_t k,
 elx_status_t elx_dpermute_rows(void* stream, int inverse, const int32_t* dDests,
                                int64_t nswaps, int64_t m, int64_t n, double* dA, int64_t lda);
 
+/* Ruiz equilibration in place: four sweeps scaling the columns, then the rows, by their
+ * damped max norms; on return A holds diag(dRow)^-1 A0 diag(dCol)^-1 (bitwise the reference's
+ * result). dRow (m) and dCol (n) are device vectors. Replaces El::RuizEquil(Matrix<double>& A,
+ * Matrix<double>& dRow, Matrix<double>& dCol, bool) --
+ * src/lapack_like/equilibrate/RuizEquil.cpp:23-60, include/El/lapack_like/equilibrate.hpp:17-21. */
+elx_status_t elx_druiz_equil(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
+                             double* dRow, double* dCol);
+
 /* ------------------------------------------------------------------------------------
  * Mixed precision: low-precision LU + FP64 iterative refinement (no reference
  * counterpart for dense matrices; the refinement loop follows the semantics of
@@ -125,7 +133,13 @@ typedef enum {
     ELX_FACTOR_FP32 = 1,  /* FP32-accurate trailing updates: each operand split into fp16 high
                              and low parts, three fp16 MFMA products (hi*hi + hi*lo + lo*hi) */
     ELX_FACTOR_FP16 = 2,
-    ELX_FACTOR_BF16 = 3
+    ELX_FACTOR_BF16 = 3,
+    /* Flag, OR'ed into a factor type of elx_dgesv_mixed (not elx_dposv_mixed): the
+     * low-precision LU factors the Ruiz-equilibrated diag(dRow)^-1 A diag(dCol)^-1 of an FP64
+     * copy of A (elx_druiz_equil), the preconditioner applies the scales, and the residuals
+     * and the FP64 fallback use A itself. For badly scaled A; costs an n x n FP64 copy and
+     * eight passes over it. */
+    ELX_FACTOR_EQUILIBRATE = 0x100
 } elx_factor_type_t;
 
 typedef enum {

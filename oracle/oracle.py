@@ -291,3 +291,38 @@ def pivot_agreement(A, dests_ref, dests_test, bsize=128, rel_tie=1e-8):
     gap = (a - b) / a if a > 0 else 0.0
     out.update(first=c, gap=float(gap), near_ties=int(gap <= rel_tie))
     return out
+
+
+def ruiz_damp(s):
+    """DampScaling (src/lapack_like/equilibrate/RuizEquil.cpp:13-21): 1 for a zero norm,
+    otherwise max(norm, eps^0.33) with eps = El::limits::Epsilon<double>() = 2^-53."""
+    tol = EPS ** 0.33
+    return np.where(s == 0.0, 1.0, np.maximum(s, tol))
+
+
+def ruiz_equil(A):
+    """El::RuizEquil (src/lapack_like/equilibrate/RuizEquil.cpp:23-60) on a copy of A: four
+    sweeps; each takes the column max norms (ColumnMaxNorms, src/blas_like/level1/
+    ColumnNorms.cpp:89-104), damps them, multiplies them into dCol and every column by the
+    reciprocal of its scale (DiagonalSolve RIGHT: A(i,j) *= 1/d(j), include/El/blas_like/
+    level1/DiagonalSolve.hpp:42-56), then the same for the rows (RowMaxNorms, RowNorms.cpp:
+    58-72; DiagonalSolve LEFT). Returns (A_eq, dRow, dCol) with A_eq = diag(dRow)^-1 A
+    diag(dCol)^-1."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = A.shape
+    dRow = np.ones(m)
+    dCol = np.ones(n)
+    for _ in range(4):
+        if m > 0 and n > 0:
+            cs = ruiz_damp(np.abs(A).max(axis=0))
+        else:
+            cs = np.ones(n)
+        dCol *= cs
+        A *= (1.0 / cs)[None, :]
+        if m > 0 and n > 0:
+            rs = ruiz_damp(np.abs(A).max(axis=1))
+        else:
+            rs = np.ones(m)
+        dRow *= rs
+        A *= (1.0 / rs)[:, None]
+    return A, dRow, dCol

@@ -1,0 +1,78 @@
+"""Ruiz equilibration on the GPU (elx_druiz_equil, El::RuizEquil): bitwise equal to the
+oracle, and the mixed solver's equilibrated factorization on badly scaled systems."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def scaled(rng, m, n, spread):
+    return np.asfortranarray((2.0 ** rng.uniform(-spread, spread, m))[:, None] *
+                             rng.standard_normal((m, n)) *
+                             (2.0 ** rng.uniform(-spread, spread, n))[None, :])
+
+
+@pytest.mark.parametrize("m,n,ld_extra", [(300, 200, 0), (1000, 1000, 24), (3, 70000, 5),
+                                          (1, 1, 0), (257, 64, 3)])
+def test_ruiz_bitwise(dev, orc, m, n, ld_extra):
+    say("ruiz", m, n, ld_extra)
+    rng = np.random.default_rng(m * 7 + n)
+    A = scaled(rng, m, n, 40)
+    if n > 3:
+        A[:, 2] = 0.0
+    if m > 3:
+        A[1, :] = 0.0
+        A[2, 0] = 1e-200
+    Ae, dR, dC = orc.ruiz_equil(A)
+    T = torch.zeros((n, m + ld_extra), dtype=torch.float64, device="cuda").t()[:m, :]
+    T.copy_(torch.from_numpy(A))
+    gR, gC = dev.ruiz_equil(T)
+    torch.cuda.synchronize()
+    assert np.array_equal(T.cpu().numpy(), Ae)
+    assert np.array_equal(gR.cpu().numpy(), dR)
+    assert np.array_equal(gC.cpu().numpy(), dC)
+
+
+@pytest.mark.parametrize("n", [700, 3000])
+def test_mixed_equilibrated_badly_scaled(dev, orc, n):
+    """Rows and columns scaled by 2^-24 .. 2^24: the fp16 LU of the equilibrated matrix is a
+    good preconditioner again; every system converges to n u without the FP64 fallback."""
+    rng = np.random.default_rng(n)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    core = (Q1 * 10.0 ** (-2 * np.arange(n) / (n - 1))[None, :]) @ Q2.T
+    A = np.asfortranarray((2.0 ** rng.uniform(-24, 24, n))[:, None] * core *
+                          (2.0 ** rng.uniform(-24, 24, n))[None, :])
+    B = rng.standard_normal((n, 3))
+    dA, dB = dev.to_device(A), dev.to_device(B)
+    dX = dev.colmajor_empty(n, 3)
+    plain = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto")
+    say(n, "without equilibration", {k: plain[k] for k in ("converged", "outer_iters",
+                                                           "inner_iters", "fell_back")})
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto", equilibrate=True)
+    X = dev.to_host(dX)
+    eta = orc.backward_error(A, X, B)
+    say(n, "equilibrated", st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["converged"] == 1 and st["fell_back"] == 0
+    assert eta <= n * EPS
+    assert np.array_equal(dev.to_host(dA), A)  # A is not modified
+
+
+def test_mixed_equilibrate_rejected_for_hpd(dev):
+    n = 64
+    A = np.asfortranarray(np.eye(n) * 2.0)
+    dA, dB = dev.to_device(A), dev.to_device(np.ones((n, 1)))
+    dX = dev.colmajor_empty(n, 1)
+    import ctypes
+    from elemental_amd._lib import IrStats, load
+    st = IrStats()
+    pA, pB, pX = dA.data_ptr(), dB.data_ptr(), dX.data_ptr()
+    r = load().elx_dposv_mixed(None, 0, n, 1, pA, n, pB, n, pX, n, 2 | 0x100, 3, 0.0, 0,
+                               ctypes.byref(st))
+    assert r == 4  # ELX_ERROR_NOT_SUPPORTED

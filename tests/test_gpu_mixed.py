@@ -205,6 +205,37 @@ def test_hpd_mixed_solve(dev, orc, uplo, factor, n):
     assert np.array_equal(dev.to_host(dA), Ast, equal_nan=True)  # A is not modified
 
 
+@pytest.mark.parametrize("n", [300, 2500, 9000])
+@pytest.mark.parametrize("kind", ["permutation", "lower_half_zero"])
+def test_mixed_structured_pivoting(dev, orc, n, kind):
+    """Structured matrices where most of the panel's workgroups hold only zeros in a panel's
+    columns: a permutation matrix (exact factors, exact solution) and a matrix whose leading
+    panels have their nonzeros only in the lower half of the rows. Every system must converge
+    without the FP64 fallback."""
+    say("structured", kind, n)
+    rng = np.random.default_rng(n)
+    if kind == "permutation":
+        A = np.zeros((n, n))
+        A[rng.permutation(n), np.arange(n)] = rng.choice([-3.0, -1.0, 2.0, 5.0], n)
+    else:
+        A = randsvd(n, 1e2, n)
+        A[: n // 2, : n // 2] = 0.0
+        A[: n // 2, n // 2:] = np.asfortranarray(randsvd(n - n // 2, 1e1, n + 1)[: n // 2])
+        A[n // 2:, : n // 2] += np.eye(n - n // 2, n // 2)
+        A = np.asfortranarray(A)
+    b = rng.standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto")
+    x = dev.to_host(dX)
+    eta = orc.backward_error(A, x, b)
+    say(st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["factor_info"] == 0 and st["converged"] == 1 and st["fell_back"] == 0
+    assert eta <= n * EPS
+    if kind == "permutation":
+        assert st["outer_iters"] <= 1
+
+
 @pytest.mark.parametrize("kind", ["gesv", "posv"])
 @pytest.mark.parametrize("factor", ["fp16", "fp32"])
 def test_block_refinement_many_rhs(dev, orc, kind, factor):

@@ -1,0 +1,52 @@
+"""The oracle's Ruiz equilibration (El::RuizEquil, src/lapack_like/equilibrate/RuizEquil.cpp:
+23-60) pinned by closed forms and invariants that do not restate its loop."""
+import numpy as np
+
+from oracle import oracle as orc
+
+TOL = (2.0 ** -53) ** 0.33
+
+
+def test_rank_one_powers_of_two():
+    # A = u v^T: the first column sweep leaves u_i / max|u| * sign(v_j), the row sweep then
+    # the sign matrix; later sweeps see unit norms. Every scale is a power of two: exact.
+    u = np.array([2.0 ** 3, -2.0 ** -5, 2.0 ** 10, 2.0 ** -2, -1.0])
+    v = np.array([-2.0 ** 4, 2.0 ** -7, 2.0 ** 2])
+    A = np.outer(u, v)
+    Ae, dR, dC = orc.ruiz_equil(A)
+    assert np.array_equal(Ae, np.sign(np.outer(u, v)))
+    assert np.array_equal(dR, np.abs(u) / np.abs(u).max())
+    assert np.array_equal(dC, np.abs(v) * np.abs(u).max())
+
+
+def test_zero_rows_and_columns_keep_unit_scales():
+    A = np.array([[0.0, 2.0, 0.0], [0.0, 0.0, 0.0], [0.0, -8.0, 0.0]])
+    Ae, dR, dC = orc.ruiz_equil(A)
+    assert np.array_equal(dC, [1.0, 8.0, 1.0])
+    assert np.array_equal(dR, [0.25, 1.0, 1.0])
+    assert np.array_equal(Ae, [[0.0, 1.0, 0.0], [0.0, 0.0, 0.0], [0.0, -1.0, 0.0]])
+
+
+def test_damping_floor():
+    # a column norm below eps^0.33 is raised to it (DampScaling)
+    Ae, dR, dC = orc.ruiz_equil(np.array([[1e-10], [0.0]]))
+    assert dC[0] == TOL
+    assert dR[1] == 1.0 and Ae[1, 0] == 0.0
+    assert abs(dR[0] * Ae[0, 0] * dC[0] - 1e-10) <= 1e-25
+
+
+def test_reconstruction_and_unit_row_norms():
+    rng = np.random.default_rng(3)
+    m, n = 60, 45
+    A = (2.0 ** rng.uniform(-30, 30, m))[:, None] * rng.standard_normal((m, n)) * \
+        (2.0 ** rng.uniform(-30, 30, n))[None, :]
+    A[:, 7] = 0.0
+    Ae, dR, dC = orc.ruiz_equil(A)
+    R = dR[:, None] * Ae * dC[None, :]
+    assert np.max(np.abs(R - A) / np.maximum(np.abs(A), 1e-300)) <= 8 * 2.0 ** -53
+    # the last step divides every row by its max norm
+    assert np.allclose(np.abs(Ae).max(axis=1), 1.0, rtol=0, atol=4 * 2.0 ** -53)
+    assert np.abs(Ae).max() <= 1.0 + 4 * 2.0 ** -53
+    # scaling balances the matrix: column norms near 1 as well
+    cm = np.abs(Ae).max(axis=0)
+    assert np.all(cm[np.arange(n) != 7] > 0.5) and cm[7] == 0.0 and dC[7] == 1.0
