@@ -146,9 +146,12 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
                                                          i64 lda, T* B, i64 ldb, int right,
                                                          int lower_eff, int mtrans, int unit) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* x = reinterpret_cast<T*>(smem_raw);  // x[v * m + i]: a vector's rows are adjacent, so
+    // Both LDS arrays have a leading dimension of m + 1, so transposed stagings (consecutive
+    // threads walking a column of one and a row of the other) hit distinct banks.
+    const int xm = m + 1;
+    T* x = reinterpret_cast<T*>(smem_raw);  // x[v * xm + i]: a vector's rows are adjacent, so
                                             // the 16 lanes of a vector hit distinct banks
-    T* Ms = x + (size_t)m * TV;             // the triangle, Ms[i + j*m] = op(A)(i, j)
+    T* Ms = x + (size_t)xm * TV;            // the triangle, Ms[i + j*xm] = op(A)(i, j)
     const int t = threadIdx.x;
     const int v = t / TP, q = t % TP;
     const i64 v0 = (i64)blockIdx.x * TV;
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
     // LDS rather than on the cache hierarchy.
     for (int idx = t; idx < m * m; idx += TNT) {
         const int i = idx % m, j = idx / m;  // walk A's columns: coalesced reads
-        Ms[mtrans ? (j + i * m) : (i + j * m)] = A[i + (i64)j * lda];
+        Ms[mtrans ? (j + i * xm) : (i + j * xm)] = A[i + (i64)j * lda];
     }
     // Coalesced staging: left side walks down each column, right side across rows.
     for (int idx = t; idx < m * TV; idx += TNT) {
@@ -171,10 +174,10 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
         const i64 gv = v0 + vv;
         T val = T(0);
         if (gv < nvec) val = right ? B[gv + (i64)i * ldb] : B[i + gv * ldb];
-        x[vv * m + i] = val;
+        x[vv * xm + i] = val;
     }
     __syncthreads();
-    auto M = [&](int i, int j) -> T { return Ms[i + j * m]; };
+    auto M = [&](int i, int j) -> T { return Ms[i + j * xm]; };
     // Column-oriented substitutions keep each vector in the registers of its 16 lanes (lane q
     // holds entries q, q+16, ...); x_j reaches the other lanes by one shuffle per step. Every
     // entry still receives its updates one FMA at a time in step order.
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int i = q + TP * k;
-            xr[k] = (i < m) ? x[v * m + i] : T(0);
+            xr[k] = (i < m) ? x[v * xm + i] : T(0);
         }
         if (lower_eff) {
 #pragma unroll
@@ -226,14 +229,14 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int i = q + TP * k;
-            if (i < m) x[v * m + i] = xr[k];
+            if (i < m) x[v * xm + i] = xr[k];
         }
     } else if (q == 0) {
         for (int i = m - 1; i >= 0; --i) {
-            T temp = x[v * m + i];
-            for (int k = i + 1; k < m; ++k) temp = __builtin_fma(-M(i, k), x[v * m + k], temp);
+            T temp = x[v * xm + i];
+            for (int k = i + 1; k < m; ++k) temp = __builtin_fma(-M(i, k), x[v * xm + k], temp);
             if (!unit) temp = right ? temp * (T(1) / M(i, i)) : temp / M(i, i);
-            x[v * m + i] = temp;
+            x[v * xm + i] = temp;
         }
     }
     __syncthreads();
@@ -248,8 +251,8 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
         }
         const i64 gv = v0 + vv;
         if (gv < nvec) {
-            if (right) B[gv + (i64)i * ldb] = x[vv * m + i];
-            else B[i + gv * ldb] = x[vv * m + i];
+            if (right) B[gv + (i64)i * ldb] = x[vv * xm + i];
+            else B[i + gv * ldb] = x[vv * xm + i];
         }
     }
 }
@@ -468,13 +471,13 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     // Left: M = op(A). Right: M = op(A)^T.
     const bool mtrans = right ? !trans : trans;
     const bool lower_eff = lower != mtrans;
-    const size_t lds = sizeof(T) * ((size_t)order * TV + (size_t)order * order);
+    const size_t lds = sizeof(T) * ((size_t)(order + 1) * TV + (size_t)order * (order + 1));
     ProfScope prof_(s, kKTrsm, (double)order * (double)order * (double)nvec);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_small_kernel<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(sizeof(T) * (128 * TV + 128 * 128)));
+                                    (int)(sizeof(T) * (129 * TV + 128 * 129)));
         (void)hipGetLastError();
         attr = true;
     }
