@@ -194,7 +194,7 @@ elx_status_t library_dsyrk(hipStream_t s, bool upper, bool trans, i64 n, i64 k, 
 elx_status_t library_strsm(hipStream_t s, bool upper, bool trans, bool unit, i64 m, i64 n,
                            const float* A, i64 lda, float* B, i64 ldb);
 void library_gemm_release();
-// e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds.
+// e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds, zero again on return.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
                                    unsigned* ctl, int* e_dev);
 

@@ -256,6 +256,10 @@ __global__ __launch_bounds__(256) void absmax_exp_kernel(i64 m, i64 n, const flo
         __threadfence();
         const unsigned bits = __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         *e_out = scale_exp_of(__uint_as_float(bits));
+        // every other workgroup has added its maximum and its count: leave both words zero
+        // for the next launch
+        __hip_atomic_store(ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ctl + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -119,7 +119,7 @@ elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float
     const int hp = split ? 0 : c.prec;  // element type of the MFMA operands
     const int* e = nullptr;
     if (hp == 0) {
-        ELX_HIP(hipMemsetAsync(c.bits, 0, 2 * sizeof(unsigned), c.s));
+        // c.bits: zeroed with the solve's control words, left zero by each exponent launch
         ELX_CHECK(launch_scale_exponent(c.s, n, k, Ut, ld, c.bits, c.e_dev));
         e = c.e_dev;
     }
@@ -229,7 +229,6 @@ elx_status_t potrf_mixed(Ctx& c) {
         const bool split = (c.prec == 2);
         const int hp = split ? 0 : c.prec;
         if (hp == 0) {
-            ELX_HIP(hipMemsetAsync(c.bits, 0, 2 * sizeof(unsigned), c.s));
             ELX_CHECK(launch_scale_exponent(c.s, n2, kb, L21, ld, c.bits, c.e_dev));
             e = c.e_dev;
         }

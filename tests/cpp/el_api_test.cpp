@@ -139,6 +139,23 @@ static int run() {
         }
         check(thrown, "logic_error on non-square Cholesky", 0.0);
     }
+    {
+        // RuizEquil of a rank-one matrix u v^T with power-of-two entries: exactly the sign
+        // matrix, dRow = |u| / max|u|, dCol = |v| max|u|
+        const double u[4] = {8.0, -1.0 / 32, 1024.0, 0.25};
+        const double v[3] = {-16.0, 1.0 / 128, 4.0};
+        Matrix<double> A(4, 3), dRow, dCol;
+        for (Int j = 0; j < 3; ++j)
+            for (Int i = 0; i < 4; ++i) A(i, j) = u[i] * v[j];
+        El::RuizEquil(A, dRow, dCol);
+        bool ok = dRow.Height() == 4 && dCol.Height() == 3;
+        for (Int j = 0; j < 3 && ok; ++j)
+            for (Int i = 0; i < 4; ++i)
+                ok = ok && A(i, j) == ((u[i] * v[j] < 0) ? -1.0 : 1.0);
+        for (Int i = 0; i < 4 && ok; ++i) ok = dRow(i, 0) == std::fabs(u[i]) / 1024.0;
+        for (Int j = 0; j < 3 && ok; ++j) ok = dCol(j, 0) == std::fabs(v[j]) * 1024.0;
+        check(ok, "RuizEquil rank-one closed form", 0.0);
+    }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
 }
