@@ -169,6 +169,12 @@ elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i
 elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, float alpha,
                                   const uint16_t* A, i64 lda, const uint16_t* B, i64 ldb,
                                   float* C, i64 ldc, const int* ea_dev, const int* eb_dev);
+// The same product with B the FP32 operand itself (element (n, k) at B[n*ldb + k], readable up
+// to the next multiple of 32 past k, 16-byte aligned rows): converted to fp16/bf16 while staged,
+// no pack, no scaling of B.
+elx_status_t launch_lpgemm_bf32(hipStream_t s, int prec, i64 m, i64 n, i64 k, float alpha,
+                                const uint16_t* A, i64 lda, const float* B, i64 ldb, float* C,
+                                i64 ldc, const int* ea_dev);
 // H(r,k) = cvt(sign * 2^e * S(r,k)), S(r,k) = S[r + k*lds] (trans false) or S[k + r*lds];
 // lo (fp16 only): the low part cvt((v - float(cvt(v))) * 2^11) of v = sign * 2^e * S(r,k).
 elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,

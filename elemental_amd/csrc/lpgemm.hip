@@ -64,6 +64,43 @@ __device__ __forceinline__ void load_slab(const uint16_t* __restrict__ P, i64 ld
     }
 }
 
+// The same slab from an FP32 K-major operand (element (r, k) at P[r*ld + k]), converted to the
+// MFMA type as it is loaded (the conversion the pack kernels use); k >= K reads as zero. The
+// row must be readable up to the next multiple of 32 past K.
+template <int PREC>
+__device__ __forceinline__ void load_slab_f32(const float* __restrict__ P, i64 ld, i64 r0, i64 R,
+                                              i64 k0, i64 K, u4 (&r)[2]) {
+    typedef typename HalfT<PREC>::T HT;
+    const int t = threadIdx.x;
+    const i64 row = r0 + (t >> 1);
+    const i64 kb = k0 + 16 * (t & 1);
+    if (row < R) {
+        const float4* src = reinterpret_cast<const float4*>(P + row * ld + kb);
+        float v[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 f = src[q];
+            v[4 * q] = f.x;
+            v[4 * q + 1] = f.y;
+            v[4 * q + 2] = f.z;
+            v[4 * q + 3] = f.w;
+        }
+        uint32_t w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float a = (kb + 2 * i < K) ? v[2 * i] : 0.0f;
+            const float b = (kb + 2 * i + 1 < K) ? v[2 * i + 1] : 0.0f;
+            w[i] = (uint32_t)__builtin_bit_cast(uint16_t, (HT)a) |
+                   ((uint32_t)__builtin_bit_cast(uint16_t, (HT)b) << 16);
+        }
+        r[0] = u4{w[0], w[1], w[2], w[3]};
+        r[1] = u4{w[4], w[5], w[6], w[7]};
+    } else {
+        r[0] = u4{0, 0, 0, 0};
+        r[1] = u4{0, 0, 0, 0};
+    }
+}
+
 __device__ __forceinline__ void store_slab(uint16_t* S, const u4 (&r)[2]) {
     const int t = threadIdx.x;
     u4* dst = reinterpret_cast<u4*>(S + (t >> 1) * LDK + 16 * (t & 1));
@@ -71,10 +108,11 @@ __device__ __forceinline__ void store_slab(uint16_t* S, const u4 (&r)[2]) {
     dst[1] = r[1];
 }
 
-template <int PREC>
+// BF32: B is the FP32 operand itself (element (n, k) at B[n*ldb + k]), converted while staged.
+template <int PREC, bool BF32>
 __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float alpha,
                                                     const uint16_t* __restrict__ A, i64 lda,
-                                                    const uint16_t* __restrict__ B, i64 ldb,
+                                                    const void* __restrict__ Bv, i64 ldb,
                                                     float* __restrict__ C, i64 ldc,
                                                     const int* ea_dev, const int* eb_dev,
                                                     int tiles_m, int tiles_n) {
@@ -103,9 +141,12 @@ __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float a
 #pragma unroll
         for (int mi = 0; mi < 8; ++mi) acc[ni][mi] = f4{0.f, 0.f, 0.f, 0.f};
 
+    const uint16_t* B = static_cast<const uint16_t*>(Bv);
+    const float* Bf = static_cast<const float*>(Bv);
     u4 ra[2], rb[2];
     load_slab(A, lda, m0, M, 0, ra);
-    load_slab(B, ldb, n0, N, 0, rb);
+    if constexpr (BF32) load_slab_f32<PREC>(Bf, ldb, n0, N, 0, K, rb);
+    else load_slab(B, ldb, n0, N, 0, rb);
     store_slab(As[0], ra);
     store_slab(Bs[0], rb);
     __syncthreads();
@@ -114,7 +155,8 @@ __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float a
         const bool more = (k0 + BK < K);
         if (more) {
             load_slab(A, lda, m0, M, k0 + BK, ra);
-            load_slab(B, ldb, n0, N, k0 + BK, rb);
+            if constexpr (BF32) load_slab_f32<PREC>(Bf, ldb, n0, N, k0 + BK, K, rb);
+            else load_slab(B, ldb, n0, N, k0 + BK, rb);
         }
         const uint16_t* as = As[cur];
         const uint16_t* bs = Bs[cur];
@@ -277,12 +319,35 @@ elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, 
     ELX_ARG(tm * tn < ((i64)1 << 30));
     ProfScope prof_(s, kKLpgemm, 2.0 * (double)m * (double)n * (double)k);
     const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
+    const void* Bv = B;
     if (prec == 0)
-        hipLaunchKernelGGL(lpgemm_kernel<0>, dim3(blocks), dim3(NT), 0, s, m, n, k, alpha, A, lda,
-                           B, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
+        hipLaunchKernelGGL((lpgemm_kernel<0, false>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
+                           A, lda, Bv, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
     else
-        hipLaunchKernelGGL(lpgemm_kernel<1>, dim3(blocks), dim3(NT), 0, s, m, n, k, alpha, A, lda,
-                           B, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
+        hipLaunchKernelGGL((lpgemm_kernel<1, false>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
+                           A, lda, Bv, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_lpgemm_bf32(hipStream_t s, int prec, i64 m, i64 n, i64 k, float alpha,
+                                const uint16_t* A, i64 lda, const float* B, i64 ldb, float* C,
+                                i64 ldc, const int* ea_dev) {
+    if (m <= 0 || n <= 0 || k <= 0) return ELX_SUCCESS;
+    ELX_ARG(lda % 8 == 0 && ldb % 4 == 0 && lda >= ((k + BK - 1) / BK) * BK &&
+            ldb >= ((k + BK - 1) / BK) * BK && reinterpret_cast<uintptr_t>(B) % 16 == 0);
+    const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    ELX_ARG(tm * tn < ((i64)1 << 30));
+    ProfScope prof_(s, kKLpgemm, 2.0 * (double)m * (double)n * (double)k);
+    const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
+    const void* Bv = B;
+    const int* eb = nullptr;
+    if (prec == 0)
+        hipLaunchKernelGGL((lpgemm_kernel<0, true>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
+                           A, lda, Bv, ldb, C, ldc, ea_dev, eb, (int)tm, (int)tn);
+    else
+        hipLaunchKernelGGL((lpgemm_kernel<1, true>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
+                           A, lda, Bv, ldb, C, ldc, ea_dev, eb, (int)tm, (int)tn);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }

@@ -123,7 +123,11 @@ elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float
         ELX_CHECK(launch_scale_exponent(c.s, n, k, Ut, ld, c.bits, c.e_dev));
         e = c.e_dev;
     }
-    ELX_CHECK(launch_pack(c.s, hp, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Lh, ldpack));
+    // L needs no pack for the hand-written product: it is K-major in W already, and the GEMM
+    // converts it while staging
+    const bool l_direct = !split && k < 512;
+    if (!l_direct)
+        ELX_CHECK(launch_pack(c.s, hp, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Lh, ldpack));
     ELX_CHECK(launch_pack(c.s, hp, false, n, k, kpad, Ut, ld, 1.0f, e, c.Uh, ldpack));
     if (split) {
         ELX_CHECK(launch_pack(c.s, 0, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Ll, ldpack, true));
@@ -147,6 +151,7 @@ elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float
         ELX_CHECK(launch_lpgemm_packed(c.s, 0, n, m, k, -1.0f / 2048.0f, c.Ul, ldpack, c.Lh,
                                        ldpack, Ct, ld, e, nullptr));
     }
+    if (l_direct) return launch_lpgemm_bf32(c.s, hp, n, m, k, -1.0f, c.Uh, ldpack, Lt, ld, Ct, ld, e);
     return launch_lpgemm_packed(c.s, hp, n, m, k, -1.0f, c.Uh, ldpack, c.Lh, ldpack, Ct, ld, e,
                                 nullptr);
 }
