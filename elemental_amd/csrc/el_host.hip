@@ -446,6 +446,22 @@ void RuizEquil(Matrix<double>& A, Matrix<double>& dRow, Matrix<double>& dCol, bo
     dC.to_host(dCol);
 }
 
+// El::SymmetricRuizEquil (src/lapack_like/equilibrate/SymmetricRuizEquil.cpp:29-55): A becomes
+// diag(d)^{-1} A diag(d)^{-1}; d is resized to n x 1.
+void SymmetricRuizEquil(Matrix<double>& A, Matrix<double>& d, Int maxiter, bool progress) {
+    (void)progress;
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    if (maxiter < 0) LogicError("maxiter must be non-negative");
+    const Int n = A.Height();
+    d.Resize(n, 1);
+    DevMatrix<double> dA(A), dD(d);
+    check(elx_dsymmetric_ruiz_equil(nullptr, n, dA.ptr(), dA.ld, dD.ptr(), (int32_t)maxiter),
+          "SymmetricRuizEquil");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dA.to_host(A);
+    dD.to_host(d);
+}
+
 }  // namespace El
 
 // ====================================================================== C interface

@@ -3,7 +3,9 @@
 // rows by theirs, every scale damped (DampScaling, RuizEquil.cpp:13-21: a zero norm gives 1,
 // others at least eps^0.33 with eps = 2^-53), the scales accumulated into dCol and dRow, and
 // every entry multiplied by the reciprocal of its scale (DiagonalSolve multiplies by 1/d).
-// On return A holds diag(dRow)^{-1} A0 diag(dCol)^{-1}.
+// On return A holds diag(dRow)^{-1} A0 diag(dCol)^{-1}. The symmetric form
+// (El::SymmetricRuizEquil) takes the square root of each damped column norm as the one scale
+// of that row and column.
 //
 // The sweeps are HBM-bound passes over A. The row scaling of one sweep is fused with the
 // column norms of the next, and the column scaling with the row norms, so a sweep reads and
@@ -88,6 +90,58 @@ __global__ void ruiz_rowscale_kernel(i64 m, i64 n, T* __restrict__ A, i64 lda,
     A[i + j * lda] *= rinv[i];
 }
 
+// Symmetric sweeps (El::SymmetricRuizEquil, src/lapack_like/equilibrate/SymmetricRuizEquil.cpp:
+// 29-55): colmax[j] = max_i |A(i,j)| after the previous sweep's scaling, applied as the
+// reference does it, right then left: A(i,j) = (A(i,j) * sinv[j]) * sinv[i].
+__global__ __launch_bounds__(ENT) void ruiz_sym_cols_kernel(i64 n, double* __restrict__ A, i64 lda,
+                                                            const double* __restrict__ sinv,
+                                                            double* __restrict__ colmax) {
+    const i64 j = blockIdx.x;
+    double* col = A + j * lda;
+    const double sj = sinv ? sinv[j] : 1.0;
+    double mx = 0.0;
+    for (i64 i = threadIdx.x; i < n; i += ENT) {
+        double v = col[i];
+        if (sinv) {
+            v = v * sj;
+            v = v * sinv[i];
+            col[i] = v;
+        }
+        mx = fmax(mx, fabs(v));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    __shared__ double wm[ENT / 64];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = wm[0];
+        for (int w = 1; w < ENT / 64; ++w) r = fmax(r, wm[w]);
+        colmax[j] = r;
+    }
+}
+
+// s = sqrt(damp(colmax)); d *= s; sinv = 1/s
+__global__ void ruiz_sym_damp_kernel(i64 n, const double* __restrict__ colmax, double tol,
+                                     double* __restrict__ d, double* __restrict__ sinv) {
+    const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
+    if (i >= n) return;
+    double s = colmax[i];
+    s = (s == 0.0) ? 1.0 : (s < tol ? tol : s);
+    s = sqrt(s);
+    d[i] *= s;
+    sinv[i] = 1.0 / s;
+}
+
+__global__ void ruiz_sym_scale_kernel(i64 n, double* __restrict__ A, i64 lda,
+                                      const double* __restrict__ sinv, i64 j0) {
+    const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
+    const i64 j = j0 + blockIdx.y;
+    if (i >= n) return;
+    double* a = A + i + j * lda;
+    *a = (*a * sinv[j]) * sinv[i];
+}
+
 template <typename T>
 __global__ void ruiz_ones_kernel(i64 len, T* __restrict__ d) {
     const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
@@ -136,6 +190,30 @@ elx_status_t launch_ruiz_equil(hipStream_t s, i64 m, i64 n, double* A, i64 lda, 
     return ELX_SUCCESS;
 }
 
+elx_status_t launch_symmetric_ruiz_equil(hipStream_t s, i64 n, double* A, i64 lda, double* d,
+                                         int maxiter, double* work) {
+    if (n <= 0) return ELX_SUCCESS;
+    ELX_ARG(n < (1ll << 31));
+    const double tol = ruiz_tolerance();
+    double* colmax = work;     // n
+    double* sinv = work + n;   // n
+    hipLaunchKernelGGL(ruiz_ones_kernel<double>, dim3(cdiv(n, ENT)), dim3(ENT), 0, s, n, d);
+    for (int it = 0; it < maxiter; ++it) {
+        hipLaunchKernelGGL(ruiz_sym_cols_kernel, dim3((unsigned)n), dim3(ENT), 0, s, n, A, lda,
+                           it ? sinv : nullptr, colmax);
+        hipLaunchKernelGGL(ruiz_sym_damp_kernel, dim3(cdiv(n, ENT)), dim3(ENT), 0, s, n, colmax, tol,
+                           d, sinv);
+    }
+    if (maxiter > 0)
+        for (i64 j0 = 0; j0 < n; j0 += 65535) {
+            const i64 nj = std::min<i64>(65535, n - j0);
+            hipLaunchKernelGGL(ruiz_sym_scale_kernel, dim3(cdiv(n, ENT), (unsigned)nj), dim3(ENT), 0, s,
+                               n, A, lda, sinv, j0);
+        }
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
 namespace {
 __global__ void reciprocal_kernel(i64 n, const double* __restrict__ in, double* __restrict__ out) {
     const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
@@ -168,6 +246,17 @@ elx_status_t elx_druiz_equil(void* stream, int64_t m, int64_t n, double* dA, int
                                sizeof(double) * std::max<size_t>(elx::ruiz_work_doubles(m, n), 1), &w));
     return elx::launch_ruiz_equil(static_cast<hipStream_t>(stream), m, n, dA, lda, dRow, dCol,
                                   static_cast<double*>(w));
+}
+
+elx_status_t elx_dsymmetric_ruiz_equil(void* stream, int64_t n, double* dA, int64_t lda, double* dD,
+                                       int32_t maxiter) {
+    ELX_ARG(n >= 0 && lda >= std::max<int64_t>(n, 1) && maxiter >= 0);
+    ELX_ARG(n == 0 || (dA && dD));
+    void* w = nullptr;
+    ELX_CHECK(elx::scratch_get(elx::kSlotRuizWork, sizeof(double) * (size_t)std::max<int64_t>(2 * n, 1),
+                               &w));
+    return elx::launch_symmetric_ruiz_equil(static_cast<hipStream_t>(stream), n, dA, lda, dD, maxiter,
+                                            static_cast<double*>(w));
 }
 
 }  // extern "C"

@@ -297,7 +297,6 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
                                 int32_t max_iters, elx_ir_stats_t* stats) {
     const bool equil = (factor_flags & ELX_FACTOR_EQUILIBRATE) != 0;
     const int factor_type = factor_flags & ~ELX_FACTOR_EQUILIBRATE;
-    if (equil && kind != 0) return ELX_ERROR_NOT_SUPPORTED;
     ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) &&
             ldb >= std::max<int64_t>(n, 1) && ldx >= std::max<int64_t>(n, 1));
     ELX_ARG(factor_type == ELX_FACTOR_FP16 || factor_type == ELX_FACTOR_BF16 ||
@@ -363,8 +362,9 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(launch_convert(s, n, dA, lda, c.W, c.ldw, part, nch));
     ELX_HIP(hipMemsetAsync(anorm_d, 0, sizeof(double), s));
     ELX_CHECK(launch_rowabs_max(s, n, nch, part, anorm_d));
-    // Equilibrated: W from diag(dRow)^-1 A diag(dCol)^-1 (an FP64 copy); the preconditioner
-    // scales by sr = 1/dRow before and sc = 1/dCol after the triangular solves.
+    // Equilibrated: W from diag(dRow)^-1 A diag(dCol)^-1 (an FP64 copy; symmetric, dRow = dCol,
+    // for the Cholesky); the preconditioner scales by sr = 1/dRow before and sc = 1/dCol after
+    // the triangular solves.
     double* sr = nullptr;
     double* sc = nullptr;
     if (equil) {
@@ -376,11 +376,18 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
         ELX_CHECK(get(kSlotEquilW, ruiz_work_doubles(n, n), &ew));
         ELX_HIP(hipMemcpy2DAsync(Aeq, sizeof(double) * n, dA, sizeof(double) * lda,
                                  sizeof(double) * n, n, hipMemcpyDeviceToDevice, s));
-        ELX_CHECK(launch_ruiz_equil(s, n, n, Aeq, n, ev, ev + n, ew));
         sr = ev + 2 * n;
-        sc = ev + 3 * n;
-        ELX_CHECK(launch_reciprocal(s, n, ev, sr));
-        ELX_CHECK(launch_reciprocal(s, n, ev + n, sc));
+        if (kind == 0) {
+            ELX_CHECK(launch_ruiz_equil(s, n, n, Aeq, n, ev, ev + n, ew));
+            sc = ev + 3 * n;
+            ELX_CHECK(launch_reciprocal(s, n, ev, sr));
+            ELX_CHECK(launch_reciprocal(s, n, ev + n, sc));
+        } else {
+            // symmetric scaling keeps the Cholesky's input symmetric: A = D Aeq D
+            ELX_CHECK(launch_symmetric_ruiz_equil(s, n, Aeq, n, ev, 3, ew));
+            ELX_CHECK(launch_reciprocal(s, n, ev, sr));
+            sc = sr;
+        }
         ELX_CHECK(launch_convert(s, n, Aeq, n, c.W, c.ldw, part, nch));
     }
     if (kind == 0) {
@@ -804,7 +811,6 @@ elx_status_t elx_dposv_mixed(void* stream, int uplo, int64_t n, int64_t nrhs, co
                              int32_t max_iters, elx_ir_stats_t* stats) {
     ELX_ARG(uplo == kLower || uplo == kUpper);
     ELX_ARG(n >= 0 && lda >= std::max<int64_t>(n, 1));
-    if (factor_type & ELX_FACTOR_EQUILIBRATE) return ELX_ERROR_NOT_SUPPORTED;
     if (n == 0) return mixed_solve(1, stream, n, nrhs, dA, lda, dB, ldb, dX, ldx, factor_type,
                                    refine, tol, max_iters, stats);
     // the full symmetric matrix from the stored triangle (the residuals read all of it)

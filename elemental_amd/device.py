@@ -150,6 +150,16 @@ def ruiz_equil(A, stream=None):
     return dRow, dCol
 
 
+def symmetric_ruiz_equil(A, maxiter=3, stream=None):
+    """El::SymmetricRuizEquil in place on a square column-major FP64 device matrix; returns d."""
+    import torch
+    pA, lda = _mat(A)
+    n = A.shape[0]
+    d = torch.empty(n, dtype=torch.float64, device=A.device)
+    call("elx_dsymmetric_ruiz_equil", _stream(stream), n, pA, lda, d.data_ptr(), maxiter)
+    return d
+
+
 def permute_rows(dests, A, inverse=False, stream=None):
     pA, lda = _mat(A)
     call("elx_dpermute_rows", _stream(stream), 1 if inverse else 0, dests.data_ptr(),
@@ -178,17 +188,18 @@ def gesv_mixed(A, B, X, factor="fp16", refine="auto", tol=0.0, max_iters=0, stre
 
 
 def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_iters=0,
-               stream=None):
+               stream=None, equilibrate=False):
     """Low-precision Cholesky + FP64 refinement of an HPD A (its `uplo` triangle); returns the
-    stats dict (synchronous), with "status" 0 or 5 (tolerance not reached)."""
+    stats dict (synchronous), with "status" 0 or 5 (tolerance not reached). With equilibrate,
+    the Cholesky factors the symmetrically equilibrated matrix (ELX_FACTOR_EQUILIBRATE)."""
     pA, lda = _mat(A)
     pB, ldb = _mat(B)
     pX, ldx = _mat(X)
     st = IrStats()
     import ctypes
     status = load().elx_dposv_mixed(_stream(stream), uplo, A.shape[0], B.shape[1], pA, lda, pB,
-                                     ldb, pX, ldx, FACTOR[factor], REFINE[refine], tol, max_iters,
-                                     ctypes.byref(st))
+                                     ldb, pX, ldx, FACTOR[factor] | (0x100 if equilibrate else 0),
+                                     REFINE[refine], tol, max_iters, ctypes.byref(st))
     if status not in (0, 5):
         from ._lib import ElxError
         raise ElxError("elx_dposv_mixed", status)
@@ -199,7 +210,7 @@ def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_i
 
 __all__ = [
     "load", "getrf", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
-    "permute_rows", "ruiz_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
+    "permute_rows", "ruiz_equil", "symmetric_ruiz_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
     "Batch", "potrf_vbatched", "potrs_vbatched", "gesv_mixed_batched",
 ]
 

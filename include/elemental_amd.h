@@ -122,6 +122,15 @@ elx_status_t elx_dpermute_rows(void* stream, int inverse, const int32_t* dDests,
 elx_status_t elx_druiz_equil(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
                              double* dRow, double* dCol);
 
+/* Symmetric Ruiz equilibration in place: maxiter sweeps (the reference's default is 3), each
+ * scaling row and column j by the square root of column j's damped max norm; on return A
+ * holds diag(d)^-1 A0 diag(d)^-1 (bitwise the reference's result) and dD (n) the scales.
+ * Replaces El::SymmetricRuizEquil(Matrix<double>& A, Matrix<double>& d, Int maxiter, bool) --
+ * src/lapack_like/equilibrate/SymmetricRuizEquil.cpp:29-55, include/El/lapack_like/
+ * equilibrate.hpp:80-84. */
+elx_status_t elx_dsymmetric_ruiz_equil(void* stream, int64_t n, double* dA, int64_t lda,
+                                       double* dD, int32_t maxiter);
+
 /* ------------------------------------------------------------------------------------
  * Mixed precision: low-precision LU + FP64 iterative refinement (no reference
  * counterpart for dense matrices; the refinement loop follows the semantics of
@@ -134,11 +143,12 @@ typedef enum {
                              and low parts, three fp16 MFMA products (hi*hi + hi*lo + lo*hi) */
     ELX_FACTOR_FP16 = 2,
     ELX_FACTOR_BF16 = 3,
-    /* Flag, OR'ed into a factor type of elx_dgesv_mixed (not elx_dposv_mixed): the
-     * low-precision LU factors the Ruiz-equilibrated diag(dRow)^-1 A diag(dCol)^-1 of an FP64
-     * copy of A (elx_druiz_equil), the preconditioner applies the scales, and the residuals
-     * and the FP64 fallback use A itself. For badly scaled A; costs an n x n FP64 copy and
-     * eight passes over it. */
+    /* Flag, OR'ed into a factor type of the mixed solvers: the low-precision LU factors the
+     * Ruiz-equilibrated diag(dRow)^-1 A diag(dCol)^-1 of an FP64 copy of A (elx_druiz_equil),
+     * the Cholesky the symmetrically equilibrated diag(d)^-1 A diag(d)^-1
+     * (elx_dsymmetric_ruiz_equil, 3 sweeps); the preconditioner applies the scales, and the
+     * residuals and the FP64 fallback use A itself. For badly scaled A; costs an n x n FP64
+     * copy and up to eight passes over it. */
     ELX_FACTOR_EQUILIBRATE = 0x100
 } elx_factor_type_t;
 

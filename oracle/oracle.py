@@ -326,3 +326,22 @@ def ruiz_equil(A):
         dRow *= rs
         A *= (1.0 / rs)[:, None]
     return A, dRow, dCol
+
+
+def symmetric_ruiz_equil(A, maxiter=3):
+    """El::SymmetricRuizEquil (src/lapack_like/equilibrate/SymmetricRuizEquil.cpp:29-55) on a
+    copy of the square A: maxiter sweeps (default 3, include/El/lapack_like/equilibrate.hpp:
+    80-84); each takes the column max norms, damps them (DampScaling, :13-21), takes their
+    square roots (SquareRootScaling, :23-27), multiplies them into d, then every entry by the
+    reciprocal of its column's scale (DiagonalSolve RIGHT) and after that of its row's
+    (DiagonalSolve LEFT). Returns (A_eq, d) with A_eq = diag(d)^-1 A diag(d)^-1."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = A.shape[0]
+    d = np.ones(n)
+    for _ in range(maxiter):
+        s = np.sqrt(ruiz_damp(np.abs(A).max(axis=0))) if n > 0 else np.ones(0)
+        d *= s
+        inv = 1.0 / s
+        A *= inv[None, :]
+        A *= inv[:, None]
+    return A, d

@@ -64,15 +64,49 @@ def test_mixed_equilibrated_badly_scaled(dev, orc, n):
     assert np.array_equal(dev.to_host(dA), A)  # A is not modified
 
 
-def test_mixed_equilibrate_rejected_for_hpd(dev):
-    n = 64
-    A = np.asfortranarray(np.eye(n) * 2.0)
-    dA, dB = dev.to_device(A), dev.to_device(np.ones((n, 1)))
-    dX = dev.colmajor_empty(n, 1)
-    import ctypes
-    from elemental_amd._lib import IrStats, load
-    st = IrStats()
-    pA, pB, pX = dA.data_ptr(), dB.data_ptr(), dX.data_ptr()
-    r = load().elx_dposv_mixed(None, 0, n, 1, pA, n, pB, n, pX, n, 2 | 0x100, 3, 0.0, 0,
-                               ctypes.byref(st))
-    assert r == 4  # ELX_ERROR_NOT_SUPPORTED
+@pytest.mark.parametrize("n,ld_extra,maxiter", [(300, 0, 3), (1000, 17, 3), (1, 0, 3),
+                                                (257, 3, 1), (64, 0, 0)])
+def test_symmetric_ruiz_bitwise(dev, orc, n, ld_extra, maxiter):
+    say("symmetric ruiz", n, ld_extra, maxiter)
+    rng = np.random.default_rng(n + maxiter)
+    G = rng.standard_normal((n, n))
+    D = 2.0 ** rng.uniform(-40, 40, n)
+    A = np.asfortranarray(D[:, None] * (G @ G.T / n + np.eye(n)) * D[None, :])
+    if n > 4:
+        A[3, :] = 0.0
+        A[:, 3] = 0.0
+    Ae, d = orc.symmetric_ruiz_equil(A, maxiter=maxiter)
+    T = torch.zeros((n, n + ld_extra), dtype=torch.float64, device="cuda").t()[:n, :]
+    T.copy_(torch.from_numpy(A))
+    gd = dev.symmetric_ruiz_equil(T, maxiter=maxiter)
+    torch.cuda.synchronize()
+    assert np.array_equal(T.cpu().numpy(), Ae)
+    assert np.array_equal(gd.cpu().numpy(), d)
+
+
+@pytest.mark.parametrize("uplo", [0, 1])
+def test_mixed_hpd_equilibrated_badly_scaled(dev, orc, uplo):
+    """An HPD matrix with its rows and columns scaled by 2^-20 .. 2^20: the fp16 Cholesky of the
+    symmetrically equilibrated matrix preconditions the refinement; every column reaches n u
+    without the FP64 fallback, and only the uplo triangle is read."""
+    n = 1500
+    rng = np.random.default_rng(11 + uplo)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    core = (Q * 10.0 ** (-2 * np.arange(n) / (n - 1))[None, :]) @ Q.T
+    D = 2.0 ** rng.uniform(-20, 20, n)
+    A = D[:, None] * ((core + core.T) / 2) * D[None, :]
+    Ast = A.copy(order="F")
+    if uplo == 0:
+        Ast[np.triu_indices(n, 1)] = np.nan
+    else:
+        Ast[np.tril_indices(n, -1)] = np.nan
+    B = rng.standard_normal((n, 2))
+    dA, dB = dev.to_device(Ast), dev.to_device(B)
+    dX = dev.colmajor_empty(n, 2)
+    st = dev.posv_mixed(dA, dB, dX, uplo=uplo, factor="fp16", refine="auto", equilibrate=True)
+    X = dev.to_host(dX)
+    eta = orc.backward_error(A, X, B)
+    say("hpd equilibrated", uplo, st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["converged"] == 1 and st["fell_back"] == 0 and st["factor_info"] == 0
+    assert eta <= n * EPS
+    assert np.array_equal(dev.to_host(dA), Ast, equal_nan=True)

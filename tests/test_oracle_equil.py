@@ -50,3 +50,43 @@ def test_reconstruction_and_unit_row_norms():
     # scaling balances the matrix: column norms near 1 as well
     cm = np.abs(Ae).max(axis=0)
     assert np.all(cm[np.arange(n) != 7] > 0.5) and cm[7] == 0.0 and dC[7] == 1.0
+
+
+def test_symmetric_diagonal_powers_of_four():
+    # a diagonal of powers of four: the first sweep's scales are exact square roots and leave
+    # the identity; later sweeps see unit norms
+    dg = np.array([4.0 ** 5, 4.0 ** -3, 1.0, 4.0 ** 9])
+    Ae, d = orc.symmetric_ruiz_equil(np.diag(dg))
+    assert np.array_equal(Ae, np.eye(4))
+    assert np.array_equal(d, np.sqrt(dg))
+
+
+def test_symmetric_rank_one_powers_of_four():
+    # A = u u^T with u a signed power of 16 per entry: sweep 1 scales column j by
+    # sqrt(|u_j| max|u|), sweep 2 by the square root of the current column norm; for these u
+    # both sweeps' scales are powers of two, so the result is exact: A_eq(i,j) =
+    # sign(u_i u_j) r_i r_j with r = sqrt(|u| / max|u|) after sweep 1 and r -> r^(1/2) after 2
+    u = np.array([16.0 ** 4, -16.0 ** -2, 16.0 ** 2, 1.0])
+    for iters in (1, 2):
+        Ae, d = orc.symmetric_ruiz_equil(np.outer(u, u), maxiter=iters)
+        r = np.sqrt(np.abs(u) / np.abs(u).max())
+        for _ in range(iters - 1):
+            r = np.sqrt(r)
+        assert np.array_equal(Ae, np.sign(np.outer(u, u)) * np.outer(r, r))
+        assert np.array_equal(d[:, None] * Ae * d[None, :], np.outer(u, u))
+
+
+def test_symmetric_reconstruction():
+    rng = np.random.default_rng(5)
+    n = 50
+    G = rng.standard_normal((n, n))
+    D = 2.0 ** rng.uniform(-25, 25, n)
+    A = D[:, None] * (G @ G.T / n + np.eye(n)) * D[None, :]
+    Ae, d = orc.symmetric_ruiz_equil(A)
+    R = d[:, None] * Ae * d[None, :]
+    assert np.max(np.abs(R - A) / np.abs(A)) <= 8 * 2.0 ** -53
+    # each sweep takes the square root of the column-norm imbalance: after three the spread
+    # (in bits) is at most a quarter of the original, and no norm exceeds 1
+    cm0, cm = np.abs(A).max(axis=0), np.abs(Ae).max(axis=0)
+    assert cm.max() <= 1.0 + 1e-12
+    assert np.log2(cm.max() / cm.min()) <= np.log2(cm0.max() / cm0.min()) / 4
