@@ -64,6 +64,7 @@ _SIGS = {
     "elx_dpermute_rows": [c_vp, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64],
     "elx_druiz_equil": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
     "elx_dsymmetric_ruiz_equil": [c_vp, c_i64, c_vp, c_i64, c_vp, c_i32],
+    "elx_dgeom_equil": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
     "elx_dgesv_mixed": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int, c_int,
                         c_dbl, c_i32, ctypes.POINTER(IrStats)],
     "elx_dposv_mixed": [c_vp, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int,

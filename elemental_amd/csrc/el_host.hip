@@ -446,6 +446,21 @@ void RuizEquil(Matrix<double>& A, Matrix<double>& dRow, Matrix<double>& dCol, bo
     dC.to_host(dCol);
 }
 
+// El::GeomEquil (src/lapack_like/equilibrate/GeomEquil.cpp:26-116): A becomes
+// diag(dRow)^{-1} A diag(dCol)^{-1}; dRow, dCol are resized to m x 1 and n x 1.
+void GeomEquil(Matrix<double>& A, Matrix<double>& dRow, Matrix<double>& dCol, bool progress) {
+    (void)progress;
+    const Int m = A.Height(), n = A.Width();
+    dRow.Resize(m, 1);
+    dCol.Resize(n, 1);
+    DevMatrix<double> dA(A), dR(dRow), dC(dCol);
+    check(elx_dgeom_equil(nullptr, m, n, dA.ptr(), dA.ld, dR.ptr(), dC.ptr()), "GeomEquil");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dA.to_host(A);
+    dR.to_host(dRow);
+    dC.to_host(dCol);
+}
+
 // El::SymmetricRuizEquil (src/lapack_like/equilibrate/SymmetricRuizEquil.cpp:29-55): A becomes
 // diag(d)^{-1} A diag(d)^{-1}; d is resized to n x 1.
 void SymmetricRuizEquil(Matrix<double>& A, Matrix<double>& d, Int maxiter, bool progress) {

@@ -3,7 +3,8 @@
 // rows by theirs, every scale damped (DampScaling, RuizEquil.cpp:13-21: a zero norm gives 1,
 // others at least eps^0.33 with eps = 2^-53), the scales accumulated into dCol and dRow, and
 // every entry multiplied by the reciprocal of its scale (DiagonalSolve multiplies by 1/d).
-// On return A holds diag(dRow)^{-1} A0 diag(dCol)^{-1}. The symmetric form
+// On return A holds diag(dRow)^{-1} A0 diag(dCol)^{-1}. Geometric equilibration (El::GeomEquil)
+// is here too. The symmetric form
 // (El::SymmetricRuizEquil) takes the square root of each damped column norm as the one scale
 // of that row and column.
 //
@@ -142,6 +143,129 @@ __global__ void ruiz_sym_scale_kernel(i64 n, double* __restrict__ A, i64 lda,
     *a = (*a * sinv[j]) * sinv[i];
 }
 
+// Geometric equilibration (El::GeomEquil, src/lapack_like/equilibrate/GeomEquil.cpp:26-116):
+// a column (row) with max |a| = mx > 0 and smallest nonzero |a| = mn is multiplied by 1/s,
+// s = max(sqrt(mn * mx), sqrt(1/1000) * mx) (s = mx in the closing column pass), and s goes
+// into dCol (dRow); all-zero columns and rows are left alone.
+__device__ __forceinline__ double geom_scale(double mx, double mn, double sqrt_damp, int final_pass) {
+    if (final_pass) return mx;
+    const double prop = sqrt(mn * mx);
+    const double floor_ = sqrt_damp * mx;
+    return prop < floor_ ? floor_ : prop;
+}
+
+// max |v| and the smallest nonzero |v| of a workgroup (mn = +inf when there is none)
+__device__ __forceinline__ void block_maxmin(double& mx, double& mn) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off));
+        mn = fmin(mn, __shfl_xor(mn, off));
+    }
+    __shared__ double wx[ENT / 64], wn[ENT / 64];
+    if ((threadIdx.x & 63) == 0) {
+        wx[threadIdx.x >> 6] = mx;
+        wn[threadIdx.x >> 6] = mn;
+    }
+    __syncthreads();
+    mx = wx[0];
+    mn = wn[0];
+    for (int w = 1; w < ENT / 64; ++w) {
+        mx = fmax(mx, wx[w]);
+        mn = fmin(mn, wn[w]);
+    }
+    __syncthreads();
+}
+
+// one workgroup per column: statistics, then the column scaled by 1/s
+__global__ __launch_bounds__(ENT) void geom_cols_kernel(i64 m, double* __restrict__ A, i64 lda,
+                                                        double* __restrict__ dCol, double sqrt_damp,
+                                                        int final_pass) {
+    const i64 j = blockIdx.x;
+    double* col = A + j * lda;
+    double mx = 0.0, mn = INFINITY;
+    for (i64 i = threadIdx.x; i < m; i += ENT) {
+        const double v = fabs(col[i]);
+        mx = fmax(mx, v);
+        if (v > 0.0) mn = fmin(mn, v);
+    }
+    block_maxmin(mx, mn);
+    if (!(mx > 0.0)) return;
+    const double sc = geom_scale(mx, mn, sqrt_damp, final_pass);
+    const double inv = 1.0 / sc;
+    for (i64 i = threadIdx.x; i < m; i += ENT) col[i] *= inv;
+    if (threadIdx.x == 0) dCol[j] *= sc;
+}
+
+// partial row statistics over 64-column chunks: pmax[c][i], pmin[c][i]
+__global__ __launch_bounds__(ENT) void geom_rowstats_kernel(i64 m, i64 n, const double* __restrict__ A,
+                                                            i64 lda, double* __restrict__ pmax,
+                                                            double* __restrict__ pmin) {
+    const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
+    const i64 j0 = (i64)blockIdx.y * ECH;
+    if (i >= m) return;
+    const i64 j1 = j0 + ECH < n ? j0 + ECH : n;
+    double mx = 0.0, mn = INFINITY;
+    for (i64 j = j0; j < j1; ++j) {
+        const double v = fabs(A[i + j * lda]);
+        mx = fmax(mx, v);
+        if (v > 0.0) mn = fmin(mn, v);
+    }
+    pmax[(i64)blockIdx.y * m + i] = mx;
+    pmin[(i64)blockIdx.y * m + i] = mn;
+}
+
+// row scales from the partials: rinv = 1/s (1 for an all-zero row), dRow *= s
+__global__ void geom_rowscale_kernel(i64 m, int nch, const double* __restrict__ pmax,
+                                     const double* __restrict__ pmin, double sqrt_damp,
+                                     double* __restrict__ dRow, double* __restrict__ rinv) {
+    const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
+    if (i >= m) return;
+    double mx = 0.0, mn = INFINITY;
+    for (int c = 0; c < nch; ++c) {
+        mx = fmax(mx, pmax[(i64)c * m + i]);
+        mn = fmin(mn, pmin[(i64)c * m + i]);
+    }
+    if (mx > 0.0) {
+        const double sc = geom_scale(mx, mn, sqrt_damp, 0);
+        rinv[i] = 1.0 / sc;
+        dRow[i] *= sc;
+    } else {
+        rinv[i] = 1.0;
+    }
+}
+
+// A(i,j) *= rinv[i] (columns j0 ..), with the whole matrix's max and smallest nonzero |a|
+// folded into out[0], out[1] (as ordered bit patterns: the values are non-negative)
+__global__ __launch_bounds__(ENT) void geom_rows_apply_kernel(i64 m, double* __restrict__ A, i64 lda,
+                                                              const double* __restrict__ rinv,
+                                                              i64 j0, int scale,
+                                                              unsigned long long* out) {
+    const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
+    const i64 j = j0 + blockIdx.y;
+    double mx = 0.0, mn = INFINITY;
+    if (i < m) {
+        double* a = A + i + j * lda;
+        double v = *a;
+        if (scale) {
+            v *= rinv[i];
+            *a = v;
+        }
+        v = fabs(v);
+        mx = v;
+        if (v > 0.0) mn = v;
+    }
+    block_maxmin(mx, mn);
+    if (threadIdx.x == 0) {
+        atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+        atomicMin(out + 1, (unsigned long long)__double_as_longlong(mn));
+    }
+}
+
+__global__ void geom_stats_init_kernel(unsigned long long* out) {
+    out[0] = 0ull;
+    out[1] = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
 template <typename T>
 __global__ void ruiz_ones_kernel(i64 len, T* __restrict__ d) {
     const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
@@ -214,6 +338,64 @@ elx_status_t launch_symmetric_ruiz_equil(hipStream_t s, i64 n, double* A, i64 ld
     return ELX_SUCCESS;
 }
 
+// El::GeomEquil: up to six sweeps (at least four when the ratio max/min-nonzero keeps falling
+// by 10% or more), then every column normalised to max |a| = 1. The stopping test reads the
+// ratio on the host after each sweep.
+elx_status_t launch_geom_equil(hipStream_t s, i64 m, i64 n, double* A, i64 lda, double* dRow,
+                               double* dCol, double* work) {
+    if (m > 0) hipLaunchKernelGGL(ruiz_ones_kernel<double>, dim3(cdiv(m, ENT)), dim3(ENT), 0, s, m, dRow);
+    if (n > 0) hipLaunchKernelGGL(ruiz_ones_kernel<double>, dim3(cdiv(n, ENT)), dim3(ENT), 0, s, n, dCol);
+    ELX_HIP(hipGetLastError());
+    if (m <= 0 || n <= 0) return ELX_SUCCESS;
+    ELX_ARG(n < (1ll << 31) && (n + ECH - 1) / ECH < 65536);
+    const int nch = (int)((n + ECH - 1) / ECH);
+    double* rinv = work;              // m
+    double* pmax = rinv + m;          // nch * m
+    double* pmin = pmax + nch * m;    // nch * m
+    unsigned long long* st = reinterpret_cast<unsigned long long*>(pmin + nch * m);  // 2
+    const double sqrt_damp = std::sqrt(1.0 / 1000.0);
+    // max |a| and smallest nonzero |a| of the whole matrix (optionally after a row scaling)
+    auto stats = [&](int scale, double* mx, double* mn) -> elx_status_t {
+        hipLaunchKernelGGL(geom_stats_init_kernel, dim3(1), dim3(1), 0, s, st);
+        for (i64 j0 = 0; j0 < n; j0 += 65535) {
+            const i64 nj = std::min<i64>(65535, n - j0);
+            hipLaunchKernelGGL(geom_rows_apply_kernel, dim3(cdiv(m, ENT), (unsigned)nj), dim3(ENT), 0, s,
+                               m, A, lda, rinv, j0, scale, st);
+        }
+        ELX_HIP(hipGetLastError());
+        unsigned long long h[2];
+        ELX_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+        ELX_HIP(hipStreamSynchronize(s));
+        *mx = __builtin_bit_cast(double, h[0]);
+        *mn = __builtin_bit_cast(double, h[1]);
+        return ELX_SUCCESS;
+    };
+    double mx = 0.0, mn = 0.0;
+    ELX_CHECK(stats(0, &mx, &mn));
+    if (mx == 0.0) return ELX_SUCCESS;
+    double ratio = mx / std::min(mn, mx);
+    for (int it = 0; it < 6; ++it) {
+        hipLaunchKernelGGL(geom_cols_kernel, dim3((unsigned)n), dim3(ENT), 0, s, m, A, lda, dCol,
+                           sqrt_damp, 0);
+        hipLaunchKernelGGL(geom_rowstats_kernel, dim3(cdiv(m, ENT), (unsigned)nch), dim3(ENT), 0, s, m,
+                           n, A, lda, pmax, pmin);
+        hipLaunchKernelGGL(geom_rowscale_kernel, dim3(cdiv(m, ENT)), dim3(ENT), 0, s, m, nch, pmax, pmin,
+                           sqrt_damp, dRow, rinv);
+        ELX_CHECK(stats(1, &mx, &mn));
+        const double new_ratio = mx / std::min(mn, mx);
+        if (it >= 3 && new_ratio >= ratio * (9.0 / 10.0)) break;
+        ratio = new_ratio;
+    }
+    hipLaunchKernelGGL(geom_cols_kernel, dim3((unsigned)n), dim3(ENT), 0, s, m, A, lda, dCol, sqrt_damp, 1);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+size_t geom_work_doubles(i64 m, i64 n) {
+    const i64 nch = (n + ECH - 1) / ECH;
+    return (size_t)(m + 2 * nch * m + 2);
+}
+
 namespace {
 __global__ void reciprocal_kernel(i64 n, const double* __restrict__ in, double* __restrict__ out) {
     const i64 i = (i64)blockIdx.x * ENT + threadIdx.x;
@@ -257,6 +439,17 @@ elx_status_t elx_dsymmetric_ruiz_equil(void* stream, int64_t n, double* dA, int6
                                &w));
     return elx::launch_symmetric_ruiz_equil(static_cast<hipStream_t>(stream), n, dA, lda, dD, maxiter,
                                             static_cast<double*>(w));
+}
+
+elx_status_t elx_dgeom_equil(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
+                             double* dRow, double* dCol) {
+    ELX_ARG(m >= 0 && n >= 0 && lda >= std::max<int64_t>(m, 1));
+    ELX_ARG((m == 0 || dRow) && (n == 0 || dCol) && (m == 0 || n == 0 || dA));
+    void* w = nullptr;
+    ELX_CHECK(elx::scratch_get(elx::kSlotRuizWork,
+                               sizeof(double) * std::max<size_t>(elx::geom_work_doubles(m, n), 1), &w));
+    return elx::launch_geom_equil(static_cast<hipStream_t>(stream), m, n, dA, lda, dRow, dCol,
+                                  static_cast<double*>(w));
 }
 
 }  // extern "C"

@@ -55,6 +55,10 @@ elx_status_t launch_update_f64(hipStream_t s, i64 n, const float* d, const doubl
 elx_status_t launch_ruiz_equil(hipStream_t s, i64 m, i64 n, double* A, i64 lda, double* dRow,
                                double* dCol, double* work);
 size_t ruiz_work_doubles(i64 m, i64 n);
+// El::GeomEquil of the m x n A in place; work holds geom_work_doubles(m, n) doubles
+elx_status_t launch_geom_equil(hipStream_t s, i64 m, i64 n, double* A, i64 lda, double* dRow,
+                               double* dCol, double* work);
+size_t geom_work_doubles(i64 m, i64 n);
 // El::SymmetricRuizEquil of the n x n A in place (maxiter sweeps); work holds 2n doubles
 elx_status_t launch_symmetric_ruiz_equil(hipStream_t s, i64 n, double* A, i64 lda, double* d,
                                          int maxiter, double* work);

@@ -150,6 +150,17 @@ def ruiz_equil(A, stream=None):
     return dRow, dCol
 
 
+def geom_equil(A, stream=None):
+    """El::GeomEquil in place on a column-major FP64 device matrix; returns (dRow, dCol)."""
+    import torch
+    pA, lda = _mat(A)
+    m, n = A.shape
+    dRow = torch.empty(m, dtype=torch.float64, device=A.device)
+    dCol = torch.empty(n, dtype=torch.float64, device=A.device)
+    call("elx_dgeom_equil", _stream(stream), m, n, pA, lda, dRow.data_ptr(), dCol.data_ptr())
+    return dRow, dCol
+
+
 def symmetric_ruiz_equil(A, maxiter=3, stream=None):
     """El::SymmetricRuizEquil in place on a square column-major FP64 device matrix; returns d."""
     import torch
@@ -210,7 +221,7 @@ def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_i
 
 __all__ = [
     "load", "getrf", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
-    "permute_rows", "ruiz_equil", "symmetric_ruiz_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
+    "permute_rows", "ruiz_equil", "symmetric_ruiz_equil", "geom_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
     "Batch", "potrf_vbatched", "potrs_vbatched", "gesv_mixed_batched",
 ]
 

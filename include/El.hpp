@@ -179,6 +179,10 @@ void HPDSolve(UpperOrLower uplo, Orientation orientation, const Matrix<F>& A, Ma
 // (include/El/lapack_like/equilibrate.hpp:17-21). Defined for double.
 void RuizEquil(Matrix<double>& A, Matrix<double>& dRow, Matrix<double>& dCol,
                bool progress = false);
+// A := diag(dRow)^{-1} A diag(dCol)^{-1}, geometric scaling (GeomEquil.cpp:26-116).
+// Defined for double.
+void GeomEquil(Matrix<double>& A, Matrix<double>& dRow, Matrix<double>& dCol,
+               bool progress = false);
 // A := diag(d)^{-1} A diag(d)^{-1}, maxiter symmetric sweeps
 // (include/El/lapack_like/equilibrate.hpp:80-84). Defined for double.
 void SymmetricRuizEquil(Matrix<double>& A, Matrix<double>& d, Int maxiter = 3,

@@ -122,6 +122,16 @@ elx_status_t elx_dpermute_rows(void* stream, int inverse, const int32_t* dDests,
 elx_status_t elx_druiz_equil(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
                              double* dRow, double* dCol);
 
+/* Geometric equilibration in place: up to six sweeps scaling each column, then each row, by
+ * max(sqrt(min nonzero |a| * max |a|), sqrt(1/1000) max |a|) (stopping after the fourth once
+ * a sweep reduces max/min-nonzero by less than 10%), then every column to max |a| = 1; on
+ * return A holds diag(dRow)^-1 A0 diag(dCol)^-1, bitwise the reference's result. Synchronous
+ * (the stopping test reads two numbers per sweep). Replaces El::GeomEquil(Matrix<double>& A,
+ * Matrix<double>& dRow, Matrix<double>& dCol, bool) --
+ * src/lapack_like/equilibrate/GeomEquil.cpp:26-116, include/El/lapack_like/equilibrate.hpp. */
+elx_status_t elx_dgeom_equil(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
+                             double* dRow, double* dCol);
+
 /* Symmetric Ruiz equilibration in place: maxiter sweeps (the reference's default is 3), each
  * scaling row and column j by the square root of column j's damped max norm; on return A
  * holds diag(d)^-1 A0 diag(d)^-1 (bitwise the reference's result) and dD (n) the scales.

@@ -345,3 +345,54 @@ def symmetric_ruiz_equil(A, maxiter=3):
         A *= inv[None, :]
         A *= inv[:, None]
     return A, d
+
+
+def geom_equil(A):
+    """El::GeomEquil (src/lapack_like/equilibrate/GeomEquil.cpp:26-116) on a copy of A: ratio
+    max|a| / min nonzero |a| (MinAbsNonzero, src/lapack_like/equilibrate/Util.hpp:15-31);
+    up to six sweeps, each scaling every nonzero column and then every nonzero row by
+    max(sqrt(mn*mx), sqrt(1/1000)*mx) (entries multiplied by 1/scale, the scale multiplied into
+    dCol / dRow), stopping after sweep iter >= 3 once the new ratio is >= 9/10 of the previous;
+    then every nonzero column multiplied by 1/max|a| (its max into dCol). Returns
+    (A_eq, dRow, dCol) with A_eq = diag(dRow)^-1 A diag(dCol)^-1."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = A.shape
+    dRow = np.ones(m)
+    dCol = np.ones(n)
+    if m == 0 or n == 0:
+        return A, dRow, dCol
+
+    def maxmin(X, axis=None):
+        ab = np.abs(X)
+        mx = ab.max(axis=axis)
+        mn = np.where(ab > 0, ab, np.inf).min(axis=axis)
+        return mx, np.minimum(mn, mx)
+
+    mx, mn = maxmin(A)
+    if mx == 0.0:
+        return A, dRow, dCol
+    ratio = mx / mn
+    sqrt_damp = np.sqrt(1.0 / 1000.0)
+    for it in range(6):
+        cmx, cmn = maxmin(A, axis=0)
+        nz = cmx > 0
+        sc = np.maximum(np.sqrt(cmn * cmx), sqrt_damp * cmx)
+        inv = np.where(nz, 1.0 / np.where(nz, sc, 1.0), 1.0)
+        A *= inv[None, :]
+        dCol = np.where(nz, dCol * sc, dCol)
+        rmx, rmn = maxmin(A, axis=1)
+        nz = rmx > 0
+        sc = np.maximum(np.sqrt(rmn * rmx), sqrt_damp * rmx)
+        inv = np.where(nz, 1.0 / np.where(nz, sc, 1.0), 1.0)
+        A *= inv[:, None]
+        dRow = np.where(nz, dRow * sc, dRow)
+        mx, mn = maxmin(A)
+        new_ratio = mx / mn
+        if it >= 3 and new_ratio >= ratio * (9.0 / 10.0):
+            break
+        ratio = new_ratio
+    cmx, _ = maxmin(A, axis=0)
+    nz = cmx > 0
+    A *= np.where(nz, 1.0 / np.where(nz, cmx, 1.0), 1.0)[None, :]
+    dCol = np.where(nz, dCol * cmx, dCol)
+    return A, dRow, dCol

@@ -110,3 +110,31 @@ def test_mixed_hpd_equilibrated_badly_scaled(dev, orc, uplo):
     assert st["converged"] == 1 and st["fell_back"] == 0 and st["factor_info"] == 0
     assert eta <= n * EPS
     assert np.array_equal(dev.to_host(dA), Ast, equal_nan=True)
+
+
+@pytest.mark.parametrize("m,n,ld_extra", [(300, 200, 0), (1000, 1000, 24), (3, 70000, 5),
+                                          (1, 1, 0), (257, 64, 3), (64, 300, 0)])
+def test_geom_bitwise(dev, orc, m, n, ld_extra):
+    say("geom", m, n, ld_extra)
+    rng = np.random.default_rng(m * 3 + n)
+    A = scaled(rng, m, n, 40)
+    if n > 3:
+        A[:, 2] = 0.0
+    if m > 3:
+        A[1, :] = 0.0
+        A[2, 0] = 1e-200
+    Ae, dR, dC = orc.geom_equil(A)
+    T = torch.zeros((n, m + ld_extra), dtype=torch.float64, device="cuda").t()[:m, :]
+    T.copy_(torch.from_numpy(A))
+    gR, gC = dev.geom_equil(T)
+    torch.cuda.synchronize()
+    assert np.array_equal(T.cpu().numpy(), Ae)
+    assert np.array_equal(gR.cpu().numpy(), dR)
+    assert np.array_equal(gC.cpu().numpy(), dC)
+
+
+def test_geom_zero_matrix(dev):
+    T = torch.zeros((5, 4), dtype=torch.float64, device="cuda").t().contiguous().t()
+    gR, gC = dev.geom_equil(T)
+    torch.cuda.synchronize()
+    assert torch.all(T == 0) and torch.all(gR == 1) and torch.all(gC == 1)

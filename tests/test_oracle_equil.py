@@ -90,3 +90,47 @@ def test_symmetric_reconstruction():
     cm0, cm = np.abs(A).max(axis=0), np.abs(Ae).max(axis=0)
     assert cm.max() <= 1.0 + 1e-12
     assert np.log2(cm.max() / cm.min()) <= np.log2(cm0.max() / cm0.min()) / 4
+
+
+def test_geom_rank_one_closed_form():
+    # A = u v^T with the spread of u below 1000 (no damping): the column pass divides column j
+    # by sqrt(min|u| max|u|) |v_j|, the row pass then leaves the sign matrix; later sweeps and
+    # the closing column pass see unit entries. All scales are powers of two here: exact.
+    u = np.array([4.0, -0.25, 16.0, 1.0])
+    v = np.array([-2.0 ** 7, 2.0 ** -9, 8.0])
+    Ae, dR, dC = orc.geom_equil(np.outer(u, v))
+    g = 2.0  # sqrt(0.25 * 16)
+    assert np.array_equal(Ae, np.sign(np.outer(u, v)))
+    assert np.array_equal(dR, np.abs(u) / g)
+    assert np.array_equal(dC, np.abs(v) * g)
+
+
+def test_geom_damping_and_zero_matrix():
+    # one column [1, 1e-6]: sqrt(mn*mx) = 1e-3 is below sqrt(1/1000)*mx, so the damped scale
+    # is used; the row pass then brings both entries to magnitude 1
+    Ae, dR, dC = orc.geom_equil(np.array([[1.0], [1e-6]]))
+    assert dC[0] == np.sqrt(1.0 / 1000.0)
+    assert np.allclose(np.abs(Ae[:, 0]), 1.0, rtol=0, atol=4 * 2.0 ** -53)
+    assert np.allclose(dR * Ae[:, 0] * dC[0], [1.0, 1e-6], rtol=4 * 2.0 ** -53, atol=0)
+    Z = np.zeros((3, 2))
+    Ae, dR, dC = orc.geom_equil(Z)
+    assert np.array_equal(Ae, Z) and np.array_equal(dR, np.ones(3)) and np.array_equal(dC, np.ones(2))
+
+
+def test_geom_reconstruction_and_unit_columns():
+    rng = np.random.default_rng(9)
+    m, n = 40, 30
+    A = (2.0 ** rng.uniform(-30, 30, m))[:, None] * rng.standard_normal((m, n)) * \
+        (2.0 ** rng.uniform(-30, 30, n))[None, :]
+    A[:, 4] = 0.0
+    A[7, :] = 0.0
+    Ae, dR, dC = orc.geom_equil(A)
+    R = dR[:, None] * Ae * dC[None, :]
+    nzm = A != 0
+    assert np.max(np.abs(R - A)[nzm] / np.abs(A)[nzm]) <= 32 * 2.0 ** -53
+    assert np.all(Ae[~nzm] == 0)
+    cm = np.abs(Ae).max(axis=0)
+    assert np.allclose(cm[np.arange(n) != 4], 1.0, rtol=0, atol=2 * 2.0 ** -53) and cm[4] == 0
+    # the spread of the entries shrinks by orders of magnitude
+    ab = np.abs(A[nzm]); abe = np.abs(Ae[nzm])
+    assert np.log2(abe.max() / abe.min()) < 0.5 * np.log2(ab.max() / ab.min())
