@@ -328,3 +328,49 @@ def test_batched_mixed_lu_ir(dev, orc):
         say(i, ns[i], f"{kappas[i]:.1e}", stats[i]["fell_back"], eta / (ns[i] * EPS))
         assert stats[i]["converged"] == 1 and eta <= ns[i] * EPS, (i, ns[i], kappas[i], stats[i])
     assert stats[0]["fell_back"] in (1, 2) and stats[1]["fell_back"] == 0
+
+
+def padded(a, extra):
+    """A column-major device copy of a with leading dimension rows + extra."""
+    a = np.asarray(a)
+    m, n = a.shape
+    t = torch.zeros((n, m + extra), dtype=torch.float64, device="cuda").t()[:m, :]
+    t.copy_(torch.from_numpy(np.ascontiguousarray(a)))
+    return t
+
+
+@pytest.mark.parametrize("n", [1, 2, 127, 129, 300])
+@pytest.mark.parametrize("kind", ["gesv", "posv"])
+def test_mixed_edge_shapes(dev, orc, n, kind):
+    """Orders around the 128-row block (and 1), leading dimensions above the order for A, B
+    and X, several right-hand sides: the solve meets n u and leaves the padding untouched."""
+    say("edge", kind, n)
+    rng = np.random.default_rng(1000 + n)
+    if kind == "gesv":
+        A = rng.standard_normal((n, n)) + n * np.eye(n)
+    else:
+        G = rng.standard_normal((n, n))
+        A = G @ G.T / n + np.eye(n)
+    B = rng.standard_normal((n, 3))
+    dA, dB = padded(A, 3), padded(B, 5)
+    dX = padded(np.zeros((n, 3)), 2)
+    if kind == "gesv":
+        st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto")
+    else:
+        st = dev.posv_mixed(dA, dB, dX, factor="fp16", refine="auto")
+    X = dX.cpu().numpy()
+    eta = orc.backward_error(A, X, B)
+    assert st["converged"] == 1 and eta <= max(n, 1) * EPS, (st, eta)
+    # the padding rows of X's storage are untouched
+    base = dX.as_strided((n + 2, 3), (1, n + 2))
+    assert torch.all(base[n:, :] == 0)
+
+
+def test_mixed_no_right_hand_sides(dev):
+    n = 50
+    A = np.eye(n) * 3.0
+    dA = dev.to_device(A)
+    dB = dev.colmajor_empty(n, 0)
+    dX = dev.colmajor_empty(n, 0)
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto")
+    assert st["converged"] == 1 and st["status"] == 0
