@@ -156,6 +156,34 @@ static int run() {
         for (Int j = 0; j < 3 && ok; ++j) ok = dCol(j, 0) == std::fabs(v[j]) * 1024.0;
         check(ok, "RuizEquil rank-one closed form", 0.0);
     }
+    {
+        // GeomEquil of u v^T with max|u| / min|u| = 64: the sign matrix, dRow = |u| / 2,
+        // dCol = 2 |v| (sqrt(min|u| max|u|) = 2)
+        const double u[4] = {4.0, -0.25, 16.0, 1.0};
+        const double v[3] = {-128.0, 1.0 / 512, 8.0};
+        Matrix<double> A(4, 3), dRow, dCol;
+        for (Int j = 0; j < 3; ++j)
+            for (Int i = 0; i < 4; ++i) A(i, j) = u[i] * v[j];
+        El::GeomEquil(A, dRow, dCol);
+        bool ok = dRow.Height() == 4 && dCol.Height() == 3;
+        for (Int j = 0; j < 3 && ok; ++j)
+            for (Int i = 0; i < 4; ++i)
+                ok = ok && A(i, j) == ((u[i] * v[j] < 0) ? -1.0 : 1.0);
+        for (Int i = 0; i < 4 && ok; ++i) ok = dRow(i, 0) == std::fabs(u[i]) / 2.0;
+        for (Int j = 0; j < 3 && ok; ++j) ok = dCol(j, 0) == std::fabs(v[j]) * 2.0;
+        check(ok, "GeomEquil rank-one closed form", 0.0);
+        // SymmetricRuizEquil of a diagonal of powers of four: the identity, d = sqrt(diag)
+        Matrix<double> D(3, 3), d;
+        const double dg[3] = {1024.0, 1.0 / 64, 16.0};
+        for (Int j = 0; j < 3; ++j)
+            for (Int i = 0; i < 3; ++i) D(i, j) = (i == j) ? dg[i] : 0.0;
+        El::SymmetricRuizEquil(D, d);
+        ok = d.Height() == 3;
+        for (Int j = 0; j < 3 && ok; ++j)
+            for (Int i = 0; i < 3; ++i) ok = ok && D(i, j) == ((i == j) ? 1.0 : 0.0);
+        for (Int i = 0; i < 3 && ok; ++i) ok = d(i, 0) == std::sqrt(dg[i]);
+        check(ok, "SymmetricRuizEquil diagonal closed form", 0.0);
+    }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
 }
