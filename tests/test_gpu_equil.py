@@ -138,3 +138,37 @@ def test_geom_zero_matrix(dev):
     gR, gC = dev.geom_equil(T)
     torch.cuda.synchronize()
     assert torch.all(T == 0) and torch.all(gR == 1) and torch.all(gC == 1)
+
+
+@pytest.mark.parametrize("kind", ["gesv", "posv"])
+def test_mixed_equilibrated_escalation(dev, orc, kind):
+    """Scaled (2^-3 .. 2^3) and ill-conditioned (kappa 1e6) systems: the FP16 factors of the
+    equilibrated matrix do not precondition well enough, so AUTO refactors FP32-accurately
+    (still equilibrated) or falls back to FP64, and the result meets n u. (Wider scalings make
+    the normwise backward error too weak a test: with rows spanning 2^32, even a one-shot FP16
+    solve scores about n u.)"""
+    n = 800
+    rng = np.random.default_rng(31 if kind == "gesv" else 32)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    sig = 10.0 ** (-6 * np.arange(n) / (n - 1))
+    if kind == "gesv":
+        Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        core = (Q1 * sig[None, :]) @ Q2.T
+        Dr, Dc = 2.0 ** rng.uniform(-3, 3, n), 2.0 ** rng.uniform(-3, 3, n)
+    else:
+        core = (Q1 * sig[None, :]) @ Q1.T
+        core = (core + core.T) / 2
+        Dr = Dc = 2.0 ** rng.uniform(-3, 3, n)
+    A = np.asfortranarray(Dr[:, None] * core * Dc[None, :])
+    B = rng.standard_normal((n, 2))
+    dA, dB = dev.to_device(A), dev.to_device(B)
+    dX = dev.colmajor_empty(n, 2)
+    if kind == "gesv":
+        st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto", equilibrate=True)
+    else:
+        st = dev.posv_mixed(dA, dB, dX, factor="fp16", refine="auto", equilibrate=True)
+    X = dev.to_host(dX)
+    eta = orc.backward_error(A, X, B)
+    say("equilibrated escalation", kind, st, "oracle eta/(n u) =", eta / (n * EPS))
+    assert st["converged"] == 1 and eta <= n * EPS
+    assert st["fell_back"] in (1, 2)
