@@ -494,6 +494,11 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     }
 
     // ---- refinement, one right-hand side at a time ----
+    // An IR step and a GMRES step cost the same (one preconditioner application, one FP64
+    // product with A), so AUTO keeps IR while a step reduces the backward error at least 3x and
+    // only then switches to GMRES-IR; plain IR stops once a step gains less than 2x.
+    constexpr double kIrKeep = 1.0 / 3.0;
+    const double ir_rate = (refine == ELX_REFINE_AUTO) ? kIrKeep : 0.5;
     const int m_restart = 50;
     double* V = nullptr;
     double* Z = nullptr;
@@ -515,9 +520,9 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     // row-stored factors: W read column-major with leading dimension ldw is L^T in its strict
     // upper triangle and U^T in its lower one, so L y = P b is a transposed unit-upper solve and
     // U x = y a transposed lower one) and one FP64 residual product R = B - A X for the group.
-    // A column leaves the block when its backward error reaches tol, or when it stops falling
-    // tenfold per step or runs out of steps: those columns start again in the per-column loop
-    // below (GMRES, escalation, FP64 fallback).
+    // A column leaves the block when its backward error reaches tol, or when a step reduces it
+    // less than IR continues with (3x under AUTO, 2x for IR) or it runs out of steps: those
+    // columns start again in the per-column loop below (GMRES, escalation, FP64 fallback).
     std::vector<char> done((size_t)nrhs, 0);
     if (nrhs > 1 && refine != ELX_REFINE_GMRES) {
         const i64 G = std::min<i64>(nrhs, 64);
@@ -571,7 +576,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
                         st.outer_iters = std::max(st.outer_iters, it);
                         continue;
                     }
-                    if (it >= max_iters || (it > 0 && !(e < 0.1 * eta_prev[(size_t)k]))) {
+                    if (it >= max_iters || (it > 0 && !(e < ir_rate * eta_prev[(size_t)k]))) {
                         active[(size_t)k] = 0;  // to the per-column loop
                         continue;
                     }
@@ -706,7 +711,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
             }
             if (refine == ELX_REFINE_NONE || it >= max_iters) break;
             if (!use_gmres) {
-                const bool slow = it > 0 && !(eta < 0.1 * eta_prev);
+                const bool slow = it > 0 && !(eta < kIrKeep * eta_prev);
                 if (slow) {
                     if (refine == ELX_REFINE_AUTO) use_gmres = true;
                     else if (!(eta < 0.5 * eta_prev)) break;  // IR stagnated
