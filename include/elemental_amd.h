@@ -166,7 +166,8 @@ typedef enum {
     ELX_REFINE_NONE = 0,
     ELX_REFINE_IR = 1,      /* classical iterative refinement */
     ELX_REFINE_GMRES = 2,   /* GMRES-IR: each correction by GMRES preconditioned with the LU */
-    ELX_REFINE_AUTO = 3     /* IR, switching to GMRES-IR when IR stagnates; when GMRES-IR
+    ELX_REFINE_AUTO = 3     /* IR while a step reduces the backward error at least 3x, then
+                               GMRES-IR (an IR step and a GMRES step cost the same); when GMRES-IR
                                stalls, an FP16/BF16 factorization is redone FP32-accurately
                                (ELX_FACTOR_FP32) and refined again, and the FP64 LU
                                (elx_dgetrf of a copy of A) is the last resort; an exactly zero
