@@ -11,6 +11,7 @@ c_i64 = ctypes.c_int64
 c_i32 = ctypes.c_int32
 c_int = ctypes.c_int
 c_dbl = ctypes.c_double
+c_flt = ctypes.c_float
 c_vp = ctypes.c_void_p
 
 ELX_STATUS = {
@@ -62,6 +63,20 @@ _SIGS = {
                   c_i64],
     "elx_dsyrk": [c_vp, c_int, c_int, c_i64, c_i64, c_dbl, c_vp, c_i64, c_dbl, c_vp, c_i64],
     "elx_dpermute_rows": [c_vp, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64],
+    "elx_dgetrf_nopiv": [c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp],
+    "elx_sgetrf": [c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp],
+    "elx_sgetrf_nopiv": [c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp],
+    "elx_sgetrs": [c_vp, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64],
+    "elx_sgesv": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp],
+    "elx_spotrf": [c_vp, c_int, c_i64, c_vp, c_i64, c_i64, c_vp],
+    "elx_spotrs": [c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64],
+    "elx_sposv": [c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp],
+    "elx_sgemm": [c_vp, c_int, c_int, c_i64, c_i64, c_i64, c_flt, c_vp, c_i64, c_vp, c_i64,
+                  c_flt, c_vp, c_i64],
+    "elx_strsm": [c_vp, c_int, c_int, c_int, c_int, c_i64, c_i64, c_flt, c_vp, c_i64, c_vp,
+                  c_i64],
+    "elx_ssyrk": [c_vp, c_int, c_int, c_i64, c_i64, c_flt, c_vp, c_i64, c_flt, c_vp, c_i64],
+    "elx_spermute_rows": [c_vp, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64],
     "elx_druiz_equil": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
     "elx_dsymmetric_ruiz_equil": [c_vp, c_i64, c_vp, c_i64, c_vp, c_i32],
     "elx_dgeom_equil": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],

@@ -467,6 +467,34 @@ elx_status_t batch_area(BatchArea** out) {
     return ELX_SUCCESS;
 }
 
+}  // namespace
+
+namespace elx {
+// Frees every batch schedule (device and pinned host copies, their events) and work queue.
+void batched_release() {
+    std::lock_guard<std::mutex> lk(g_bmu);
+    for (auto& kv : g_batch) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(kv.first);
+        BatchArea& a = kv.second;
+        for (Schedule& sc : a.sched) {
+            if (sc.copied) (void)hipEventSynchronize(sc.copied);
+            if (sc.dev) (void)hipFree(sc.dev);
+            if (sc.host) (void)hipHostFree(sc.host);
+            if (sc.copied) (void)hipEventDestroy(sc.copied);
+            sc = Schedule();
+        }
+        if (a.queue) (void)hipFree(a.queue);
+        a.queue = nullptr;
+        (void)hipSetDevice(cur);
+    }
+    g_batch.clear();
+}
+}  // namespace elx
+
+namespace {
+
 // Valid items in decreasing order of n (ties by index), on the device.
 elx_status_t schedule(hipStream_t s, Schedule& sc, int64_t count, const int32_t* n,
                       const int32_t* lda, const int32_t* nrhs, const int32_t* ldb) {

@@ -108,6 +108,7 @@ const Matrix<T> Matrix<T>::operator()(Range I, Range J) const {
 }
 
 template class Matrix<double>;
+template class Matrix<float>;
 template class Matrix<Int>;
 
 // ------------------------------------------------------------------ Permutation
@@ -216,9 +217,114 @@ int read_info(const DevBuf& info) {
     return v;
 }
 
-void check_blocksize() {
-    if (Blocksize() < 1 || Blocksize() > 128)
-        LogicError("Blocksize() must be in [1,128] for the GPU factorizations");
+// The device ABI for each field (elemental_amd.h).
+struct Dev {
+    static elx_status_t getrf(int64_t m, int64_t n, double* A, int64_t ld, int64_t nb, int32_t* d,
+                              int32_t* info) {
+        return elx_dgetrf(nullptr, m, n, A, ld, nb, d, info);
+    }
+    static elx_status_t getrf(int64_t m, int64_t n, float* A, int64_t ld, int64_t nb, int32_t* d,
+                              int32_t* info) {
+        return elx_sgetrf(nullptr, m, n, A, ld, nb, d, info);
+    }
+    static elx_status_t getrf_nopiv(int64_t m, int64_t n, double* A, int64_t ld, int64_t nb,
+                                    int32_t* info) {
+        return elx_dgetrf_nopiv(nullptr, m, n, A, ld, nb, info);
+    }
+    static elx_status_t getrf_nopiv(int64_t m, int64_t n, float* A, int64_t ld, int64_t nb,
+                                    int32_t* info) {
+        return elx_sgetrf_nopiv(nullptr, m, n, A, ld, nb, info);
+    }
+    static elx_status_t getrs(int o, int64_t n, int64_t k, const double* A, int64_t lda,
+                              const int32_t* d, double* B, int64_t ldb) {
+        return elx_dgetrs(nullptr, o, n, k, A, lda, d, B, ldb);
+    }
+    static elx_status_t getrs(int o, int64_t n, int64_t k, const float* A, int64_t lda,
+                              const int32_t* d, float* B, int64_t ldb) {
+        return elx_sgetrs(nullptr, o, n, k, A, lda, d, B, ldb);
+    }
+    static elx_status_t gesv(int64_t n, int64_t k, double* A, int64_t lda, double* B, int64_t ldb,
+                             int64_t nb, int32_t* info) {
+        return elx_dgesv(nullptr, n, k, A, lda, B, ldb, nb, nullptr, info);
+    }
+    static elx_status_t gesv(int64_t n, int64_t k, float* A, int64_t lda, float* B, int64_t ldb,
+                             int64_t nb, int32_t* info) {
+        return elx_sgesv(nullptr, n, k, A, lda, B, ldb, nb, nullptr, info);
+    }
+    static elx_status_t potrf(int u, int64_t n, double* A, int64_t lda, int64_t nb, int32_t* info) {
+        return elx_dpotrf(nullptr, u, n, A, lda, nb, info);
+    }
+    static elx_status_t potrf(int u, int64_t n, float* A, int64_t lda, int64_t nb, int32_t* info) {
+        return elx_spotrf(nullptr, u, n, A, lda, nb, info);
+    }
+    static elx_status_t potrs(int u, int o, int64_t n, int64_t k, const double* A, int64_t lda,
+                              double* B, int64_t ldb) {
+        return elx_dpotrs(nullptr, u, o, n, k, A, lda, B, ldb);
+    }
+    static elx_status_t potrs(int u, int o, int64_t n, int64_t k, const float* A, int64_t lda,
+                              float* B, int64_t ldb) {
+        return elx_spotrs(nullptr, u, o, n, k, A, lda, B, ldb);
+    }
+    static elx_status_t posv(int u, int o, int64_t n, int64_t k, double* A, int64_t lda, double* B,
+                             int64_t ldb, int64_t nb, int32_t* info) {
+        return elx_dposv(nullptr, u, o, n, k, A, lda, B, ldb, nb, info);
+    }
+    static elx_status_t posv(int u, int o, int64_t n, int64_t k, float* A, int64_t lda, float* B,
+                             int64_t ldb, int64_t nb, int32_t* info) {
+        return elx_sposv(nullptr, u, o, n, k, A, lda, B, ldb, nb, info);
+    }
+    static elx_status_t gemm(int ta, int tb, int64_t m, int64_t n, int64_t k, double al,
+                             const double* A, int64_t lda, const double* B, int64_t ldb, double be,
+                             double* C, int64_t ldc) {
+        return elx_dgemm(nullptr, ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+    }
+    static elx_status_t gemm(int ta, int tb, int64_t m, int64_t n, int64_t k, float al,
+                             const float* A, int64_t lda, const float* B, int64_t ldb, float be,
+                             float* C, int64_t ldc) {
+        return elx_sgemm(nullptr, ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+    }
+    static elx_status_t trsm(int sd, int u, int o, int dg, int64_t m, int64_t n, double al,
+                             const double* A, int64_t lda, double* B, int64_t ldb) {
+        return elx_dtrsm(nullptr, sd, u, o, dg, m, n, al, A, lda, B, ldb);
+    }
+    static elx_status_t trsm(int sd, int u, int o, int dg, int64_t m, int64_t n, float al,
+                             const float* A, int64_t lda, float* B, int64_t ldb) {
+        return elx_strsm(nullptr, sd, u, o, dg, m, n, al, A, lda, B, ldb);
+    }
+    static elx_status_t syrk(int u, int o, int64_t n, int64_t k, double al, const double* A,
+                             int64_t lda, double be, double* C, int64_t ldc) {
+        return elx_dsyrk(nullptr, u, o, n, k, al, A, lda, be, C, ldc);
+    }
+    static elx_status_t syrk(int u, int o, int64_t n, int64_t k, float al, const float* A,
+                             int64_t lda, float be, float* C, int64_t ldc) {
+        return elx_ssyrk(nullptr, u, o, n, k, al, A, lda, be, C, ldc);
+    }
+    static elx_status_t permute(int inv, const int32_t* d, int64_t ns, int64_t m, int64_t n,
+                                double* A, int64_t lda) {
+        return elx_dpermute_rows(nullptr, inv, d, ns, m, n, A, lda);
+    }
+    static elx_status_t permute(int inv, const int32_t* d, int64_t ns, int64_t m, int64_t n,
+                                float* A, int64_t lda) {
+        return elx_spermute_rows(nullptr, inv, d, ns, m, n, A, lda);
+    }
+};
+
+// Status of a device call as the reference's exception. Device-side numerical failures
+// arrive through the info word instead (see read_info).
+void check_status(elx_status_t st, const char* what) { check(st, what); }
+
+Int blocksize_checked() {
+    if (Blocksize() < 1) LogicError("Blocksize() must be positive");
+    return Blocksize();
+}
+
+// The info word of a device factorization as the reference reports it: 0 success, j+1 the
+// first failing column (the caller throws its exception), and -1 (the device panel gave up
+// waiting for a peer workgroup) a runtime error.
+int read_info_checked(const DevBuf& info, const char* what) {
+    const int v = read_info(info);
+    if (v < 0) RuntimeError(std::string(what) + ": device panel did not complete");
+    return v;
 }
 
 }  // namespace
@@ -226,13 +332,17 @@ void check_blocksize() {
 template <typename T>
 void Permutation::PermuteRows(Matrix<T>& A, Int offset) const {
     if (A.Height() == 0 || A.Width() == 0 || dests_.empty()) return;
-    if (offset != 0) LogicError("Only offset 0 is supported");
-    if ((Int)dests_.size() > A.Height()) LogicError("Permutation larger than matrix");
-    DevMatrix<double> dA(A);
+    // origins j+offset, destinations dests[j]+offset (Permutation.cpp:544-580): the swap
+    // sequence applied to the rows from `offset` down
+    if (offset < 0 || offset > A.Height()) LogicError("Permutation offset out of range");
+    if ((Int)dests_.size() > A.Height() - offset) LogicError("Permutation larger than matrix");
+    for (Int d : dests_)
+        if (d < 0 || d >= A.Height() - offset) LogicError("Swap destination out of range");
+    DevMatrix<T> dA(A);
     DevBuf dd(sizeof(int32_t) * dests_.size());
     hcheck(hipMemcpy(dd.p, dests_.data(), sizeof(int32_t) * dests_.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    check(elx_dpermute_rows(nullptr, 0, static_cast<int32_t*>(dd.p), (int64_t)dests_.size(), dA.m,
-                            dA.n, dA.ptr(), dA.ld),
+    check(Dev::permute(0, static_cast<int32_t*>(dd.p), (int64_t)dests_.size(), dA.m - offset, dA.n,
+                       dA.ptr() + offset, dA.ld),
           "PermuteRows");
     hcheck(hipDeviceSynchronize(), "sync");
     dA.to_host(A);
@@ -241,195 +351,236 @@ void Permutation::PermuteRows(Matrix<T>& A, Int offset) const {
 template <typename T>
 void Permutation::InversePermuteRows(Matrix<T>& A, Int offset) const {
     if (A.Height() == 0 || A.Width() == 0 || dests_.empty()) return;
-    if (offset != 0) LogicError("Only offset 0 is supported");
-    if ((Int)dests_.size() > A.Height()) LogicError("Permutation larger than matrix");
-    DevMatrix<double> dA(A);
+    if (offset < 0 || offset > A.Height()) LogicError("Permutation offset out of range");
+    if ((Int)dests_.size() > A.Height() - offset) LogicError("Permutation larger than matrix");
+    for (Int d : dests_)
+        if (d < 0 || d >= A.Height() - offset) LogicError("Swap destination out of range");
+    DevMatrix<T> dA(A);
     DevBuf dd(sizeof(int32_t) * dests_.size());
     hcheck(hipMemcpy(dd.p, dests_.data(), sizeof(int32_t) * dests_.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    check(elx_dpermute_rows(nullptr, 1, static_cast<int32_t*>(dd.p), (int64_t)dests_.size(), dA.m,
-                            dA.n, dA.ptr(), dA.ld),
+    check(Dev::permute(1, static_cast<int32_t*>(dd.p), (int64_t)dests_.size(), dA.m - offset, dA.n,
+                       dA.ptr() + offset, dA.ld),
           "InversePermuteRows");
     hcheck(hipDeviceSynchronize(), "sync");
     dA.to_host(A);
 }
 template void Permutation::PermuteRows(Matrix<double>&, Int) const;
 template void Permutation::InversePermuteRows(Matrix<double>&, Int) const;
+template void Permutation::PermuteRows(Matrix<float>&, Int) const;
+template void Permutation::InversePermuteRows(Matrix<float>&, Int) const;
 
 // ------------------------------------------------------------------ level 3
-template <>
-void Gemm(Orientation oA, Orientation oB, double alpha, const Matrix<double>& A,
-          const Matrix<double>& B, double beta, Matrix<double>& C) {
+template <typename T>
+void Gemm(Orientation oA, Orientation oB, T alpha, const Matrix<T>& A, const Matrix<T>& B, T beta,
+          Matrix<T>& C) {
     const Int m = C.Height(), n = C.Width();
     const Int k = (oA == NORMAL) ? A.Width() : A.Height();
     if (((oA == NORMAL) ? A.Height() : A.Width()) != m || ((oB == NORMAL) ? B.Width() : B.Height()) != n ||
         ((oB == NORMAL) ? B.Height() : B.Width()) != k)
         LogicError("Nonconformal Gemm");
-    DevMatrix<double> dA(A), dB(B), dC(C);
-    check(elx_dgemm(nullptr, oA, oB, m, n, k, alpha, dA.ptr(), dA.ld, dB.ptr(), dB.ld, beta, dC.ptr(), dC.ld), "Gemm");
+    DevMatrix<T> dA(A), dB(B), dC(C);
+    check(Dev::gemm(oA, oB, m, n, k, alpha, dA.ptr(), dA.ld, dB.ptr(), dB.ld, beta, dC.ptr(), dC.ld), "Gemm");
     hcheck(hipDeviceSynchronize(), "sync");
     dC.to_host(C);
 }
 
-template <>
+template <typename F>
 void Trsm(LeftOrRight side, UpperOrLower uplo, Orientation orientation, UnitOrNonUnit diag,
-          double alpha, const Matrix<double>& A, Matrix<double>& B, bool checkIfSingular) {
+          F alpha, const Matrix<F>& A, Matrix<F>& B, bool checkIfSingular) {
     if (A.Height() != A.Width()) LogicError("Triangular matrix must be square");
     if ((side == LEFT ? B.Height() : B.Width()) != A.Height()) LogicError("Nonconformal Trsm");
     if (checkIfSingular && diag != UNIT)
         for (Int j = 0; j < A.Height(); ++j)
-            if (A.Get(j, j) == 0.0) throw SingularMatrixException();
-    DevMatrix<double> dA(A), dB(B);
-    check(elx_dtrsm(nullptr, side, uplo, orientation, diag, B.Height(), B.Width(), alpha, dA.ptr(),
-                    dA.ld, dB.ptr(), dB.ld),
+            if (A.Get(j, j) == F(0)) throw SingularMatrixException();
+    DevMatrix<F> dA(A), dB(B);
+    check(Dev::trsm(side, uplo, orientation, diag, B.Height(), B.Width(), alpha, dA.ptr(), dA.ld,
+                    dB.ptr(), dB.ld),
           "Trsm");
     hcheck(hipDeviceSynchronize(), "sync");
     dB.to_host(B);
 }
 
-template <>
-void Syrk(UpperOrLower uplo, Orientation orientation, double alpha, const Matrix<double>& A,
-          double beta, Matrix<double>& C, bool) {
+template <typename T>
+void Syrk(UpperOrLower uplo, Orientation orientation, T alpha, const Matrix<T>& A, T beta,
+          Matrix<T>& C, bool) {
     const Int n = C.Height();
     const Int k = (orientation == NORMAL) ? A.Width() : A.Height();
     if (C.Width() != n || ((orientation == NORMAL) ? A.Height() : A.Width()) != n)
         LogicError("Nonconformal Syrk");
-    DevMatrix<double> dA(A), dC(C);
-    check(elx_dsyrk(nullptr, uplo, orientation, n, k, alpha, dA.ptr(), dA.ld, beta, dC.ptr(), dC.ld), "Syrk");
+    DevMatrix<T> dA(A), dC(C);
+    check(Dev::syrk(uplo, orientation, n, k, alpha, dA.ptr(), dA.ld, beta, dC.ptr(), dC.ld), "Syrk");
     hcheck(hipDeviceSynchronize(), "sync");
     dC.to_host(C);
 }
 
-template <>
-void Herk(UpperOrLower uplo, Orientation orientation, double alpha, const Matrix<double>& A,
-          double beta, Matrix<double>& C) {
+template <typename T>
+void Herk(UpperOrLower uplo, Orientation orientation, T alpha, const Matrix<T>& A, T beta,
+          Matrix<T>& C) {
     Syrk(uplo, orientation, alpha, A, beta, C, true);
 }
 
 // ------------------------------------------------------------------ factorizations
-template <>
-void LU(Matrix<double>& A, Permutation& P) {
-    check_blocksize();
+template <typename F>
+void LU(Matrix<F>& A) {
+    const Int nb = blocksize_checked();
+    const Int m = A.Height(), n = A.Width();
+    if (std::min(m, n) == 0) return;
+    DevMatrix<F> dA(A);
+    DevBuf info(sizeof(int32_t));
+    check(Dev::getrf_nopiv(m, n, dA.ptr(), dA.ld, nb, static_cast<int32_t*>(info.p)), "LU");
+    hcheck(hipDeviceSynchronize(), "sync");
+    const int inf = read_info_checked(info, "LU");
+    dA.to_host(A);  // partially overwritten when the factorization stopped
+    if (inf > 0) throw SingularMatrixException();
+}
+
+template <typename F>
+void LU(Matrix<F>& A, Permutation& P) {
+    const Int nb = blocksize_checked();
     const Int m = A.Height(), n = A.Width(), minDim = std::min(m, n);
     P.MakeIdentity(m);
     if (minDim == 0) return;
-    DevMatrix<double> dA(A);
+    DevMatrix<F> dA(A);
     DevBuf dests(sizeof(int32_t) * minDim), info(sizeof(int32_t));
-    check(elx_dgetrf(nullptr, m, n, dA.ptr(), dA.ld, Blocksize(), static_cast<int32_t*>(dests.p),
+    check(Dev::getrf(m, n, dA.ptr(), dA.ld, nb, static_cast<int32_t*>(dests.p),
                      static_cast<int32_t*>(info.p)),
           "LU");
     hcheck(hipDeviceSynchronize(), "sync");
-    const int inf = read_info(info);
+    const int inf = read_info_checked(info, "LU");
     std::vector<Int> d(minDim);
     hcheck(hipMemcpy(d.data(), dests.p, sizeof(int32_t) * minDim, hipMemcpyDeviceToHost), "hipMemcpy");
     dA.to_host(A);
     if (inf > 0) {
+        // the swaps recorded before the throw: those of every step up to the failing column
         d.resize(inf);
         P.SetSwapDestinations(m, d);
         throw SingularMatrixException();
     }
-    if (inf < 0) RuntimeError("LU: device panel did not complete");
     P.SetSwapDestinations(m, d);
 }
 
 namespace lu {
-template <>
-void SolveAfter(Orientation orientation, const Matrix<double>& A, const Permutation& P,
-                Matrix<double>& B) {
+template <typename F>
+void SolveAfter(Orientation orientation, const Matrix<F>& A, Matrix<F>& B) {
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    if (A.Height() != B.Height()) LogicError("A and B must be the same height");
+    const Int n = A.Height();
+    if (n == 0 || B.Width() == 0) return;
+    DevMatrix<F> dA(A), dB(B);
+    check(Dev::getrs(orientation, n, B.Width(), dA.ptr(), dA.ld, nullptr, dB.ptr(), dB.ld),
+          "lu::SolveAfter");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dB.to_host(B);
+}
+
+template <typename F>
+void SolveAfter(Orientation orientation, const Matrix<F>& A, const Permutation& P, Matrix<F>& B) {
     if (A.Height() != A.Width()) LogicError("A must be square");
     if (A.Height() != B.Height()) LogicError("A and B must be the same height");
     const Int n = A.Height();
     if (n == 0 || B.Width() == 0) return;
     const std::vector<Int>& d = P.SwapDestinations();
-    DevMatrix<double> dA(A), dB(B);
-    DevBuf dd(sizeof(int32_t) * std::max<size_t>(d.size(), 1));
-    if (!d.empty())
-        hcheck(hipMemcpy(dd.p, d.data(), sizeof(int32_t) * d.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    if ((Int)d.size() != n) {
-        // A partial swap sequence: pad with identity swaps.
-        std::vector<Int> full(n);
-        for (Int i = 0; i < n; ++i) full[i] = (i < (Int)d.size()) ? d[i] : i;
-        DevBuf df(sizeof(int32_t) * n);
-        hcheck(hipMemcpy(df.p, full.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice), "hipMemcpy");
-        check(elx_dgetrs(nullptr, orientation, n, B.Width(), dA.ptr(), dA.ld,
-                         static_cast<int32_t*>(df.p), dB.ptr(), dB.ld),
-              "lu::SolveAfter");
-        hcheck(hipDeviceSynchronize(), "sync");
-    } else {
-        check(elx_dgetrs(nullptr, orientation, n, B.Width(), dA.ptr(), dA.ld,
-                         static_cast<int32_t*>(dd.p), dB.ptr(), dB.ld),
-              "lu::SolveAfter");
-        hcheck(hipDeviceSynchronize(), "sync");
-    }
+    if ((Int)d.size() > n) LogicError("Permutation larger than A");
+    // A partial swap sequence is padded with identity swaps.
+    std::vector<Int> full(n);
+    for (Int i = 0; i < n; ++i) full[i] = (i < (Int)d.size()) ? d[i] : i;
+    DevMatrix<F> dA(A), dB(B);
+    DevBuf df(sizeof(int32_t) * n);
+    hcheck(hipMemcpy(df.p, full.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    check(Dev::getrs(orientation, n, B.Width(), dA.ptr(), dA.ld, static_cast<int32_t*>(df.p),
+                     dB.ptr(), dB.ld),
+          "lu::SolveAfter");
+    hcheck(hipDeviceSynchronize(), "sync");
     dB.to_host(B);
 }
 }  // namespace lu
 
-template <>
-void Cholesky(UpperOrLower uplo, Matrix<double>& A) {
-    check_blocksize();
+template <typename F>
+void Cholesky(UpperOrLower uplo, Matrix<F>& A) {
+    const Int nb = blocksize_checked();
     if (A.Height() != A.Width()) LogicError("A must be square");
     const Int n = A.Height();
     if (n == 0) return;
-    DevMatrix<double> dA(A);
+    DevMatrix<F> dA(A);
     DevBuf info(sizeof(int32_t));
-    check(elx_dpotrf(nullptr, uplo, n, dA.ptr(), dA.ld, Blocksize(), static_cast<int32_t*>(info.p)),
-          "Cholesky");
+    check(Dev::potrf(uplo, n, dA.ptr(), dA.ld, nb, static_cast<int32_t*>(info.p)), "Cholesky");
     hcheck(hipDeviceSynchronize(), "sync");
-    const int inf = read_info(info);
+    const int inf = read_info_checked(info, "Cholesky");
     dA.to_host(A);
     if (inf > 0) throw NonHPDMatrixException("A was not numerically HPD");
 }
 
 namespace cholesky {
-template <>
-void SolveAfter(UpperOrLower uplo, Orientation orientation, const Matrix<double>& A,
-                Matrix<double>& B) {
+template <typename F>
+void SolveAfter(UpperOrLower uplo, Orientation orientation, const Matrix<F>& A, Matrix<F>& B) {
     if (A.Height() != A.Width()) LogicError("A must be square");
     if (A.Height() != B.Height()) LogicError("A and B must be the same height");
     if (A.Height() == 0 || B.Width() == 0) return;
-    DevMatrix<double> dA(A), dB(B);
-    check(elx_dpotrs(nullptr, uplo, orientation, A.Height(), B.Width(), dA.ptr(), dA.ld, dB.ptr(),
-                     dB.ld),
+    DevMatrix<F> dA(A), dB(B);
+    check(Dev::potrs(uplo, orientation, A.Height(), B.Width(), dA.ptr(), dA.ld, dB.ptr(), dB.ld),
           "cholesky::SolveAfter");
     hcheck(hipDeviceSynchronize(), "sync");
     dB.to_host(B);
 }
 }  // namespace cholesky
 
-template <>
-void LinearSolve(const Matrix<double>& A, Matrix<double>& B) {
-    check_blocksize();
+template <typename F>
+void LinearSolve(const Matrix<F>& A, Matrix<F>& B) {
+    const Int nb = blocksize_checked();
     if (A.Height() != A.Width()) LogicError("A must be square");
     if (A.Height() != B.Height()) LogicError("A and B must be the same height");
     const Int n = A.Height();
     if (n == 0) return;
-    DevMatrix<double> dA(A), dB(B);  // the device copy plays the role of ACopy
+    DevMatrix<F> dA(A), dB(B);  // the device copy plays the role of ACopy
     DevBuf info(sizeof(int32_t));
-    check(elx_dgesv(nullptr, n, B.Width(), dA.ptr(), dA.ld, dB.ptr(), dB.ld, Blocksize(), nullptr,
+    check(Dev::gesv(n, B.Width(), dA.ptr(), dA.ld, dB.ptr(), dB.ld, nb,
                     static_cast<int32_t*>(info.p)),
           "LinearSolve");
     hcheck(hipDeviceSynchronize(), "sync");
-    if (read_info(info) > 0) throw SingularMatrixException();
+    const int inf = read_info_checked(info, "LinearSolve");
+    // B is partially updated when the elimination stopped, as in the reference
     dB.to_host(B);
+    if (inf > 0) throw SingularMatrixException();
 }
 
-template <>
-void HPDSolve(UpperOrLower uplo, Orientation orientation, const Matrix<double>& A,
-              Matrix<double>& B) {
-    check_blocksize();
+template <typename F>
+void HPDSolve(UpperOrLower uplo, Orientation orientation, const Matrix<F>& A, Matrix<F>& B) {
+    const Int nb = blocksize_checked();
     if (A.Height() != A.Width()) LogicError("A must be square");
     if (A.Height() != B.Height()) LogicError("A and B must be the same height");
     const Int n = A.Height();
     if (n == 0) return;
-    DevMatrix<double> dA(A), dB(B);
+    DevMatrix<F> dA(A), dB(B);
     DevBuf info(sizeof(int32_t));
-    check(elx_dposv(nullptr, uplo, orientation, n, B.Width(), dA.ptr(), dA.ld, dB.ptr(), dB.ld,
-                    Blocksize(), static_cast<int32_t*>(info.p)),
+    check(Dev::posv(uplo, orientation, n, B.Width(), dA.ptr(), dA.ld, dB.ptr(), dB.ld, nb,
+                    static_cast<int32_t*>(info.p)),
           "HPDSolve");
     hcheck(hipDeviceSynchronize(), "sync");
-    if (read_info(info) > 0) throw NonHPDMatrixException("A was not numerically HPD");
+    if (read_info_checked(info, "HPDSolve") > 0) throw NonHPDMatrixException("A was not numerically HPD");
     dB.to_host(B);
 }
+
+// Explicit instantiations, as the reference's PROTO macros provide (LU.cpp:233-313,
+// Cholesky.cpp:206-266, Linear.cpp:320-343, HPD.cpp:109-133, Gemm.cpp:261-298,
+// Trsm.cpp:400-435, Syrk.cpp, Herk.cpp).
+#define EL_AMD_PROTO(F)                                                                        \
+    template void Gemm(Orientation, Orientation, F, const Matrix<F>&, const Matrix<F>&, F,       \
+                       Matrix<F>&);                                                            \
+    template void Trsm(LeftOrRight, UpperOrLower, Orientation, UnitOrNonUnit, F,               \
+                       const Matrix<F>&, Matrix<F>&, bool);                                    \
+    template void Syrk(UpperOrLower, Orientation, F, const Matrix<F>&, F, Matrix<F>&, bool);  \
+    template void Herk(UpperOrLower, Orientation, F, const Matrix<F>&, F, Matrix<F>&);        \
+    template void LU(Matrix<F>&);                                                              \
+    template void LU(Matrix<F>&, Permutation&);                                                \
+    template void lu::SolveAfter(Orientation, const Matrix<F>&, Matrix<F>&);                   \
+    template void lu::SolveAfter(Orientation, const Matrix<F>&, const Permutation&, Matrix<F>&); \
+    template void Cholesky(UpperOrLower, Matrix<F>&);                                          \
+    template void cholesky::SolveAfter(UpperOrLower, Orientation, const Matrix<F>&, Matrix<F>&); \
+    template void LinearSolve(const Matrix<F>&, Matrix<F>&);                                   \
+    template void HPDSolve(UpperOrLower, Orientation, const Matrix<F>&, Matrix<F>&);
+EL_AMD_PROTO(double)
+EL_AMD_PROTO(float)
+#undef EL_AMD_PROTO
 
 // El::RuizEquil (src/lapack_like/equilibrate/RuizEquil.cpp:23-60): A becomes
 // diag(dRow)^{-1} A diag(dCol)^{-1}; dRow, dCol are resized to m x 1 and n x 1.
@@ -514,6 +665,8 @@ using El::Permutation;
 
 static Matrix<double>* M(ElMatrix_d A) { return reinterpret_cast<Matrix<double>*>(A); }
 static const Matrix<double>* M(ElConstMatrix_d A) { return reinterpret_cast<const Matrix<double>*>(A); }
+static Matrix<float>* M(ElMatrix_s A) { return reinterpret_cast<Matrix<float>*>(A); }
+static const Matrix<float>* M(ElConstMatrix_s A) { return reinterpret_cast<const Matrix<float>*>(A); }
 static Matrix<Int>* Mi(ElMatrix_i A) { return reinterpret_cast<Matrix<Int>*>(A); }
 static const Matrix<Int>* Mi(ElConstMatrix_i A) { return reinterpret_cast<const Matrix<Int>*>(A); }
 static Permutation* Pm(ElPermutation P) { return reinterpret_cast<Permutation*>(P); }
@@ -546,6 +699,17 @@ ElError ElMatrixLDim_d(ElConstMatrix_d A, ElInt* ld) { EL_TRY(*ld = M(A)->LDim()
 ElError ElMatrixBuffer_d(ElMatrix_d A, double** b) { EL_TRY(*b = M(A)->Buffer()) }
 ElError ElMatrixLockedBuffer_d(ElConstMatrix_d A, const double** b) { EL_TRY(*b = M(A)->LockedBuffer()) }
 
+ElError ElMatrixCreate_s(ElMatrix_s* A) { EL_TRY(*A = reinterpret_cast<ElMatrix_s>(new Matrix<float>())) }
+ElError ElMatrixDestroy_s(ElConstMatrix_s A) { EL_TRY(delete M(A)) }
+ElError ElMatrixResize_s(ElMatrix_s A, ElInt h, ElInt w) { EL_TRY(M(A)->Resize(h, w)) }
+ElError ElMatrixResizeWithLDim_s(ElMatrix_s A, ElInt h, ElInt w, ElInt ld) { EL_TRY(M(A)->Resize(h, w, ld)) }
+ElError ElMatrixAttach_s(ElMatrix_s A, ElInt h, ElInt w, float* buf, ElInt ld) { EL_TRY(M(A)->Attach(h, w, buf, ld)) }
+ElError ElMatrixHeight_s(ElConstMatrix_s A, ElInt* h) { EL_TRY(*h = M(A)->Height()) }
+ElError ElMatrixWidth_s(ElConstMatrix_s A, ElInt* w) { EL_TRY(*w = M(A)->Width()) }
+ElError ElMatrixLDim_s(ElConstMatrix_s A, ElInt* ld) { EL_TRY(*ld = M(A)->LDim()) }
+ElError ElMatrixBuffer_s(ElMatrix_s A, float** b) { EL_TRY(*b = M(A)->Buffer()) }
+ElError ElMatrixLockedBuffer_s(ElConstMatrix_s A, const float** b) { EL_TRY(*b = M(A)->LockedBuffer()) }
+
 ElError ElMatrixCreate_i(ElMatrix_i* A) { EL_TRY(*A = reinterpret_cast<ElMatrix_i>(new Matrix<Int>())) }
 ElError ElMatrixDestroy_i(ElConstMatrix_i A) { EL_TRY(delete Mi(A)) }
 ElError ElMatrixHeight_i(ElConstMatrix_i A, ElInt* h) { EL_TRY(*h = Mi(A)->Height()) }
@@ -569,42 +733,64 @@ ElError ElPermutationPermuteRows_d(ElConstPermutation P, ElMatrix_d A) { EL_TRY(
 ElError ElPermutationInversePermuteRows_d(ElConstPermutation P, ElMatrix_d A) {
     EL_TRY(Pm(P)->InversePermuteRows(*M(A)))
 }
+ElError ElPermutationPermuteRows_s(ElConstPermutation P, ElMatrix_s A) { EL_TRY(Pm(P)->PermuteRows(*M(A))) }
+ElError ElPermutationInversePermuteRows_s(ElConstPermutation P, ElMatrix_s A) {
+    EL_TRY(Pm(P)->InversePermuteRows(*M(A)))
+}
 
 ElError ElBlocksize(ElInt* b) { EL_TRY(*b = El::Blocksize()) }
 ElError ElSetBlocksize(ElInt b) { EL_TRY(El::SetBlocksize(b)) }
 ElError ElPushBlocksizeStack(ElInt b) { EL_TRY(El::PushBlocksizeStack(b)) }
 ElError ElPopBlocksizeStack(void) { EL_TRY(El::PopBlocksizeStack()) }
 
-ElError ElLUPartialPiv_d(ElMatrix_d A, ElPermutation P) { EL_TRY(El::LU(*M(A), *Pm(P))) }
-ElError ElSolveAfterLUPartialPiv_d(ElOrientation o, ElConstMatrix_d A, ElConstPermutation P,
-                                   ElMatrix_d B) {
-    EL_TRY(El::lu::SolveAfter(O(o), *M(A), *Pm(P), *M(B)))
-}
-ElError ElCholesky_d(ElUpperOrLower u, ElMatrix_d A) { EL_TRY(El::Cholesky(U(u), *M(A))) }
-ElError ElSolveAfterCholesky_d(ElUpperOrLower u, ElOrientation o, ElConstMatrix_d A, ElMatrix_d B) {
-    EL_TRY(El::cholesky::SolveAfter(U(u), O(o), *M(A), *M(B)))
-}
-ElError ElLinearSolve_d(ElConstMatrix_d A, ElMatrix_d B) { EL_TRY(El::LinearSolve(*M(A), *M(B))) }
-ElError ElHPDSolve_d(ElUpperOrLower u, ElOrientation o, ElConstMatrix_d A, ElMatrix_d B) {
-    EL_TRY(El::HPDSolve(U(u), O(o), *M(A), *M(B)))
-}
-
-ElError ElGemm_d(ElOrientation oA, ElOrientation oB, double alpha, ElConstMatrix_d A,
-                 ElConstMatrix_d B, double beta, ElMatrix_d C) {
-    EL_TRY(El::Gemm(O(oA), O(oB), alpha, *M(A), *M(B), beta, *M(C)))
-}
-ElError ElTrsm_d(ElLeftOrRight side, ElUpperOrLower u, ElOrientation o, ElUnitOrNonUnit diag,
-                 double alpha, ElConstMatrix_d A, ElMatrix_d B) {
-    EL_TRY(El::Trsm(static_cast<El::LeftOrRight>(side), U(u), O(o),
-                    static_cast<El::UnitOrNonUnit>(diag), alpha, *M(A), *M(B)))
-}
-ElError ElHerk_d(ElUpperOrLower u, ElOrientation o, double alpha, ElConstMatrix_d A, double beta,
-                 ElMatrix_d C) {
-    EL_TRY(El::Herk(U(u), O(o), alpha, *M(A), beta, *M(C)))
-}
-ElError ElSyrk_d(ElUpperOrLower u, ElOrientation o, double alpha, ElConstMatrix_d A, double beta,
-                 ElMatrix_d C) {
-    EL_TRY(El::Syrk(U(u), O(o), alpha, *M(A), beta, *M(C)))
-}
+// The factorization, solve and level-3 entry points for one field (factor-C.cpp:70-135,
+// 268-323 and solve-C.cpp:18-46 define them per field through C_PROTO_FIELD).
+#define EL_AMD_C_PROTO(SIG, F)                                                                 \
+    ElError ElLU_##SIG(ElMatrix_##SIG A) { EL_TRY(El::LU(*M(A))) }                             \
+    ElError ElLUPartialPiv_##SIG(ElMatrix_##SIG A, ElPermutation P) {                          \
+        EL_TRY(El::LU(*M(A), *Pm(P)))                                                          \
+    }                                                                                          \
+    ElError ElSolveAfterLU_##SIG(ElOrientation o, ElConstMatrix_##SIG A, ElMatrix_##SIG B) {   \
+        EL_TRY(El::lu::SolveAfter(O(o), *M(A), *M(B)))                                         \
+    }                                                                                          \
+    ElError ElSolveAfterLUPartialPiv_##SIG(ElOrientation o, ElConstMatrix_##SIG A,             \
+                                           ElConstPermutation P, ElMatrix_##SIG B) {           \
+        EL_TRY(El::lu::SolveAfter(O(o), *M(A), *Pm(P), *M(B)))                                 \
+    }                                                                                          \
+    ElError ElCholesky_##SIG(ElUpperOrLower u, ElMatrix_##SIG A) {                             \
+        EL_TRY(El::Cholesky(U(u), *M(A)))                                                      \
+    }                                                                                          \
+    ElError ElSolveAfterCholesky_##SIG(ElUpperOrLower u, ElOrientation o,                      \
+                                       ElConstMatrix_##SIG A, ElMatrix_##SIG B) {              \
+        EL_TRY(El::cholesky::SolveAfter(U(u), O(o), *M(A), *M(B)))                             \
+    }                                                                                          \
+    ElError ElLinearSolve_##SIG(ElConstMatrix_##SIG A, ElMatrix_##SIG B) {                     \
+        EL_TRY(El::LinearSolve(*M(A), *M(B)))                                                  \
+    }                                                                                          \
+    ElError ElHPDSolve_##SIG(ElUpperOrLower u, ElOrientation o, ElConstMatrix_##SIG A,         \
+                             ElMatrix_##SIG B) {                                               \
+        EL_TRY(El::HPDSolve(U(u), O(o), *M(A), *M(B)))                                         \
+    }                                                                                          \
+    ElError ElGemm_##SIG(ElOrientation oA, ElOrientation oB, F alpha, ElConstMatrix_##SIG A,   \
+                         ElConstMatrix_##SIG B, F beta, ElMatrix_##SIG C) {                    \
+        EL_TRY(El::Gemm(O(oA), O(oB), alpha, *M(A), *M(B), beta, *M(C)))                      \
+    }                                                                                          \
+    ElError ElTrsm_##SIG(ElLeftOrRight side, ElUpperOrLower u, ElOrientation o,                \
+                         ElUnitOrNonUnit diag, F alpha, ElConstMatrix_##SIG A,                 \
+                         ElMatrix_##SIG B) {                                                   \
+        EL_TRY(El::Trsm(static_cast<El::LeftOrRight>(side), U(u), O(o),                        \
+                        static_cast<El::UnitOrNonUnit>(diag), alpha, *M(A), *M(B)))            \
+    }                                                                                          \
+    ElError ElHerk_##SIG(ElUpperOrLower u, ElOrientation o, F alpha, ElConstMatrix_##SIG A,    \
+                         F beta, ElMatrix_##SIG C) {                                           \
+        EL_TRY(El::Herk(U(u), O(o), alpha, *M(A), beta, *M(C)))                                \
+    }                                                                                          \
+    ElError ElSyrk_##SIG(ElUpperOrLower u, ElOrientation o, F alpha, ElConstMatrix_##SIG A,    \
+                         F beta, ElMatrix_##SIG C) {                                           \
+        EL_TRY(El::Syrk(U(u), O(o), alpha, *M(A), beta, *M(C)))                                \
+    }
+EL_AMD_C_PROTO(s, float)
+EL_AMD_C_PROTO(d, double)
+#undef EL_AMD_C_PROTO
 
 }  // extern "C"

@@ -123,17 +123,34 @@ elx_status_t launch_resident(const void* fn, i64 grid, int block, void** args, s
 elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                           const double* A, i64 lda, const double* B, i64 ldb, double beta,
                           double* C, i64 ldc, int tri);
+// The same for FP64 and FP32 with the info word of a factorization sequence (`stop`, may be
+// null): once it is non-zero the call leaves C unchanged.
+elx_status_t launch_gemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
+                         const double* A, i64 lda, const double* B, i64 ldb, double beta,
+                         double* C, i64 ldc, int tri, const int32_t* stop);
+elx_status_t launch_gemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, float alpha,
+                         const float* A, i64 lda, const float* B, i64 ldb, float beta, float* C,
+                         i64 ldc, int tri, const int32_t* stop);
+// Two device doubles per device for the coefficients of gated library calls.
+elx_status_t gate_slot(double** out);
 
 // Small triangular solves for many independent vectors (triangle order <= 128).
-// side/uplo/orient/diag as in Elemental. B is m x n.
+// side/uplo/orient/diag as in Elemental. B is m x n. `stop`: the info word of a
+// factorization sequence (may be null); once it is non-zero the call changes nothing.
 elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, int diag, i64 m,
-                               i64 n, const double* A, i64 lda, double* B, i64 ldb);
+                               i64 n, const double* A, i64 lda, double* B, i64 ldb,
+                               const int32_t* stop = nullptr);
+elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, int diag, i64 m,
+                               i64 n, const float* A, i64 lda, float* B, i64 ldb,
+                               const int32_t* stop = nullptr);
 elx_status_t launch_trsm_small_f32(hipStream_t s, int side, int uplo, int orient, int diag,
                                    i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb);
 
 // General triangular solve (blocked by 128 with GEMM updates for larger triangles).
 elx_status_t dtrsm(hipStream_t s, int side, int uplo, int orient, int diag, i64 m, i64 n,
                    double alpha, const double* A, i64 lda, double* B, i64 ldb);
+elx_status_t strsm(hipStream_t s, int side, int uplo, int orient, int diag, i64 m, i64 n,
+                   float alpha, const float* A, i64 lda, float* B, i64 ldb);
 
 // LU panel of height mp, width w (<= 128) at A (column-major, lda). Records panel-relative
 // pivots (plus offset `goff`) into dests, the row-move list into pairs (2*w int2), and the
@@ -141,19 +158,31 @@ elx_status_t dtrsm(hipStream_t s, int side, int uplo, int orient, int diag, i64 
 elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
                                  int32_t* dests, i64 goff, int2* pairs, int32_t* info);
 
+elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, double* A, i64 lda, int32_t* dests,
+                             i64 goff, int2* pairs, int32_t* info);
+elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, float* A, i64 lda, int32_t* dests,
+                             i64 goff, int2* pairs, int32_t* info);
+
 // The same panel for an fp32 matrix stored row by row: element (r, j) at A[r*lda + j].
 elx_status_t launch_lu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
                                           int32_t* dests, i64 goff, int2* pairs, int32_t* info);
 // The row-move list applied to an fp32 matrix stored row by row (row i contiguous at
-// A + i*lda): columns [c0,c1) and [c2,c3) of each moved row, starting at row r0.
+// A + i*lda): columns [c0,c1) and [c2,c3) of each moved row, starting at row r0; moves that
+// reach outside the `rows` rows from r0 on are skipped.
 elx_status_t launch_row_moves_f32_rowmajor(hipStream_t s, const int2* pairs, int npairs,
-                                           float* A, i64 lda, i64 r0, i64 c0, i64 c1, i64 c2,
-                                           i64 c3);
+                                           float* A, i64 lda, i64 r0, i64 rows, i64 c0, i64 c1,
+                                           i64 c2, i64 c3);
 
 // Apply a row-move list (pairs[i] = {dst, src}, panel-relative, -1 = none) to columns
-// [c0,c1) and [c2,c3) of A starting at row `r0`.
+// [c0,c1) and [c2,c3) of A starting at row `r0`; moves outside rows [r0, r0+rows) are skipped.
 elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, double* A,
-                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3);
+                                  i64 lda, i64 r0, i64 rows, i64 c0, i64 c1, i64 c2, i64 c3);
+elx_status_t launch_row_moves(hipStream_t s, const int2* pairs, int npairs, double* A, i64 lda,
+                              i64 r0, i64 rows, i64 c0, i64 c1, i64 c2, i64 c3,
+                              const int32_t* stop);
+elx_status_t launch_row_moves(hipStream_t s, const int2* pairs, int npairs, float* A, i64 lda,
+                              i64 r0, i64 rows, i64 c0, i64 c1, i64 c2, i64 c3,
+                              const int32_t* stop);
 
 // Explicit permutation vector from a swap sequence (p[i] = source row of row i).
 elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswaps, i64 size,
@@ -162,6 +191,10 @@ elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswap
 // B_out(i,:) = B_in(p[i],:)
 elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i64 n,
                                     const double* Bin, i64 ldin, double* Bout, i64 ldout);
+elx_status_t launch_gather_rows(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                const double* Bin, i64 ldin, double* Bout, i64 ldout);
+elx_status_t launch_gather_rows(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                const float* Bin, i64 ldin, float* Bout, i64 ldout);
 
 // C(fp32, column-major) += alpha * 2^-(ea+eb) * A * B^T with A (m x k) and B (n x k) fp16
 // (prec 0) or bf16 (prec 1) stored K-major (ld a multiple of 8, >= k rounded up to 32,
@@ -190,16 +223,20 @@ elx_status_t library_gemm_lowp(hipStream_t s, int prec, i64 m, i64 n, i64 k, con
 // e_dev is null): the alphas of the library GEMM for the packed, power-of-two scaled operands.
 elx_status_t launch_gemm_alphas(hipStream_t s, const int* e_dev, int mul, float* out);
 // FP64 C = alpha op(A) op(B) + beta C (rocblas_dgemm) and the symmetric rank-k update of one
-// triangle of C (rocblas_dsyrk: trans false -> A A^T, true -> A^T A), host alpha and beta.
+// triangle of C (rocblas_dsyrk: trans false -> A A^T, true -> A^T A), host alpha and beta, or,
+// when coef_dev is given, alpha = coef_dev[0] and beta = coef_dev[1] read on the device.
 elx_status_t library_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                            const double* A, i64 lda, const double* B, i64 ldb, double beta,
-                           double* C, i64 ldc);
+                           double* C, i64 ldc, const double* coef_dev = nullptr);
 elx_status_t library_dsyrk(hipStream_t s, bool upper, bool trans, i64 n, i64 k, double alpha,
-                           const double* A, i64 lda, double beta, double* C, i64 ldc);
+                           const double* A, i64 lda, double beta, double* C, i64 ldc,
+                           const double* coef_dev = nullptr);
 // FP32 B := op(A)^{-1} B from the left (rocblas_strsm), op = transpose when trans
 elx_status_t library_strsm(hipStream_t s, bool upper, bool trans, bool unit, i64 m, i64 n,
                            const float* A, i64 lda, float* B, i64 ldb);
 void library_gemm_release();
+// Frees the batched solvers' schedules (pinned host and device copies) and queues.
+void batched_release();
 // e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds, zero again on return.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
                                    unsigned* ctl, int* e_dev);
@@ -209,5 +246,14 @@ elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda
                               int32_t* info);
 elx_status_t launch_potf2_f32(hipStream_t s, int uplo, i64 n, float* A, i64 lda, i64 goff,
                               int32_t* info);
+elx_status_t launch_potf2(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,
+                          int32_t* info);
+elx_status_t launch_potf2(hipStream_t s, int uplo, i64 n, float* A, i64 lda, i64 goff,
+                          int32_t* info);
+// LU without pivoting of the n x n (n <= 128) block at A (lu::Unb); info as the panel's.
+elx_status_t launch_getf2_nopiv(hipStream_t s, i64 n, double* A, i64 lda, i64 goff,
+                                int32_t* info);
+elx_status_t launch_getf2_nopiv(hipStream_t s, i64 n, float* A, i64 lda, i64 goff,
+                                int32_t* info);
 
 }  // namespace elx

@@ -86,20 +86,21 @@ elx_status_t library_gemm_lowp(hipStream_t s, int prec, i64 m, i64 n, i64 k, con
 
 elx_status_t library_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                            const double* A, i64 lda, const double* B, i64 ldb, double beta,
-                           double* C, i64 ldc) {
+                           double* C, i64 ldc, const double* coef_dev) {
     ELX_ARG(m < INT_MAX && n < INT_MAX && k < INT_MAX && lda < INT_MAX && ldb < INT_MAX &&
             ldc < INT_MAX);
     rocblas_handle h = nullptr;
     ELX_CHECK(rb_handle(&h));
     std::lock_guard<std::mutex> lk(g_rb_mu);
     if (rocblas_set_stream(h, s) != rocblas_status_success ||
-        rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success)
+        rocblas_set_pointer_mode(h, coef_dev ? rocblas_pointer_mode_device
+                                             : rocblas_pointer_mode_host) != rocblas_status_success)
         return ELX_ERROR_HIP;
     const rocblas_status st = rocblas_dgemm(
         h, ta ? rocblas_operation_transpose : rocblas_operation_none,
         tb ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
-        (rocblas_int)k, &alpha, A, (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C,
-        (rocblas_int)ldc);
+        (rocblas_int)k, coef_dev ? coef_dev : &alpha, A, (rocblas_int)lda, B, (rocblas_int)ldb,
+        coef_dev ? coef_dev + 1 : &beta, C, (rocblas_int)ldc);
     if (st != rocblas_status_success) {
         std::fprintf(stderr, "elemental_amd: rocblas_dgemm failed (%d)\n", (int)st);
         return ELX_ERROR_HIP;
@@ -108,18 +109,21 @@ elx_status_t library_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k,
 }
 
 elx_status_t library_dsyrk(hipStream_t s, bool upper, bool trans, i64 n, i64 k, double alpha,
-                           const double* A, i64 lda, double beta, double* C, i64 ldc) {
+                           const double* A, i64 lda, double beta, double* C, i64 ldc,
+                           const double* coef_dev) {
     ELX_ARG(n < INT_MAX && k < INT_MAX && lda < INT_MAX && ldc < INT_MAX);
     rocblas_handle h = nullptr;
     ELX_CHECK(rb_handle(&h));
     std::lock_guard<std::mutex> lk(g_rb_mu);
     if (rocblas_set_stream(h, s) != rocblas_status_success ||
-        rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success)
+        rocblas_set_pointer_mode(h, coef_dev ? rocblas_pointer_mode_device
+                                             : rocblas_pointer_mode_host) != rocblas_status_success)
         return ELX_ERROR_HIP;
     const rocblas_status st = rocblas_dsyrk(
         h, upper ? rocblas_fill_upper : rocblas_fill_lower,
         trans ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)n,
-        (rocblas_int)k, &alpha, A, (rocblas_int)lda, &beta, C, (rocblas_int)ldc);
+        (rocblas_int)k, coef_dev ? coef_dev : &alpha, A, (rocblas_int)lda,
+        coef_dev ? coef_dev + 1 : &beta, C, (rocblas_int)ldc);
     if (st != rocblas_status_success) {
         std::fprintf(stderr, "elemental_amd: rocblas_dsyrk failed (%d)\n", (int)st);
         return ELX_ERROR_HIP;

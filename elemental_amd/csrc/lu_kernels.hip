@@ -9,16 +9,25 @@ namespace {
 // Row moves: the net effect of one panel's swap sequence on the columns outside the panel
 // (PB.PermuteRows(AB0) / PB.PermuteRows(AB2), src/lapack_like/factor/LU.cpp:136-137).
 // One wave per column. All sources of a column are read (and staged in LDS) before any
-// destination is written, so overlapping source/destination rows are safe.
+// destination is written, so overlapping source/destination rows are safe. A move whose
+// source or destination lies outside the `rows` rows below r0 is skipped, so a list that a
+// panel left unwritten can never reach past the column.
 // ---------------------------------------------------------------------------------------
+__device__ __forceinline__ int2 move_in_range(int2 p, i64 rows) {
+    return (p.x >= 0 && p.y >= 0 && p.x < rows && p.y < rows) ? p : int2{-1, -1};
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void row_moves_kernel(const int2* __restrict__ pairs,
                                                         int npairs, T* A, i64 lda, i64 r0,
-                                                        i64 c0, i64 c1, i64 c2, i64 c3) {
+                                                        i64 rows, i64 c0, i64 c1, i64 c2,
+                                                        i64 c3, const int32_t* __restrict__ stop) {
     __shared__ T stage[4][256];
     __shared__ int2 pr[256];
+    if (stop && *stop != 0) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int i = t; i < 256; i += 256) pr[i] = (i < npairs) ? pairs[i] : int2{-1, -1};
+    for (int i = t; i < 256; i += 256)
+        pr[i] = (i < npairs) ? move_in_range(pairs[i], rows) : int2{-1, -1};
     __syncthreads();
     const i64 n1 = c1 - c0, total = n1 + (c3 - c2);
     const i64 cidx = (i64)blockIdx.x * 4 + wave;
@@ -47,14 +56,15 @@ __global__ __launch_bounds__(256) void row_moves_kernel(const int2* __restrict__
 // mixed-precision factorization keeps its working copy: each move is a contiguous segment
 // copy. A workgroup owns a 64-column slice of every moved row; wave w reads the slice of moves
 // 32w .. 32w+31 (a float per lane, 256 contiguous bytes per move), and all writes follow one
-// barrier, so overlapping sources and destinations are safe.
+// barrier, so overlapping sources and destinations are safe. Moves outside the `rows` rows
+// below r0 are skipped.
 __global__ __launch_bounds__(512) void row_moves_rowmajor_kernel(const int2* __restrict__ pairs,
                                                                  int npairs, float* A, i64 lda,
-                                                                 i64 r0, i64 c0, i64 c1, i64 c2,
-                                                                 i64 c3) {
+                                                                 i64 r0, i64 rows, i64 c0, i64 c1,
+                                                                 i64 c2, i64 c3) {
     __shared__ int2 pr[256];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (t < 256) pr[t] = (t < npairs) ? pairs[t] : int2{-1, -1};
+    if (t < 256) pr[t] = (t < npairs) ? move_in_range(pairs[t], rows) : int2{-1, -1};
     __syncthreads();
     const i64 s1 = (c1 - c0 + 63) / 64;  // slices of the first range
     const i64 sl = blockIdx.x;
@@ -144,8 +154,10 @@ constexpr int TV = TNT / TP;     // vectors per workgroup
 template <typename T>
 __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const T* __restrict__ A,
                                                          i64 lda, T* B, i64 ldb, int right,
-                                                         int lower_eff, int mtrans, int unit) {
+                                                         int lower_eff, int mtrans, int unit,
+                                                         const int32_t* __restrict__ stop) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    if (stop && *stop != 0) return;
     // Both LDS arrays have a leading dimension of m + 1, so transposed stagings (consecutive
     // threads walking a column of one and a row of the other) hit distinct banks.
     const int xm = m + 1;
@@ -293,6 +305,8 @@ __global__ __launch_bounds__(256) void potf2_kernel(int n, T* A, i64 lda, int up
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* S = reinterpret_cast<T*>(smem_raw);  // lower triangle L(r, c) at S[r + c*LD]
     __shared__ int s_fail;
+    // an earlier block already failed: the reference has thrown, nothing more changes
+    if (*info != 0) return;
     const int LD = n + 1, t = threadIdx.x, wave = t >> 6, lane = t & 63;
     for (int idx = t; idx < n * n; idx += 256) {
         const int i = idx % n, j = idx / n;
@@ -394,28 +408,92 @@ __global__ __launch_bounds__(256) void potf2_kernel(int n, T* A, i64 lda, int up
     }
 }
 
+
+// ---------------------------------------------------------------------------------------
+// LU without pivoting of an n x n (n <= 128) diagonal block in one workgroup, in LDS:
+// lu::Unb (src/lapack_like/factor/LU/Local.hpp:44-61). Per column j: alpha = a(j,j) (exactly
+// zero -> SingularMatrixException in the reference: info = goff+j+1 and the block is left as
+// it stands), a21 *= 1/alpha (blas::Scal), then Geru(-1): a(i,k) = fma(a(i,j), -a(j,k), a(i,k))
+// for every k with a(j,k) != 0, the reference BLAS dger's skip rule.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void getf2_nopiv_kernel(int n, T* A, i64 lda, i64 goff,
+                                                          int32_t* info) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* S = reinterpret_cast<T*>(smem_raw);  // S[i + j*LD]
+    if (*info != 0) return;
+    const int LD = n + 1, t = threadIdx.x;
+    for (int idx = t; idx < n * n; idx += 256) {
+        const int i = idx % n, j = idx / n;
+        S[i + j * LD] = A[i + (i64)j * lda];
+    }
+    __syncthreads();
+    int fail = -1;
+    for (int j = 0; j < n; ++j) {
+        const T alpha = S[j + j * LD];
+        if (alpha == T(0)) {
+            fail = j;
+            break;
+        }
+        const T inv = T(1) / alpha;
+        for (int i = j + 1 + t; i < n; i += 256) S[i + j * LD] = inv * S[i + j * LD];
+        __syncthreads();
+        const int h = n - (j + 1);
+        for (int idx = t; idx < h * h; idx += 256) {
+            const int i = j + 1 + idx % h, k = j + 1 + idx / h;
+            const T y = S[j + k * LD];
+            if (y != T(0)) S[i + k * LD] = __builtin_fma(S[i + j * LD], -y, S[i + k * LD]);
+        }
+        __syncthreads();
+    }
+    if (fail >= 0 && t == 0) atomicCAS(info, 0, (int32_t)(goff + fail + 1));
+    for (int idx = t; idx < n * n; idx += 256) {
+        const int i = idx % n, j = idx / n;
+        A[i + (i64)j * lda] = S[i + j * LD];
+    }
+}
+
 }  // namespace
 
 namespace elx {
 
-elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, double* A,
-                                  i64 lda, i64 r0, i64 c0, i64 c1, i64 c2, i64 c3) {
-    ELX_ARG(npairs <= 256);
+template <typename T>
+static elx_status_t row_moves_impl(hipStream_t s, const int2* pairs, int npairs, T* A, i64 lda,
+                                   i64 r0, i64 rows, i64 c0, i64 c1, i64 c2, i64 c3,
+                                   const int32_t* stop) {
+    ELX_ARG(npairs <= 256 && rows >= 0);
     const i64 total = std::max<i64>(c1 - c0, 0) + std::max<i64>(c3 - c2, 0);
     if (total <= 0 || npairs <= 0) return ELX_SUCCESS;
     if (c1 < c0) c1 = c0;
     if (c3 < c2) c3 = c2;
-    ProfScope prof_(s, kKRowMoves, 2.0 * npairs * (double)total * sizeof(double));
-    hipLaunchKernelGGL(row_moves_kernel<double>, dim3(cdiv(total, 4)), dim3(256), 0, s, pairs,
-                       npairs, A, lda, r0, c0, c1, c2, c3);
+    ProfScope prof_(s, kKRowMoves, 2.0 * npairs * (double)total * sizeof(T));
+    hipLaunchKernelGGL(row_moves_kernel<T>, dim3(cdiv(total, 4)), dim3(256), 0, s, pairs, npairs,
+                       A, lda, r0, rows, c0, c1, c2, c3, stop);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
 
+elx_status_t launch_row_moves_f64(hipStream_t s, const int2* pairs, int npairs, double* A,
+                                  i64 lda, i64 r0, i64 rows, i64 c0, i64 c1, i64 c2, i64 c3) {
+    return row_moves_impl<double>(s, pairs, npairs, A, lda, r0, rows, c0, c1, c2, c3, nullptr);
+}
+
+elx_status_t launch_row_moves(hipStream_t s, const int2* pairs, int npairs, double* A, i64 lda,
+                              i64 r0, i64 rows, i64 c0, i64 c1, i64 c2, i64 c3,
+                              const int32_t* stop) {
+    return row_moves_impl<double>(s, pairs, npairs, A, lda, r0, rows, c0, c1, c2, c3, stop);
+}
+
+elx_status_t launch_row_moves(hipStream_t s, const int2* pairs, int npairs, float* A, i64 lda,
+                              i64 r0, i64 rows, i64 c0, i64 c1, i64 c2, i64 c3,
+                              const int32_t* stop) {
+    return row_moves_impl<float>(s, pairs, npairs, A, lda, r0, rows, c0, c1, c2, c3, stop);
+}
+
 elx_status_t launch_row_moves_f32_rowmajor(hipStream_t s, const int2* pairs, int npairs,
-                                           float* A, i64 lda, i64 r0, i64 c0, i64 c1, i64 c2,
-                                           i64 c3) {
-    ELX_ARG(npairs <= 256);
+                                           float* A, i64 lda, i64 r0, i64 rows, i64 c0, i64 c1,
+                                           i64 c2, i64 c3) {
+    ELX_ARG(npairs <= 256 && rows >= 0);
     if (c1 < c0) c1 = c0;
     if (c3 < c2) c3 = c2;
     const i64 total = (c1 - c0) + (c3 - c2);
@@ -423,7 +501,7 @@ elx_status_t launch_row_moves_f32_rowmajor(hipStream_t s, const int2* pairs, int
     const i64 slices = cdiv(c1 - c0, 64) + cdiv(c3 - c2, 64);
     ProfScope prof_(s, kKRowMoves, 2.0 * npairs * (double)total * sizeof(float));
     hipLaunchKernelGGL(row_moves_rowmajor_kernel, dim3((unsigned)slices), dim3(512), 0, s, pairs,
-                       npairs, A, lda, r0, c0, c1, c2, c3);
+                       npairs, A, lda, r0, rows, c0, c1, c2, c3);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
@@ -448,19 +526,36 @@ elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswap
     return ELX_SUCCESS;
 }
 
-elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i64 n,
-                                    const double* Bin, i64 ldin, double* Bout, i64 ldout) {
+template <typename T>
+static elx_status_t gather_rows_impl(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                     const T* Bin, i64 ldin, T* Bout, i64 ldout) {
     if (m <= 0 || n <= 0) return ELX_SUCCESS;
     ELX_ARG(n <= 65535);
-    hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(cdiv(m, 256), (unsigned)n), dim3(256), 0,
-                       s, perm, m, n, Bin, ldin, Bout, ldout);
+    hipLaunchKernelGGL(gather_rows_kernel<T>, dim3(cdiv(m, 256), (unsigned)n), dim3(256), 0, s,
+                       perm, m, n, Bin, ldin, Bout, ldout);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
 
+elx_status_t launch_gather_rows_f64(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                    const double* Bin, i64 ldin, double* Bout, i64 ldout) {
+    return gather_rows_impl<double>(s, perm, m, n, Bin, ldin, Bout, ldout);
+}
+
+elx_status_t launch_gather_rows(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                const double* Bin, i64 ldin, double* Bout, i64 ldout) {
+    return gather_rows_impl<double>(s, perm, m, n, Bin, ldin, Bout, ldout);
+}
+
+elx_status_t launch_gather_rows(hipStream_t s, const int32_t* perm, i64 m, i64 n,
+                                const float* Bin, i64 ldin, float* Bout, i64 ldout) {
+    return gather_rows_impl<float>(s, perm, m, n, Bin, ldin, Bout, ldout);
+}
+
 template <typename T>
 static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orient, int diag,
-                                    i64 m, i64 n, const T* A, i64 lda, T* B, i64 ldb) {
+                                    i64 m, i64 n, const T* A, i64 lda, T* B, i64 ldb,
+                                    const int32_t* stop) {
     if (m <= 0 || n <= 0) return ELX_SUCCESS;
     const bool right = (side == kRight);
     const i64 order = right ? n : m;
@@ -483,19 +578,26 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     }
     hipLaunchKernelGGL(trsm_small_kernel<T>, dim3(cdiv(nvec, TV)), dim3(TNT), lds, s, (int)order,
                        nvec, A, lda, B, ldb, right ? 1 : 0, lower_eff ? 1 : 0, mtrans ? 1 : 0,
-                       diag == kUnit ? 1 : 0);
+                       diag == kUnit ? 1 : 0, stop);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
 
 elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, int diag, i64 m,
-                               i64 n, const double* A, i64 lda, double* B, i64 ldb) {
-    return trsm_small_impl<double>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb);
+                               i64 n, const double* A, i64 lda, double* B, i64 ldb,
+                               const int32_t* stop) {
+    return trsm_small_impl<double>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb, stop);
+}
+
+elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, int diag, i64 m,
+                               i64 n, const float* A, i64 lda, float* B, i64 ldb,
+                               const int32_t* stop) {
+    return trsm_small_impl<float>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb, stop);
 }
 
 elx_status_t launch_trsm_small_f32(hipStream_t s, int side, int uplo, int orient, int diag,
                                    i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb) {
-    return trsm_small_impl<float>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb);
+    return trsm_small_impl<float>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb, nullptr);
 }
 
 elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,
@@ -515,6 +617,46 @@ elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda
                        uplo == kUpper ? 1 : 0, goff, info);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
+}
+
+elx_status_t launch_potf2(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,
+                          int32_t* info) {
+    return launch_potf2_f64(s, uplo, n, A, lda, goff, info);
+}
+
+elx_status_t launch_potf2(hipStream_t s, int uplo, i64 n, float* A, i64 lda, i64 goff,
+                          int32_t* info) {
+    return launch_potf2_f32(s, uplo, n, A, lda, goff, info);
+}
+
+template <typename T>
+static elx_status_t getf2_nopiv_impl(hipStream_t s, i64 n, T* A, i64 lda, i64 goff,
+                                     int32_t* info) {
+    if (n <= 0) return ELX_SUCCESS;
+    ELX_ARG(n <= 128);
+    const size_t lds = sizeof(T) * (size_t)n * (n + 1);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&getf2_nopiv_kernel<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(sizeof(T) * 128 * 129));
+        (void)hipGetLastError();
+        attr = true;
+    }
+    hipLaunchKernelGGL(getf2_nopiv_kernel<T>, dim3(1), dim3(256), lds, s, (int)n, A, lda, goff,
+                       info);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+elx_status_t launch_getf2_nopiv(hipStream_t s, i64 n, double* A, i64 lda, i64 goff,
+                                int32_t* info) {
+    return getf2_nopiv_impl<double>(s, n, A, lda, goff, info);
+}
+
+elx_status_t launch_getf2_nopiv(hipStream_t s, i64 n, float* A, i64 lda, i64 goff,
+                                int32_t* info) {
+    return getf2_nopiv_impl<float>(s, n, A, lda, goff, info);
 }
 
 elx_status_t launch_potf2_f32(hipStream_t s, int uplo, i64 n, float* A, i64 lda, i64 goff,

@@ -51,7 +51,7 @@ struct PanelArgs {
     int G;
     int R;
     uint32_t seq_base;  // sequence number of column 0 of this launch
-    u4* rec;            // [REP][2][G][2] candidate records
+    u4* rec;            // [REP][2][G][RU] candidate records
     u4* rows;           // [3][G][WMAX] candidate row chunks
 };
 
@@ -61,11 +61,22 @@ using elx::xchg::ld16;
 using elx::xchg::mix32;
 using elx::xchg::st16;
 
-// Candidate record, two 16-byte units: {value (double bits), label, tag} and {look-ahead
-// entry, previous multiplier} (doubles). The tag covers both units, so a reader that sees a
-// current first unit with a stale second one rejects the pair.
-__device__ __forceinline__ uint32_t rec_tag(u4 r0, u4 r1, uint32_t seq) {
-    return seq ^ mix32(r0[0], r0[1], r0[2] ^ mix32(r1[0] ^ r1[2], r1[1], r1[3]));
+// Candidate record, three 16-byte units, each checked on its own for the column's sequence
+// number: {value (double bits), label, seq ^ hash} and {look-ahead entry (double bits), seq,
+// hash} and {previous multiplier (double bits), seq, hash}. The units are stored by separate
+// instructions whose order of arrival is not fixed, so each carries the sequence number (as a
+// plain word where there is room): a unit left from an earlier column is rejected exactly,
+// never through a hash comparison, and the hash only guards against a unit read while half
+// written.
+constexpr int RU = 3;  // 16-byte units per record
+__device__ __forceinline__ uint32_t rec0_tag(u4 r0, uint32_t seq) {
+    return seq ^ mix32(r0[0], r0[1], r0[2]);
+}
+__device__ __forceinline__ u4 rec_unit(uint32_t lo, uint32_t hi, uint32_t seq, uint32_t salt) {
+    return u4{lo, hi, seq, mix32(lo, hi ^ salt, seq)};
+}
+__device__ __forceinline__ bool rec_unit_ok(u4 r, uint32_t seq, uint32_t salt) {
+    return r[2] == seq && r[3] == mix32(r[0], r[1] ^ salt, seq);
 }
 __device__ __forceinline__ uint32_t dlo(double v) { return (uint32_t)(uint64_t)__double_as_longlong(v); }
 __device__ __forceinline__ uint32_t dhi(double v) { return (uint32_t)((uint64_t)__double_as_longlong(v) >> 32); }
@@ -218,6 +229,15 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     const i64 row0 = (i64)g * R;
     const int rows = (int)((p.mp - row0) < R ? (p.mp - row0) : R);
 
+    // An earlier panel of this factorization met an exactly zero pivot (or gave up): the
+    // reference has thrown, so this panel changes nothing and leaves identity swaps and an
+    // empty move list.
+    if (*p.info != 0) {
+        for (int i = t; i < 2 * W; i += NT) p.pairs[i] = int2{-1, -1};
+        for (int c = t; c < W; c += NT) p.dests[c] = (int32_t)(p.goff + c);
+        return;
+    }
+
     if (p.rowmajor) {
         for (int idx = t; idx < rows * W; idx += NT) {
             const int j = idx % W, r = idx / W;
@@ -238,7 +258,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
 
     typedef Chunk<T> CK;
     const int q1 = (W + CK::VPC - 1) / CK::VPC;
-    const uint32_t nrec = (uint32_t)REP * 2u * 2u * (uint32_t)G;  // records: [REP][2][G][2]
+    const uint32_t nrec = (uint32_t)REP * 2u * (uint32_t)RU * (uint32_t)G;  // [REP][2][G][RU]
     const uint32_t nrows = 3u * (uint32_t)G * WMAX;  // row chunks in use: [3][G][WMAX]
     // Deferred-update pipeline. At the start of step c every live row (label >= c) holds
     // column c updated through step c-1 and columns >= c+1 updated through step c-2: the
@@ -278,13 +298,13 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                             ? __builtin_fma(-lp[cand], up[cc + 1], tile[cand * LD + cc + 1])
                             : T(0);
         if (lane < REP) {
-            const uint32_t rec = 2u * (uint32_t)(((lane * 2) + (cc & 1)) * G + g);
+            const uint32_t rec = (uint32_t)RU * (uint32_t)(((lane * 2) + (cc & 1)) * G + g);
             const double a = (double)aprev, l = (cand >= 0) ? (double)lc[cand] : 0.0;
             const double val = (cand >= 0) ? (double)tile[cand * LD + cc] : 0.0;
-            const u4 r1 = u4{dlo(a), dhi(a), dlo(l), dhi(l)};
             u4 r0 = u4{dlo(val), dhi(val), (uint32_t)cand_lbl, 0u};
-            r0[3] = rec_tag(r0, r1, seq);
-            st16(p.rec, nrec, rec + 1, r1);
+            r0[3] = rec0_tag(r0, seq);
+            st16(p.rec, nrec, rec + 1, rec_unit(dlo(a), dhi(a), seq, 0x5bd1e995u));
+            st16(p.rec, nrec, rec + 2, rec_unit(dlo(l), dhi(l), seq, 0x1b873593u));
             st16(p.rec, nrec, rec, r0);
         }
         if (cand >= 0) {
@@ -334,8 +354,11 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         }
     }
 
+    // The bounded waits: each stage counts its own polls from zero. Stage (1) is decided per
+    // wave and combined by a vote; stage (2) runs the same iterations in every wave (each ends
+    // in a vote), so every wave reaches the same decision at the same barrier.
     bool aborted = false;
-    long long spins = 0;
+    int stop_col = -1;  // column of an exactly zero pivot (the panel stops there)
     int iter = 0;
     // the pivot row of the previous step, in flight: chunk registers, owner, its multiplier
     u4 fch = u4{0u, 0u, 0u, 0u};
@@ -354,6 +377,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             T bv = T(-1);
             int bL = INT_MAX, bq = -1;
             bool wave_aborted = false;
+            long long spins = 0;
             if (wave < PW) {
                 constexpr int STRIDE = PW * 64;
                 bool got0 = !(t < G), got1 = !(t + STRIDE < G);
@@ -363,16 +387,19 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                         const bool got = k ? got1 : got0;
                         if (got) continue;
                         const int q = t + k * STRIDE;
-                        const uint32_t rq = 2u * (uint32_t)((((g % REP) * 2) + s) * G + q);
+                        const uint32_t rq = (uint32_t)RU * (uint32_t)((((g % REP) * 2) + s) * G + q);
                         const u4 r0 = ld16(p.rec, nrec, rq);
                         const u4 r1 = ld16(p.rec, nrec, rq + 1);
-                        if (r0[3] != rec_tag(r0, r1, seq)) continue;
+                        const u4 r2 = ld16(p.rec, nrec, rq + 2);
+                        if (r0[3] != rec0_tag(r0, seq) || !rec_unit_ok(r1, seq, 0x5bd1e995u) ||
+                            !rec_unit_ok(r2, seq, 0x1b873593u))
+                            continue;
                         if (k) got1 = true;
                         else got0 = true;
                         const T val = (T)dbits(r0[0], r0[1]);
                         la_v[q] = val;
                         la_a[q] = (T)dbits(r1[0], r1[1]);
-                        la_l[q] = (T)dbits(r1[2], r1[3]);
+                        la_l[q] = (T)dbits(r2[0], r2[1]);
                         const int L = (int)r0[2];
                         if (L != INT_MAX) {
                             const T v = fabs(val);
@@ -414,6 +441,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         if (f_w >= 0) {
             const int q = f_q0 + t;
             bool done = !(t < 128 && q < q1);
+            long long spins = 0;
             while (true) {
                 if (!done) {
                     if (!CK::ok(fch, q, f_seq))
@@ -442,6 +470,25 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             if (c > 0)
                 for (int j = c + t; j < W; j += NT) up[j] = T(0);  // no pivot row (all NaN)
             __syncthreads();
+        }
+        // An exactly zero pivot (or a column whose candidates are all NaN): lu::Panel throws
+        // SingularMatrixException here (Panel.hpp:52-53), after the row interchange of this
+        // column (none: the pivot is the current row) and with every earlier step applied
+        // to the whole panel. Finish the pending step c-1 update of columns c+1.. (u_{c-1} is
+        // complete in up now) and stop; the columns outside the panel are not permuted.
+        if (none || alpha == T(0)) {
+            const int wr = W - c - 1;
+            if (c > 0 && wr > 0)
+                for (int idx = t; idx < rows * wr; idx += NT) {
+                    const int r = idx / wr, j = c + 1 + idx % wr;
+                    if (lbl[r] >= c) tile[r * LD + j] = __builtin_fma(-lp[r], up[j], tile[r * LD + j]);
+                }
+            if (g == 0 && t == 0) {
+                p.dests[c] = (int32_t)(p.goff + piv);
+                atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
+            }
+            stop_col = c;
+            break;
         }
         // (3) issue the fetch of this step's pivot row (consumed at the next step)
         f_w = none ? -1 : gw;
@@ -494,10 +541,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                     }
                 }
             }
-            if (g == 0 && t == 0) {
-                p.dests[c] = (int32_t)(p.goff + piv);
-                if (none || alpha == T(0)) atomicCAS(p.info, 0, (int32_t)(p.goff + c + 1));
-            }
+            if (g == 0 && t == 0) p.dests[c] = (int32_t)(p.goff + piv);
             wave_best(bv, bL, bR);
             if (lane == 0) {
                 rb_v[wave] = bv;
@@ -538,6 +582,11 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     __syncthreads();
 
     if (aborted) {
+        // The panel gave up (a peer never arrived): nothing was written back, so A is
+        // unchanged. Every workgroup leaves the same identity swaps and an empty move list (the
+        // launches queued after the panel then move nothing) and info = -1.
+        for (int i = t; i < 2 * W; i += NT) p.pairs[i] = int2{-1, -1};
+        for (int c = t; c < W; c += NT) p.dests[c] = (int32_t)(p.goff + c);
         if (t == 0) atomicCAS(p.info, 0, -1);
         return;
     }
@@ -552,6 +601,11 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             const int r = idx % rows, j = idx / rows;
             p.A[(i64)lbl[r] + (i64)j * p.lda] = tile[r * LD + j];
         }
+    }
+    if (stop_col >= 0) {
+        for (int i = t; i < 2 * W; i += NT) p.pairs[i] = int2{-1, -1};
+        for (int c = stop_col + 1 + t; c < W; c += NT) p.dests[c] = (int32_t)(p.goff + c);
+        return;
     }
     for (int r = t; r < rows; r += NT) {
         const int phys = (int)(row0 + r), L = lbl[r];
@@ -591,7 +645,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowma
     }
     elx::PanelSync* ps = nullptr;
     elx::ProfScope prof_(s, elx::kKPanel, 2.0 * (double)mp * (double)w * sizeof(T));
-    const size_t rec_bytes = sizeof(u4) * REP * 2 * 2 * (size_t)di.num_cus;
+    const size_t rec_bytes = sizeof(u4) * REP * 2 * RU * (size_t)di.num_cus;
     // Row chunks rotate over three slots: a winner's row of column c is read until the
     // readers reach column c+1, while its owner may already publish column c+2.
     const size_t row_bytes = sizeof(u4) * 3 * (size_t)di.num_cus * WMAX;
@@ -610,7 +664,7 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowma
     a.R = (int)R;
     a.seq_base = (uint32_t)((++ps->epoch) * 1024);
     a.rec = reinterpret_cast<u4*>(ps->slots);
-    a.rows = a.rec + REP * 2 * 2 * (size_t)di.num_cus;
+    a.rows = a.rec + REP * 2 * RU * (size_t)di.num_cus;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
@@ -631,6 +685,16 @@ namespace elx {
 elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
                                  int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
     return launch_panel<double>(s, mp, w, A, lda, 0, dests, goff, pairs, info);
+}
+
+elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, double* A, i64 lda, int32_t* dests,
+                             i64 goff, int2* pairs, int32_t* info) {
+    return launch_panel<double>(s, mp, w, A, lda, 0, dests, goff, pairs, info);
+}
+
+elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, float* A, i64 lda, int32_t* dests,
+                             i64 goff, int2* pairs, int32_t* info) {
+    return launch_panel<float>(s, mp, w, A, lda, 0, dests, goff, pairs, info);
 }
 
 elx_status_t launch_lu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,

@@ -26,7 +26,7 @@ enum {
     kSlotW = 10, kSlotLh, kSlotUh, kSlotLinv, kSlotUinv, kSlotIdx, kSlotVec, kSlotPart,
     kSlotSmall, kSlotXcopy, kSlotKrylovV, kSlotKrylovZ, kSlotKrylovW, kSlotKrylovP,
     kSlotF64, kSlotF64Idx, kSlotSym, kSlotDiag, kSlotLl, kSlotUl, kSlotBlockY, kSlotBlockR,
-    kSlotBlockN, kSlotBlockQ, kSlotEquilA, kSlotEquilV, kSlotEquilW
+    kSlotBlockN, kSlotBlockQ, kSlotEquilA, kSlotEquilV, kSlotEquilW, kSlotXsave, kSlotBlockXs
 };
 
 // Uncached buffers of the persistent triangular solves (never freed before the process
@@ -173,8 +173,8 @@ elx_status_t getrf_mixed(Ctx& c) {
             int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
             ELX_CHECK(launch_lu_panel_f32_rowmajor(c.s, n - j0, w, at(j0, j0), ld, c.dests + j0,
                                                    j0, pr, c.info));
-            ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, K0, j0,
-                                                    j0 + w, K1));
+            ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, n - j0, K0,
+                                                    j0, j0 + w, K1));
             const i64 wr = K1 - (j0 + w);
             if (wr > 0) {
                 // U12 = L11^{-1} A12, as U12^T L11^T = A12^T on the stored transposes
@@ -188,8 +188,8 @@ elx_status_t getrf_mixed(Ctx& c) {
         for (i64 j0 = K0; j0 < K1; j0 += 128) {
             const i64 w = std::min<i64>(128, K1 - j0);
             const int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
-            ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, 0, K0, K1,
-                                                    n));
+            ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, n - j0, 0,
+                                                    K0, K1, n));
         }
         // Inverses of the outer block's unit-lower diagonal blocks: final now (later swaps only
         // move rows below K1), used for U12 here and by the triangular solves afterwards.
@@ -351,6 +351,8 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(get(kSlotPart, (size_t)nch * (size_t)n, &part));
     float* y = nullptr;
     ELX_CHECK(get(kSlotSmall, (size_t)n, &y));
+    double* xs = nullptr;  // the iterate before an IR correction (accepted only if it helps)
+    ELX_CHECK(get(kSlotXsave, (size_t)n, &xs));
     TrsvArea* ta = nullptr;
     ELX_CHECK(trsv_area(n, &ta));
     int* err = ta->err;
@@ -419,15 +421,23 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(d2h(&finfo, c.info, 1, s));
     ELX_CHECK(d2h(&anorm, anorm_d, 1, s));
     st.factor_info = finfo;
+    // the cooperative panel gave up waiting for a peer workgroup: a device error, not a
+    // numerical outcome
+    if (finfo < 0) {
+        if (stats) *stats = st;
+        return ELX_ERROR_HIP;
+    }
 
     // r = b - A x in FP64 and its normwise backward error
-    auto backward = [&](const double* x, const double* b, double* eta) -> elx_status_t {
+    auto backward = [&](const double* x, const double* b, double* eta,
+                        double* rnorm = nullptr) -> elx_status_t {
         ELX_HIP(hipMemsetAsync(norms, 0, sizeof(double) * 3, s));
         ELX_CHECK(launch_residual(s, n, dA, lda, x, b, r, part, nch, norms));
         double hn[3];
         ELX_CHECK(d2h(hn, norms, 3, s));
         const double den = anorm * hn[1] + hn[2];
         *eta = den > 0 ? hn[0] / den : (hn[0] > 0 ? INFINITY : 0.0);
+        if (rnorm) *rnorm = hn[0];
         return ELX_SUCCESS;
     };
     // FP64 fallback (refine = AUTO): El::LU of a copy of A, lu::SolveAfter, then refinement
@@ -436,8 +446,10 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     double* F64 = nullptr;
     int32_t* d64 = nullptr;
     bool f64_ready = false;
+    bool f64_failed = false;  // the FP64 factorization met an exactly zero pivot: not retried
     auto fp64_solve = [&](const double* b, double* x, double* eta, bool* ok) -> elx_status_t {
         *ok = false;
+        if (f64_failed) return ELX_SUCCESS;
         if (!f64_ready) {
             ELX_CHECK(get(kSlotF64, (size_t)n * (size_t)n, &F64));
             ELX_CHECK(get(kSlotF64Idx, (size_t)n + 8, &d64));
@@ -448,7 +460,11 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
             else ELX_CHECK(elx_dpotrf(s, kLower, n, F64, n, 128, d64 + n));
             int32_t info64 = 0;
             ELX_CHECK(d2h(&info64, d64 + n, 1, s));
-            if (info64 != 0) return ELX_SUCCESS;  // exactly singular in FP64 as well
+            if (info64 < 0) return ELX_ERROR_HIP;  // the device panel did not complete
+            if (info64 != 0) {                     // exactly singular in FP64 as well
+                f64_failed = true;
+                return ELX_SUCCESS;
+            }
             f64_ready = true;
         }
         auto f64_apply = [&](double* v) -> elx_status_t {
@@ -534,6 +550,8 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
         ELX_CHECK(get(kSlotBlockR, (size_t)n * (size_t)G, &Rb));
         ELX_CHECK(get(kSlotBlockN, (size_t)(3 * G), &cn));
         ELX_CHECK(get(kSlotBlockQ, (size_t)G, &qd));
+        double* Xs = nullptr;  // the block's X before its last update
+        ELX_CHECK(get(kSlotBlockXs, (size_t)n * (size_t)G, &Xs));
         // X(:, q) [+]= M^{-1} Src(:, q) for nq columns
         auto block_precond = [&](const double* Src, i64 lds, const int32_t* q, i64 nq, double* X0,
                                  int accumulate) -> elx_status_t {
@@ -542,7 +560,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
             ELX_CHECK(library_strsm(s, false, true, false, n, nq, c.W, c.ldw, Yb, n));
             return launch_update_f64_2d(s, n, nq, q, Yb, n, sc, X0, ldx, accumulate);
         };
-        std::vector<double> hn((size_t)(3 * G)), eta_prev((size_t)G);
+        std::vector<double> hn((size_t)(3 * G)), eta_prev((size_t)G), rn_prev((size_t)G);
         std::vector<char> active((size_t)G);
         std::vector<int32_t> q;
         for (i64 c0 = 0; c0 < nrhs; c0 += G) {
@@ -576,16 +594,27 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
                         st.outer_iters = std::max(st.outer_iters, it);
                         continue;
                     }
+                    if (it > 0 && !(rn < rn_prev[(size_t)k])) {
+                        // Refined.hpp:96-105: a candidate is kept only when the residual
+                        // decreased; the column goes back to its previous iterate
+                        ELX_HIP(hipMemcpyAsync(X0 + k * ldx, Xs + k * n, sizeof(double) * n,
+                                               hipMemcpyDeviceToDevice, s));
+                        active[(size_t)k] = 0;  // to the per-column loop
+                        continue;
+                    }
                     if (it >= max_iters || (it > 0 && !(e < ir_rate * eta_prev[(size_t)k]))) {
                         active[(size_t)k] = 0;  // to the per-column loop
                         continue;
                     }
                     eta_prev[(size_t)k] = e;
+                    rn_prev[(size_t)k] = rn;
                     q.push_back((int32_t)k);
                 }
                 if (q.empty()) break;
                 ELX_HIP(hipMemcpyAsync(qd, q.data(), sizeof(int32_t) * q.size(),
                                        hipMemcpyHostToDevice, s));
+                ELX_HIP(hipMemcpy2DAsync(Xs, sizeof(double) * n, X0, sizeof(double) * ldx,
+                                         sizeof(double) * n, g, hipMemcpyDeviceToDevice, s));
                 ELX_CHECK(block_precond(Rb, n, qd, (i64)q.size(), X0, 1));
             }
         }
@@ -699,12 +728,26 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
         };
         // x0 = M^{-1} b
         ELX_CHECK(precond(b, x, 0));
-        double eta = INFINITY, eta_prev = INFINITY;
+        double eta = INFINITY, eta_prev = INFINITY, rn = INFINITY, rn_prev = INFINITY;
         int it = 0;
         bool ok = false;
         bool use_gmres = (refine == ELX_REFINE_GMRES);
+        bool ir_step = false;  // the last update was an IR correction (x_prev saved in xs)
         for (;;) {
-            ELX_CHECK(backward(x, b, &eta));
+            ELX_CHECK(backward(x, b, &eta, &rn));
+            if (ir_step && !(rn < rn_prev)) {
+                // Refined.hpp:96-105: the candidate x + dx is kept only when the residual
+                // decreased; otherwise the previous iterate stands
+                ELX_HIP(hipMemcpyAsync(x, xs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                if (refine != ELX_REFINE_AUTO) {
+                    eta = eta_prev;
+                    break;
+                }
+                use_gmres = true;  // AUTO: continue from the previous iterate with GMRES-IR
+                ELX_CHECK(backward(x, b, &eta, &rn));
+                eta_prev = INFINITY;
+            }
+            ir_step = false;
             if (eta <= tol) {
                 ok = true;
                 break;
@@ -720,8 +763,11 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
                 break;  // GMRES-IR stagnated
             }
             eta_prev = eta;
+            rn_prev = rn;
             if (!use_gmres) {
+                ELX_HIP(hipMemcpyAsync(xs, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
                 ELX_CHECK(precond(r, x, 1));
+                ir_step = true;
             } else {
                 // aim to reach tol in this outer step
                 const double rel = std::min(1e-2, std::max(1e-13, 0.1 * tol / eta));

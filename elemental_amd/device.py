@@ -41,12 +41,30 @@ def _mat(t, dtype=torch.float64):
     return t.data_ptr(), ld
 
 
+def _pre(t):
+    """Entry-point prefix for the tensor's field: elx_d (float64) or elx_s (float32)."""
+    if t.dtype == torch.float64:
+        return "elx_d"
+    if t.dtype == torch.float32:
+        return "elx_s"
+    raise ValueError(f"expected float64 or float32, got {t.dtype}")
+
+
+def _m(t):
+    """(pointer, leading dimension) of a float64 or float32 column-major tensor."""
+    return _mat(t, t.dtype if t.dtype in (torch.float32, torch.float64) else torch.float64)
+
+
 def colmajor_empty(m, n, dtype=torch.float64, device="cuda"):
     return torch.empty((n, m), dtype=dtype, device=device).t()
 
 
-def to_device(a, dtype=torch.float64, device="cuda"):
+def to_device(a, dtype=None, device="cuda"):
+    """Column-major device copy of a numpy array (float32 stays float32, anything else becomes
+    float64 unless dtype is given)."""
     a = np.asarray(a)
+    if dtype is None:
+        dtype = torch.float32 if a.dtype == np.float32 else torch.float64
     if a.ndim == 1:
         a = a[:, None]
     t = colmajor_empty(a.shape[0], a.shape[1], dtype=dtype, device=device)
@@ -59,83 +77,95 @@ def to_host(t):
 
 
 def getrf(A, nb=128, stream=None):
-    """In-place PA = LU. Returns (dests int32 tensor, info int32 tensor)."""
-    pA, lda = _mat(A)
+    """In-place PA = LU (float64 or float32). Returns (dests int32 tensor, info int32 tensor)."""
+    pA, lda = _m(A)
     m, n = A.shape
     dests = torch.zeros(max(min(m, n), 1), dtype=torch.int32, device=A.device)
     info = torch.zeros(1, dtype=torch.int32, device=A.device)
-    call("elx_dgetrf", _stream(stream), m, n, pA, lda, nb, dests.data_ptr(), info.data_ptr())
+    call(_pre(A) + "getrf", _stream(stream), m, n, pA, lda, nb, dests.data_ptr(), info.data_ptr())
     return dests[: min(m, n)], info
 
 
+def getrf_nopiv(A, nb=128, stream=None):
+    """In-place A = LU without pivoting. Returns the info int32 tensor."""
+    pA, lda = _m(A)
+    m, n = A.shape
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call(_pre(A) + "getrf_nopiv", _stream(stream), m, n, pA, lda, nb, info.data_ptr())
+    return info
+
+
 def getrs(A, dests, B, orient=NORMAL, stream=None):
-    pA, lda = _mat(A)
-    pB, ldb = _mat(B)
-    call("elx_dgetrs", _stream(stream), orient, A.shape[0], B.shape[1], pA, lda,
-         dests.data_ptr(), pB, ldb)
+    """lu::SolveAfter; dests None for the factors of getrf_nopiv."""
+    pA, lda = _m(A)
+    pB, ldb = _m(B)
+    call(_pre(A) + "getrs", _stream(stream), orient, A.shape[0], B.shape[1], pA, lda,
+         None if dests is None else dests.data_ptr(), pB, ldb)
     return B
 
 
 def gesv(A, B, nb=128, stream=None, want_dests=False):
     """Row-echelon LU on [A|B] and upper solve (A and B overwritten)."""
-    pA, lda = _mat(A)
-    pB, ldb = _mat(B)
+    pA, lda = _m(A)
+    pB, ldb = _m(B)
     n = A.shape[0]
     dests = torch.zeros(max(n, 1), dtype=torch.int32, device=A.device)
     info = torch.zeros(1, dtype=torch.int32, device=A.device)
-    call("elx_dgesv", _stream(stream), n, B.shape[1], pA, lda, pB, ldb, nb, dests.data_ptr(),
-         info.data_ptr())
+    call(_pre(A) + "gesv", _stream(stream), n, B.shape[1], pA, lda, pB, ldb, nb,
+         dests.data_ptr(), info.data_ptr())
     return (dests[:n], info) if want_dests else info
 
 
 def potrf(A, uplo=LOWER, nb=128, stream=None):
-    pA, lda = _mat(A)
+    pA, lda = _m(A)
     info = torch.zeros(1, dtype=torch.int32, device=A.device)
-    call("elx_dpotrf", _stream(stream), uplo, A.shape[0], pA, lda, nb, info.data_ptr())
+    call(_pre(A) + "potrf", _stream(stream), uplo, A.shape[0], pA, lda, nb, info.data_ptr())
     return info
 
 
 def potrs(A, B, uplo=LOWER, orient=NORMAL, stream=None):
-    pA, lda = _mat(A)
-    pB, ldb = _mat(B)
-    call("elx_dpotrs", _stream(stream), uplo, orient, A.shape[0], B.shape[1], pA, lda, pB, ldb)
+    pA, lda = _m(A)
+    pB, ldb = _m(B)
+    call(_pre(A) + "potrs", _stream(stream), uplo, orient, A.shape[0], B.shape[1], pA, lda, pB,
+         ldb)
     return B
 
 
 def posv(A, B, uplo=LOWER, orient=NORMAL, nb=128, stream=None):
-    pA, lda = _mat(A)
-    pB, ldb = _mat(B)
+    pA, lda = _m(A)
+    pB, ldb = _m(B)
     info = torch.zeros(1, dtype=torch.int32, device=A.device)
-    call("elx_dposv", _stream(stream), uplo, orient, A.shape[0], B.shape[1], pA, lda, pB, ldb,
-         nb, info.data_ptr())
+    call(_pre(A) + "posv", _stream(stream), uplo, orient, A.shape[0], B.shape[1], pA, lda, pB,
+         ldb, nb, info.data_ptr())
     return info
 
 
 def gemm(transA, transB, alpha, A, B, beta, C, stream=None):
-    pA, lda = _mat(A)
-    pB, ldb = _mat(B)
-    pC, ldc = _mat(C)
+    pA, lda = _m(A)
+    pB, ldb = _m(B)
+    pC, ldc = _m(C)
     m, n = C.shape
     k = A.shape[1] if transA == NORMAL else A.shape[0]
-    call("elx_dgemm", _stream(stream), transA, transB, m, n, k, alpha, pA, lda, pB, ldb, beta,
-         pC, ldc)
+    call(_pre(C) + "gemm", _stream(stream), transA, transB, m, n, k, alpha, pA, lda, pB, ldb,
+         beta, pC, ldc)
     return C
 
 
 def trsm(side, uplo, orient, diag, alpha, A, B, stream=None):
-    pA, lda = _mat(A)
-    pB, ldb = _mat(B)
+    pA, lda = _m(A)
+    pB, ldb = _m(B)
     m, n = B.shape
-    call("elx_dtrsm", _stream(stream), side, uplo, orient, diag, m, n, alpha, pA, lda, pB, ldb)
+    call(_pre(B) + "trsm", _stream(stream), side, uplo, orient, diag, m, n, alpha, pA, lda, pB,
+         ldb)
     return B
 
 
 def syrk(uplo, orient, alpha, A, beta, C, stream=None):
-    pA, lda = _mat(A)
-    pC, ldc = _mat(C)
+    pA, lda = _m(A)
+    pC, ldc = _m(C)
     n = C.shape[0]
     k = A.shape[1] if orient == NORMAL else A.shape[0]
-    call("elx_dsyrk", _stream(stream), uplo, orient, n, k, alpha, pA, lda, beta, pC, ldc)
+    call(_pre(C) + "syrk", _stream(stream), uplo, orient, n, k, alpha, pA, lda, beta, pC, ldc)
     return C
 
 
@@ -172,8 +202,8 @@ def symmetric_ruiz_equil(A, maxiter=3, stream=None):
 
 
 def permute_rows(dests, A, inverse=False, stream=None):
-    pA, lda = _mat(A)
-    call("elx_dpermute_rows", _stream(stream), 1 if inverse else 0, dests.data_ptr(),
+    pA, lda = _m(A)
+    call(_pre(A) + "permute_rows", _stream(stream), 1 if inverse else 0, dests.data_ptr(),
          dests.numel(), A.shape[0], A.shape[1], pA, lda)
     return A
 
@@ -220,7 +250,7 @@ def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_i
 
 
 __all__ = [
-    "load", "getrf", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
+    "load", "getrf", "getrf_nopiv", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
     "permute_rows", "ruiz_equil", "symmetric_ruiz_equil", "geom_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
     "Batch", "potrf_vbatched", "potrs_vbatched", "gesv_mixed_batched",
 ]

@@ -9,6 +9,7 @@
 //   SingularMatrixException,
 //   NonHPDMatrixException         include/El/core/environment/decl.hpp:209-231
 //   LU, lu::SolveAfter            include/El/lapack_like/factor.hpp:364-392,418-460
+//                                 (with and without pivoting; F = float and double)
 //   Cholesky, cholesky::SolveAfter include/El/lapack_like/factor.hpp:21-40
 //   LinearSolve, HPDSolve         include/El/lapack_like/solve.hpp:19-38,188-212
 //   Gemm, Trsm, Herk, Syrk        include/El/blas_like/level3.hpp:34-120,326-334,455-468
@@ -136,7 +137,9 @@ class Permutation {
     std::vector<Int> dests_;
 };
 
-// Algorithmic blocksize stack (default 128).
+// Algorithmic blocksize stack (default 128). Any positive blocksize is accepted; the GPU
+// factorizations run blocks wider than 128 columns as 128-column panels, which gives the same
+// pivots as the wider blocking in exact arithmetic.
 Int Blocksize();
 void SetBlocksize(Int blocksize);
 void PushBlocksizeStack(Int blocksize);
@@ -156,10 +159,14 @@ template <typename T>
 void Herk(UpperOrLower uplo, Orientation orientation, T alpha, const Matrix<T>& A, T beta,
           Matrix<T>& C);
 
-// ---- factorizations and solves ----
+// ---- factorizations and solves (instantiated for F = float and double) ----
+template <typename F>
+void LU(Matrix<F>& A);
 template <typename F>
 void LU(Matrix<F>& A, Permutation& P);
 namespace lu {
+template <typename F>
+void SolveAfter(Orientation orientation, const Matrix<F>& A, Matrix<F>& B);
 template <typename F>
 void SolveAfter(Orientation orientation, const Matrix<F>& A, const Permutation& P, Matrix<F>& B);
 }  // namespace lu

@@ -12,7 +12,16 @@
  *   *dInfo == 0      success
  *   *dInfo == j+1    column j (0-based, global) had an exactly zero pivot (LU) or a
  *                    non-positive diagonal (Cholesky) -- the point at which Elemental throws
- *                    SingularMatrixException / NonHPDMatrixException.
+ *                    SingularMatrixException / NonHPDMatrixException. The rest of the
+ *                    enqueued sequence then changes nothing, so A (and B) are left as the
+ *                    reference leaves them when it throws.
+ *   *dInfo == -1     the cooperative LU panel gave up waiting for a peer workgroup (a GPU
+ *                    shared with other work can delay one); A is unchanged from the start of
+ *                    that panel on. A runtime error, not a numerical result.
+ * Block sizes nb > 128 run as 128-column panels (the same pivots in exact arithmetic).
+ *
+ * Every routine of the plain path exists for FP64 (elx_d*) and FP32 (elx_s*), as the
+ * reference instantiates its sequential path for both fields.
  * Pivots use Elemental's swap-sequence format: dests[j] is the 0-based row swapped with
  * row j at step j, i.e. Permutation::SwapDestinations() (LAPACK ipiv - 1).
  */
@@ -65,12 +74,26 @@ elx_status_t elx_profile_query(int kernel_class, int64_t* launches, double* tota
  * dDests: min(m,n) int32 (device). dInfo: one int32 (device). */
 elx_status_t elx_dgetrf(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
                         int64_t nb, int32_t* dDests, int32_t* dInfo);
+elx_status_t elx_sgetrf(void* stream, int64_t m, int64_t n, float* dA, int64_t lda, int64_t nb,
+                        int32_t* dDests, int32_t* dInfo);
+
+/* A = LU without pivoting: blocked with lu::Unb on the diagonal blocks,
+ * Trsm(RIGHT,UPPER,NORMAL,NON_UNIT) of A21, Trsm(LEFT,LOWER,NORMAL,UNIT) of A12, Gemm for A22.
+ * Replaces El::LU(Matrix<F>&) -- src/lapack_like/factor/LU.cpp:21-44, LU/Local.hpp:44-61. */
+elx_status_t elx_dgetrf_nopiv(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
+                              int64_t nb, int32_t* dInfo);
+elx_status_t elx_sgetrf_nopiv(void* stream, int64_t m, int64_t n, float* dA, int64_t lda,
+                              int64_t nb, int32_t* dInfo);
 
 /* B := op(A)^{-1} P B (orient NORMAL) or P^T op(A)^{-1} B (TRANSPOSE/ADJOINT) with the
  * factors of elx_dgetrf. Replaces lu::SolveAfter(Orientation, const Matrix<double>&,
- * const Permutation&, Matrix<double>&) -- src/lapack_like/factor/LU/SolveAfter.hpp:66-92. */
+ * const Permutation&, Matrix<double>&) -- src/lapack_like/factor/LU/SolveAfter.hpp:66-92.
+ * dDests NULL: no permutation, the factors of elx_dgetrf_nopiv -- lu::SolveAfter(Orientation,
+ * const Matrix<F>&, Matrix<F>&), src/lapack_like/factor/LU/SolveAfter.hpp:15-38. */
 elx_status_t elx_dgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const double* dA,
                         int64_t lda, const int32_t* dDests, double* dB, int64_t ldb);
+elx_status_t elx_sgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const float* dA,
+                        int64_t lda, const int32_t* dDests, float* dB, int64_t ldb);
 
 /* Short-circuited LU on [A|B] followed by one upper Trsm: B := A^{-1} B, A overwritten
  * with its (row-echelon) factors. Replaces lin_solve::Overwrite / RowEchelon --
@@ -78,11 +101,15 @@ elx_status_t elx_dgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const
  * a copy of A). dDests may be NULL. */
 elx_status_t elx_dgesv(void* stream, int64_t n, int64_t nrhs, double* dA, int64_t lda,
                        double* dB, int64_t ldb, int64_t nb, int32_t* dDests, int32_t* dInfo);
+elx_status_t elx_sgesv(void* stream, int64_t n, int64_t nrhs, float* dA, int64_t lda, float* dB,
+                       int64_t ldb, int64_t nb, int32_t* dDests, int32_t* dInfo);
 
 /* A = L L^T (uplo LOWER) or U^T U (UPPER), right-looking variant 3 with block size nb.
  * Replaces El::Cholesky(UpperOrLower, Matrix<double>&) -- src/lapack_like/factor/
  * Cholesky.cpp:26-38, Cholesky/LowerVariant3.hpp:15-68, Cholesky/UpperVariant3.hpp:15-73. */
 elx_status_t elx_dpotrf(void* stream, int uplo, int64_t n, double* dA, int64_t lda, int64_t nb,
+                        int32_t* dInfo);
+elx_status_t elx_spotrf(void* stream, int uplo, int64_t n, float* dA, int64_t lda, int64_t nb,
                         int32_t* dInfo);
 
 /* B := A^{-1} B with the Cholesky factor. Replaces cholesky::SolveAfter(UpperOrLower,
@@ -90,11 +117,15 @@ elx_status_t elx_dpotrf(void* stream, int uplo, int64_t n, double* dA, int64_t l
  * src/lapack_like/factor/Cholesky/SolveAfter.hpp:15-43. */
 elx_status_t elx_dpotrs(void* stream, int uplo, int orient, int64_t n, int64_t nrhs,
                         const double* dA, int64_t lda, double* dB, int64_t ldb);
+elx_status_t elx_spotrs(void* stream, int uplo, int orient, int64_t n, int64_t nrhs,
+                        const float* dA, int64_t lda, float* dB, int64_t ldb);
 
 /* Cholesky factor + solve in place (A overwritten). Replaces hpd_solve::Overwrite --
  * src/lapack_like/solve/HPD.cpp:15-25 (El::HPDSolve, :47-57, is this after a copy of A). */
 elx_status_t elx_dposv(void* stream, int uplo, int orient, int64_t n, int64_t nrhs, double* dA,
                        int64_t lda, double* dB, int64_t ldb, int64_t nb, int32_t* dInfo);
+elx_status_t elx_sposv(void* stream, int uplo, int orient, int64_t n, int64_t nrhs, float* dA,
+                       int64_t lda, float* dB, int64_t ldb, int64_t nb, int32_t* dInfo);
 
 /* Level-3 pieces on device matrices. Replace El::Gemm / El::Trsm / El::Herk on
  * Matrix<double> -- src/blas_like/level3/Gemm.cpp:19-72, Trsm.cpp:24-64, Herk.cpp:14-21 ->
@@ -107,12 +138,22 @@ elx_status_t elx_dtrsm(void* stream, int side, int uplo, int orient, int diag, i
                        int64_t ldb);
 elx_status_t elx_dsyrk(void* stream, int uplo, int orient, int64_t n, int64_t k, double alpha,
                        const double* dA, int64_t lda, double beta, double* dC, int64_t ldc);
+elx_status_t elx_sgemm(void* stream, int transA, int transB, int64_t m, int64_t n, int64_t k,
+                       float alpha, const float* dA, int64_t lda, const float* dB, int64_t ldb,
+                       float beta, float* dC, int64_t ldc);
+elx_status_t elx_strsm(void* stream, int side, int uplo, int orient, int diag, int64_t m,
+                       int64_t n, float alpha, const float* dA, int64_t lda, float* dB,
+                       int64_t ldb);
+elx_status_t elx_ssyrk(void* stream, int uplo, int orient, int64_t n, int64_t k, float alpha,
+                       const float* dA, int64_t lda, float beta, float* dC, int64_t ldc);
 
 /* Row permutation of a device matrix by a swap sequence (forward or inverse order).
  * Replaces Permutation::PermuteRows / InversePermuteRows on Matrix<double> --
  * src/lapack_like/perm/Permutation.cpp:544-631. */
 elx_status_t elx_dpermute_rows(void* stream, int inverse, const int32_t* dDests,
                                int64_t nswaps, int64_t m, int64_t n, double* dA, int64_t lda);
+elx_status_t elx_spermute_rows(void* stream, int inverse, const int32_t* dDests,
+                               int64_t nswaps, int64_t m, int64_t n, float* dA, int64_t lda);
 
 /* Ruiz equilibration in place: four sweeps scaling the columns, then the rows, by their
  * damped max norms; on return A holds diag(dRow)^-1 A0 diag(dCol)^-1 (bitwise the reference's

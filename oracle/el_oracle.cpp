@@ -15,12 +15,17 @@
 //   cholesky::SolveAfter                src/lapack_like/factor/Cholesky/SolveAfter.hpp:15-43
 //   El::HPDSolve                        src/lapack_like/solve/HPD.cpp:15-25,47-57
 //   Permutation (swap sequence)         src/lapack_like/perm/Permutation.cpp:166-250,544-631,731-756
-//   Gaussian / Uniform / GEPPGrowth     src/matrices/random/independent/Gaussian.cpp:16-47,
-//                                       include/El/core/random/impl.hpp:127-137,185-207,
+//   El::LU(Matrix&) without pivoting    src/lapack_like/factor/LU.cpp:21-44, LU/Local.hpp:44-61
+//   lu::SolveAfter(orient,A,B)          src/lapack_like/factor/LU/SolveAfter.hpp:15-38
+//   Gaussian / GEPPGrowth               src/matrices/random/independent/Gaussian.cpp:16-47,
+//                                       include/El/core/random/impl.hpp:185-207,
 //                                       src/core/random.cpp:24-32,
 //                                       src/matrices/deterministic/misc/GEPPGrowth.cpp:15-30
 //
-// The BLAS underneath is selectable:
+// Every algorithm is a template over the field and is exported for double (orc_*) and float
+// (orc_s*), as the reference instantiates its sequential path for both.
+//
+// The BLAS underneath is selectable (double; float always runs the naive form):
 //   * "naive": the netlib reference-BLAS loop orders, with every a += b*c written as one
 //     fused multiply-add (the form vectorised host BLAS kernels use). Deterministic,
 //     single-threaded; this is the backend the parity tests use.
@@ -66,14 +71,15 @@ struct Blas {
 
 #define IDX(i, j, ld) ((i) + (i64)(j) * (ld))
 
-// ---------------- naive (netlib loop order, FMA) ----------------
-i64 nv_idamax(i64 n, const double* x, i64 incx) {
+// ---------------- naive (netlib loop order, FMA), for T = double and float ----------------
+template <typename T>
+i64 nv_idamax(i64 n, const T* x, i64 incx) {
     // Strict '>' keeps the first index among equal magnitudes (BLAS i?amax semantics),
     // src/core/imports/blas/MaxInd.hpp:26-46.
-    double best = -1.0;
+    T best = T(-1);
     i64 bi = -1;
     for (i64 i = 0; i < n; ++i) {
-        double v = std::fabs(x[i * incx]);
+        T v = std::fabs(x[i * incx]);
         if (v > best) {
             best = v;
             bi = i;
@@ -82,31 +88,35 @@ i64 nv_idamax(i64 n, const double* x, i64 incx) {
     return bi;
 }
 
-void nv_dswap(i64 n, double* x, i64 incx, double* y, i64 incy) {
+template <typename T>
+void nv_dswap(i64 n, T* x, i64 incx, T* y, i64 incy) {
     for (i64 i = 0; i < n; ++i) std::swap(x[i * incx], y[i * incy]);
 }
 
-void nv_dscal(i64 n, double a, double* x, i64 incx) {
+template <typename T>
+void nv_dscal(i64 n, T a, T* x, i64 incx) {
     for (i64 i = 0; i < n; ++i) x[i * incx] = a * x[i * incx];
 }
 
-void nv_dger(i64 m, i64 n, double alpha, const double* x, i64 incx, const double* y,
-             i64 incy, double* A, i64 lda) {
+template <typename T>
+void nv_dger(i64 m, i64 n, T alpha, const T* x, i64 incx, const T* y, i64 incy, T* A,
+             i64 lda) {
     for (i64 j = 0; j < n; ++j) {
-        double yj = y[j * incy];
-        if (yj == 0.0) continue;
-        double temp = alpha * yj;
-        double* a = A + j * lda;
+        T yj = y[j * incy];
+        if (yj == T(0)) continue;
+        T temp = alpha * yj;
+        T* a = A + j * lda;
         for (i64 i = 0; i < m; ++i) a[i] = std::fma(x[i * incx], temp, a[i]);
     }
 }
 
-void nv_dsyr(char uplo, i64 n, double alpha, const double* x, i64 incx, double* A, i64 lda) {
+template <typename T>
+void nv_dsyr(char uplo, i64 n, T alpha, const T* x, i64 incx, T* A, i64 lda) {
     for (i64 j = 0; j < n; ++j) {
-        double xj = x[j * incx];
-        if (xj == 0.0) continue;
-        double temp = alpha * xj;
-        double* a = A + j * lda;
+        T xj = x[j * incx];
+        if (xj == T(0)) continue;
+        T temp = alpha * xj;
+        T* a = A + j * lda;
         if (uplo == 'L') {
             for (i64 i = j; i < n; ++i) a[i] = std::fma(x[i * incx], temp, a[i]);
         } else {
@@ -115,96 +125,99 @@ void nv_dsyr(char uplo, i64 n, double alpha, const double* x, i64 incx, double* 
     }
 }
 
-void nv_dgemm(char ta, char tb, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda,
-              const double* B, i64 ldb, double beta, double* C, i64 ldc) {
+template <typename T>
+void nv_dgemm(char ta, char tb, i64 m, i64 n, i64 k, T alpha, const T* A, i64 lda, const T* B,
+              i64 ldb, T beta, T* C, i64 ldc) {
     if (m == 0 || n == 0) return;
     const bool nota = (ta == 'N'), notb = (tb == 'N');
     for (i64 j = 0; j < n; ++j) {
-        double* c = C + j * ldc;
+        T* c = C + j * ldc;
         if (notb && nota) {
-            if (beta == 0.0) for (i64 i = 0; i < m; ++i) c[i] = 0.0;
-            else if (beta != 1.0) for (i64 i = 0; i < m; ++i) c[i] *= beta;
+            if (beta == T(0)) for (i64 i = 0; i < m; ++i) c[i] = T(0);
+            else if (beta != T(1)) for (i64 i = 0; i < m; ++i) c[i] *= beta;
             for (i64 l = 0; l < k; ++l) {
-                double temp = alpha * B[IDX(l, j, ldb)];
-                const double* a = A + l * lda;
+                T temp = alpha * B[IDX(l, j, ldb)];
+                const T* a = A + l * lda;
                 for (i64 i = 0; i < m; ++i) c[i] = std::fma(temp, a[i], c[i]);
             }
         } else if (!notb && nota) {
-            if (beta == 0.0) for (i64 i = 0; i < m; ++i) c[i] = 0.0;
-            else if (beta != 1.0) for (i64 i = 0; i < m; ++i) c[i] *= beta;
+            if (beta == T(0)) for (i64 i = 0; i < m; ++i) c[i] = T(0);
+            else if (beta != T(1)) for (i64 i = 0; i < m; ++i) c[i] *= beta;
             for (i64 l = 0; l < k; ++l) {
-                double temp = alpha * B[IDX(j, l, ldb)];
-                const double* a = A + l * lda;
+                T temp = alpha * B[IDX(j, l, ldb)];
+                const T* a = A + l * lda;
                 for (i64 i = 0; i < m; ++i) c[i] = std::fma(temp, a[i], c[i]);
             }
         } else {
             for (i64 i = 0; i < m; ++i) {
-                double temp = 0.0;
+                T temp = T(0);
                 for (i64 l = 0; l < k; ++l) {
-                    double av = A[IDX(l, i, lda)];
-                    double bv = notb ? B[IDX(l, j, ldb)] : B[IDX(j, l, ldb)];
+                    T av = A[IDX(l, i, lda)];
+                    T bv = notb ? B[IDX(l, j, ldb)] : B[IDX(j, l, ldb)];
                     temp = std::fma(av, bv, temp);
                 }
-                c[i] = (beta == 0.0) ? alpha * temp : alpha * temp + beta * c[i];
+                c[i] = (beta == T(0)) ? alpha * temp : alpha * temp + beta * c[i];
             }
         }
     }
 }
 
-void nv_dsyrk(char uplo, char trans, i64 n, i64 k, double alpha, const double* A, i64 lda,
-              double beta, double* C, i64 ldc) {
+template <typename T>
+void nv_dsyrk(char uplo, char trans, i64 n, i64 k, T alpha, const T* A, i64 lda, T beta, T* C,
+              i64 ldc) {
     for (i64 j = 0; j < n; ++j) {
-        double* c = C + j * ldc;
+        T* c = C + j * ldc;
         i64 i0 = (uplo == 'L') ? j : 0, i1 = (uplo == 'L') ? n : j + 1;
         if (trans == 'N') {
-            if (beta == 0.0) for (i64 i = i0; i < i1; ++i) c[i] = 0.0;
-            else if (beta != 1.0) for (i64 i = i0; i < i1; ++i) c[i] *= beta;
+            if (beta == T(0)) for (i64 i = i0; i < i1; ++i) c[i] = T(0);
+            else if (beta != T(1)) for (i64 i = i0; i < i1; ++i) c[i] *= beta;
             for (i64 l = 0; l < k; ++l) {
-                double ajl = A[IDX(j, l, lda)];
-                if (ajl == 0.0) continue;
-                double temp = alpha * ajl;
-                const double* a = A + l * lda;
+                T ajl = A[IDX(j, l, lda)];
+                if (ajl == T(0)) continue;
+                T temp = alpha * ajl;
+                const T* a = A + l * lda;
                 for (i64 i = i0; i < i1; ++i) c[i] = std::fma(temp, a[i], c[i]);
             }
         } else {
             for (i64 i = i0; i < i1; ++i) {
-                double temp = 0.0;
+                T temp = T(0);
                 for (i64 l = 0; l < k; ++l)
                     temp = std::fma(A[IDX(l, i, lda)], A[IDX(l, j, lda)], temp);
-                c[i] = (beta == 0.0) ? alpha * temp : alpha * temp + beta * c[i];
+                c[i] = (beta == T(0)) ? alpha * temp : alpha * temp + beta * c[i];
             }
         }
     }
 }
 
-void nv_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, double alpha,
-              const double* A, i64 lda, double* B, i64 ldb) {
+template <typename T>
+void nv_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, T alpha, const T* A,
+              i64 lda, T* B, i64 ldb) {
     if (m == 0 || n == 0) return;
     const bool nounit = (diag == 'N');
-    if (alpha != 1.0)
+    if (alpha != T(1))
         for (i64 j = 0; j < n; ++j)
             for (i64 i = 0; i < m; ++i) B[IDX(i, j, ldb)] *= alpha;
     if (side == 'L') {
         if (trans == 'N') {
             if (uplo == 'U') {
                 for (i64 j = 0; j < n; ++j) {
-                    double* b = B + j * ldb;
+                    T* b = B + j * ldb;
                     for (i64 k = m - 1; k >= 0; --k) {
-                        if (b[k] == 0.0) continue;
+                        if (b[k] == T(0)) continue;
                         if (nounit) b[k] = b[k] / A[IDX(k, k, lda)];
-                        double bk = -b[k];
-                        const double* a = A + k * lda;
+                        T bk = -b[k];
+                        const T* a = A + k * lda;
                         for (i64 i = 0; i < k; ++i) b[i] = std::fma(bk, a[i], b[i]);
                     }
                 }
             } else {
                 for (i64 j = 0; j < n; ++j) {
-                    double* b = B + j * ldb;
+                    T* b = B + j * ldb;
                     for (i64 k = 0; k < m; ++k) {
-                        if (b[k] == 0.0) continue;
+                        if (b[k] == T(0)) continue;
                         if (nounit) b[k] = b[k] / A[IDX(k, k, lda)];
-                        double bk = -b[k];
-                        const double* a = A + k * lda;
+                        T bk = -b[k];
+                        const T* a = A + k * lda;
                         for (i64 i = k + 1; i < m; ++i) b[i] = std::fma(bk, a[i], b[i]);
                     }
                 }
@@ -212,10 +225,10 @@ void nv_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, double 
         } else {  // B := inv(A^T) B
             if (uplo == 'U') {
                 for (i64 j = 0; j < n; ++j) {
-                    double* b = B + j * ldb;
+                    T* b = B + j * ldb;
                     for (i64 i = 0; i < m; ++i) {
-                        double temp = b[i];
-                        const double* a = A + i * lda;
+                        T temp = b[i];
+                        const T* a = A + i * lda;
                         for (i64 k = 0; k < i; ++k) temp = std::fma(-a[k], b[k], temp);
                         if (nounit) temp = temp / a[i];
                         b[i] = temp;
@@ -223,10 +236,10 @@ void nv_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, double 
                 }
             } else {
                 for (i64 j = 0; j < n; ++j) {
-                    double* b = B + j * ldb;
+                    T* b = B + j * ldb;
                     for (i64 i = m - 1; i >= 0; --i) {
-                        double temp = b[i];
-                        const double* a = A + i * lda;
+                        T temp = b[i];
+                        const T* a = A + i * lda;
                         for (i64 k = i + 1; k < m; ++k) temp = std::fma(-a[k], b[k], temp);
                         if (nounit) temp = temp / a[i];
                         b[i] = temp;
@@ -238,29 +251,29 @@ void nv_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, double 
         if (trans == 'N') {
             if (uplo == 'U') {
                 for (i64 j = 0; j < n; ++j) {
-                    double* bj = B + j * ldb;
+                    T* bj = B + j * ldb;
                     for (i64 k = 0; k < j; ++k) {
-                        double akj = A[IDX(k, j, lda)];
-                        if (akj == 0.0) continue;
-                        const double* bk = B + k * ldb;
+                        T akj = A[IDX(k, j, lda)];
+                        if (akj == T(0)) continue;
+                        const T* bk = B + k * ldb;
                         for (i64 i = 0; i < m; ++i) bj[i] = std::fma(-akj, bk[i], bj[i]);
                     }
                     if (nounit) {
-                        double temp = 1.0 / A[IDX(j, j, lda)];
+                        T temp = T(1) / A[IDX(j, j, lda)];
                         for (i64 i = 0; i < m; ++i) bj[i] = temp * bj[i];
                     }
                 }
             } else {
                 for (i64 j = n - 1; j >= 0; --j) {
-                    double* bj = B + j * ldb;
+                    T* bj = B + j * ldb;
                     for (i64 k = j + 1; k < n; ++k) {
-                        double akj = A[IDX(k, j, lda)];
-                        if (akj == 0.0) continue;
-                        const double* bk = B + k * ldb;
+                        T akj = A[IDX(k, j, lda)];
+                        if (akj == T(0)) continue;
+                        const T* bk = B + k * ldb;
                         for (i64 i = 0; i < m; ++i) bj[i] = std::fma(-akj, bk[i], bj[i]);
                     }
                     if (nounit) {
-                        double temp = 1.0 / A[IDX(j, j, lda)];
+                        T temp = T(1) / A[IDX(j, j, lda)];
                         for (i64 i = 0; i < m; ++i) bj[i] = temp * bj[i];
                     }
                 }
@@ -268,29 +281,29 @@ void nv_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, double 
         } else {  // B := B inv(A^T)
             if (uplo == 'U') {
                 for (i64 k = n - 1; k >= 0; --k) {
-                    double* bk = B + k * ldb;
+                    T* bk = B + k * ldb;
                     if (nounit) {
-                        double temp = 1.0 / A[IDX(k, k, lda)];
+                        T temp = T(1) / A[IDX(k, k, lda)];
                         for (i64 i = 0; i < m; ++i) bk[i] = temp * bk[i];
                     }
                     for (i64 j = 0; j < k; ++j) {
-                        double ajk = A[IDX(j, k, lda)];
-                        if (ajk == 0.0) continue;
-                        double* bj = B + j * ldb;
+                        T ajk = A[IDX(j, k, lda)];
+                        if (ajk == T(0)) continue;
+                        T* bj = B + j * ldb;
                         for (i64 i = 0; i < m; ++i) bj[i] = std::fma(-ajk, bk[i], bj[i]);
                     }
                 }
             } else {
                 for (i64 k = 0; k < n; ++k) {
-                    double* bk = B + k * ldb;
+                    T* bk = B + k * ldb;
                     if (nounit) {
-                        double temp = 1.0 / A[IDX(k, k, lda)];
+                        T temp = T(1) / A[IDX(k, k, lda)];
                         for (i64 i = 0; i < m; ++i) bk[i] = temp * bk[i];
                     }
                     for (i64 j = k + 1; j < n; ++j) {
-                        double ajk = A[IDX(j, k, lda)];
-                        if (ajk == 0.0) continue;
-                        double* bj = B + j * ldb;
+                        T ajk = A[IDX(j, k, lda)];
+                        if (ajk == T(0)) continue;
+                        T* bj = B + j * ldb;
                         for (i64 i = 0; i < m; ++i) bj[i] = std::fma(-ajk, bk[i], bj[i]);
                     }
                 }
@@ -299,7 +312,8 @@ void nv_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, double 
     }
 }
 
-Blas g_naive = {nv_idamax, nv_dswap, nv_dscal, nv_dger, nv_dsyr, nv_dtrsm, nv_dgemm, nv_dsyrk};
+Blas g_naive = {nv_idamax<double>, nv_dswap<double>, nv_dscal<double>, nv_dger<double>,
+                nv_dsyr<double>,   nv_dtrsm<double>, nv_dgemm<double>, nv_dsyrk<double>};
 
 // ---------------- host BLAS through dlopen (Fortran ABI, LP64) ----------------
 typedef int fint;
@@ -413,13 +427,75 @@ bool open_fortran_blas(const char* path, const char* prefix) {
     return true;
 }
 
+// ---------------------------------------------------------------------------------------
+// Typed BLAS dispatch: double through the selected backend, float through the naive one (the
+// FP32 parity tests use the naive backend only).
+// ---------------------------------------------------------------------------------------
+template <typename T>
+struct Ops;
+template <>
+struct Ops<double> {
+    static i64 iamax(i64 n, const double* x, i64 incx) { return g_blas->idamax(n, x, incx); }
+    static void swap(i64 n, double* x, i64 incx, double* y, i64 incy) {
+        g_blas->dswap(n, x, incx, y, incy);
+    }
+    static void scal(i64 n, double a, double* x, i64 incx) { g_blas->dscal(n, a, x, incx); }
+    static void ger(i64 m, i64 n, double al, const double* x, i64 incx, const double* y,
+                    i64 incy, double* A, i64 lda) {
+        g_blas->dger(m, n, al, x, incx, y, incy, A, lda);
+    }
+    static void syr(char u, i64 n, double al, const double* x, i64 incx, double* A, i64 lda) {
+        g_blas->dsyr(u, n, al, x, incx, A, lda);
+    }
+    static void trsm(char sd, char u, char t, char d, i64 m, i64 n, double al, const double* A,
+                     i64 lda, double* B, i64 ldb) {
+        g_blas->dtrsm(sd, u, t, d, m, n, al, A, lda, B, ldb);
+    }
+    static void gemm(char ta, char tb, i64 m, i64 n, i64 k, double al, const double* A, i64 lda,
+                     const double* B, i64 ldb, double be, double* C, i64 ldc) {
+        g_blas->dgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+    }
+    static void syrk(char u, char t, i64 n, i64 k, double al, const double* A, i64 lda,
+                     double be, double* C, i64 ldc) {
+        g_blas->dsyrk(u, t, n, k, al, A, lda, be, C, ldc);
+    }
+};
+template <>
+struct Ops<float> {
+    static i64 iamax(i64 n, const float* x, i64 incx) { return nv_idamax<float>(n, x, incx); }
+    static void swap(i64 n, float* x, i64 incx, float* y, i64 incy) {
+        nv_dswap<float>(n, x, incx, y, incy);
+    }
+    static void scal(i64 n, float a, float* x, i64 incx) { nv_dscal<float>(n, a, x, incx); }
+    static void ger(i64 m, i64 n, float al, const float* x, i64 incx, const float* y, i64 incy,
+                    float* A, i64 lda) {
+        nv_dger<float>(m, n, al, x, incx, y, incy, A, lda);
+    }
+    static void syr(char u, i64 n, float al, const float* x, i64 incx, float* A, i64 lda) {
+        nv_dsyr<float>(u, n, al, x, incx, A, lda);
+    }
+    static void trsm(char sd, char u, char t, char d, i64 m, i64 n, float al, const float* A,
+                     i64 lda, float* B, i64 ldb) {
+        nv_dtrsm<float>(sd, u, t, d, m, n, al, A, lda, B, ldb);
+    }
+    static void gemm(char ta, char tb, i64 m, i64 n, i64 k, float al, const float* A, i64 lda,
+                     const float* B, i64 ldb, float be, float* C, i64 ldc) {
+        nv_dgemm<float>(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+    }
+    static void syrk(char u, char t, i64 n, i64 k, float al, const float* A, i64 lda, float be,
+                     float* C, i64 ldc) {
+        nv_dsyrk<float>(u, t, n, k, al, A, lda, be, C, ldc);
+    }
+};
+
 std::mt19937& generator() {
     static std::mt19937 gen((21L << 16) | 0L);
     return gen;
 }
 
 // ---------------------------------------------------------------------------------------
-// Elemental's sequential algorithms
+// Elemental's sequential algorithms (templates over the field, instantiated for double and
+// float as the reference's PROTO macros do, e.g. src/lapack_like/factor/LU.cpp:233-313)
 // ---------------------------------------------------------------------------------------
 
 // lu::Panel, src/lapack_like/factor/LU/Panel.hpp:15-60. piv[c] receives the swap
@@ -427,72 +503,250 @@ std::mt19937& generator() {
 // an exactly zero pivot at panel column c (the point where SingularMatrixException is thrown).
 // When trace_col is in [0,n), the candidate column of that step (rows k..m-1, before the
 // pivot search) is copied to trace_out and the panel stops, returning -1.
-int lu_panel(double* A, i64 m, i64 n, i64 lda, int32_t* piv, i64 trace_col = -1,
-             double* trace_out = nullptr) {
+template <typename T>
+int lu_panel(T* A, i64 m, i64 n, i64 lda, int32_t* piv, i64 trace_col = -1,
+             T* trace_out = nullptr) {
     for (i64 k = 0; k < n; ++k) {
         const i64 horz = n - (k + 1), vert = m - (k + 1);
         if (k == trace_col) {
             for (i64 i = k; i < m; ++i) trace_out[i - k] = A[IDX(i, k, lda)];
             return -1;
         }
-        const i64 maxInd = g_blas->idamax(vert + 1, &A[IDX(k, k, lda)], 1);
+        const i64 maxInd = Ops<T>::iamax(vert + 1, &A[IDX(k, k, lda)], 1);
         const i64 iPiv = maxInd + k;
         piv[k] = (int32_t)iPiv;
-        if (iPiv != k) g_blas->dswap(n, &A[k], lda, &A[iPiv], lda);
-        const double alpha = A[IDX(k, k, lda)];
-        if (alpha == 0.0) return (int)(k + 1);
-        const double alphaInv = 1.0 / alpha;
-        g_blas->dscal(vert, alphaInv, &A[IDX(k + 1, k, lda)], 1);
-        g_blas->dger(vert, horz, -1.0, &A[IDX(k + 1, k, lda)], 1, &A[IDX(k, k + 1, lda)], lda,
-                     &A[IDX(k + 1, k + 1, lda)], lda);
+        if (iPiv != k) Ops<T>::swap(n, &A[k], lda, &A[iPiv], lda);
+        const T alpha = A[IDX(k, k, lda)];
+        if (alpha == T(0)) return (int)(k + 1);
+        const T alphaInv = T(1) / alpha;
+        Ops<T>::scal(vert, alphaInv, &A[IDX(k + 1, k, lda)], 1);
+        Ops<T>::ger(vert, horz, T(-1), &A[IDX(k + 1, k, lda)], 1, &A[IDX(k, k + 1, lda)], lda,
+                    &A[IDX(k + 1, k + 1, lda)], lda);
+    }
+    return 0;
+}
+
+// lu::Unb (LU without pivoting), src/lapack_like/factor/LU/Local.hpp:44-61: Scal by
+// 1/alpha, then Geru(-1). Returns 0, or j+1 for an exactly zero pivot at column j.
+template <typename T>
+int lu_unb(T* A, i64 m, i64 n, i64 lda) {
+    for (i64 j = 0; j < std::min(m, n); ++j) {
+        const T alpha = A[IDX(j, j, lda)];
+        if (alpha == T(0)) return (int)(j + 1);
+        Ops<T>::scal(m - (j + 1), T(1) / alpha, &A[IDX(j + 1, j, lda)], 1);
+        Ops<T>::ger(m - (j + 1), n - (j + 1), T(-1), &A[IDX(j + 1, j, lda)], 1,
+                    &A[IDX(j, j + 1, lda)], lda, &A[IDX(j + 1, j + 1, lda)], lda);
     }
     return 0;
 }
 
 // Permutation::PermuteRows with implicit origins j (Permutation.cpp:544-580) applied
 // through RowSwap (src/blas_like/level1/Swap.cpp:99-116).
-void permute_rows(const int32_t* dests, i64 nswaps, double* A, i64 height, i64 width,
-                  i64 lda) {
+template <typename T>
+void permute_rows(const int32_t* dests, i64 nswaps, T* A, i64 height, i64 width, i64 lda) {
     if (height == 0 || width == 0) return;
     for (i64 j = 0; j < nswaps; ++j)
-        if (dests[j] != j) g_blas->dswap(width, &A[j], lda, &A[dests[j]], lda);
+        if (dests[j] != j) Ops<T>::swap(width, &A[j], lda, &A[dests[j]], lda);
 }
 
 // Permutation::InversePermuteRows (Permutation.cpp:586-620): the swaps in reverse order.
-void inverse_permute_rows(const int32_t* dests, i64 nswaps, double* A, i64 height,
-                          i64 width, i64 lda) {
+template <typename T>
+void inverse_permute_rows(const int32_t* dests, i64 nswaps, T* A, i64 height, i64 width,
+                          i64 lda) {
     if (height == 0 || width == 0) return;
     for (i64 j = nswaps - 1; j >= 0; --j)
-        if (dests[j] != j) g_blas->dswap(width, &A[j], lda, &A[dests[j]], lda);
+        if (dests[j] != j) Ops<T>::swap(width, &A[j], lda, &A[dests[j]], lda);
 }
 
-int chol_unb_lower(double* A, i64 n, i64 lda) {
+template <typename T>
+int chol_unb_lower(T* A, i64 n, i64 lda) {
     // LowerVariant3Unblocked, Cholesky/LowerVariant3.hpp:15-41
     for (i64 j = 0; j < n; ++j) {
-        double alpha = A[IDX(j, j, lda)];
-        if (alpha <= 0.0) return (int)(j + 1);
+        T alpha = A[IDX(j, j, lda)];
+        if (alpha <= T(0)) return (int)(j + 1);
         alpha = std::sqrt(alpha);
         A[IDX(j, j, lda)] = alpha;
         const i64 h = n - (j + 1);
-        g_blas->dscal(h, 1.0 / alpha, &A[IDX(j + 1, j, lda)], 1);
-        g_blas->dsyr('L', h, -1.0, &A[IDX(j + 1, j, lda)], 1, &A[IDX(j + 1, j + 1, lda)], lda);
+        Ops<T>::scal(h, T(1) / alpha, &A[IDX(j + 1, j, lda)], 1);
+        Ops<T>::syr('L', h, T(-1), &A[IDX(j + 1, j, lda)], 1, &A[IDX(j + 1, j + 1, lda)], lda);
     }
     return 0;
 }
 
-int chol_unb_upper(double* A, i64 n, i64 lda) {
+template <typename T>
+int chol_unb_upper(T* A, i64 n, i64 lda) {
     // UpperVariant3Unblocked, Cholesky/UpperVariant3.hpp:15-46 (Conj is the identity for
     // real data)
     for (i64 j = 0; j < n; ++j) {
-        double alpha = A[IDX(j, j, lda)];
-        if (alpha <= 0.0) return (int)(j + 1);
+        T alpha = A[IDX(j, j, lda)];
+        if (alpha <= T(0)) return (int)(j + 1);
         alpha = std::sqrt(alpha);
         A[IDX(j, j, lda)] = alpha;
         const i64 w = n - (j + 1);
-        g_blas->dscal(w, 1.0 / alpha, &A[IDX(j, j + 1, lda)], lda);
-        g_blas->dsyr('U', w, -1.0, &A[IDX(j, j + 1, lda)], lda, &A[IDX(j + 1, j + 1, lda)], lda);
+        Ops<T>::scal(w, T(1) / alpha, &A[IDX(j, j + 1, lda)], lda);
+        Ops<T>::syr('U', w, T(-1), &A[IDX(j, j + 1, lda)], lda, &A[IDX(j + 1, j + 1, lda)], lda);
     }
     return 0;
+}
+
+// El::LU(Matrix<F>&, Permutation&), src/lapack_like/factor/LU.cpp:104-142.
+// dests[0..minDim) receives Permutation::SwapDestinations() (0-based). At an exactly zero
+// pivot the function returns (global column + 1) with A as the reference leaves it when
+// lu::Panel throws: the earlier blocks finished, the current panel's rows swapped up to the
+// failing column, the columns outside the panel not yet permuted for this block.
+template <typename T>
+int lu_impl(T* A, i64 m, i64 n, i64 lda, i64 bsize, int32_t* dests, i64 trace_col,
+            T* trace_out) {
+    const i64 minDim = std::min(m, n);
+    std::vector<int32_t> piv(bsize);
+    for (i64 k = 0; k < minDim; k += bsize) {
+        const i64 nb = std::min(bsize, minDim - k);
+        T* AB0 = A + k;
+        T* AB1 = A + IDX(k, k, lda);
+        T* AB2 = A + IDX(k, k + nb, lda);
+        const i64 tc = (trace_col >= k && trace_col < k + nb) ? trace_col - k : -1;
+        const int pinfo = lu_panel(AB1, m - k, nb, lda, piv.data(), tc, trace_out);
+        if (pinfo < 0) return -1;
+        for (i64 c = 0; c < nb; ++c) dests[k + c] = (int32_t)(k + piv[c]);
+        if (pinfo) return (int)(k + pinfo);
+        permute_rows(piv.data(), nb, AB0, m - k, k, lda);
+        permute_rows(piv.data(), nb, AB2, m - k, n - k - nb, lda);
+        T* A11 = AB1;
+        T* A12 = AB2;
+        T* A21 = A + IDX(k + nb, k, lda);
+        T* A22 = A + IDX(k + nb, k + nb, lda);
+        Ops<T>::trsm('L', 'L', 'N', 'U', nb, n - k - nb, T(1), A11, lda, A12, lda);
+        Ops<T>::gemm('N', 'N', m - k - nb, n - k - nb, nb, T(-1), A21, lda, A12, lda, T(1), A22,
+                     lda);
+    }
+    return 0;
+}
+
+// El::LU(Matrix<F>&) without pivoting, src/lapack_like/factor/LU.cpp:21-44: lu::Unb on A11,
+// Trsm(RIGHT,UPPER,NORMAL,NON_UNIT) for A21, Trsm(LEFT,LOWER,NORMAL,UNIT) for A12, Gemm.
+template <typename T>
+int lu_nopiv_impl(T* A, i64 m, i64 n, i64 lda, i64 bsize) {
+    const i64 minDim = std::min(m, n);
+    for (i64 k = 0; k < minDim; k += bsize) {
+        const i64 nb = std::min(bsize, minDim - k);
+        T* A11 = A + IDX(k, k, lda);
+        T* A12 = A + IDX(k, k + nb, lda);
+        T* A21 = A + IDX(k + nb, k, lda);
+        T* A22 = A + IDX(k + nb, k + nb, lda);
+        const int info = lu_unb(A11, nb, nb, lda);
+        if (info) return (int)(k + info);
+        Ops<T>::trsm('R', 'U', 'N', 'N', m - k - nb, nb, T(1), A11, lda, A21, lda);
+        Ops<T>::trsm('L', 'L', 'N', 'U', nb, n - k - nb, T(1), A11, lda, A12, lda);
+        Ops<T>::gemm('N', 'N', m - k - nb, n - k - nb, nb, T(-1), A21, lda, A12, lda, T(1), A22,
+                     lda);
+    }
+    return 0;
+}
+
+// lu::SolveAfter(orientation, A, [P,] B), src/lapack_like/factor/LU/SolveAfter.hpp:15-38
+// (no pivots: dests null) and :66-92. orient: 0 NORMAL, 1 TRANSPOSE, 2 ADJOINT (identical
+// for real data).
+template <typename T>
+int lu_solve_after(int orient, const T* A, i64 n, i64 lda, const int32_t* dests, i64 ndests,
+                   T* B, i64 nrhs, i64 ldb) {
+    if (orient == 0) {
+        if (dests) permute_rows(dests, ndests, B, n, nrhs, ldb);
+        Ops<T>::trsm('L', 'L', 'N', 'U', n, nrhs, T(1), A, lda, B, ldb);
+        Ops<T>::trsm('L', 'U', 'N', 'N', n, nrhs, T(1), A, lda, B, ldb);
+    } else {
+        Ops<T>::trsm('L', 'U', 'T', 'N', n, nrhs, T(1), A, lda, B, ldb);
+        Ops<T>::trsm('L', 'L', 'T', 'U', n, nrhs, T(1), A, lda, B, ldb);
+        if (dests) inverse_permute_rows(dests, ndests, B, n, nrhs, ldb);
+    }
+    return 0;
+}
+
+// El::LinearSolve(A, B): ACopy(A), RowEchelon(ACopy, B), Trsm(L,U,N,N)
+// (src/lapack_like/solve/Linear.cpp:30-76,176-183,212-218). dests (optional, may be null)
+// receives the swap destinations RowEchelon records.
+template <typename T>
+int linear_solve(const T* A, i64 n, i64 lda, T* B, i64 nrhs, i64 ldb, i64 bsize,
+                 int32_t* dests) {
+    const i64 ld = std::max<i64>(n, 1);
+    std::vector<T> C((size_t)(ld * n));
+    for (i64 j = 0; j < n; ++j)
+        for (i64 i = 0; i < n; ++i) C[IDX(i, j, ld)] = A[IDX(i, j, lda)];
+    T* Ac = C.data();
+    std::vector<int32_t> piv(bsize);
+    for (i64 k = 0; k < n; k += bsize) {
+        const i64 nb = std::min(bsize, n - k);
+        T* AB1 = Ac + IDX(k, k, ld);
+        T* AB2 = Ac + IDX(k, k + nb, ld);
+        T* BB = B + k;
+        const int pinfo = lu_panel(AB1, n - k, nb, ld, piv.data());
+        if (dests)
+            for (i64 c = 0; c < nb; ++c) dests[k + c] = (int32_t)(k + piv[c]);
+        if (pinfo) return (int)(k + pinfo);
+        permute_rows(piv.data(), nb, AB2, n - k, n - k - nb, ld);
+        permute_rows(piv.data(), nb, BB, n - k, nrhs, ldb);
+        T* A11 = AB1;
+        T* A12 = AB2;
+        T* A21 = Ac + IDX(k + nb, k, ld);
+        T* A22 = Ac + IDX(k + nb, k + nb, ld);
+        T* B1 = B + k;
+        T* B2 = B + k + nb;
+        Ops<T>::trsm('L', 'L', 'N', 'U', nb, n - k - nb, T(1), A11, ld, A12, ld);
+        Ops<T>::trsm('L', 'L', 'N', 'U', nb, nrhs, T(1), A11, ld, B1, ldb);
+        Ops<T>::gemm('N', 'N', n - k - nb, n - k - nb, nb, T(-1), A21, ld, A12, ld, T(1), A22, ld);
+        Ops<T>::gemm('N', 'N', n - k - nb, nrhs, nb, T(-1), A21, ld, B1, ldb, T(1), B2, ldb);
+    }
+    Ops<T>::trsm('L', 'U', 'N', 'N', n, nrhs, T(1), Ac, ld, B, ldb);
+    return 0;
+}
+
+// El::Cholesky(uplo, A): uplo 0 LOWER, 1 UPPER (Cholesky.cpp:26-38, variant 3).
+template <typename T>
+int cholesky(int uplo, T* A, i64 n, i64 lda, i64 bsize) {
+    for (i64 k = 0; k < n; k += bsize) {
+        const i64 nb = std::min(bsize, n - k);
+        T* A11 = A + IDX(k, k, lda);
+        T* A22 = A + IDX(k + nb, k + nb, lda);
+        if (uplo == 0) {
+            const int info = chol_unb_lower(A11, nb, lda);
+            if (info) return (int)(k + info);
+            T* A21 = A + IDX(k + nb, k, lda);
+            Ops<T>::trsm('R', 'L', 'T', 'N', n - k - nb, nb, T(1), A11, lda, A21, lda);
+            Ops<T>::syrk('L', 'N', n - k - nb, nb, T(-1), A21, lda, T(1), A22, lda);
+        } else {
+            const int info = chol_unb_upper(A11, nb, lda);
+            if (info) return (int)(k + info);
+            T* A12 = A + IDX(k, k + nb, lda);
+            Ops<T>::trsm('L', 'U', 'T', 'N', nb, n - k - nb, T(1), A11, lda, A12, lda);
+            Ops<T>::syrk('U', 'T', n - k - nb, nb, T(-1), A12, lda, T(1), A22, lda);
+        }
+    }
+    return 0;
+}
+
+// cholesky::SolveAfter(uplo, orientation, A, B), Cholesky/SolveAfter.hpp:15-43.
+// (TRANSPOSE conjugates B before and after, the identity for real data.)
+template <typename T>
+int cholesky_solve_after(int uplo, const T* A, i64 n, i64 lda, T* B, i64 nrhs, i64 ldb) {
+    if (uplo == 0) {
+        Ops<T>::trsm('L', 'L', 'N', 'N', n, nrhs, T(1), A, lda, B, ldb);
+        Ops<T>::trsm('L', 'L', 'T', 'N', n, nrhs, T(1), A, lda, B, ldb);
+    } else {
+        Ops<T>::trsm('L', 'U', 'T', 'N', n, nrhs, T(1), A, lda, B, ldb);
+        Ops<T>::trsm('L', 'U', 'N', 'N', n, nrhs, T(1), A, lda, B, ldb);
+    }
+    return 0;
+}
+
+// El::HPDSolve(uplo, orientation, A, B), src/lapack_like/solve/HPD.cpp:15-25,47-57.
+template <typename T>
+int hpd_solve(int uplo, const T* A, i64 n, i64 lda, T* B, i64 nrhs, i64 ldb, i64 bsize) {
+    const i64 ld = std::max<i64>(n, 1);
+    std::vector<T> C((size_t)(ld * n));
+    for (i64 j = 0; j < n; ++j)
+        for (i64 i = 0; i < n; ++i) C[IDX(i, j, ld)] = A[IDX(i, j, lda)];
+    const int info = cholesky(uplo, C.data(), n, ld, bsize);
+    if (info) return info;
+    return cholesky_solve_after(uplo, C.data(), n, ld, B, nrhs, ldb);
 }
 
 }  // namespace
@@ -562,16 +816,6 @@ void orc_gaussian(double* A, i64 m, i64 n, i64 lda, double mean, double stddev) 
         }
 }
 
-// MakeUniform over the real "ball" [center-radius, center+radius]
-// (Uniform.cpp:16-31, random/impl.hpp:127-137,222-224).
-void orc_uniform(double* A, i64 m, i64 n, i64 lda, double center, double radius) {
-    for (i64 j = 0; j < n; ++j)
-        for (i64 i = 0; i < m; ++i) {
-            std::uniform_real_distribution<double> d(center - radius, center + radius);
-            A[IDX(i, j, lda)] = d(generator());
-        }
-}
-
 // GEPPGrowth, src/matrices/deterministic/misc/GEPPGrowth.cpp:15-30.
 void orc_geppgrowth(double* A, i64 n, i64 lda) {
     for (i64 j = 0; j < n; ++j)
@@ -587,37 +831,11 @@ void orc_geppgrowth(double* A, i64 n, i64 lda) {
 // the reference throws SingularMatrixException / NonHPDMatrixException.
 // ---------------------------------------------------------------------------------------
 
-// El::LU(Matrix<F>&, Permutation&), src/lapack_like/factor/LU.cpp:104-142.
-// dests[0..minDim) receives Permutation::SwapDestinations() (0-based).
-static int lu_impl(double* A, i64 m, i64 n, i64 lda, i64 bsize, int32_t* dests, i64 trace_col,
-                   double* trace_out) {
-    const i64 minDim = std::min(m, n);
-    std::vector<int32_t> piv(bsize);
-    for (i64 k = 0; k < minDim; k += bsize) {
-        const i64 nb = std::min(bsize, minDim - k);
-        double* AB0 = A + k;
-        double* AB1 = A + IDX(k, k, lda);
-        double* AB2 = A + IDX(k, k + nb, lda);
-        const i64 tc = (trace_col >= k && trace_col < k + nb) ? trace_col - k : -1;
-        const int pinfo = lu_panel(AB1, m - k, nb, lda, piv.data(), tc, trace_out);
-        if (pinfo < 0) return -1;
-        for (i64 c = 0; c < nb; ++c) dests[k + c] = (int32_t)(k + piv[c]);
-        if (pinfo) return (int)(k + pinfo);
-        permute_rows(piv.data(), nb, AB0, m - k, k, lda);
-        permute_rows(piv.data(), nb, AB2, m - k, n - k - nb, lda);
-        double* A11 = AB1;
-        double* A12 = AB2;
-        double* A21 = A + IDX(k + nb, k, lda);
-        double* A22 = A + IDX(k + nb, k + nb, lda);
-        g_blas->dtrsm('L', 'L', 'N', 'U', nb, n - k - nb, 1.0, A11, lda, A12, lda);
-        g_blas->dgemm('N', 'N', m - k - nb, n - k - nb, nb, -1.0, A21, lda, A12, lda, 1.0, A22,
-                      lda);
-    }
-    return 0;
-}
-
 int orc_lu(double* A, i64 m, i64 n, i64 lda, i64 bsize, int32_t* dests) {
-    return lu_impl(A, m, n, lda, bsize, dests, -1, nullptr);
+    return lu_impl(A, m, n, lda, bsize, dests, -1, (double*)nullptr);
+}
+int orc_slu(float* A, i64 m, i64 n, i64 lda, i64 bsize, int32_t* dests) {
+    return lu_impl(A, m, n, lda, bsize, dests, -1, (float*)nullptr);
 }
 
 // Runs El::LU up to global step `col` and copies that step's candidate column (rows
@@ -627,108 +845,65 @@ int orc_lu_trace_column(double* A, i64 m, i64 n, i64 lda, i64 bsize, i64 col, do
     std::vector<int32_t> d((size_t)std::min(m, n));
     return lu_impl(A, m, n, lda, bsize, d.data(), col, out);
 }
+int orc_slu_trace_column(float* A, i64 m, i64 n, i64 lda, i64 bsize, i64 col, float* out) {
+    std::vector<int32_t> d((size_t)std::min(m, n));
+    return lu_impl(A, m, n, lda, bsize, d.data(), col, out);
+}
 
-// lu::SolveAfter(orientation, A, P, B), src/lapack_like/factor/LU/SolveAfter.hpp:66-92.
-// orient: 0 NORMAL, 1 TRANSPOSE, 2 ADJOINT (identical for real data).
+// El::LU(A) without pivoting (LU.cpp:21-44).
+int orc_lu_nopiv(double* A, i64 m, i64 n, i64 lda, i64 bsize) {
+    return lu_nopiv_impl(A, m, n, lda, bsize);
+}
+int orc_slu_nopiv(float* A, i64 m, i64 n, i64 lda, i64 bsize) {
+    return lu_nopiv_impl(A, m, n, lda, bsize);
+}
+
+// lu::SolveAfter(orientation, A, P, B) (dests non-null) or (orientation, A, B) (dests null).
 int orc_lu_solve_after(int orient, const double* A, i64 n, i64 lda, const int32_t* dests,
                        i64 ndests, double* B, i64 nrhs, i64 ldb) {
-    if (orient == 0) {
-        permute_rows(dests, ndests, B, n, nrhs, ldb);
-        g_blas->dtrsm('L', 'L', 'N', 'U', n, nrhs, 1.0, A, lda, B, ldb);
-        g_blas->dtrsm('L', 'U', 'N', 'N', n, nrhs, 1.0, A, lda, B, ldb);
-    } else {
-        g_blas->dtrsm('L', 'U', 'T', 'N', n, nrhs, 1.0, A, lda, B, ldb);
-        g_blas->dtrsm('L', 'L', 'T', 'U', n, nrhs, 1.0, A, lda, B, ldb);
-        inverse_permute_rows(dests, ndests, B, n, nrhs, ldb);
-    }
-    return 0;
+    return lu_solve_after(orient, A, n, lda, dests, ndests, B, nrhs, ldb);
+}
+int orc_slu_solve_after(int orient, const float* A, i64 n, i64 lda, const int32_t* dests,
+                        i64 ndests, float* B, i64 nrhs, i64 ldb) {
+    return lu_solve_after(orient, A, n, lda, dests, ndests, B, nrhs, ldb);
 }
 
-// El::LinearSolve(A, B): ACopy(A), RowEchelon(ACopy, B), Trsm(L,U,N,N)
-// (src/lapack_like/solve/Linear.cpp:30-76,176-183,212-218). dests (optional, may be null)
-// receives the swap destinations RowEchelon records.
-int orc_linear_solve(const double* A, i64 n, i64 lda, double* B, i64 nrhs, i64 ldb,
-                     i64 bsize, int32_t* dests) {
-    const i64 ld = std::max<i64>(n, 1);
-    std::vector<double> C((size_t)(ld * n));
-    for (i64 j = 0; j < n; ++j)
-        for (i64 i = 0; i < n; ++i) C[IDX(i, j, ld)] = A[IDX(i, j, lda)];
-    double* Ac = C.data();
-    std::vector<int32_t> piv(bsize);
-    for (i64 k = 0; k < n; k += bsize) {
-        const i64 nb = std::min(bsize, n - k);
-        double* AB1 = Ac + IDX(k, k, ld);
-        double* AB2 = Ac + IDX(k, k + nb, ld);
-        double* BB = B + k;
-        const int pinfo = lu_panel(AB1, n - k, nb, ld, piv.data());
-        if (dests)
-            for (i64 c = 0; c < nb; ++c) dests[k + c] = (int32_t)(k + piv[c]);
-        if (pinfo) return (int)(k + pinfo);
-        permute_rows(piv.data(), nb, AB2, n - k, n - k - nb, ld);
-        permute_rows(piv.data(), nb, BB, n - k, nrhs, ldb);
-        double* A11 = AB1;
-        double* A12 = AB2;
-        double* A21 = Ac + IDX(k + nb, k, ld);
-        double* A22 = Ac + IDX(k + nb, k + nb, ld);
-        double* B1 = B + k;
-        double* B2 = B + k + nb;
-        g_blas->dtrsm('L', 'L', 'N', 'U', nb, n - k - nb, 1.0, A11, ld, A12, ld);
-        g_blas->dtrsm('L', 'L', 'N', 'U', nb, nrhs, 1.0, A11, ld, B1, ldb);
-        g_blas->dgemm('N', 'N', n - k - nb, n - k - nb, nb, -1.0, A21, ld, A12, ld, 1.0, A22, ld);
-        g_blas->dgemm('N', 'N', n - k - nb, nrhs, nb, -1.0, A21, ld, B1, ldb, 1.0, B2, ldb);
-    }
-    g_blas->dtrsm('L', 'U', 'N', 'N', n, nrhs, 1.0, Ac, ld, B, ldb);
-    return 0;
+int orc_linear_solve(const double* A, i64 n, i64 lda, double* B, i64 nrhs, i64 ldb, i64 bsize,
+                     int32_t* dests) {
+    return linear_solve(A, n, lda, B, nrhs, ldb, bsize, dests);
+}
+int orc_slinear_solve(const float* A, i64 n, i64 lda, float* B, i64 nrhs, i64 ldb, i64 bsize,
+                      int32_t* dests) {
+    return linear_solve(A, n, lda, B, nrhs, ldb, bsize, dests);
 }
 
-// El::Cholesky(uplo, A): uplo 0 LOWER, 1 UPPER (Cholesky.cpp:26-38, variant 3).
 int orc_cholesky(int uplo, double* A, i64 n, i64 lda, i64 bsize) {
-    for (i64 k = 0; k < n; k += bsize) {
-        const i64 nb = std::min(bsize, n - k);
-        double* A11 = A + IDX(k, k, lda);
-        double* A22 = A + IDX(k + nb, k + nb, lda);
-        if (uplo == 0) {
-            const int info = chol_unb_lower(A11, nb, lda);
-            if (info) return (int)(k + info);
-            double* A21 = A + IDX(k + nb, k, lda);
-            g_blas->dtrsm('R', 'L', 'T', 'N', n - k - nb, nb, 1.0, A11, lda, A21, lda);
-            g_blas->dsyrk('L', 'N', n - k - nb, nb, -1.0, A21, lda, 1.0, A22, lda);
-        } else {
-            const int info = chol_unb_upper(A11, nb, lda);
-            if (info) return (int)(k + info);
-            double* A12 = A + IDX(k, k + nb, lda);
-            g_blas->dtrsm('L', 'U', 'T', 'N', nb, n - k - nb, 1.0, A11, lda, A12, lda);
-            g_blas->dsyrk('U', 'T', n - k - nb, nb, -1.0, A12, lda, 1.0, A22, lda);
-        }
-    }
-    return 0;
+    return cholesky(uplo, A, n, lda, bsize);
+}
+int orc_scholesky(int uplo, float* A, i64 n, i64 lda, i64 bsize) {
+    return cholesky(uplo, A, n, lda, bsize);
 }
 
-// cholesky::SolveAfter(uplo, orientation, A, B), Cholesky/SolveAfter.hpp:15-43.
-// (TRANSPOSE conjugates B before and after, the identity for real data.)
-int orc_cholesky_solve_after(int uplo, int orient, const double* A, i64 n, i64 lda,
-                             double* B, i64 nrhs, i64 ldb) {
+int orc_cholesky_solve_after(int uplo, int orient, const double* A, i64 n, i64 lda, double* B,
+                             i64 nrhs, i64 ldb) {
     (void)orient;
-    if (uplo == 0) {
-        g_blas->dtrsm('L', 'L', 'N', 'N', n, nrhs, 1.0, A, lda, B, ldb);
-        g_blas->dtrsm('L', 'L', 'T', 'N', n, nrhs, 1.0, A, lda, B, ldb);
-    } else {
-        g_blas->dtrsm('L', 'U', 'T', 'N', n, nrhs, 1.0, A, lda, B, ldb);
-        g_blas->dtrsm('L', 'U', 'N', 'N', n, nrhs, 1.0, A, lda, B, ldb);
-    }
-    return 0;
+    return cholesky_solve_after(uplo, A, n, lda, B, nrhs, ldb);
+}
+int orc_scholesky_solve_after(int uplo, int orient, const float* A, i64 n, i64 lda, float* B,
+                              i64 nrhs, i64 ldb) {
+    (void)orient;
+    return cholesky_solve_after(uplo, A, n, lda, B, nrhs, ldb);
 }
 
-// El::HPDSolve(uplo, orientation, A, B), src/lapack_like/solve/HPD.cpp:15-25,47-57.
 int orc_hpd_solve(int uplo, int orient, const double* A, i64 n, i64 lda, double* B, i64 nrhs,
                   i64 ldb, i64 bsize) {
-    const i64 ld = std::max<i64>(n, 1);
-    std::vector<double> C((size_t)(ld * n));
-    for (i64 j = 0; j < n; ++j)
-        for (i64 i = 0; i < n; ++i) C[IDX(i, j, ld)] = A[IDX(i, j, lda)];
-    const int info = orc_cholesky(uplo, C.data(), n, ld, bsize);
-    if (info) return info;
-    return orc_cholesky_solve_after(uplo, orient, C.data(), n, ld, B, nrhs, ldb);
+    (void)orient;
+    return hpd_solve(uplo, A, n, lda, B, nrhs, ldb, bsize);
+}
+int orc_shpd_solve(int uplo, int orient, const float* A, i64 n, i64 lda, float* B, i64 nrhs,
+                   i64 ldb, i64 bsize) {
+    (void)orient;
+    return hpd_solve(uplo, A, n, lda, B, nrhs, ldb, bsize);
 }
 
 // Permutation::ExplicitVector (Permutation.cpp:731-745): p with (PA)(i,:) = A(p(i),:).
@@ -885,6 +1060,18 @@ void orc_dtrsm(char side, char uplo, char trans, char diag, i64 m, i64 n, double
 void orc_dsyrk(char uplo, char trans, i64 n, i64 k, double alpha, const double* A, i64 lda,
                double beta, double* C, i64 ldc) {
     g_blas->dsyrk(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+void orc_sgemm(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda,
+               const float* B, i64 ldb, float beta, float* C, i64 ldc) {
+    Ops<float>::gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+void orc_strsm(char side, char uplo, char trans, char diag, i64 m, i64 n, float alpha,
+               const float* A, i64 lda, float* B, i64 ldb) {
+    Ops<float>::trsm(side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);
+}
+void orc_ssyrk(char uplo, char trans, i64 n, i64 k, float alpha, const float* A, i64 lda,
+               float beta, float* C, i64 ldc) {
+    Ops<float>::syrk(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
 }
 
 // Host LAPACK dgetrf / dpotrf of the loaded backend (library cross-check and secondary CPU

@@ -3,7 +3,9 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
 cpu_baseline leg of bench.py, as the checker or as the timed CPU baseline. Every function
 restates the reference (file:line citations are in el_oracle.cpp). Arrays are numpy,
-column-major (Fortran order), float64; pivots are int32 swap destinations.
+column-major (Fortran order), float64 or float32 (the factorizations, solves and level-3
+routines run in the dtype of their input, through the orc_* or orc_s* entry points); pivots
+are int32 swap destinations.
 """
 import ctypes
 import os
@@ -17,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 _i64 = ctypes.c_int64
 _dbl = ctypes.c_double
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
 _lib = None
@@ -47,6 +50,29 @@ def lib():
 def _p(a):
     assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "need Fortran float64"
     return a.ctypes.data_as(_dp)
+
+
+def _pt(a):
+    """Pointer to a Fortran-ordered float64 or float32 array."""
+    assert a.flags.f_contiguous or a.ndim == 1, "need Fortran order"
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    assert a.dtype == np.float32, a.dtype
+    return a.ctypes.data_as(_fp)
+
+
+def _dt(A):
+    """float32 for float32 input, float64 otherwise."""
+    return np.float32 if np.asarray(A).dtype == np.float32 else np.float64
+
+
+def _fn(name, dt):
+    """orc_<name> for float64, orc_s<name> for float32."""
+    return getattr(lib(), "orc_" + ("s" if dt == np.float32 else "") + name)
+
+
+def _sc(x, dt):
+    return ctypes.c_float(x) if dt == np.float32 else _dbl(x)
 
 
 def _ip_(a):
@@ -80,12 +106,6 @@ def gaussian(m, n, mean=0.0, stddev=1.0):
     return A
 
 
-def uniform(m, n, center=0.0, radius=1.0):
-    A = np.zeros((m, n), order="F")
-    lib().orc_uniform(_p(A), _i64(m), _i64(n), _i64(_ld(A)), _dbl(center), _dbl(radius))
-    return A
-
-
 def gepp_growth(n):
     A = np.zeros((n, n), order="F")
     lib().orc_geppgrowth(_p(A), _i64(n), _i64(_ld(A)))
@@ -93,71 +113,95 @@ def gepp_growth(n):
 
 
 def lu(A, bsize=128):
-    """El::LU(A, P) on a copy; returns (factors, dests, info)."""
-    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    """El::LU(A, P) on a copy, in A's dtype; returns (factors, dests, info)."""
+    dt = _dt(A)
+    F = np.array(A, dtype=dt, order="F", copy=True)
     m, n = F.shape
     d = np.zeros(max(min(m, n), 1), np.int32)
-    info = lib().orc_lu(_p(F), _i64(m), _i64(n), _i64(_ld(F)), _i64(bsize), _ip_(d))
+    info = _fn("lu", dt)(_pt(F), _i64(m), _i64(n), _i64(_ld(F)), _i64(bsize), _ip_(d))
     return F, d[: min(m, n)], info
 
 
-def lu_trace_column(A, col, bsize=128):
-    F = np.array(A, dtype=np.float64, order="F", copy=True)
+def lu_nopiv(A, bsize=128):
+    """El::LU(A) without pivoting on a copy, in A's dtype; returns (factors, info)."""
+    dt = _dt(A)
+    F = np.array(A, dtype=dt, order="F", copy=True)
     m, n = F.shape
-    out = np.zeros(m - col)
-    lib().orc_lu_trace_column(_p(F), _i64(m), _i64(n), _i64(_ld(F)), _i64(bsize), _i64(col), _p(out))
+    info = _fn("lu_nopiv", dt)(_pt(F), _i64(m), _i64(n), _i64(_ld(F)), _i64(bsize))
+    return F, info
+
+
+def lu_trace_column(A, col, bsize=128):
+    dt = _dt(A)
+    F = np.array(A, dtype=dt, order="F", copy=True)
+    m, n = F.shape
+    out = np.zeros(m - col, dtype=dt)
+    _fn("lu_trace_column", dt)(_pt(F), _i64(m), _i64(n), _i64(_ld(F)), _i64(bsize), _i64(col),
+                               _pt(out))
     return out
 
 
 def lu_solve_after(F, dests, B, orient=0):
-    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    """lu::SolveAfter(orient, F, P, B); dests None gives the unpivoted form."""
+    dt = _dt(F)
+    F = np.asfortranarray(F, dtype=dt)
+    X = np.array(B, dtype=dt, order="F", copy=True)
     if X.ndim == 1:
         X = X.reshape(-1, 1, order="F")
     n = F.shape[0]
-    d = np.ascontiguousarray(dests, np.int32)
-    lib().orc_lu_solve_after(orient, _p(F), _i64(n), _i64(_ld(F)), _ip_(d), _i64(d.size), _p(X),
-                             _i64(X.shape[1]), _i64(_ld(X)))
+    if dests is None:
+        dp, nd = ctypes.cast(None, _ip), 0
+    else:
+        d = np.ascontiguousarray(dests, np.int32)
+        dp, nd = _ip_(d), d.size
+    _fn("lu_solve_after", dt)(orient, _pt(F), _i64(n), _i64(_ld(F)), dp, _i64(nd), _pt(X),
+                              _i64(X.shape[1]), _i64(_ld(X)))
     return X
 
 
 def linear_solve(A, B, bsize=128):
-    """El::LinearSolve(A, B); returns (X, dests, info)."""
-    A = np.asfortranarray(A, dtype=np.float64)
-    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    """El::LinearSolve(A, B) in A's dtype; returns (X, dests, info)."""
+    dt = _dt(A)
+    A = np.asfortranarray(A, dtype=dt)
+    X = np.array(B, dtype=dt, order="F", copy=True)
     if X.ndim == 1:
         X = X.reshape(-1, 1, order="F")
     n = A.shape[0]
     d = np.zeros(max(n, 1), np.int32)
-    info = lib().orc_linear_solve(_p(A), _i64(n), _i64(_ld(A)), _p(X), _i64(X.shape[1]),
-                                  _i64(_ld(X)), _i64(bsize), _ip_(d))
+    info = _fn("linear_solve", dt)(_pt(A), _i64(n), _i64(_ld(A)), _pt(X), _i64(X.shape[1]),
+                                   _i64(_ld(X)), _i64(bsize), _ip_(d))
     return X, d[:n], info
 
 
 def cholesky(A, uplo=0, bsize=128):
-    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    dt = _dt(A)
+    F = np.array(A, dtype=dt, order="F", copy=True)
     n = F.shape[0]
-    info = lib().orc_cholesky(uplo, _p(F), _i64(n), _i64(_ld(F)), _i64(bsize))
+    info = _fn("cholesky", dt)(uplo, _pt(F), _i64(n), _i64(_ld(F)), _i64(bsize))
     return F, info
 
 
 def cholesky_solve_after(F, B, uplo=0, orient=0):
-    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    dt = _dt(F)
+    F = np.asfortranarray(F, dtype=dt)
+    X = np.array(B, dtype=dt, order="F", copy=True)
     if X.ndim == 1:
         X = X.reshape(-1, 1, order="F")
     n = F.shape[0]
-    lib().orc_cholesky_solve_after(uplo, orient, _p(F), _i64(n), _i64(_ld(F)), _p(X),
-                                   _i64(X.shape[1]), _i64(_ld(X)))
+    _fn("cholesky_solve_after", dt)(uplo, orient, _pt(F), _i64(n), _i64(_ld(F)), _pt(X),
+                                    _i64(X.shape[1]), _i64(_ld(X)))
     return X
 
 
 def hpd_solve(A, B, uplo=0, orient=0, bsize=128):
-    A = np.asfortranarray(A, dtype=np.float64)
-    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    dt = _dt(A)
+    A = np.asfortranarray(A, dtype=dt)
+    X = np.array(B, dtype=dt, order="F", copy=True)
     if X.ndim == 1:
         X = X.reshape(-1, 1, order="F")
     n = A.shape[0]
-    info = lib().orc_hpd_solve(uplo, orient, _p(A), _i64(n), _i64(_ld(A)), _p(X),
-                               _i64(X.shape[1]), _i64(_ld(X)), _i64(bsize))
+    info = _fn("hpd_solve", dt)(uplo, orient, _pt(A), _i64(n), _i64(_ld(A)), _pt(X),
+                                _i64(X.shape[1]), _i64(_ld(X)), _i64(bsize))
     return X, info
 
 
@@ -230,33 +274,40 @@ def blas(name, *args):
 
 
 def gemm(ta, tb, alpha, A, B, beta, C):
-    C = np.array(C, dtype=np.float64, order="F", copy=True)
-    A = np.asfortranarray(A, dtype=np.float64)
-    B = np.asfortranarray(B, dtype=np.float64)
+    dt = _dt(C)
+    C = np.array(C, dtype=dt, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=dt)
+    B = np.asfortranarray(B, dtype=dt)
     m, n = C.shape
     k = A.shape[1] if ta == "N" else A.shape[0]
-    lib().orc_dgemm(ctypes.c_char(ta.encode()), ctypes.c_char(tb.encode()), _i64(m), _i64(n), _i64(k),
-                    _dbl(alpha), _p(A), _i64(_ld(A)), _p(B), _i64(_ld(B)), _dbl(beta), _p(C), _i64(_ld(C)))
+    f = lib().orc_sgemm if dt == np.float32 else lib().orc_dgemm
+    f(ctypes.c_char(ta.encode()), ctypes.c_char(tb.encode()), _i64(m), _i64(n), _i64(k),
+      _sc(alpha, dt), _pt(A), _i64(_ld(A)), _pt(B), _i64(_ld(B)), _sc(beta, dt), _pt(C),
+      _i64(_ld(C)))
     return C
 
 
 def trsm(side, uplo, trans, diag, alpha, A, B):
-    X = np.array(B, dtype=np.float64, order="F", copy=True)
-    A = np.asfortranarray(A, dtype=np.float64)
+    dt = _dt(B)
+    X = np.array(B, dtype=dt, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=dt)
     m, n = X.shape
-    lib().orc_dtrsm(ctypes.c_char(side.encode()), ctypes.c_char(uplo.encode()),
-                    ctypes.c_char(trans.encode()), ctypes.c_char(diag.encode()), _i64(m), _i64(n),
-                    _dbl(alpha), _p(A), _i64(_ld(A)), _p(X), _i64(_ld(X)))
+    f = lib().orc_strsm if dt == np.float32 else lib().orc_dtrsm
+    f(ctypes.c_char(side.encode()), ctypes.c_char(uplo.encode()), ctypes.c_char(trans.encode()),
+      ctypes.c_char(diag.encode()), _i64(m), _i64(n), _sc(alpha, dt), _pt(A), _i64(_ld(A)),
+      _pt(X), _i64(_ld(X)))
     return X
 
 
 def syrk(uplo, trans, alpha, A, beta, C):
-    C = np.array(C, dtype=np.float64, order="F", copy=True)
-    A = np.asfortranarray(A, dtype=np.float64)
+    dt = _dt(C)
+    C = np.array(C, dtype=dt, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=dt)
     n = C.shape[0]
     k = A.shape[1] if trans == "N" else A.shape[0]
-    lib().orc_dsyrk(ctypes.c_char(uplo.encode()), ctypes.c_char(trans.encode()), _i64(n), _i64(k),
-                    _dbl(alpha), _p(A), _i64(_ld(A)), _dbl(beta), _p(C), _i64(_ld(C)))
+    f = lib().orc_ssyrk if dt == np.float32 else lib().orc_dsyrk
+    f(ctypes.c_char(uplo.encode()), ctypes.c_char(trans.encode()), _i64(n), _i64(k),
+      _sc(alpha, dt), _pt(A), _i64(_ld(A)), _sc(beta, dt), _pt(C), _i64(_ld(C)))
     return C
 
 
@@ -271,12 +322,15 @@ def host_dgetrf(A):
 # ---------------------------------------------------------------------------------------
 # Near-tie classification of pivot disagreements
 # ---------------------------------------------------------------------------------------
-def pivot_agreement(A, dests_ref, dests_test, bsize=128, rel_tie=1e-8):
+def pivot_agreement(A, dests_ref, dests_test, bsize=128, rel_tie=None):
     """Compare two pivot sequences of LU(A). Returns dict(mismatches, near_ties, first,
     gap): at the first disagreement the oracle's candidate column is recomputed and the
     relative magnitude gap between the two chosen rows reported. A disagreement is a
-    near-tie when that gap is below rel_tie. Later steps run on different Schur complements,
-    so only the first disagreement is classified; later ones are counted."""
+    near-tie when that gap is below rel_tie (1e-8 for float64, 1e-4 for float32 input). Later
+    steps run on different Schur complements, so only the first disagreement is classified;
+    later ones are counted."""
+    if rel_tie is None:
+        rel_tie = 1e-4 if _dt(A) == np.float32 else 1e-8
     dr = np.asarray(dests_ref)
     dt = np.asarray(dests_test)
     mism = np.nonzero(dr != dt)[0]

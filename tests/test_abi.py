@@ -27,7 +27,15 @@ def test_headers_declare_the_path():
     for required in ["elx_dgetrf", "elx_dgetrs", "elx_dgesv", "elx_dpotrf", "elx_dpotrs",
                      "elx_dposv", "elx_dgemm", "elx_dtrsm", "elx_dsyrk", "elx_dgesv_mixed",
                      "ElLUPartialPiv_d", "ElSolveAfterLUPartialPiv_d", "ElCholesky_d",
-                     "ElLinearSolve_d", "ElHPDSolve_d"]:
+                     "ElLinearSolve_d", "ElHPDSolve_d",
+                     # the float instantiations and the unpivoted LU
+                     "elx_sgetrf", "elx_sgetrs", "elx_sgesv", "elx_spotrf", "elx_spotrs",
+                     "elx_sposv", "elx_sgemm", "elx_strsm", "elx_ssyrk", "elx_spermute_rows",
+                     "elx_dgetrf_nopiv", "elx_sgetrf_nopiv",
+                     "ElLU_d", "ElLU_s", "ElSolveAfterLU_d", "ElSolveAfterLU_s",
+                     "ElLUPartialPiv_s", "ElSolveAfterLUPartialPiv_s", "ElCholesky_s",
+                     "ElSolveAfterCholesky_s", "ElLinearSolve_s", "ElHPDSolve_s", "ElGemm_s",
+                     "ElTrsm_s", "ElHerk_s", "ElSyrk_s", "ElMatrixCreate_s"]:
         assert required in names
 
 
@@ -51,7 +59,12 @@ def test_cpp_api_is_exported():
                 "El::Cholesky<double>(El::UpperOrLower, El::Matrix<double>&)",
                 "El::LinearSolve<double>(El::Matrix<double> const&, El::Matrix<double>&)",
                 "El::lu::SolveAfter<double>(", "El::cholesky::SolveAfter<double>(",
-                "El::HPDSolve<double>("]:
+                "El::HPDSolve<double>(",
+                "El::LU<float>(El::Matrix<float>&, El::Permutation&)",
+                "El::LU<float>(El::Matrix<float>&)", "El::LU<double>(El::Matrix<double>&)",
+                "El::lu::SolveAfter<float>(El::Orientation, El::Matrix<float> const&, El::Matrix<float>&)",
+                "El::Cholesky<float>(", "El::LinearSolve<float>(", "El::HPDSolve<float>(",
+                "El::Gemm<float>(", "El::Trsm<float>(", "El::Herk<float>(", "El::Syrk<float>("]:
         assert sym in out, sym
 
 
@@ -62,8 +75,10 @@ def test_argument_errors_without_device_work():
     # lda < m
     st = lib.elx_dgetrf(None, 10, 10, None, 5, 128, None, ctypes.byref(info))
     assert st == 1
-    # block size out of range
+    # block size out of range (any positive block size is accepted)
     st = lib.elx_dgetrf(None, 10, 10, None, 10, 0, None, ctypes.byref(info))
+    assert st == 1
+    st = lib.elx_sgetrf(None, 10, 10, None, 5, 128, None, ctypes.byref(info))
     assert st == 1
     st = lib.elx_dpotrf(None, 7, 10, None, 10, 128, ctypes.byref(info))
     assert st == 1

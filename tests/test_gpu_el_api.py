@@ -125,3 +125,85 @@ def test_cpp_api_driver():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+class MatS:
+    """An ElMatrix_s attached to a Fortran-ordered float32 numpy array."""
+
+    def __init__(self, lib, a):
+        self.lib = lib
+        self.a = np.array(a, dtype=np.float32, order="F", copy=True)
+        self.h = vp()
+        assert lib.ElMatrixCreate_s(ctypes.byref(self.h)) == 0
+        m, n = self.a.shape
+        assert lib.ElMatrixAttach_s(self.h, m, n,
+                                    self.a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                    max(m, 1)) == 0
+
+    def __del__(self):
+        self.lib.ElMatrixDestroy_s(self.h)
+
+
+def test_c_api_float_entry_points(lib, orc):
+    """The _s entry points (C_PROTO_FIELD(s, float) in factor-C.cpp / solve-C.cpp)."""
+    lib.ElMatrixCreate_s.argtypes = [ctypes.POINTER(vp)]
+    lib.ElMatrixDestroy_s.argtypes = [vp]
+    lib.ElMatrixAttach_s.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    for name in ("ElLUPartialPiv_s", "ElLinearSolve_s"):
+        getattr(lib, name).argtypes = [vp, vp]
+    lib.ElSolveAfterLUPartialPiv_s.argtypes = [ctypes.c_int, vp, vp, vp]
+    lib.ElCholesky_s.argtypes = [ctypes.c_int, vp]
+    lib.ElHPDSolve_s.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
+    n = 300
+    orc.rng_seed(50)
+    A = orc.gaussian(n, n).astype(np.float32)
+    B = orc.gaussian(n, 2).astype(np.float32)
+    F_ref, d_ref, _ = orc.lu(A)
+    M = MatS(lib, A)
+    P = vp()
+    assert lib.ElPermutationCreate(ctypes.byref(P)) == 0
+    assert lib.ElLUPartialPiv_s(M.h, P) == 0
+    d = perm_dests(lib, P)
+    agree = orc.pivot_agreement(A, d_ref, d)
+    assert agree["mismatches"] == 0 or agree["near_ties"] == 1, agree
+    assert np.abs(M.a - F_ref).max() <= 1e3 * n * 2.0 ** -24 * np.abs(F_ref).max()
+    X = MatS(lib, B)
+    assert lib.ElSolveAfterLUPartialPiv_s(0, M.h, P, X.h) == 0
+    X_ref = orc.lu_solve_after(F_ref, d_ref, B)
+    assert np.abs(X.a - X_ref).max() <= 1e-2 * np.abs(X_ref).max()
+    Y = MatS(lib, B)
+    assert lib.ElLinearSolve_s(MatS(lib, A).h, Y.h) == 0
+    assert np.abs(Y.a - X_ref).max() <= 1e-2 * np.abs(X_ref).max()
+    G = orc.gaussian(n, n)
+    H = (G @ G.T / n + np.eye(n)).astype(np.float32)
+    C = MatS(lib, H)
+    assert lib.ElCholesky_s(0, C.h) == 0
+    Z = MatS(lib, B)
+    assert lib.ElHPDSolve_s(0, 0, MatS(lib, H).h, Z.h) == 0
+    Z_ref, _ = orc.hpd_solve(H, B)
+    assert np.abs(Z.a - Z_ref).max() <= 1e-3 * np.abs(Z_ref).max()
+    lib.ElPermutationDestroy(P)
+
+
+def test_c_api_lu_without_pivoting(lib, orc):
+    """ElLU_d / ElSolveAfterLU_d (factor-C.cpp:271,289): LU.cpp:21-44 and
+    LU/SolveAfter.hpp:15-38."""
+    lib.ElLU_d.argtypes = [vp]
+    lib.ElSolveAfterLU_d.argtypes = [ctypes.c_int, vp, vp]
+    rng = np.random.default_rng(51)
+    n = 260
+    A = rng.standard_normal((n, n))
+    A += np.diag(np.abs(A).sum(0) + 1.0)
+    A = np.asfortranarray(A)
+    B = np.asfortranarray(rng.standard_normal((n, 3)))
+    F_ref, info = orc.lu_nopiv(A)
+    M = Mat(lib, A)
+    assert lib.ElLU_d(M.h) == 0
+    assert np.abs(M.a - F_ref).max() <= 1e3 * n * EPS * np.abs(F_ref).max()
+    X = Mat(lib, B)
+    assert lib.ElSolveAfterLU_d(0, M.h, X.h) == 0
+    assert orc.backward_error(A, X.a, B) <= n * EPS
+    Z = np.zeros((4, 4))
+    Zm = Mat(lib, Z)
+    assert lib.ElLU_d(Zm.h) == 5  # SingularMatrixException -> EL_RUNTIME_ERROR

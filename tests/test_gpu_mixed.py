@@ -374,3 +374,30 @@ def test_mixed_no_right_hand_sides(dev):
     dX = dev.colmajor_empty(n, 0)
     st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="auto")
     assert st["converged"] == 1 and st["status"] == 0
+
+
+def test_ir_keeps_only_decreasing_candidates(dev, orc):
+    """Refined.hpp:96-105: a refinement candidate x + dx replaces x only when its residual is
+    smaller. With fp16 factors of a kappa = 1e10 system the preconditioned operator is far
+    from the identity: the first IR correction dx = M^-1 (b - A x0) (computed here by the same
+    factorization applied to the residual) makes the residual larger, so plain IR must return
+    x0 = M^-1 b itself, bit for bit the unrefined solve."""
+    n = 400
+    A = randsvd(n, 1e10, 21)
+    b = np.random.default_rng(22).standard_normal((n, 1))
+    dA, dB = dev.to_device(A), dev.to_device(b)
+    dX0 = dev.colmajor_empty(n, 1)
+    dev.gesv_mixed(dA, dB, dX0, factor="fp16", refine="none")
+    x0 = dev.to_host(dX0)
+    r0 = b - A @ x0
+    dD = dev.colmajor_empty(n, 1)
+    dev.gesv_mixed(dA, dev.to_device(r0), dD, factor="fp16", refine="none")
+    x1 = x0 + dev.to_host(dD)
+    grow = np.abs(b - A @ x1).max() / np.abs(r0).max()
+    say("keep-if-decreasing: residual ratio of the first candidate", grow)
+    assert grow > 2.0, "premise: the first correction must increase the residual"
+    dX = dev.colmajor_empty(n, 1)
+    st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="ir")
+    say(st)
+    assert np.array_equal(dev.to_host(dX), x0)
+    assert st["converged"] == 0 and st["status"] == 5

@@ -99,9 +99,13 @@ def check_lu(dev, orc, A, nb=128, expect_exact=False):
     if agree["mismatches"]:
         say("pivot disagreement", agree)
         assert agree["near_ties"] == 1, f"pivot disagreement that is not a near-tie: {agree}"
+        # the GPU's own factors must still be a backward-stable LU with its own pivots
+        if m == n and n > 0:
+            err = orc.lu_factor_error(A, F, d)
+            assert err <= 8 * n * EPS, err
         return agree
     if expect_exact:
-        assert np.array_equal(F, F_ref)
+        assert np.array_equal(F, F_ref), "a single panel must match lu::Panel bit for bit"
     if m == n and n > 0:
         err = orc.lu_factor_error(A, F, d)
         assert err <= 8 * n * EPS, err
@@ -117,6 +121,18 @@ def test_lu_shapes(dev, orc):
         say("lu", m, n, nb)
         A = orc.gaussian(m, n)
         check_lu(dev, orc, A, nb)
+
+
+def test_lu_single_panel_bitwise(dev, orc):
+    """n <= nb (and tall single panels: one cooperative panel over many workgroups) runs only
+    the panel kernel, whose per-entry FMA order is lu::Panel's: the factors are asserted equal
+    to the oracle's bit for bit."""
+    orc.rng_seed(17)
+    for (m, n, nb) in [(1, 1, 128), (7, 7, 128), (100, 100, 128), (128, 128, 128),
+                       (300, 64, 64), (4000, 128, 128), (20000, 96, 96), (32768, 128, 128)]:
+        say("single panel bitwise", m, n, nb)
+        A = orc.gaussian(m, n)
+        check_lu(dev, orc, A, nb, expect_exact=True)
 
 
 def test_lu_config1_n1024(dev, orc):

@@ -24,12 +24,6 @@ def test_gaussian_stream_matches_independent_restatement(orc, golden):
     assert np.array_equal(C.ravel(), ref[:2]) and not np.array_equal(B, C)
 
 
-def test_uniform_in_ball(orc):
-    orc.rng_seed(7)
-    U = orc.uniform(500, 3)
-    assert U.min() >= -1.0 and U.max() <= 1.0 and abs(U.mean()) < 0.1
-
-
 def test_lu_pivots_and_factors_match_exact_rational(orc, golden):
     for idx in range(int(golden["lu_count"])):
         A = golden[f"lu{idx}_A"]
@@ -293,3 +287,126 @@ def test_host_dgetrf_matches_scipy(orc):
         orc.set_backend("naive")
     lu_, piv = sl.lu_factor(A)
     assert info == 0 and np.array_equal(ip, piv) and np.allclose(F, lu_, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------
+# float instantiations (the orc_s* entry points) and the unpivoted LU
+# ------------------------------------------------------------------------------------------
+EPS32 = 2.0 ** -24
+
+
+def test_float_gepp_growth_closed_form(orc):
+    # the closed form holds in float too (all values are small integers and powers of two)
+    for n in (1, 2, 7, 40):
+        A = orc.gepp_growth(n).astype(np.float32)
+        F, d, info = orc.lu(A, bsize=8)
+        assert F.dtype == np.float32 and info == 0
+        assert np.array_equal(d, np.arange(n, dtype=np.int32))
+        assert np.array_equal(np.tril(F, -1), np.tril(-np.ones((n, n), np.float32), -1))
+        assert np.array_equal(F[:, n - 1], (2.0 ** np.arange(n)).astype(np.float32))
+
+
+def test_float_cholesky_exact(orc, golden):
+    # integer G with small entries: G G^T and every step of the factorization are exact in float
+    for idx in range(int(golden["chol_count"])):
+        G = golden[f"chol{idx}_G"]
+        A = golden[f"chol{idx}_A"]
+        if np.abs(A).max() >= 2 ** 24:
+            continue
+        for bs in (1, 3, 128):
+            F, info = orc.cholesky(A.astype(np.float32), uplo=0, bsize=bs)
+            assert info == 0 and np.array_equal(np.tril(F), G.astype(np.float32)), (idx, bs)
+            F, info = orc.cholesky(A.astype(np.float32), uplo=1, bsize=bs)
+            assert info == 0 and np.array_equal(np.triu(F), G.T.astype(np.float32)), (idx, bs)
+
+
+def test_float_lu_matches_exact_rational(orc, golden):
+    """The golden matrices are small integers (exact in float): the float oracle must find the
+    exact pivots and factors within a float rounding budget."""
+    for idx in range(int(golden["lu_count"])):
+        A = golden[f"lu{idx}_A"]
+        assert np.array_equal(A.astype(np.float32).astype(np.float64), A)
+        F, d, info = orc.lu(A.astype(np.float32), bsize=3)
+        assert info == 0
+        assert np.array_equal(d, golden[f"lu{idx}_dests"]), idx
+        Fe = golden[f"lu{idx}_F"]
+        n = A.shape[0]
+        assert np.allclose(F, Fe, rtol=0, atol=64 * n * EPS32 * max(1.0, np.abs(Fe).max())), idx
+
+
+def test_float_solves_against_float64(orc):
+    orc.rng_seed(21)
+    A = orc.gaussian(90, 90)
+    B = orc.gaussian(90, 3)
+    H = np.asfortranarray(A @ A.T / 90 + np.eye(90))
+    X64, _, _ = orc.linear_solve(A, B, bsize=16)
+    X32, d32, info = orc.linear_solve(A.astype(np.float32), B.astype(np.float32), bsize=16)
+    assert info == 0 and X32.dtype == np.float32
+    cond = np.linalg.cond(A)
+    assert np.abs(X32 - X64).max() <= 90 * cond * EPS32 * np.abs(X64).max()
+    Y64, _ = orc.hpd_solve(H, B, bsize=16)
+    Y32, info = orc.hpd_solve(H.astype(np.float32), B.astype(np.float32), bsize=16)
+    assert info == 0 and np.abs(Y32 - Y64).max() <= 90 * np.linalg.cond(H) * EPS32 * np.abs(Y64).max()
+    for side in "LR":
+        T = np.asfortranarray(np.triu(A) + 10 * np.eye(90)).astype(np.float32)
+        C = np.asfortranarray(B if side == "L" else B.T).astype(np.float32)
+        Xs = orc.trsm(side, "U", "N", "N", 1.0, T, C)
+        R = (np.triu(T).astype(np.float64) @ Xs) if side == "L" else (Xs @ np.triu(T).astype(np.float64))
+        assert np.allclose(R, C, atol=1e-4)
+
+
+def _exact_lu_pair(n, seed):
+    """L unit lower and U upper with small integers and power-of-two diagonal: A = L U is exact,
+    and so is every multiplier and update of the unpivoted elimination."""
+    rng = np.random.default_rng(seed)
+    L = np.tril(rng.integers(-3, 4, (n, n)).astype(np.float64), -1) + np.eye(n)
+    U = np.triu(rng.integers(-3, 4, (n, n)).astype(np.float64), 1) + np.diag(2.0 ** rng.integers(0, 3, n))
+    return L, U, np.asfortranarray(L @ U)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lu_nopiv_exact_known_factors(orc, dtype):
+    for n in (1, 5, 12):
+        L, U, A = _exact_lu_pair(n, n)
+        if np.abs(A).max() >= 2 ** 20:
+            continue
+        for bs in (1, 3, 128):
+            F, info = orc.lu_nopiv(A.astype(dtype), bsize=bs)
+            assert info == 0 and F.dtype == dtype
+            assert np.array_equal(np.tril(F, -1), np.tril(L, -1).astype(dtype)), (n, bs)
+            assert np.array_equal(np.triu(F), U.astype(dtype)), (n, bs)
+
+
+def test_lu_nopiv_equals_pivoted_on_column_dominant(orc):
+    """A strictly column diagonally dominant matrix stays so under elimination, so partial
+    pivoting picks the diagonal every step; with the same restated BLAS the unpivoted form
+    (LU.cpp:21-44) then gives the pivoted one's factors bit for bit."""
+    rng = np.random.default_rng(13)
+    for n, bs in [(40, 8), (130, 128), (75, 16)]:
+        A = rng.standard_normal((n, n))
+        A += np.diag(np.abs(A).sum(0) + 1.0)
+        A = np.asfortranarray(A)
+        F, d, info = orc.lu(A, bsize=bs)
+        assert info == 0 and np.array_equal(d, np.arange(n, dtype=np.int32))
+        G, info2 = orc.lu_nopiv(A, bsize=bs)
+        assert info2 == 0 and np.array_equal(F, G)
+        B = np.asfortranarray(rng.standard_normal((n, 2)))
+        assert np.array_equal(orc.lu_solve_after(G, None, B), orc.lu_solve_after(F, d, B))
+
+
+def test_lu_nopiv_singular_reports_column(orc):
+    L, U, A = _exact_lu_pair(9, 3)
+    U[4, 4] = 0.0  # exact zero pivot at column 4 once columns 0..3 are eliminated
+    A = np.asfortranarray(L @ U)
+    for bs in (1, 3, 128):
+        F, info = orc.lu_nopiv(A, bsize=bs)
+        assert info == 5, bs
+        # lu::Unb throws inside the diagonal block, before the block's Trsm of A21: the
+        # columns of earlier blocks are finished (exact multipliers), and inside the failing
+        # block only its own rows carry the multipliers of columns before the zero pivot
+        k0 = (4 // bs) * bs
+        assert np.array_equal(np.tril(F, -1)[:, :k0], np.tril(L, -1)[:, :k0]), bs
+        r1 = min(k0 + bs, 9)
+        assert np.array_equal(np.tril(F, -1)[k0:r1, k0:4], np.tril(L, -1)[k0:r1, k0:4]), bs
+        if r1 < 9 and k0 < 4:
+            assert not np.array_equal(F[r1:, k0:4], L[r1:, k0:4]), bs  # A21 not yet solved
