@@ -29,6 +29,33 @@ CLASSES = ["lu_panel", "dgemm_f64", "lpgemm", "trsm_small", "row_moves", "trsv",
            "pack"]
 
 
+def make_factors(n, seed):
+    """U, V Haar-orthogonal (QR of seeded Gaussians with sign-fixed R) for randsvd systems."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+
+    def haar():
+        G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+        Q, R = torch.linalg.qr(G)
+        del G
+        return Q * torch.sign(torch.diagonal(R))[None, :]
+
+    return haar(), haar(), g
+
+
+def randsvd_from(U, V, kappa):
+    """A = U diag(sigma) V^T with sigma geometric from 1 to 1/kappa, column-major FP64."""
+    import torch
+    from elemental_amd import device as dev
+    n = U.shape[0]
+    sig = kappa ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+    A = dev.colmajor_empty(n, n)
+    A.copy_((U * sig[None, :]) @ V.T.contiguous())
+    torch.cuda.synchronize()
+    return A
+
+
 def make_problem(n, kappa, seed, matrix="randsvd"):
     import torch
     from elemental_amd import device as dev
@@ -161,22 +188,19 @@ def traffic_from_profile(cls):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(n_metric, n_sample):
-    """The oracle (restated El::LU + lu::SolveAfter) over the host BLAS on all allowed cores,
-    one solve at n_sample, scaled to solves/s at n_metric by the n^3 flop ratio."""
-    import numpy as np
-    from oracle import oracle as orc
-    # Pick the fastest host BLAS available on this machine (a short dgemm calibration).
+def pick_host_blas(orc, cores):
+    """The fastest host BLAS on this machine (a short dgemm calibration among MKL, OpenBLAS
+    and scipy's bundled OpenBLAS); returns (backend path, dgemm GFLOP/s) or (None, None)."""
     import glob
+    import numpy as np
     cands = ["mkl", "openblas"] + sorted(
         glob.glob("/usr/local/lib/python3*/dist-packages/scipy.libs/libscipy_openblas*.so"))
-    cores = len(os.sched_getaffinity(0))
     best, best_t = None, None
     m = 2048
     Ac = np.asfortranarray(np.random.default_rng(1).standard_normal((m, m)))
     for cand in cands:
         try:
-            name = orc.set_backend(cand)
+            orc.set_backend(cand)
         except RuntimeError:
             continue
         orc.set_threads(cores)
@@ -187,10 +211,23 @@ def cpu_baseline(n_metric, n_sample):
         if best_t is None or t < best_t:
             best, best_t = cand, t
     if best is None:
+        return None, None
+    return orc.set_backend(best), 2 * m ** 3 / best_t / 1e9
+
+
+def cpu_baseline(n_metric, A_sample, b_sample, n_small, reps=3):
+    """The reference's CPU path on the GPU box's host cores: the oracle's restated El::LU +
+    lu::SolveAfter (nb = 128, the reference's call sequence) over the fastest host BLAS, on a
+    randsvd sample of the same family at a bounded size. Medians of `reps` runs: (a) all cores
+    at the sample size, (b) one thread at n_small (the sample's leading block), (c) the same
+    BLAS's LAPACK dgetrf + a solve on all cores at the sample size. Each is scaled by the n^3
+    ratio to solves/s at n_metric."""
+    import numpy as np
+    from oracle import oracle as orc
+    cores = len(os.sched_getaffinity(0))
+    backend, gemm_gflops = pick_host_blas(orc, cores)
+    if backend is None:
         return None
-    backend = orc.set_backend(best)
-    gemm_gflops = 2 * m ** 3 / best_t / 1e9
-    orc.set_threads(cores)
     cpu_model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -199,20 +236,178 @@ def cpu_baseline(n_metric, n_sample):
                 break
     except OSError:
         pass
-    rng = np.random.default_rng(11)
-    A = np.asfortranarray(rng.standard_normal((n_sample, n_sample)))
-    b = np.asfortranarray(rng.standard_normal((n_sample, 1)))
-    t0 = time.perf_counter()
-    F, d, info = orc.lu(A)
-    x = orc.lu_solve_after(F, d, b)
-    t = time.perf_counter() - t0
-    scale = (n_sample / n_metric) ** 3
-    return {"value": round(scale / t, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+    physical = None
+    try:
+        ids = set()
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                phys = line.split(":")[1].strip()
+            elif line.startswith("core id"):
+                core = line.split(":")[1].strip()
+                ids.add((phys, core))
+        physical = len(ids) or None
+    except OSError:
+        pass
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), [round(t, 3) for t in ts]
+
+    def el_lu_solve(A, b):
+        F, d, info = orc.lu(A)
+        return orc.lu_solve_after(F, d, b), info
+
+    ns = A_sample.shape[0]
+    orc.set_threads(cores)
+    t_all, runs_all = timed(lambda: el_lu_solve(A_sample, b_sample))
+    x, info = el_lu_solve(A_sample, b_sample)
+    eta_all = orc.backward_error(A_sample, x, b_sample)
+    As = np.asfortranarray(A_sample[:n_small, :n_small])
+    bs = np.asfortranarray(b_sample[:n_small])
+    orc.set_threads(1)
+    t_one, runs_one = timed(lambda: el_lu_solve(As, bs))
+    orc.set_threads(cores)
+
+    def host_lapack():
+        F, ip, inf = orc.host_dgetrf(A_sample)
+        if inf == -100:
+            return None
+        return orc.lu_solve_after(F, ip, b_sample)
+    lapack = None
+    if host_lapack() is not None:
+        t_lap, runs_lap = timed(host_lapack)
+        lapack = {"value": round((ns / n_metric) ** 3 / t_lap, 6), "unit": "solves/s",
+                  "seconds_at_sample_median": round(t_lap, 3), "runs": runs_lap,
+                  "routine": "host LAPACK dgetrf + the restated lu::SolveAfter"}
+    orc.set_backend("naive")
+    return {"value": round((ns / n_metric) ** 3 / t_all, 6), "unit": "solves/s",
+            "cores": cores, "kind": "port",
             "sample": f"oracle El::LU + lu::SolveAfter (nb=128) over {os.path.basename(backend)} "
-                      f"with {cores} threads, one N(0,1) system at n={n_sample} in {t:.2f} s, "
-                      f"scaled by (n_sample/n)^3 to n={n_metric}",
-            "seconds_at_sample": round(t, 3), "info": int(info), "cpu_model": cpu_model,
-            "host_dgemm_gflops": round(gemm_gflops, 1)}
+                      f"with {cores} threads on a randsvd system (same kappa) at n={ns}, median of "
+                      f"{reps} runs, scaled by (n_sample/n)^3 to n={n_metric}",
+            "seconds_at_sample_median": round(t_all, 3), "runs": runs_all,
+            "backward_error_at_sample": eta_all, "info": int(info),
+            "one_thread": {"value": round((n_small / n_metric) ** 3 / t_one, 8),
+                           "unit": "solves/s", "cores": 1, "n_sample": n_small,
+                           "seconds_at_sample_median": round(t_one, 3), "runs": runs_one},
+            "host_lapack_dgetrf": lapack, "blas": backend, "host_dgemm_gflops": round(gemm_gflops, 1),
+            "cpu_model": cpu_model, "logical_cpus": os.cpu_count(), "physical_cores": physical}
+
+
+def config1_check():
+    """Config 1 (n = 1024, A and b from Elemental's Gaussian stream, seed (21<<16)|0): the GPU
+    El::LU + lu::SolveAfter against the oracle's restatement -- pivot agreement with near-ties
+    counted, double-double backward errors of both, and the reference's own LU test criterion
+    (tests/lapack_like/LU.cpp:43-53)."""
+    import numpy as np
+    from elemental_amd import device as dev
+    from oracle import oracle as orc
+    orc.set_backend("naive")
+    n = 1024
+    orc.rng_seed()
+    A = orc.gaussian(n, n)
+    b = orc.gaussian(n, 1)
+    F_ref, d_ref, info_ref = orc.lu(A)
+    x_ref = orc.lu_solve_after(F_ref, d_ref, b)
+    dA = dev.to_device(A)
+    d, info = dev.getrf(dA)
+    dB = dev.to_device(b)
+    dev.getrs(dA, d, dB)
+    x = dev.to_host(dB)
+    d = d.cpu().numpy()
+    agree = orc.pivot_agreement(A, d_ref, d)
+    eta = orc.backward_error(A, x, b)
+    rel = np.abs(A @ x - b).max() / (2.0 ** -53 * n * max(np.abs(A).sum(0).max(), np.abs(x).sum(0).max()))
+    return {"n": n, "pivot_mismatches": agree["mismatches"], "near_ties": agree["near_ties"],
+            "first_mismatch": agree["first"], "backward_error_gpu": eta,
+            "backward_error_oracle": orc.backward_error(A, x_ref, b),
+            "tolerance": n * 2.0 ** -53, "el_lu_test_rel_error": rel,
+            "factor_error_gpu": orc.lu_factor_error(A, dev.to_host(dA), d),
+            "max_abs_diff_x_vs_oracle": float(np.abs(x - x_ref).max())}
+
+
+def measure_peaks():
+    """Re-measured peaks on this box: large library GEMMs (torch.matmul, FP64/FP32/FP16/BF16,
+    8192^3, best of 3) and a 4 GiB device-to-device copy (read + write bytes)."""
+    import torch
+    out = {}
+    for dt, key in ((torch.float64, "fp64_TFs"), (torch.float32, "fp32_TFs"),
+                    (torch.float16, "fp16_TFs"), (torch.bfloat16, "bf16_TFs")):
+        m = 8192
+        a = torch.randn(m, m, device="cuda", dtype=torch.float32).to(dt)
+        bm = torch.randn(m, m, device="cuda", dtype=torch.float32).to(dt)
+        c = a @ bm
+        best = None
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            c = a @ bm
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1) * 1e-3
+            best = t if best is None else min(best, t)
+        out[key] = {"datasheet": PEAKS[key], "measured": round(2 * m ** 3 / best / 1e12, 1)}
+        del a, bm, c
+    src = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = None
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3
+        best = t if best is None else min(best, t)
+    out["hbm_GBs"] = {"datasheet": PEAKS["hbm_GBs"], "measured": round(2 * src.numel() * 8 / best / 1e9, 1)}
+    del src, dst
+    torch.cuda.empty_cache()
+    out["method"] = "torch.matmul (rocBLAS/hipBLASLt) 8192^3 and a 4 GiB copy, best of 3"
+    return out
+
+
+def config3_rows(args, U, V, A, b):
+    """The rest of config 3 at n = 32768, one timed solve each (outside the headline's timed
+    region): bf16 factors at the headline kappa, 64 right-hand sides at the headline kappa
+    (block refinement), and fp16 factors at kappa 1e4 and 1e6 (AUTO escalation)."""
+    import torch
+    from elemental_amd import device as dev
+    n = A.shape[0]
+    rows = []
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 7)
+    B64 = dev.colmajor_empty(n, 64)
+    B64.copy_(torch.randn(n, 64, dtype=torch.float64, device="cuda", generator=g))
+
+    def run(Am, rhs, factor, kappa):
+        X = dev.colmajor_empty(n, rhs.shape[1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = dev.gesv_mixed(Am, rhs, X, factor=factor, refine="auto")
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        row = {"kappa": kappa, "factor": factor, "nrhs": rhs.shape[1], "ms": round(1e3 * t, 1),
+               "converged": st["converged"], "backward_error": backward_error(Am, X, rhs),
+               "tolerance": n * 2.0 ** -53, "fell_back": st["fell_back"],
+               "outer_iters": st["outer_iters"], "inner_iters": st["inner_iters"]}
+        print(f"[bench] config3 {row}", file=sys.stderr, flush=True)
+        return row
+
+    rows.append(run(A, b, "bf16", args.kappa))
+    rows.append(run(A, B64, args.factor, args.kappa))
+    del B64
+    for kappa in (1e4, 1e6):
+        Ak = randsvd_from(U, V, kappa)
+        rows.append(run(Ak, b, args.factor, kappa))
+        del Ak
+        torch.cuda.empty_cache()
+    return rows
 
 
 def hpd_solve(args):
@@ -412,7 +607,10 @@ def main():
     ap.add_argument("--refine", default="auto", choices=["ir", "gmres", "auto"])
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-n", type=int, default=12288)
+    ap.add_argument("--cpu-n-one-thread", type=int, default=4096)
+    ap.add_argument("--skip-config3", action="store_true")
+    ap.add_argument("--skip-peaks", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--no-profile-pass", action="store_true")
@@ -458,7 +656,18 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"counters_run": True, "solve": st, "copy_bytes": src.numel() * 8}), flush=True)
         return
-    A, b = make_problem(n, args.kappa, args.seed, args.matrix)
+    peaks = None if args.skip_peaks else measure_peaks()
+    if peaks:
+        print(f"[bench] peaks {peaks}", file=sys.stderr, flush=True)
+    U = V = None
+    if args.matrix == "randsvd":
+        U, V, g = make_factors(n, args.seed)
+        A = randsvd_from(U, V, args.kappa)
+        b = dev.colmajor_empty(n, 1)
+        b.copy_(torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=g))
+        torch.cuda.synchronize()
+    else:
+        A, b = make_problem(n, args.kappa, args.seed, args.matrix)
     X = dev.colmajor_empty(n, 1)
 
     def mixed_solve():
@@ -534,6 +743,12 @@ def main():
             v["share"] = round(v["ms"] / sum(x["ms"] for x in kernels.values()), 4)
         print(f"[bench] kernel classes {json.dumps(kernels)}", file=sys.stderr, flush=True)
 
+    config3 = None
+    if not args.skip_config3 and U is not None:
+        config3 = config3_rows(args, U, V, A, b)
+    del U, V
+    torch.cuda.empty_cache()
+
     hpd = None
     if not args.skip_hpd:
         hpd = hpd_solve(args)
@@ -547,10 +762,32 @@ def main():
         lu_batch = batched_lu_ir(args)
 
     cpu = None
+    config1 = None
     if not args.skip_cpu:
         del A
         torch.cuda.empty_cache()
-        cpu = cpu_baseline(n, args.cpu_n)
+        # a randsvd sample of the same family at a bounded size, built on the device
+        Us, Vs, gs = make_factors(args.cpu_n, args.seed + 9)
+        As = randsvd_from(Us, Vs, args.kappa)
+        bs = torch.randn(args.cpu_n, 1, dtype=torch.float64, device="cuda", generator=gs)
+        A_h, b_h = dev.to_host(As), dev.to_host(bs)
+        del Us, Vs, As, bs
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(n, A_h, b_h, args.cpu_n_one_thread)
+        del A_h, b_h
+        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
+        config1 = config1_check()
+        print(f"[bench] config1 {config1}", file=sys.stderr, flush=True)
+
+    credited = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2
+    whole = {"credited_flops": credited,
+             "mixed_tflops": round(credited * value / 1e12, 2),
+             "mixed_frac_of_fp16_peak": round(credited * value / 1e12 / PEAKS["fp16_TFs"], 4),
+             "note": "HPL-MxP convention: (2/3)n^3 + 2n^2 credited whatever the precision"}
+    if plain:
+        whole["plain_fp64_tflops"] = round(credited * plain["solves_per_s"] / 1e12, 2)
+        whole["plain_fp64_frac_of_fp64_peak"] = round(
+            credited * plain["solves_per_s"] / 1e12 / PEAKS["fp64_TFs"], 4)
 
     line = {
         "metric": "FP64-accurate Ax=b solves/s: FP16-LU-IR vs FP64 LU at n=32768, 1 MI355X",
@@ -580,6 +817,10 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "whole_solve": whole,
+        "peaks": peaks,
+        "config1_check": config1,
+        "config3": config3,
         "hpd_solve": hpd,
         "batched_cholesky": batched,
         "batched_lu_ir": lu_batch,

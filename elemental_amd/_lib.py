@@ -77,6 +77,10 @@ _SIGS = {
                   c_i64],
     "elx_ssyrk": [c_vp, c_int, c_int, c_i64, c_i64, c_flt, c_vp, c_i64, c_flt, c_vp, c_i64],
     "elx_spermute_rows": [c_vp, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64],
+    "elx_dgetrf_full": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp],
+    "elx_sgetrf_full": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp],
+    "elx_dgetrs_full": [c_vp, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64],
+    "elx_sgetrs_full": [c_vp, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64],
     "elx_druiz_equil": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
     "elx_dsymmetric_ruiz_equil": [c_vp, c_i64, c_vp, c_i64, c_vp, c_i32],
     "elx_dgeom_equil": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],

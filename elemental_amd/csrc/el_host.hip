@@ -235,6 +235,22 @@ struct Dev {
                                     int32_t* info) {
         return elx_sgetrf_nopiv(nullptr, m, n, A, ld, nb, info);
     }
+    static elx_status_t getrf_full(int64_t m, int64_t n, double* A, int64_t ld, int32_t* p,
+                                   int32_t* q, int32_t* info) {
+        return elx_dgetrf_full(nullptr, m, n, A, ld, p, q, info);
+    }
+    static elx_status_t getrf_full(int64_t m, int64_t n, float* A, int64_t ld, int32_t* p,
+                                   int32_t* q, int32_t* info) {
+        return elx_sgetrf_full(nullptr, m, n, A, ld, p, q, info);
+    }
+    static elx_status_t getrs_full(int o, int64_t n, int64_t k, const double* A, int64_t lda,
+                                   const int32_t* p, const int32_t* q, double* B, int64_t ldb) {
+        return elx_dgetrs_full(nullptr, o, n, k, A, lda, p, q, B, ldb);
+    }
+    static elx_status_t getrs_full(int o, int64_t n, int64_t k, const float* A, int64_t lda,
+                                   const int32_t* p, const int32_t* q, float* B, int64_t ldb) {
+        return elx_sgetrs_full(nullptr, o, n, k, A, lda, p, q, B, ldb);
+    }
     static elx_status_t getrs(int o, int64_t n, int64_t k, const double* A, int64_t lda,
                               const int32_t* d, double* B, int64_t ldb) {
         return elx_dgetrs(nullptr, o, n, k, A, lda, d, B, ldb);
@@ -459,7 +475,59 @@ void LU(Matrix<F>& A, Permutation& P) {
     P.SetSwapDestinations(m, d);
 }
 
+// El::LU(A, P, Q): full pivoting (LU.cpp:144-152, LU/Full.hpp:15-58).
+template <typename F>
+void LU(Matrix<F>& A, Permutation& P, Permutation& Q) {
+    const Int m = A.Height(), n = A.Width(), minDim = std::min(m, n);
+    P.MakeIdentity(m);
+    Q.MakeIdentity(n);
+    if (minDim == 0) return;
+    DevMatrix<F> dA(A);
+    DevBuf dp(sizeof(int32_t) * minDim), dq(sizeof(int32_t) * minDim), info(sizeof(int32_t));
+    check(Dev::getrf_full(m, n, dA.ptr(), dA.ld, static_cast<int32_t*>(dp.p),
+                          static_cast<int32_t*>(dq.p), static_cast<int32_t*>(info.p)),
+          "LU");
+    hcheck(hipDeviceSynchronize(), "sync");
+    const int inf = read_info_checked(info, "LU");
+    std::vector<Int> p(minDim), q(minDim);
+    hcheck(hipMemcpy(p.data(), dp.p, sizeof(int32_t) * minDim, hipMemcpyDeviceToHost), "hipMemcpy");
+    hcheck(hipMemcpy(q.data(), dq.p, sizeof(int32_t) * minDim, hipMemcpyDeviceToHost), "hipMemcpy");
+    dA.to_host(A);
+    const Int nsw = inf > 0 ? inf : minDim;  // the swaps made before a throw
+    p.resize(nsw);
+    q.resize(nsw);
+    P.SetSwapDestinations(m, p);
+    Q.SetSwapDestinations(n, q);
+    if (inf > 0) throw SingularMatrixException();
+}
+
 namespace lu {
+template <typename F>
+void SolveAfter(Orientation orientation, const Matrix<F>& A, const Permutation& P,
+                const Permutation& Q, Matrix<F>& B) {
+    if (A.Height() != A.Width()) LogicError("A must be square");
+    if (A.Height() != B.Height()) LogicError("A and B must be the same height");
+    const Int n = A.Height();
+    if (n == 0 || B.Width() == 0) return;
+    auto padded = [&](const Permutation& X) {
+        const std::vector<Int>& d = X.SwapDestinations();
+        if ((Int)d.size() > n) LogicError("Permutation larger than A");
+        std::vector<Int> full(n);
+        for (Int i = 0; i < n; ++i) full[i] = (i < (Int)d.size()) ? d[i] : i;
+        return full;
+    };
+    const std::vector<Int> p = padded(P), q = padded(Q);
+    DevMatrix<F> dA(A), dB(B);
+    DevBuf dp(sizeof(int32_t) * n), dq(sizeof(int32_t) * n);
+    hcheck(hipMemcpy(dp.p, p.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    hcheck(hipMemcpy(dq.p, q.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    check(Dev::getrs_full(orientation, n, B.Width(), dA.ptr(), dA.ld, static_cast<int32_t*>(dp.p),
+                          static_cast<int32_t*>(dq.p), dB.ptr(), dB.ld),
+          "lu::SolveAfter");
+    hcheck(hipDeviceSynchronize(), "sync");
+    dB.to_host(B);
+}
+
 template <typename F>
 void SolveAfter(Orientation orientation, const Matrix<F>& A, Matrix<F>& B) {
     if (A.Height() != A.Width()) LogicError("A must be square");
@@ -572,6 +640,9 @@ void HPDSolve(UpperOrLower uplo, Orientation orientation, const Matrix<F>& A, Ma
     template void Herk(UpperOrLower, Orientation, F, const Matrix<F>&, F, Matrix<F>&);        \
     template void LU(Matrix<F>&);                                                              \
     template void LU(Matrix<F>&, Permutation&);                                                \
+    template void LU(Matrix<F>&, Permutation&, Permutation&);                                  \
+    template void lu::SolveAfter(Orientation, const Matrix<F>&, const Permutation&,            \
+                                 const Permutation&, Matrix<F>&);                              \
     template void lu::SolveAfter(Orientation, const Matrix<F>&, Matrix<F>&);                   \
     template void lu::SolveAfter(Orientation, const Matrix<F>&, const Permutation&, Matrix<F>&); \
     template void Cholesky(UpperOrLower, Matrix<F>&);                                          \
@@ -749,6 +820,14 @@ ElError ElPopBlocksizeStack(void) { EL_TRY(El::PopBlocksizeStack()) }
     ElError ElLU_##SIG(ElMatrix_##SIG A) { EL_TRY(El::LU(*M(A))) }                             \
     ElError ElLUPartialPiv_##SIG(ElMatrix_##SIG A, ElPermutation P) {                          \
         EL_TRY(El::LU(*M(A), *Pm(P)))                                                          \
+    }                                                                                          \
+    ElError ElLUFullPiv_##SIG(ElMatrix_##SIG A, ElPermutation P, ElPermutation Q) {            \
+        EL_TRY(El::LU(*M(A), *Pm(P), *Pm(Q)))                                                  \
+    }                                                                                          \
+    ElError ElSolveAfterLUFullPiv_##SIG(ElOrientation o, ElConstMatrix_##SIG A,                \
+                                        ElConstPermutation P, ElConstPermutation Q,            \
+                                        ElMatrix_##SIG B) {                                    \
+        EL_TRY(El::lu::SolveAfter(O(o), *M(A), *Pm(P), *Pm(Q), *M(B)))                         \
     }                                                                                          \
     ElError ElSolveAfterLU_##SIG(ElOrientation o, ElConstMatrix_##SIG A, ElMatrix_##SIG B) {   \
         EL_TRY(El::lu::SolveAfter(O(o), *M(A), *M(B)))                                         \

@@ -20,6 +20,7 @@ std::mutex g_mu;
 std::map<int, DeviceInfo> g_info;
 std::map<std::pair<int, int>, Scratch> g_scratch;
 std::map<int, PanelSync> g_psync;
+std::map<int, PanelSync> g_xsync;  // one-XCD panel exchange areas (ordinary memory)
 }  // namespace
 
 const DeviceInfo& device_info() {
@@ -123,6 +124,27 @@ elx_status_t panel_sync_get(int max_groups, size_t slot_bytes, PanelSync** out) 
     return ELX_SUCCESS;
 }
 
+elx_status_t xcd_sync_get(int max_groups, size_t slot_bytes, PanelSync** out) {
+    int dev = 0;
+    ELX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    PanelSync& ps = g_xsync[dev];
+    if (ps.max_groups < max_groups || ps.slot_bytes < slot_bytes) {
+        ELX_HIP(hipDeviceSynchronize());
+        if (ps.slots) ELX_HIP(hipFree(ps.slots));
+        void* slots = nullptr;
+        ELX_HIP(hipMalloc(&slots, slot_bytes));
+        ELX_HIP(hipMemset(slots, 0, slot_bytes));
+        ELX_HIP(hipDeviceSynchronize());
+        ps.slots = slots;
+        ps.max_groups = max_groups;
+        ps.slot_bytes = slot_bytes;
+        ps.epoch = 0;
+    }
+    *out = &ps;
+    return ELX_SUCCESS;
+}
+
 // ---------------------------------------------------------------- kernel-class timing
 namespace {
 struct ProfRec {
@@ -191,6 +213,9 @@ elx_status_t release_all() {
         for (auto& kv : g_scratch)
             if (kv.second.ptr) (void)hipFree(kv.second.ptr);
         g_scratch.clear();
+        for (auto& kv : g_xsync)
+            if (kv.second.slots) (void)hipFree(kv.second.slots);
+        g_xsync.clear();
     }
     library_gemm_release();
     batched_release();
@@ -491,6 +516,38 @@ elx_status_t getrs_api(void* stream, int orient, int64_t n, int64_t nrhs, const 
 }
 
 template <typename T>
+elx_status_t getrf_full_api(void* stream, int64_t m, int64_t n, T* dA, int64_t lda, int32_t* dP,
+                            int32_t* dQ, int32_t* dInfo) {
+    ELX_ARG(m >= 0 && n >= 0 && lda >= std::max<int64_t>(m, 1) && dInfo != nullptr);
+    ELX_ARG(std::min(m, n) == 0 || (dA && dP && dQ));
+    return getrf_full(static_cast<hipStream_t>(stream), m, n, dA, lda, dP, dQ, dInfo);
+}
+
+// lu::SolveAfter(orientation, A, P, Q, B), src/lapack_like/factor/LU/SolveAfter.hpp:123-151
+template <typename T>
+elx_status_t getrs_full_api(void* stream, int orient, int64_t n, int64_t nrhs, const T* dA,
+                            int64_t lda, const int32_t* dP, const int32_t* dQ, T* dB,
+                            int64_t ldb) {
+    ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) && ldb >= std::max<int64_t>(n, 1));
+    ELX_ARG(orient >= 0 && orient <= 2);
+    if (n == 0 || nrhs == 0) return ELX_SUCCESS;
+    ELX_ARG(dA && dP && dQ && dB);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (orient == kNormal) {
+        ELX_CHECK(permute_rows_impl(s, 0, dP, n, n, nrhs, dB, ldb));
+        ELX_CHECK(trsm_blocked(s, kLeft, kLower, kNormal, kUnit, n, nrhs, T(1), dA, lda, dB, ldb));
+        ELX_CHECK(trsm_blocked(s, kLeft, kUpper, kNormal, kNonUnit, n, nrhs, T(1), dA, lda, dB, ldb));
+        ELX_CHECK(permute_rows_impl(s, 1, dQ, n, n, nrhs, dB, ldb));
+    } else {
+        ELX_CHECK(permute_rows_impl(s, 0, dQ, n, n, nrhs, dB, ldb));
+        ELX_CHECK(trsm_blocked(s, kLeft, kUpper, orient, kNonUnit, n, nrhs, T(1), dA, lda, dB, ldb));
+        ELX_CHECK(trsm_blocked(s, kLeft, kLower, orient, kUnit, n, nrhs, T(1), dA, lda, dB, ldb));
+        ELX_CHECK(permute_rows_impl(s, 1, dP, n, n, nrhs, dB, ldb));
+    }
+    return ELX_SUCCESS;
+}
+
+template <typename T>
 elx_status_t gesv_api(void* stream, int64_t n, int64_t nrhs, T* dA, int64_t lda, T* dB,
                       int64_t ldb, int64_t nb, int32_t* dDests, int32_t* dInfo) {
     ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) && ldb >= std::max<int64_t>(n, 1));
@@ -673,6 +730,25 @@ elx_status_t elx_dgetrf_nopiv(void* stream, int64_t m, int64_t n, double* dA, in
 elx_status_t elx_sgetrf_nopiv(void* stream, int64_t m, int64_t n, float* dA, int64_t lda,
                               int64_t nb, int32_t* dInfo) {
     return getrf_nopiv_api<float>(stream, m, n, dA, lda, nb, dInfo);
+}
+
+elx_status_t elx_dgetrf_full(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
+                             int32_t* dP, int32_t* dQ, int32_t* dInfo) {
+    return getrf_full_api<double>(stream, m, n, dA, lda, dP, dQ, dInfo);
+}
+elx_status_t elx_sgetrf_full(void* stream, int64_t m, int64_t n, float* dA, int64_t lda,
+                             int32_t* dP, int32_t* dQ, int32_t* dInfo) {
+    return getrf_full_api<float>(stream, m, n, dA, lda, dP, dQ, dInfo);
+}
+elx_status_t elx_dgetrs_full(void* stream, int orient, int64_t n, int64_t nrhs, const double* dA,
+                             int64_t lda, const int32_t* dP, const int32_t* dQ, double* dB,
+                             int64_t ldb) {
+    return getrs_full_api<double>(stream, orient, n, nrhs, dA, lda, dP, dQ, dB, ldb);
+}
+elx_status_t elx_sgetrs_full(void* stream, int orient, int64_t n, int64_t nrhs, const float* dA,
+                             int64_t lda, const int32_t* dP, const int32_t* dQ, float* dB,
+                             int64_t ldb) {
+    return getrs_full_api<float>(stream, orient, n, nrhs, dA, lda, dP, dQ, dB, ldb);
 }
 
 elx_status_t elx_dgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const double* dA,

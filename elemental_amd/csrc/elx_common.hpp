@@ -63,6 +63,8 @@ struct Scratch {
 elx_status_t scratch_get(int slot, size_t bytes, void** out);
 // scratch slot of the Ruiz equilibration's work vectors
 constexpr int kSlotRuizWork = 64;
+// scratch slot of the full-pivoting LU's per-workgroup pivot candidates
+constexpr int kSlotFullPiv = 65;
 
 // Per-device persistent state of the cooperative panel kernel's exchange area.
 struct PanelSync {
@@ -73,6 +75,8 @@ struct PanelSync {
     int max_groups = 0;
 };
 elx_status_t panel_sync_get(int max_groups, size_t slot_bytes, PanelSync** out);
+// The exchange area of the one-XCD panel: ordinary device memory (it stays in one XCD's L2).
+elx_status_t xcd_sync_get(int max_groups, size_t slot_bytes, PanelSync** out);
 
 inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 
@@ -124,13 +128,15 @@ elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, 
                           const double* A, i64 lda, const double* B, i64 ldb, double beta,
                           double* C, i64 ldc, int tri);
 // The same for FP64 and FP32 with the info word of a factorization sequence (`stop`, may be
-// null): once it is non-zero the call leaves C unchanged.
+// null): once it is non-zero the call leaves C unchanged. allow_library false keeps every
+// product on the hand-written kernel (no library handle, no library workspace).
 elx_status_t launch_gemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                          const double* A, i64 lda, const double* B, i64 ldb, double beta,
-                         double* C, i64 ldc, int tri, const int32_t* stop);
+                         double* C, i64 ldc, int tri, const int32_t* stop,
+                         bool allow_library = true);
 elx_status_t launch_gemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, float alpha,
                          const float* A, i64 lda, const float* B, i64 ldb, float beta, float* C,
-                         i64 ldc, int tri, const int32_t* stop);
+                         i64 ldc, int tri, const int32_t* stop, bool allow_library = true);
 // Two device doubles per device for the coefficients of gated library calls.
 elx_status_t gate_slot(double** out);
 
@@ -163,9 +169,14 @@ elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, double* A, i64 lda, i
 elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, float* A, i64 lda, int32_t* dests,
                              i64 goff, int2* pairs, int32_t* info);
 
-// The same panel for an fp32 matrix stored row by row: element (r, j) at A[r*lda + j].
+// The same panel for an fp32 matrix stored row by row: element (r, j) at A[r*lda + j]. With
+// xcd, the panel runs on the CUs of one XCD and exchanges its pivot candidates through that
+// XCD's L2 (w <= 32 and at most 1024 rows per CU: lu_panel_xcd_fits).
 elx_status_t launch_lu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
-                                          int32_t* dests, i64 goff, int2* pairs, int32_t* info);
+                                          int32_t* dests, i64 goff, int2* pairs, int32_t* info,
+                                          bool xcd = false);
+bool lu_panel_xcd_fits(i64 mp, i64 w);
+bool xcd_panels_enabled();
 // The row-move list applied to an fp32 matrix stored row by row (row i contiguous at
 // A + i*lda): columns [c0,c1) and [c2,c3) of each moved row, starting at row r0; moves that
 // reach outside the `rows` rows from r0 on are skipped.
@@ -250,6 +261,11 @@ elx_status_t launch_potf2(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i6
                           int32_t* info);
 elx_status_t launch_potf2(hipStream_t s, int uplo, i64 n, float* A, i64 lda, i64 goff,
                           int32_t* info);
+// LU with full pivoting (lu::Full): p, q receive the row and column swap destinations.
+elx_status_t getrf_full(hipStream_t s, i64 m, i64 n, double* A, i64 lda, int32_t* p, int32_t* q,
+                        int32_t* info);
+elx_status_t getrf_full(hipStream_t s, i64 m, i64 n, float* A, i64 lda, int32_t* p, int32_t* q,
+                        int32_t* info);
 // LU without pivoting of the n x n (n <= 128) block at A (lu::Unb); info as the panel's.
 elx_status_t launch_getf2_nopiv(hipStream_t s, i64 n, double* A, i64 lda, i64 goff,
                                 int32_t* info);

@@ -23,14 +23,21 @@ __device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b, uint32_t c) {
     return h;
 }
 constexpr int kSysScope = 1 | 16;  // sc0 | sc1
+// Workgroups on one XCD share its L2: stores go through the (write-through) vector L1 to L2,
+// and loads with sc0 set miss the L1 and read L2. Ordinary device memory, coherent among the
+// workgroups of that XCD only.
+constexpr int kXcdLoad = 1;   // sc0
+constexpr int kXcdStore = 0;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t area(const u4* base, uint32_t count) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<u4*>(base), 0, (int)(count * 16u), 0x00020000);
 }
+template <int AUX = kSysScope>
 __device__ __forceinline__ void st16(const u4* base, uint32_t count, uint32_t i, u4 v) {
-    __builtin_amdgcn_raw_buffer_store_b128(v, area(base, count), (int)(i * 16u), 0, kSysScope);
+    __builtin_amdgcn_raw_buffer_store_b128(v, area(base, count), (int)(i * 16u), 0, AUX);
 }
+template <int AUX = kSysScope>
 __device__ __forceinline__ u4 ld16(const u4* base, uint32_t count, uint32_t i) {
-    return __builtin_amdgcn_raw_buffer_load_b128(area(base, count), (int)(i * 16u), 0, kSysScope);
+    return __builtin_amdgcn_raw_buffer_load_b128(area(base, count), (int)(i * 16u), 0, AUX);
 }
 
 }  // namespace xchg

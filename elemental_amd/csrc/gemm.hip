@@ -221,7 +221,7 @@ __global__ void gate_kernel(const int32_t* __restrict__ stop, double alpha, doub
 template <typename T>
 elx_status_t gemm_impl(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, T alpha,
                        const T* A, i64 lda, const T* B, i64 ldb, T beta, T* C, i64 ldc, int tri,
-                       const int32_t* stop) {
+                       const int32_t* stop, bool allow_library) {
     using namespace elx;
     if (m <= 0 || n <= 0) return ELX_SUCCESS;
     if (k <= 0 || alpha == T(0)) {
@@ -240,7 +240,7 @@ elx_status_t gemm_impl(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, T a
         // transposed, one triangle of C) goes to dsyrk. In a factorization sequence the
         // coefficients come from the device (gate_kernel), so a recorded failure leaves C as it
         // is.
-        if ((double)m * (double)n * (double)k >= (double)(1 << 24)) {
+        if (allow_library && (double)m * (double)n * (double)k >= (double)(1 << 24)) {
             const bool syrk = tri != 0 && A == B && lda == ldb && m == n && ta != tb && (tri == 1) == tb;
             if (tri == 0 || syrk) {
                 const double* coef = nullptr;
@@ -285,20 +285,23 @@ namespace elx {
 
 elx_status_t launch_gemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                          const double* A, i64 lda, const double* B, i64 ldb, double beta,
-                         double* C, i64 ldc, int tri, const int32_t* stop) {
-    return gemm_impl<double>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, stop);
+                         double* C, i64 ldc, int tri, const int32_t* stop, bool allow_library) {
+    return gemm_impl<double>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, stop,
+                             allow_library);
 }
 
 elx_status_t launch_gemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, float alpha,
                          const float* A, i64 lda, const float* B, i64 ldb, float beta, float* C,
-                         i64 ldc, int tri, const int32_t* stop) {
-    return gemm_impl<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, stop);
+                         i64 ldc, int tri, const int32_t* stop, bool allow_library) {
+    return gemm_impl<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, stop,
+                            allow_library);
 }
 
 elx_status_t launch_dgemm(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, double alpha,
                           const double* A, i64 lda, const double* B, i64 ldb, double beta,
                           double* C, i64 ldc, int tri) {
-    return gemm_impl<double>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, nullptr);
+    return gemm_impl<double>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, nullptr,
+                             true);
 }
 
 }  // namespace elx

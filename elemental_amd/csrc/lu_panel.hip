@@ -24,6 +24,8 @@
 // permuted in one gather pass.
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
+#include <cstring>
 
 #include "elx_common.hpp"
 #include "exchange.hpp"
@@ -34,6 +36,9 @@ constexpr int NT = 256;
 constexpr int WMAX = 128;                 // maximum panel width
 constexpr long long SPIN_LIMIT = 1 << 20;  // bounded wait: every wave exits
 constexpr int RPL = 5;                     // rows per lane held in registers (R <= 320)
+constexpr int XCD_WMAX = 32;               // one-XCD panel: at most 32 columns,
+constexpr int XCD_RMIN = 32;               // 32 to 1024 rows per workgroup
+constexpr int XCD_RMAX = 1024;
 constexpr int REP = 4;                     // copies of each record (readers spread over them)
 using elx::xchg::u4;
 
@@ -124,9 +129,9 @@ __device__ __forceinline__ bool better(T v, int l, T bv, int bl) {
     return (v > bv) || (v == bv && l < bl);
 }
 
-// Wave-wide argmax of (v, l) with payload q (q >= 0 marks a candidate; v >= 0, l < 2^23 - 1,
-// q < 512): every lane ends with the wave's best. The candidate is packed into an order key
-// -- value bits (non-negative floats order like their bit patterns), then 2^23-1-l, then q --
+// Wave-wide argmax of (v, l) with payload q (q >= 0 marks a candidate; v >= 0, l < 2^22 - 1,
+// q < 1024): every lane ends with the wave's best. The candidate is packed into an order key
+// -- value bits (non-negative floats order like their bit patterns), then 2^22-1-l, then q --
 // reduced by DPP butterflies inside each 16-lane row and readlanes across the four rows.
 struct OKey {
     uint32_t v1, v0, l;  // value bits (high word zero for float), label key | q
@@ -163,7 +168,7 @@ __device__ __forceinline__ void wave_best(T& v, int& l, int& q) {
         } else {
             k.v0 = __float_as_uint((float)v);
         }
-        k.l = ((0x7FFFFFu - (uint32_t)l) << 9) | (uint32_t)q;
+        k.l = ((0x3FFFFFu - (uint32_t)l) << 10) | (uint32_t)q;
     }
     k = okey_step<0xB1, WIDE>(k);   // quad_perm [1,0,3,2]
     k = okey_step<0x4E, WIDE>(k);   // quad_perm [2,3,0,1]
@@ -182,8 +187,8 @@ __device__ __forceinline__ void wave_best(T& v, int& l, int& q) {
             v = (T)__longlong_as_double((long long)(((uint64_t)k.v1 << 32) | k.v0));
         else
             v = (T)__uint_as_float(k.v0);
-        l = (int)(0x7FFFFFu - (k.l >> 9));
-        q = (int)(k.l & 511u);
+        l = (int)(0x3FFFFFu - (k.l >> 10));
+        q = (int)(k.l & 1023u);
     }
 }
 
@@ -202,8 +207,14 @@ __device__ __forceinline__ void pick(const T* wv, const int* wl, const int* wq, 
         }
 }
 
-template <typename T>
+// XM = 0: the workgroups are spread over all XCDs and exchange through uncached memory.
+// XM = 1: the workgroups sit on one XCD (those with blockIdx % 8 == 0 of a grid dealt
+// round-robin over the 8 XCDs; the rest return at once) and exchange through that XCD's L2,
+// a few hundred nanoseconds per hop instead of the microseconds of a trip to memory.
+template <typename T, int XM>
 __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
+    constexpr int LDB = XM ? elx::xchg::kXcdLoad : elx::xchg::kSysScope;
+    constexpr int STB = XM ? elx::xchg::kXcdStore : elx::xchg::kSysScope;
     constexpr int NW = NT / 64;
     constexpr int PW = NW - 1;  // waves that read the exchange; the last wave publishes
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -224,7 +235,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
     int* rb_q = rb_l + NW;                     // [NW]
     int* wb_ok = rb_q + NW;                    // [2][NW] per-wave votes
 
-    const int g = blockIdx.x, t = threadIdx.x, G = p.G;
+    if (XM && (blockIdx.x & 7) != 0) return;  // not on the panel's XCD
+    const int g = XM ? (int)(blockIdx.x >> 3) : (int)blockIdx.x, t = threadIdx.x, G = p.G;
     const int wave = t >> 6, lane = t & 63;
     const i64 row0 = (i64)g * R;
     const int rows = (int)((p.mp - row0) < R ? (p.mp - row0) : R);
@@ -303,9 +315,9 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             const double val = (cand >= 0) ? (double)tile[cand * LD + cc] : 0.0;
             u4 r0 = u4{dlo(val), dhi(val), (uint32_t)cand_lbl, 0u};
             r0[3] = rec0_tag(r0, seq);
-            st16(p.rec, nrec, rec + 1, rec_unit(dlo(a), dhi(a), seq, 0x5bd1e995u));
-            st16(p.rec, nrec, rec + 2, rec_unit(dlo(l), dhi(l), seq, 0x1b873593u));
-            st16(p.rec, nrec, rec, r0);
+            st16<STB>(p.rec, nrec, rec + 1, rec_unit(dlo(a), dhi(a), seq, 0x5bd1e995u));
+            st16<STB>(p.rec, nrec, rec + 2, rec_unit(dlo(l), dhi(l), seq, 0x1b873593u));
+            st16<STB>(p.rec, nrec, rec, r0);
         }
         if (cand >= 0) {
             const T lb = lp[cand];
@@ -322,7 +334,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                         v[e] = (j < W) ? tile[cand * LD + j] : T(0);
                     }
                 }
-                st16(p.rows, nrows, myrow + q, CK::make(v, q, seq));
+                st16<STB>(p.rows, nrows, myrow + q, CK::make(v, q, seq));
             }
         }
     };
@@ -388,9 +400,9 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
                         if (got) continue;
                         const int q = t + k * STRIDE;
                         const uint32_t rq = (uint32_t)RU * (uint32_t)((((g % REP) * 2) + s) * G + q);
-                        const u4 r0 = ld16(p.rec, nrec, rq);
-                        const u4 r1 = ld16(p.rec, nrec, rq + 1);
-                        const u4 r2 = ld16(p.rec, nrec, rq + 2);
+                        const u4 r0 = ld16<LDB>(p.rec, nrec, rq);
+                        const u4 r1 = ld16<LDB>(p.rec, nrec, rq + 1);
+                        const u4 r2 = ld16<LDB>(p.rec, nrec, rq + 2);
                         if (r0[3] != rec0_tag(r0, seq) || !rec_unit_ok(r1, seq, 0x5bd1e995u) ||
                             !rec_unit_ok(r2, seq, 0x1b873593u))
                             continue;
@@ -445,7 +457,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
             while (true) {
                 if (!done) {
                     if (!CK::ok(fch, q, f_seq))
-                        fch = ld16(p.rows, nrows, (uint32_t)(((c - 1) % 3) * G + f_w) * WMAX + q);
+                        fch = ld16<LDB>(p.rows, nrows, (uint32_t)(((c - 1) % 3) * G + f_w) * WMAX + q);
                     if (CK::ok(fch, q, f_seq)) {
                         T v[CK::VPC];
                         CK::get(fch, v);
@@ -496,7 +508,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         f_l = lprev_w;
         f_q0 = (c + 1) / CK::VPC;
         if (f_w >= 0 && t < 128 && f_q0 + t < q1)
-            fch = ld16(p.rows, nrows, (uint32_t)((c % 3) * G + f_w) * WMAX + f_q0 + t);
+            fch = ld16<LDB>(p.rows, nrows, (uint32_t)((c % 3) * G + f_w) * WMAX + f_q0 + t);
         // (4) per row: the swap as a relabel (the row at position c moves to position piv,
         // the winner becomes row c), the multiplier, column c+1 through step c, and the
         // candidate of column c+1. An exactly zero pivot (the reference throws
@@ -622,21 +634,34 @@ size_t panel_lds_bytes(int R, int W) {
 
 template <typename T>
 elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowmajor,
-                          int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
+                          int32_t* dests, i64 goff, int2* pairs, int32_t* info, bool xcd) {
     ELX_ARG(w >= 1 && w <= WMAX && mp >= w);
     const elx::DeviceInfo& di = elx::device_info();
     const size_t lds_cap = (size_t)di.max_lds_per_block;
-    // A step costs about the same for 16 to 128 rows per workgroup, while the exchange gets
-    // slower with more workgroups: short panels use fewer, fuller workgroups.
-    const i64 RMIN = 128;
-    i64 G = std::min<i64>(di.num_cus, (mp + RMIN - 1) / RMIN);
-    if (G < 1) G = 1;
-    i64 R = (mp + G - 1) / G;
-    G = (mp + R - 1) / R;  // every workgroup owns at least one row
-    ELX_ARG(G <= NT);      // one record per thread in the exchange
-    ELX_ARG(mp < (1 << 23) - 1);
+    i64 G, R;
+    if (xcd) {
+        // one XCD's CUs, one workgroup each, at least 32 rows per workgroup
+        ELX_ARG(w <= XCD_WMAX);
+        const i64 cus = std::max(di.num_cus / 8, 1);
+        G = std::min<i64>(cus, (mp + XCD_RMIN - 1) / XCD_RMIN);
+        if (G < 1) G = 1;
+        R = (mp + G - 1) / G;
+        G = (mp + R - 1) / R;
+        ELX_ARG(R <= XCD_RMAX);
+    } else {
+        // A step costs about the same for 16 to 128 rows per workgroup, while the exchange
+        // gets slower with more workgroups: short panels use fewer, fuller workgroups.
+        const i64 RMIN = 128;
+        G = std::min<i64>(di.num_cus, (mp + RMIN - 1) / RMIN);
+        if (G < 1) G = 1;
+        R = (mp + G - 1) / G;
+        G = (mp + R - 1) / R;  // every workgroup owns at least one row
+        ELX_ARG(R <= 64 * RPL);
+    }
+    ELX_ARG(G <= NT);  // one record per thread in the exchange
+    ELX_ARG(mp < (1 << 22) - 1);
     const size_t lds = panel_lds_bytes<T>((int)R, (int)w);
-    if (lds > lds_cap || R > 64 * RPL) {
+    if (lds > lds_cap) {
         std::fprintf(stderr,
                      "elemental_amd: LU panel of height %lld needs %zu bytes of LDS per CU "
                      "(limit %zu)\n",
@@ -649,7 +674,8 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowma
     // Row chunks rotate over three slots: a winner's row of column c is read until the
     // readers reach column c+1, while its owner may already publish column c+2.
     const size_t row_bytes = sizeof(u4) * 3 * (size_t)di.num_cus * WMAX;
-    ELX_CHECK(elx::panel_sync_get(di.num_cus, rec_bytes + row_bytes, &ps));
+    if (xcd) ELX_CHECK(elx::xcd_sync_get(di.num_cus, rec_bytes + row_bytes, &ps));
+    else ELX_CHECK(elx::panel_sync_get(di.num_cus, rec_bytes + row_bytes, &ps));
     PanelArgs<T> a;
     a.mp = mp;
     a.w = (int)w;
@@ -665,16 +691,16 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowma
     a.seq_base = (uint32_t)((++ps->epoch) * 1024);
     a.rec = reinterpret_cast<u4*>(ps->slots);
     a.rows = a.rec + REP * 2 * RU * (size_t)di.num_cus;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lu_panel_kernel<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cap);
+    const void* fn = xcd ? reinterpret_cast<const void*>(&lu_panel_kernel<T, 1>)
+                         : reinterpret_cast<const void*>(&lu_panel_kernel<T, 0>);
+    static bool attr_set[2] = {false, false};
+    if (!attr_set[xcd ? 1 : 0]) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cap);
         (void)hipGetLastError();
-        attr_set = true;
+        attr_set[xcd ? 1 : 0] = true;
     }
     void* args[] = {&a};
-    ELX_CHECK(elx::launch_resident(reinterpret_cast<const void*>(&lu_panel_kernel<T>), G, NT, args,
-                                   lds, s));
+    ELX_CHECK(elx::launch_resident(fn, xcd ? 8 * G : G, NT, args, lds, s));
     return ELX_SUCCESS;
 }
 
@@ -682,24 +708,52 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowma
 
 namespace elx {
 
-elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
-                                 int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
-    return launch_panel<double>(s, mp, w, A, lda, 0, dests, goff, pairs, info);
+// The mixed LU's panels use the one-XCD exchange unless ELX_PANEL_EXCHANGE=memory selects the
+// all-XCD exchange through uncached memory (an A/B switch between two complete panels).
+bool xcd_panels_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("ELX_PANEL_EXCHANGE");
+        return !(v && std::strcmp(v, "memory") == 0);
+    }();
+    return on;
 }
 
+template <typename T>
+bool xcd_fits(i64 mp, i64 w) {
+    const DeviceInfo& di = device_info();
+    const i64 cus = std::max(di.num_cus / 8, 1);
+    if (w < 1 || w > XCD_WMAX || mp < w || mp > XCD_RMAX * cus) return false;
+    const i64 G = std::min<i64>(cus, (mp + XCD_RMIN - 1) / XCD_RMIN);
+    const i64 R = (mp + G - 1) / G;
+    return panel_lds_bytes<T>((int)R, (int)w) <= (size_t)di.max_lds_per_block;
+}
+
+bool lu_panel_xcd_fits(i64 mp, i64 w) { return xcd_fits<float>(mp, w); }
+
+elx_status_t launch_lu_panel_f64(hipStream_t s, i64 mp, i64 w, double* A, i64 lda,
+                                 int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
+    return launch_panel<double>(s, mp, w, A, lda, 0, dests, goff, pairs, info, false);
+}
+
+// Panels of at most 32 columns (Blocksize() <= 32) run on one XCD where their rows fit.
 elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, double* A, i64 lda, int32_t* dests,
                              i64 goff, int2* pairs, int32_t* info) {
-    return launch_panel<double>(s, mp, w, A, lda, 0, dests, goff, pairs, info);
+    const bool xcd = xcd_panels_enabled() && xcd_fits<double>(mp, w);
+    return launch_panel<double>(s, mp, w, A, lda, 0, dests, goff, pairs, info, xcd);
 }
 
 elx_status_t launch_lu_panel(hipStream_t s, i64 mp, i64 w, float* A, i64 lda, int32_t* dests,
                              i64 goff, int2* pairs, int32_t* info) {
-    return launch_panel<float>(s, mp, w, A, lda, 0, dests, goff, pairs, info);
+    const bool xcd = xcd_panels_enabled() && xcd_fits<float>(mp, w);
+    return launch_panel<float>(s, mp, w, A, lda, 0, dests, goff, pairs, info, xcd);
 }
 
 elx_status_t launch_lu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
-                                          int32_t* dests, i64 goff, int2* pairs, int32_t* info) {
-    return launch_panel<float>(s, mp, w, A, lda, 1, dests, goff, pairs, info);
+                                          int32_t* dests, i64 goff, int2* pairs, int32_t* info,
+                                          bool xcd) {
+    return launch_panel<float>(s, mp, w, A, lda, 1, dests, goff, pairs, info, xcd);
 }
+
+
 
 }  // namespace elx

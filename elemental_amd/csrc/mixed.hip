@@ -97,7 +97,8 @@ struct Ctx {
     float* Uinv;
     int32_t* dests;
     int32_t* perm;
-    int2* pairs;  // (NB/128) x 256
+    int2* pairs;  // (NB/32) x 256
+    bool xcd_panels;  // one-XCD 32-column panels where they fit
     int32_t* info;
     int* e_dev;
     unsigned* bits;
@@ -157,8 +158,10 @@ elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float
 }
 
 // Two-level blocked LU with partial pivoting of W (fp32, A stored row by row, so a row
-// interchange moves contiguous memory): 128-column cooperative panels inside NB-column outer
-// blocks; the outer trailing update is one rank-NB low-precision GEMM.
+// interchange moves contiguous memory): cooperative panels inside NB-column outer blocks; the
+// outer trailing update is one rank-NB low-precision GEMM. Panels are 32 columns wide and run
+// on the CUs of one XCD, exchanging their pivot candidates through its L2, when the rows fit
+// there (up to 1024 per CU), else 128 columns over all XCDs through memory.
 elx_status_t getrf_mixed(Ctx& c) {
     const i64 n = c.n, ld = c.ldw;
     float* W = c.W;
@@ -168,11 +171,13 @@ elx_status_t getrf_mixed(Ctx& c) {
     for (i64 K0 = 0; K0 < n; K0 += c.NB) {
         const i64 nbk = std::min(c.NB, n - K0);
         const i64 K1 = K0 + nbk;
-        for (i64 j0 = K0; j0 < K1; j0 += 128) {
-            const i64 w = std::min<i64>(128, K1 - j0);
-            int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
+        const bool xcd = c.xcd_panels && lu_panel_xcd_fits(n - K0, 32);
+        const i64 P = xcd ? 32 : 128;
+        for (i64 j0 = K0; j0 < K1; j0 += P) {
+            const i64 w = std::min<i64>(P, K1 - j0);
+            int2* pr = c.pairs + ((j0 - K0) / P) * 256;
             ELX_CHECK(launch_lu_panel_f32_rowmajor(c.s, n - j0, w, at(j0, j0), ld, c.dests + j0,
-                                                   j0, pr, c.info));
+                                                   j0, pr, c.info, xcd));
             ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, n - j0, K0,
                                                     j0, j0 + w, K1));
             const i64 wr = K1 - (j0 + w);
@@ -185,9 +190,9 @@ elx_status_t getrf_mixed(Ctx& c) {
             }
         }
         // the outer block's swaps on the columns outside it, in panel order
-        for (i64 j0 = K0; j0 < K1; j0 += 128) {
-            const i64 w = std::min<i64>(128, K1 - j0);
-            const int2* pr = c.pairs + ((j0 - K0) / 128) * 256;
+        for (i64 j0 = K0; j0 < K1; j0 += P) {
+            const i64 w = std::min<i64>(P, K1 - j0);
+            const int2* pr = c.pairs + ((j0 - K0) / P) * 256;
             ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, n - j0, 0,
                                                     K0, K1, n));
         }
@@ -334,7 +339,8 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(get(kSlotLinv, (size_t)nblk * 128 * 128, &c.Linv));
     ELX_CHECK(get(kSlotUinv, (size_t)nblk * 128 * 128, &c.Uinv));
     int32_t* idx = nullptr;
-    ELX_CHECK(get(kSlotIdx, (size_t)(2 * n + 8 + (c.NB / 128 + 1) * 512), &idx));
+    ELX_CHECK(get(kSlotIdx, (size_t)(2 * n + 8 + (c.NB / 32 + 1) * 512), &idx));
+    c.xcd_panels = xcd_panels_enabled();
     c.dests = idx;
     c.perm = idx + n;
     c.info = idx + 2 * n;

@@ -95,6 +95,27 @@ def getrf_nopiv(A, nb=128, stream=None):
     return info
 
 
+def getrf_full(A, stream=None):
+    """In-place P A Q = L U with full pivoting. Returns (p, q int32 tensors, info)."""
+    pA, lda = _m(A)
+    m, n = A.shape
+    k = max(min(m, n), 1)
+    p = torch.zeros(k, dtype=torch.int32, device=A.device)
+    q = torch.zeros(k, dtype=torch.int32, device=A.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call(_pre(A) + "getrf_full", _stream(stream), m, n, pA, lda, p.data_ptr(), q.data_ptr(),
+         info.data_ptr())
+    return p[: min(m, n)], q[: min(m, n)], info
+
+
+def getrs_full(A, p, q, B, orient=NORMAL, stream=None):
+    pA, lda = _m(A)
+    pB, ldb = _m(B)
+    call(_pre(A) + "getrs_full", _stream(stream), orient, A.shape[0], B.shape[1], pA, lda,
+         p.data_ptr(), q.data_ptr(), pB, ldb)
+    return B
+
+
 def getrs(A, dests, B, orient=NORMAL, stream=None):
     """lu::SolveAfter; dests None for the factors of getrf_nopiv."""
     pA, lda = _m(A)
@@ -250,7 +271,7 @@ def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_i
 
 
 __all__ = [
-    "load", "getrf", "getrf_nopiv", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
+    "load", "getrf", "getrf_nopiv", "getrf_full", "getrs_full", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
     "permute_rows", "ruiz_equil", "symmetric_ruiz_equil", "geom_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
     "Batch", "potrf_vbatched", "potrs_vbatched", "gesv_mixed_batched",
 ]

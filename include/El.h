@@ -99,12 +99,16 @@ ElError ElPushBlocksizeStack(ElInt blocksize);
 ElError ElPopBlocksizeStack(void);
 
 /* Factorizations and solves, for float (_s) and double (_d): ElLU_ (no pivoting),
- * ElLUPartialPiv_, ElSolveAfterLU_, ElSolveAfterLUPartialPiv_ (factor-C.cpp:268-323),
+ * ElLUPartialPiv_, ElLUFullPiv_, ElSolveAfterLU_, ElSolveAfterLUPartialPiv_,
+ * ElSolveAfterLUFullPiv_ (factor-C.cpp:268-323),
  * ElCholesky_, ElSolveAfterCholesky_ (factor-C.cpp:70-135), ElLinearSolve_, ElHPDSolve_
  * (solve-C.cpp:18-46). */
 ElError ElLU_s(ElMatrix_s A);
 ElError ElLUPartialPiv_s(ElMatrix_s A, ElPermutation P);
 ElError ElSolveAfterLU_s(ElOrientation orientation, ElConstMatrix_s A, ElMatrix_s B);
+ElError ElLUFullPiv_s(ElMatrix_s A, ElPermutation P, ElPermutation Q);
+ElError ElSolveAfterLUFullPiv_s(ElOrientation orientation, ElConstMatrix_s A,
+                                ElConstPermutation P, ElConstPermutation Q, ElMatrix_s B);
 ElError ElSolveAfterLUPartialPiv_s(ElOrientation orientation, ElConstMatrix_s A,
                                    ElConstPermutation P, ElMatrix_s B);
 ElError ElCholesky_s(ElUpperOrLower uplo, ElMatrix_s A);
@@ -117,6 +121,9 @@ ElError ElHPDSolve_s(ElUpperOrLower uplo, ElOrientation orientation, ElConstMatr
 ElError ElLU_d(ElMatrix_d A);
 ElError ElLUPartialPiv_d(ElMatrix_d A, ElPermutation P);
 ElError ElSolveAfterLU_d(ElOrientation orientation, ElConstMatrix_d A, ElMatrix_d B);
+ElError ElLUFullPiv_d(ElMatrix_d A, ElPermutation P, ElPermutation Q);
+ElError ElSolveAfterLUFullPiv_d(ElOrientation orientation, ElConstMatrix_d A,
+                                ElConstPermutation P, ElConstPermutation Q, ElMatrix_d B);
 ElError ElSolveAfterLUPartialPiv_d(ElOrientation orientation, ElConstMatrix_d A,
                                    ElConstPermutation P, ElMatrix_d B);
 ElError ElCholesky_d(ElUpperOrLower uplo, ElMatrix_d A);

@@ -9,7 +9,7 @@
 //   SingularMatrixException,
 //   NonHPDMatrixException         include/El/core/environment/decl.hpp:209-231
 //   LU, lu::SolveAfter            include/El/lapack_like/factor.hpp:364-392,418-460
-//                                 (with and without pivoting; F = float and double)
+//                                 (no, partial and full pivoting; F = float and double)
 //   Cholesky, cholesky::SolveAfter include/El/lapack_like/factor.hpp:21-40
 //   LinearSolve, HPDSolve         include/El/lapack_like/solve.hpp:19-38,188-212
 //   Gemm, Trsm, Herk, Syrk        include/El/blas_like/level3.hpp:34-120,326-334,455-468
@@ -164,11 +164,16 @@ template <typename F>
 void LU(Matrix<F>& A);
 template <typename F>
 void LU(Matrix<F>& A, Permutation& P);
+template <typename F>
+void LU(Matrix<F>& A, Permutation& P, Permutation& Q);
 namespace lu {
 template <typename F>
 void SolveAfter(Orientation orientation, const Matrix<F>& A, Matrix<F>& B);
 template <typename F>
 void SolveAfter(Orientation orientation, const Matrix<F>& A, const Permutation& P, Matrix<F>& B);
+template <typename F>
+void SolveAfter(Orientation orientation, const Matrix<F>& A, const Permutation& P,
+                const Permutation& Q, Matrix<F>& B);
 }  // namespace lu
 template <typename F>
 void Cholesky(UpperOrLower uplo, Matrix<F>& A);

@@ -85,6 +85,26 @@ elx_status_t elx_dgetrf_nopiv(void* stream, int64_t m, int64_t n, double* dA, in
 elx_status_t elx_sgetrf_nopiv(void* stream, int64_t m, int64_t n, float* dA, int64_t lda,
                               int64_t nb, int32_t* dInfo);
 
+/* PAQ = LU with full (complete) pivoting: per step the first entry of largest magnitude of
+ * the trailing block in column-major order, rows and columns swapped over the whole matrix.
+ * dP, dQ: min(m,n) int32 row and column swap destinations (device).
+ * Replaces El::LU(Matrix<F>&, Permutation& P, Permutation& Q) -- src/lapack_like/factor/
+ * LU.cpp:144-152, LU/Full.hpp:15-58. */
+elx_status_t elx_dgetrf_full(void* stream, int64_t m, int64_t n, double* dA, int64_t lda,
+                             int32_t* dP, int32_t* dQ, int32_t* dInfo);
+elx_status_t elx_sgetrf_full(void* stream, int64_t m, int64_t n, float* dA, int64_t lda,
+                             int32_t* dP, int32_t* dQ, int32_t* dInfo);
+
+/* B := Q U^{-1} L^{-1} P B (NORMAL) or P^T L^{-T} U^{-T} Q^T B with the factors of
+ * elx_dgetrf_full. Replaces lu::SolveAfter(Orientation, const Matrix<F>&, const Permutation&,
+ * const Permutation&, Matrix<F>&) -- src/lapack_like/factor/LU/SolveAfter.hpp:123-151. */
+elx_status_t elx_dgetrs_full(void* stream, int orient, int64_t n, int64_t nrhs, const double* dA,
+                             int64_t lda, const int32_t* dP, const int32_t* dQ, double* dB,
+                             int64_t ldb);
+elx_status_t elx_sgetrs_full(void* stream, int orient, int64_t n, int64_t nrhs, const float* dA,
+                             int64_t lda, const int32_t* dP, const int32_t* dQ, float* dB,
+                             int64_t ldb);
+
 /* B := op(A)^{-1} P B (orient NORMAL) or P^T op(A)^{-1} B (TRANSPOSE/ADJOINT) with the
  * factors of elx_dgetrf. Replaces lu::SolveAfter(Orientation, const Matrix<double>&,
  * const Permutation&, Matrix<double>&) -- src/lapack_like/factor/LU/SolveAfter.hpp:66-92.

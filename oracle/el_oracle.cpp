@@ -17,6 +17,8 @@
 //   Permutation (swap sequence)         src/lapack_like/perm/Permutation.cpp:166-250,544-631,731-756
 //   El::LU(Matrix&) without pivoting    src/lapack_like/factor/LU.cpp:21-44, LU/Local.hpp:44-61
 //   lu::SolveAfter(orient,A,B)          src/lapack_like/factor/LU/SolveAfter.hpp:15-38
+//   El::LU(A,P,Q) (full pivoting)       src/lapack_like/factor/LU/Full.hpp:15-58, LU.cpp:144-152
+//   lu::SolveAfter(orient,A,P,Q,B)      src/lapack_like/factor/LU/SolveAfter.hpp:123-151
 //   Gaussian / GEPPGrowth               src/matrices/random/independent/Gaussian.cpp:16-47,
 //                                       include/El/core/random/impl.hpp:185-207,
 //                                       src/core/random.cpp:24-32,
@@ -643,6 +645,59 @@ int lu_nopiv_impl(T* A, i64 m, i64 n, i64 lda, i64 bsize) {
     return 0;
 }
 
+// lu::Full (LU with full pivoting), src/lapack_like/factor/LU/Full.hpp:15-58: per step k,
+// MaxAbsLoc of the trailing block A(k:,k:) (src/blas_like/level1/MaxAbsLoc.cpp:131-163: a
+// column-major scan with strict '>', so ties go to the first entry in column-major order and
+// an all-zero block gives its top-left entry), RowSwap(A,k,iPiv) and ColSwap(A,k,jPiv) over the
+// whole matrix, then Scal by 1/alpha and Geru(-1). p and q receive P's and Q's swap
+// destinations. Returns 0, or k+1 for an exactly zero pivot (SingularMatrixException).
+template <typename T>
+int lu_full_impl(T* A, i64 m, i64 n, i64 lda, int32_t* p, int32_t* q) {
+    const i64 minDim = std::min(m, n);
+    for (i64 k = 0; k < minDim; ++k) {
+        T best = T(0);
+        i64 bi = 0, bj = 0;
+        for (i64 j = k; j < n; ++j)
+            for (i64 i = k; i < m; ++i) {
+                const T v = std::fabs(A[IDX(i, j, lda)]);
+                if (v > best) {
+                    best = v;
+                    bi = i - k;
+                    bj = j - k;
+                }
+            }
+        const i64 iPiv = k + bi, jPiv = k + bj;
+        p[k] = (int32_t)iPiv;
+        q[k] = (int32_t)jPiv;
+        if (iPiv != k) Ops<T>::swap(n, &A[k], lda, &A[iPiv], lda);
+        if (jPiv != k) Ops<T>::swap(m, &A[IDX(0, k, lda)], 1, &A[IDX(0, jPiv, lda)], 1);
+        const T alpha = A[IDX(k, k, lda)];
+        if (alpha == T(0)) return (int)(k + 1);
+        Ops<T>::scal(m - (k + 1), T(1) / alpha, &A[IDX(k + 1, k, lda)], 1);
+        Ops<T>::ger(m - (k + 1), n - (k + 1), T(-1), &A[IDX(k + 1, k, lda)], 1,
+                    &A[IDX(k, k + 1, lda)], lda, &A[IDX(k + 1, k + 1, lda)], lda);
+    }
+    return 0;
+}
+
+// lu::SolveAfter(orientation, A, P, Q, B), src/lapack_like/factor/LU/SolveAfter.hpp:123-151.
+template <typename T>
+int lu_full_solve_after(int orient, const T* A, i64 n, i64 lda, const int32_t* p,
+                        const int32_t* q, i64 nswaps, T* B, i64 nrhs, i64 ldb) {
+    if (orient == 0) {
+        permute_rows(p, nswaps, B, n, nrhs, ldb);
+        Ops<T>::trsm('L', 'L', 'N', 'U', n, nrhs, T(1), A, lda, B, ldb);
+        Ops<T>::trsm('L', 'U', 'N', 'N', n, nrhs, T(1), A, lda, B, ldb);
+        inverse_permute_rows(q, nswaps, B, n, nrhs, ldb);
+    } else {
+        permute_rows(q, nswaps, B, n, nrhs, ldb);
+        Ops<T>::trsm('L', 'U', 'T', 'N', n, nrhs, T(1), A, lda, B, ldb);
+        Ops<T>::trsm('L', 'L', 'T', 'U', n, nrhs, T(1), A, lda, B, ldb);
+        inverse_permute_rows(p, nswaps, B, n, nrhs, ldb);
+    }
+    return 0;
+}
+
 // lu::SolveAfter(orientation, A, [P,] B), src/lapack_like/factor/LU/SolveAfter.hpp:15-38
 // (no pivots: dests null) and :66-92. orient: 0 NORMAL, 1 TRANSPOSE, 2 ADJOINT (identical
 // for real data).
@@ -856,6 +911,22 @@ int orc_lu_nopiv(double* A, i64 m, i64 n, i64 lda, i64 bsize) {
 }
 int orc_slu_nopiv(float* A, i64 m, i64 n, i64 lda, i64 bsize) {
     return lu_nopiv_impl(A, m, n, lda, bsize);
+}
+
+// El::LU(A, P, Q) with full pivoting and its SolveAfter.
+int orc_lu_full(double* A, i64 m, i64 n, i64 lda, int32_t* p, int32_t* q) {
+    return lu_full_impl(A, m, n, lda, p, q);
+}
+int orc_slu_full(float* A, i64 m, i64 n, i64 lda, int32_t* p, int32_t* q) {
+    return lu_full_impl(A, m, n, lda, p, q);
+}
+int orc_lu_full_solve_after(int orient, const double* A, i64 n, i64 lda, const int32_t* p,
+                            const int32_t* q, i64 nswaps, double* B, i64 nrhs, i64 ldb) {
+    return lu_full_solve_after(orient, A, n, lda, p, q, nswaps, B, nrhs, ldb);
+}
+int orc_slu_full_solve_after(int orient, const float* A, i64 n, i64 lda, const int32_t* p,
+                             const int32_t* q, i64 nswaps, float* B, i64 nrhs, i64 ldb) {
+    return lu_full_solve_after(orient, A, n, lda, p, q, nswaps, B, nrhs, ldb);
 }
 
 // lu::SolveAfter(orientation, A, P, B) (dests non-null) or (orientation, A, B) (dests null).

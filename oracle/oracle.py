@@ -131,6 +131,32 @@ def lu_nopiv(A, bsize=128):
     return F, info
 
 
+def lu_full(A):
+    """El::LU(A, P, Q) with full pivoting on a copy, in A's dtype; returns (F, p, q, info)."""
+    dt = _dt(A)
+    F = np.array(A, dtype=dt, order="F", copy=True)
+    m, n = F.shape
+    k = max(min(m, n), 1)
+    p = np.zeros(k, np.int32)
+    q = np.zeros(k, np.int32)
+    info = _fn("lu_full", dt)(_pt(F), _i64(m), _i64(n), _i64(_ld(F)), _ip_(p), _ip_(q))
+    return F, p[: min(m, n)], q[: min(m, n)], info
+
+
+def lu_full_solve_after(F, p, q, B, orient=0):
+    dt = _dt(F)
+    F = np.asfortranarray(F, dtype=dt)
+    X = np.array(B, dtype=dt, order="F", copy=True)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1, order="F")
+    pp = np.ascontiguousarray(p, np.int32)
+    qq = np.ascontiguousarray(q, np.int32)
+    _fn("lu_full_solve_after", dt)(orient, _pt(F), _i64(F.shape[0]), _i64(_ld(F)), _ip_(pp),
+                                   _ip_(qq), _i64(pp.size), _pt(X), _i64(X.shape[1]),
+                                   _i64(_ld(X)))
+    return X
+
+
 def lu_trace_column(A, col, bsize=128):
     dt = _dt(A)
     F = np.array(A, dtype=dt, order="F", copy=True)

@@ -35,7 +35,10 @@ def test_headers_declare_the_path():
                      "ElLU_d", "ElLU_s", "ElSolveAfterLU_d", "ElSolveAfterLU_s",
                      "ElLUPartialPiv_s", "ElSolveAfterLUPartialPiv_s", "ElCholesky_s",
                      "ElSolveAfterCholesky_s", "ElLinearSolve_s", "ElHPDSolve_s", "ElGemm_s",
-                     "ElTrsm_s", "ElHerk_s", "ElSyrk_s", "ElMatrixCreate_s"]:
+                     "ElTrsm_s", "ElHerk_s", "ElSyrk_s", "ElMatrixCreate_s",
+                     "elx_dgetrf_full", "elx_sgetrf_full", "elx_dgetrs_full", "elx_sgetrs_full",
+                     "ElLUFullPiv_d", "ElLUFullPiv_s", "ElSolveAfterLUFullPiv_d",
+                     "ElSolveAfterLUFullPiv_s"]:
         assert required in names
 
 
@@ -64,7 +67,8 @@ def test_cpp_api_is_exported():
                 "El::LU<float>(El::Matrix<float>&)", "El::LU<double>(El::Matrix<double>&)",
                 "El::lu::SolveAfter<float>(El::Orientation, El::Matrix<float> const&, El::Matrix<float>&)",
                 "El::Cholesky<float>(", "El::LinearSolve<float>(", "El::HPDSolve<float>(",
-                "El::Gemm<float>(", "El::Trsm<float>(", "El::Herk<float>(", "El::Syrk<float>("]:
+                "El::Gemm<float>(", "El::Trsm<float>(", "El::Herk<float>(", "El::Syrk<float>(",
+                "El::LU<double>(El::Matrix<double>&, El::Permutation&, El::Permutation&)"]:
         assert sym in out, sym
 
 

@@ -207,3 +207,26 @@ def test_c_api_lu_without_pivoting(lib, orc):
     Z = np.zeros((4, 4))
     Zm = Mat(lib, Z)
     assert lib.ElLU_d(Zm.h) == 5  # SingularMatrixException -> EL_RUNTIME_ERROR
+
+
+def test_c_api_lu_full_pivoting(lib, orc):
+    """ElLUFullPiv_d / ElSolveAfterLUFullPiv_d (factor-C.cpp:279-283, 305-312)."""
+    lib.ElLUFullPiv_d.argtypes = [vp, vp, vp]
+    lib.ElSolveAfterLUFullPiv_d.argtypes = [ctypes.c_int, vp, vp, vp, vp]
+    rng = np.random.default_rng(73)
+    n = 200
+    A = np.asfortranarray(rng.standard_normal((n, n)))
+    B = np.asfortranarray(rng.standard_normal((n, 2)))
+    F_ref, p_ref, q_ref, _ = orc.lu_full(A)
+    M = Mat(lib, A)
+    P, Q = vp(), vp()
+    lib.ElPermutationCreate(ctypes.byref(P))
+    lib.ElPermutationCreate(ctypes.byref(Q))
+    assert lib.ElLUFullPiv_d(M.h, P, Q) == 0
+    assert np.array_equal(perm_dests(lib, P), p_ref) and np.array_equal(perm_dests(lib, Q), q_ref)
+    assert np.array_equal(M.a, F_ref)
+    X = Mat(lib, B)
+    assert lib.ElSolveAfterLUFullPiv_d(0, M.h, P, Q, X.h) == 0
+    assert orc.backward_error(A, X.a, B) <= n * EPS
+    lib.ElPermutationDestroy(P)
+    lib.ElPermutationDestroy(Q)

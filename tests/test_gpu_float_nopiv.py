@@ -296,3 +296,46 @@ def test_blocksize_above_128(dev, orc):
     agree = orc.pivot_agreement(A, d_ref, d.cpu().numpy(), bsize=256)
     assert agree["mismatches"] == 0 or agree["near_ties"] == 1, agree
     assert orc.lu_factor_error(A, dev.to_host(dA), d.cpu().numpy()) <= 8 * 600 * EPS
+
+
+# -------------------------------------------------------------------------- full pivoting
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lu_full_pivoting_vs_oracle(dev, orc, dtype):
+    """lu::Full (LU/Full.hpp:15-58): every step's pivot is the first largest entry of the
+    trailing block in column-major order; pivots identical to the oracle's, and the factors
+    bit for bit (each entry receives the same FMA sequence as the restated dger)."""
+    rng = np.random.default_rng(72)
+    for (m, n) in [(1, 1), (7, 7), (60, 45), (45, 60), (300, 300), (700, 700)]:
+        say("lu full", m, n, dtype.__name__)
+        A = np.asfortranarray(rng.standard_normal((m, n))).astype(dtype)
+        F_ref, p_ref, q_ref, info_ref = orc.lu_full(A)
+        dA = dev.to_device(A)
+        p, q, info = dev.getrf_full(dA)
+        F = dev.to_host(dA)
+        assert int(info.item()) == info_ref == 0
+        assert np.array_equal(p.cpu().numpy(), p_ref) and np.array_equal(q.cpu().numpy(), q_ref)
+        assert np.array_equal(F, F_ref)
+        if m == n:
+            B = rng.standard_normal((n, 3)).astype(dtype)
+            for orient in (0, 1):
+                dB = dev.to_device(B)
+                dev.getrs_full(dev.to_device(F_ref), torch.from_numpy(p_ref).cuda(),
+                               torch.from_numpy(q_ref).cuda(), dB, orient=orient)
+                ref = orc.lu_full_solve_after(F_ref, p_ref, q_ref, B, orient=orient)
+                tol = (1e-11 if dtype == np.float64 else 1e-3) * np.abs(ref).max()
+                assert np.abs(dev.to_host(dB) - ref).max() <= tol
+
+
+def test_lu_full_pivoting_singular_and_ties(dev, orc):
+    u = np.array([1.0, -2.0, 4.0, 0.5])
+    v = np.array([2.0, 1.0, -8.0, 0.5, 3.0])
+    for A in (np.asfortranarray(np.outer(u, v)), np.ones((6, 6), order="F"),
+              np.zeros((5, 5), order="F")):
+        F_ref, p_ref, q_ref, info_ref = orc.lu_full(A)
+        dA = dev.to_device(A)
+        p, q, info = dev.getrf_full(dA)
+        assert int(info.item()) == info_ref > 0
+        k = info_ref
+        assert np.array_equal(p.cpu().numpy()[:k], p_ref[:k])
+        assert np.array_equal(q.cpu().numpy()[:k], q_ref[:k])
+        assert np.array_equal(dev.to_host(dA), F_ref)  # A as the reference leaves it

@@ -135,6 +135,26 @@ def test_lu_single_panel_bitwise(dev, orc):
         check_lu(dev, orc, A, nb, expect_exact=True)
 
 
+def test_lu_one_xcd_panels(dev, orc):
+    """Blocksize() <= 32: the panels run on the CUs of one XCD and exchange their candidates
+    through its L2 (up to 1024 rows per CU). Same bars: single panels bit for bit, blocked
+    factorizations with identical pivots."""
+    orc.rng_seed(18)
+    for (m, n, nb) in [(40, 40, 32), (1000, 32, 32), (17000, 16, 16), (32768, 32, 32)]:
+        say("one-XCD panel bitwise", m, n, nb)
+        A = orc.gaussian(m, n)
+        check_lu(dev, orc, A, nb, expect_exact=True)
+        F_ref, d_ref, _ = orc.lu(A.astype(np.float32), bsize=nb)
+        dA = dev.to_device(A.astype(np.float32))
+        d, info = dev.getrf(dA, nb=nb)
+        assert int(info.item()) == 0
+        assert np.array_equal(d.cpu().numpy(), d_ref)
+        assert np.array_equal(dev.to_host(dA), F_ref), "float one-XCD panel bit for bit"
+    for (n, nb) in [(700, 32), (1500, 16)]:
+        say("one-XCD panels, blocked LU", n, nb)
+        check_lu(dev, orc, orc.gaussian(n, n), nb)
+
+
 def test_lu_config1_n1024(dev, orc):
     """Config 1: n=1024, A and b from Elemental's Gaussian stream; pivots, factor error,
     DD backward error of the solve."""

@@ -410,3 +410,71 @@ def test_lu_nopiv_singular_reports_column(orc):
         assert np.array_equal(np.tril(F, -1)[k0:r1, k0:4], np.tril(L, -1)[k0:r1, k0:4]), bs
         if r1 < 9 and k0 < 4:
             assert not np.array_equal(F[r1:, k0:4], L[r1:, k0:4]), bs  # A21 not yet solved
+
+
+# ------------------------------------------------------------------------------------------
+# LU with full pivoting (lu::Full)
+# ------------------------------------------------------------------------------------------
+def _exact_full_pivot_lu(A):
+    """Complete pivoting in exact rational arithmetic with the reference's choice rule (the
+    first entry of largest magnitude in column-major order)."""
+    m, n = A.shape
+    M = [[Fraction(A[i, j]) for j in range(n)] for i in range(m)]
+    p, q = [], []
+    for k in range(min(m, n)):
+        best, bi, bj = Fraction(0), 0, 0
+        for j in range(k, n):
+            for i in range(k, m):
+                if abs(M[i][j]) > best:
+                    best, bi, bj = abs(M[i][j]), i - k, j - k
+        ip, jp = k + bi, k + bj
+        p.append(ip)
+        q.append(jp)
+        M[k], M[ip] = M[ip], M[k]
+        for r in range(m):
+            M[r][k], M[r][jp] = M[r][jp], M[r][k]
+        if M[k][k] == 0:
+            return M, p, q, k + 1
+        for i in range(k + 1, m):
+            M[i][k] = M[i][k] / M[k][k]
+            for j in range(k + 1, n):
+                M[i][j] -= M[i][k] * M[k][j]
+    return M, p, q, 0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lu_full_matches_exact_rational(orc, dtype):
+    rng = np.random.default_rng(70)
+    for (m, n) in [(6, 6), (7, 5), (5, 8)]:
+        A = rng.integers(-9, 10, (m, n)).astype(np.float64)
+        Me, pe, qe, infoe = _exact_full_pivot_lu(A)
+        F, p, q, info = orc.lu_full(A.astype(dtype))
+        assert info == infoe
+        assert list(p) == pe[: len(p)] and list(q) == qe[: len(q)], (m, n)
+        Fe = np.array([[float(x) for x in row] for row in Me])
+        eps = 2.0 ** -53 if dtype == np.float64 else 2.0 ** -24
+        assert np.allclose(F, Fe, rtol=0, atol=64 * max(m, n) * eps * np.abs(Fe).max())
+
+
+def test_lu_full_closed_forms(orc):
+    # diagonal: the pivots take the diagonal in decreasing magnitude, L = I, U = sorted diagonal
+    d = np.array([3.0, -7.0, 0.5, 9.0, -2.0])
+    F, p, q, info = orc.lu_full(np.asfortranarray(np.diag(d)))
+    assert info == 0 and np.array_equal(np.abs(np.diag(F)), np.sort(np.abs(d))[::-1])
+    assert np.array_equal(np.tril(F, -1), np.zeros((5, 5))) and np.array_equal(p, q)
+    # rank one: the Schur complement after one step is exactly zero -> singular at step 2
+    u = np.array([1.0, -2.0, 4.0])
+    v = np.array([2.0, 1.0, -8.0, 0.5])
+    F, p, q, info = orc.lu_full(np.asfortranarray(np.outer(u, v)))
+    assert info == 2 and p[0] == 2 and q[0] == 2  # |4 * -8| = 32 is the largest entry
+    # ties: all ones -> the first entry in column-major order, then singular
+    F, p, q, info = orc.lu_full(np.ones((4, 4), order="F"))
+    assert info == 2 and p[0] == 0 and q[0] == 0
+    # the solve: P A Q = L U
+    rng = np.random.default_rng(71)
+    A = np.asfortranarray(rng.standard_normal((30, 30)))
+    B = np.asfortranarray(rng.standard_normal((30, 3)))
+    F, p, q, info = orc.lu_full(A)
+    for orient in (0, 1):
+        X = orc.lu_full_solve_after(F, p, q, B, orient=orient)
+        assert np.allclose((A if orient == 0 else A.T) @ X, B, atol=1e-11)
