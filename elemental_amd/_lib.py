@@ -94,6 +94,14 @@ _SIGS = {
     "elx_spotrf_vbatched": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp],
     "elx_dpotrs_vbatched": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "elx_spotrs_vbatched": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "elx_dposv_vbatched_graph": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp],
+    "elx_sposv_vbatched_graph": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp],
+    "elx_graph_launch": [c_vp, c_vp],
+    "elx_graph_destroy": [c_vp],
+    "elx_batched_set_variant": [c_int, c_int, c_int, c_int],
+    "elx_batched_get_variant": [c_int, c_int, c_vp, c_vp],
 }
 
 _lib = None

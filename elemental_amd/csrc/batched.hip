@@ -1,36 +1,44 @@
 // Batched variable-size Cholesky factorization and solve (lower, FP64 and FP32).
 //
 // Each matrix runs the blocked lower Cholesky of El::Cholesky (variant 3,
-// src/lapack_like/factor/Cholesky/LowerVariant3.hpp:15-68) with 32-column blocks, in the
-// left-looking order: a block column first receives the updates of every earlier block column
-// (one GEMM with K = the number of earlier columns, on MFMA), then its 32 x 32 diagonal block
-// is factored unblocked (alpha = sqrt(a_jj), a_jj <= 0 -> not HPD, a21 *= 1/alpha, then the
-// dsyr rank-1 update, as LowerVariant3Unblocked does)
-// and the rows below are solved against it (X L11^T = A21, here as the MFMA product of A21
-// with the transposed inverse of L11, which differs from the reference's substitution only
-// in rounding). The solve follows
-// cholesky::SolveAfter (src/lapack_like/factor/Cholesky/SolveAfter.hpp:15-43): L y = b, then
-// L^T x = y, blocked by 32 rows.
+// src/lapack_like/factor/Cholesky/LowerVariant3.hpp:15-68) with NB-column blocks (NB = 16 or
+// 32), in the left-looking order: a block column first receives the updates of every earlier
+// block column (one GEMM with K = the number of earlier columns, on MFMA), then its NB x NB
+// diagonal block is factored unblocked (alpha = sqrt(a_jj), a_jj <= 0 -> not HPD,
+// a21 *= 1/alpha, then the dsyr rank-1 update, as LowerVariant3Unblocked does) and the rows
+// below are solved against it (X L11^T = A21, here as the MFMA product of A21 with the
+// transposed inverse of L11, which differs from the reference's substitution only in
+// rounding). The solve follows cholesky::SolveAfter
+// (src/lapack_like/factor/Cholesky/SolveAfter.hpp:15-43): L y = b, then L^T x = y, blocked by
+// 32 rows, up to 8 right-hand sides at a time.
 //
-// Design for MI355X: one persistent workgroup per CU slot pulls matrices from a work queue in
-// decreasing size (largest first, so the tail of the batch is made of small matrices); the
-// block column being factored is accumulated in MFMA registers and staged in LDS (at most
-// 512 x 33 values), the earlier columns stream from memory once per block column. Matrices
-// are column-major, n <= 512, addressed through a device array of pointers.
+// Design for MI355X: matrices are sorted largest first and split into size bins (n <= 128,
+// <= 256, <= 512). Each bin has its own persistent kernel whose LDS holds one block column of
+// the bin's largest order, so smaller matrices run with more workgroups per CU (3, 2, 1 for
+// FP64): while one workgroup is in the serial diagonal factor, another keeps the matrix cores
+// busy. Within a bin, workgroups pull matrices from a work queue. The block column being
+// factored is accumulated in MFMA registers and staged in LDS; the earlier columns stream from
+// memory once per block column. The block width and the workgroups per CU of each bin come
+// from a measured table (batched_tuning.hpp). A factor + solve sequence can be captured once
+// into a HIP graph and replayed (elx_?posv_vbatched_graph).
 #include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <vector>
 
+#include "batched_tuning.hpp"
 #include "elx_common.hpp"
 
 namespace {
 
-constexpr int BNB = 32;     // block columns
 constexpr int BNT = 256;    // threads per workgroup
 constexpr int BNMAX = 512;  // largest order
-constexpr int BLD = BNB + 1;
+constexpr int NBIN = 3;
+constexpr int kBinMax[NBIN] = {128, 256, 512};
+constexpr int kQueueFactor = 0;  // queue words: one per factor bin, then the solve's
+constexpr int kQueueSolve = NBIN;
+constexpr int kQueueWords = 8;
 
 template <typename T>
 struct Mma;
@@ -101,35 +109,41 @@ __device__ __forceinline__ void stage_cols(const T* A, int lda, int k0, int m, i
     }
 }
 
-// One workgroup: the lower Cholesky factor of one matrix, in place.
-template <typename T>
-__device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* Minv) {
+// One workgroup: the lower Cholesky factor of one matrix of order n <= NM, in place. P is
+// NM x (NB+1) values of LDS (the block column, also the staging area), Minv (NB+1) x NB.
+template <typename T, int NM, int NB>
+__device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Minv) {
     typedef typename Mma<T>::v4 v4;
-    constexpr int MT = BNMAX / 16 / 4;  // row tiles per wave
-    constexpr int PCAP = BNMAX * BLD;   // LDS values of P, reused as the staging area
+    constexpr int MT = NM / 64;     // row tiles per wave (NM/16 tiles over 4 waves)
+    constexpr int CT = NB / 16;     // column tiles of a block column
+    constexpr int LD = NB + 1;      // odd row stride of P and Minv
+    constexpr int LPC = BNT / NB;   // lanes per column of the inverse of L11
+    constexpr int RPL = NB / LPC;   // rows per lane of that inverse
+    constexpr int SCAP = NM * LD;   // staging capacity (values)
+    constexpr int PV = NM <= 128 ? 16 : 32;  // staged values per thread per chunk
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int lr = lane & 15, lk = lane >> 4;
     T* S = P;
     __shared__ int s_fail;
-    for (int k0 = 0; k0 < n; k0 += BNB) {
-        const int kb = min(BNB, n - k0), m = n - k0;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0), m = n - k0;
         const int nrt = (m + 15) >> 4;
         const int SM = nrt * 16;
-        // chunk width: at most 32 staged values per thread (held in registers while the
-        // previous chunk is multiplied), a multiple of 4 columns
-        const int kcw = min(256, max(4, ((32 * BNT) / SM) & ~3));
+        // chunk width: at most PV staged values per thread and the staging capacity, a
+        // multiple of 4 columns
+        const int kcw = min(256, max(4, (min(PV * BNT, SCAP) / SM) & ~3));
         const int cq = BNT / m, cr = BNT - cq * m;  // BNT = cq*m + cr: the walk over (row, column)
         // (1) C = A(k0:n, k0:k0+kb) - L(k0:n, 0:k0) L(k0:k0+kb, 0:k0)^T: the block column and
         // then chunks of L to its left pass through LDS; wave w accumulates row tiles w, w+4, ...
         // (all of them at once, in MFMA registers). Rows >= m of a staged tile and columns
         // >= kb of the block only reach accumulator entries that are never used.
-        stage_cols(A, lda, k0, m, k0, kb, BNB, SM, S);
+        stage_cols(A, lda, k0, m, k0, kb, NB, SM, S);
         __syncthreads();
-        v4 acc[MT][2];
+        v4 acc[MT][CT];
 #pragma unroll
         for (int u = 0; u < MT; ++u)
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
+            for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = (wave + 4 * u) * 16 + Mma<T>::drow(lane, r);
@@ -137,12 +151,12 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
                     acc[u][ct][r] = (row < m) ? S[col * SM + row] : T(0);
                 }
         // chunk pipeline: the loads of chunk i+1 are in flight while chunk i is multiplied
-        T pv[32];
+        T pv[PV];
         auto load_chunk = [&](int kc, int w) {
             const int total = m * w;
             int kk = t / m, r = t - kk * m;
 #pragma unroll
-            for (int u = 0; u < 32; ++u) {
+            for (int u = 0; u < PV; ++u) {
                 const int idx = t + u * BNT;
                 pv[u] = (idx < total) ? A[(k0 + r) + (size_t)(kc + kk) * lda] : T(0);
                 r += cr;
@@ -157,7 +171,7 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
             const int total = m * w;
             int kk = t / m, r = t - kk * m;
 #pragma unroll
-            for (int u = 0; u < 32; ++u) {
+            for (int u = 0; u < PV; ++u) {
                 const int idx = t + u * BNT;
                 if (idx < total) S[kk * SM + r] = pv[u];
                 r += cr;
@@ -181,14 +195,16 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
 #pragma unroll 4
             for (int kk = 0; kk < w; kk += 4) {
                 const T* Sk = S + (kk + lk) * SM;
-                const T b0 = Sk[lr], b1 = Sk[16 + lr];
+                T b[CT];
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) b[ct] = Sk[ct * 16 + lr];
 #pragma unroll
                 for (int u = 0; u < MT; ++u) {
                     const int rt = wave + 4 * u;
                     if (rt < nrt) {
                         const T a = -Sk[rt * 16 + lr];
-                        acc[u][0] = Mma<T>::mma(a, b0, acc[u][0]);
-                        acc[u][1] = Mma<T>::mma(a, b1, acc[u][1]);
+#pragma unroll
+                        for (int ct = 0; ct < CT; ++ct) acc[u][ct] = Mma<T>::mma(a, b[ct], acc[u][ct]);
                     }
                 }
             }
@@ -198,24 +214,24 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
 #pragma unroll
         for (int u = 0; u < MT; ++u)
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
+            for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = (wave + 4 * u) * 16 + Mma<T>::drow(lane, r);
                     const int col = ct * 16 + lr;
-                    if (row < m) P[row * BLD + col] = acc[u][ct][r];
+                    if (row < m) P[row * LD + col] = acc[u][ct][r];
                 }
         __syncthreads();
         // (2) the diagonal block, unblocked, by wave 0: lane i holds row i. Each step
         // broadcasts the pivot and the scaled column through the scalar unit; the dsyr update
-        // is a select, not a branch.
+        // is a select, not a branch. The factored block stays in rows 0..kb-1 of P (L11).
         if (wave == 0) {
-            T rw[BNB];
+            T rw[NB];
 #pragma unroll
-            for (int c = 0; c < BNB; ++c) rw[c] = (lane < kb) ? P[lane * BLD + c] : T(0);
+            for (int c = 0; c < NB; ++c) rw[c] = (lane < kb) ? P[lane * LD + c] : T(0);
             int fail = -1;
 #pragma unroll
-            for (int j = 0; j < BNB; ++j) {
+            for (int j = 0; j < NB; ++j) {
                 if (j < kb && fail < 0) {
                     const T ajj = bcast(rw[j], j);
                     if (ajj <= T(0)) {
@@ -230,42 +246,39 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
                         // only change the sign of a zero entry.
                         const T xi = rw[j];
 #pragma unroll
-                        for (int l = j + 1; l < BNB; ++l) rw[l] = __builtin_fma(xi, -bcast(rw[j], l), rw[l]);
+                        for (int l = j + 1; l < NB; ++l) rw[l] = __builtin_fma(xi, -bcast(rw[j], l), rw[l]);
                     }
                 }
             }
             if (lane < kb)
 #pragma unroll
-                for (int c = 0; c < BNB; ++c) {
-                    P[lane * BLD + c] = rw[c];
-                    Ls[lane * BLD + c] = rw[c];
-                }
+                for (int c = 0; c < NB; ++c) P[lane * LD + c] = rw[c];
             if (lane == 0) s_fail = fail;
         }
         __syncthreads();
         const int fail = s_fail;
-        if (fail < 0 && m > BNB) {
-            // inv(L11) (rows below exist, so kb == 32) by all threads: 8 lanes per column c,
-            // lane g of the group holds rows g, g+8, g+16, g+24 of y = L11^{-1} e_c, formed in
-            // the column order (y_k /= l_kk, then y_i -= l_ik y_k); stored as row c of Minv, so
+        if (fail < 0 && m > NB) {
+            // inv(L11) (rows below exist, so kb == NB) by all threads: LPC lanes per column c,
+            // lane g of the group holds rows g, g+LPC, ... of y = L11^{-1} e_c, formed in the
+            // column order (y_k /= l_kk, then y_i -= l_ik y_k); stored as row c of Minv, so
             // Minv(k, j) = inv(L11)(j, k)
-            const int c = t >> 3, g = t & 7;
-            T y[4];
+            const int c = t / LPC, g = t % LPC;
+            T y[RPL];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) y[q] = (g + 8 * q == c) ? T(1) : T(0);
+            for (int q = 0; q < RPL; ++q) y[q] = (g + LPC * q == c) ? T(1) : T(0);
 #pragma unroll
-            for (int k = 0; k < BNB; ++k) {
-                const int qk = k >> 3, gk = k & 7;
-                if (g == gk) y[qk] = y[qk] / Ls[k * BLD + k];
-                const T yk = __shfl(y[qk], (lane & ~7) | gk);
+            for (int k = 0; k < NB; ++k) {
+                const int qk = k / LPC, gk = k % LPC;
+                if (g == gk) y[qk] = y[qk] / P[k * LD + k];
+                const T yk = __shfl(y[qk], (lane & ~(LPC - 1)) | gk);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int i = g + 8 * q;
-                    if (i > k) y[q] = __builtin_fma(-Ls[i * BLD + k], yk, y[q]);
+                for (int q = 0; q < RPL; ++q) {
+                    const int i = g + LPC * q;
+                    if (i > k) y[q] = __builtin_fma(-P[i * LD + k], yk, y[q]);
                 }
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) Minv[c * BLD + g + 8 * q] = y[q];
+            for (int q = 0; q < RPL; ++q) Minv[c * LD + g + LPC * q] = y[q];
             __syncthreads();
         }
         if (fail >= 0) {
@@ -275,27 +288,26 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
             if (t == 0) *info = k0 + fail + 1;
             for (int idx = t; idx < m * kb; idx += BNT) {
                 const int i = idx % m, c = idx / m;
-                if (i >= c) A[(k0 + i) + (size_t)(k0 + c) * lda] = P[i * BLD + c];
+                if (i >= c) A[(k0 + i) + (size_t)(k0 + c) * lda] = P[i * LD + c];
             }
             __syncthreads();
             return;
         }
-        // (3) rows below (present only when kb == 32): X = A21 L11^{-T} as one MFMA product
+        // (3) rows below (present only when kb == NB): X = A21 L11^{-T} as one MFMA product
         // with the inverse, Minv(k, j) = inv(L11)(j, k); results go straight to A
-        for (int rt = 2 + wave; rt < nrt; rt += 4) {
-            v4 acc2[2];
+        for (int rt = CT + wave; rt < nrt; rt += 4) {
+            v4 acc2[CT];
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) acc2[ct] = v4{T(0), T(0), T(0), T(0)};
+            for (int ct = 0; ct < CT; ++ct) acc2[ct] = v4{T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int kk = 0; kk < BNB; kk += 4) {
-                const T a = P[(rt * 16 + lr) * BLD + kk + lk];
-                const T b0 = Minv[(kk + lk) * BLD + lr];
-                const T b1 = Minv[(kk + lk) * BLD + 16 + lr];
-                acc2[0] = Mma<T>::mma(a, b0, acc2[0]);
-                acc2[1] = Mma<T>::mma(a, b1, acc2[1]);
+            for (int kk = 0; kk < NB; kk += 4) {
+                const T a = P[(rt * 16 + lr) * LD + kk + lk];
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    acc2[ct] = Mma<T>::mma(a, Minv[(kk + lk) * LD + ct * 16 + lr], acc2[ct]);
             }
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
+            for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = rt * 16 + Mma<T>::drow(lane, r);
@@ -305,39 +317,64 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Ls, T* M
         // the diagonal block's lower triangle
         for (int idx = t; idx < kb * kb; idx += BNT) {
             const int i = idx % kb, c = idx / kb;
-            if (i >= c) A[(k0 + i) + (size_t)(k0 + c) * lda] = P[i * BLD + c];
+            if (i >= c) A[(k0 + i) + (size_t)(k0 + c) * lda] = P[i * LD + c];
         }
         __syncthreads();
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(BNT) void potrf_vbatched_kernel(const BatchItem* __restrict__ items,
-                                                             int count, T* const* __restrict__ As,
-                                                             int32_t* __restrict__ info,
-                                                             int* queue) {
+template <int NM>
+constexpr int bin_occ_max() {
+    return NM <= 128 ? 3 : (NM <= 256 ? 2 : 1);
+}
+
+// Persistent workgroups of one size bin: items[begin, end) (orders <= NM), pulled through the
+// bin's queue word.
+template <typename T, int NM, int NB>
+__global__ __launch_bounds__(BNT, bin_occ_max<NM>()) void potrf_vbatched_kernel(
+    const BatchItem* __restrict__ items, int begin, int end, T* const* __restrict__ As,
+    int32_t* __restrict__ info, int* queue) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* P = reinterpret_cast<T*>(smem_raw);  // BNMAX x BLD
-    T* Ls = P + BNMAX * BLD;                // BNB x BLD: L11
-    T* Minv = Ls + BNB * BLD;               // BNB x BLD: inv(L11)^T
+    T* P = reinterpret_cast<T*>(smem_raw);  // NM x (NB+1)
+    T* Minv = P + NM * (NB + 1);            // NB x (NB+1)
     __shared__ int s_i;
     for (;;) {
-        if (threadIdx.x == 0) s_i = atomicAdd(queue, 1);
+        if (threadIdx.x == 0) s_i = begin + atomicAdd(queue, 1);
         __syncthreads();
         const int i = s_i;
         __syncthreads();
-        if (i >= count) break;
+        if (i >= end) break;
         const BatchItem it = items[i];
         if (threadIdx.x == 0) info[it.index] = 0;
-        if (it.n > 0) potrf_one<T>(it.n, As[it.index], it.lda, info + it.index, P, Ls, Minv);
+        if (it.n > 0) potrf_one<T, NM, NB>(it.n, As[it.index], it.lda, info + it.index, P, Minv);
     }
 }
 
-// One workgroup: L L^T X = B for one matrix (L from potrf), nrhs columns at a time in LDS.
 template <typename T>
-__device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T* Y, T* Ls) {
-    const int t = threadIdx.x;
-    constexpr int NR = 8;
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One workgroup: L L^T X = B for one matrix (L from potrf), up to NR right-hand sides at a time
+// in LDS, blocked by SB = 32 rows. Forward: each block's rows first receive the products with
+// the solved part (256 threads as 32 rows x 8 column groups, coalesced down the columns of L),
+// then wave 0 solves the diagonal block with its rows in registers. Backward: each block
+// column's dot products with the solved part below (a wave per column, lanes down the column),
+// then wave 0 solves the transposed diagonal block with its columns in registers.
+constexpr int NR = 8, SB = 32, SLD = SB + 1;
+template <typename T>
+__device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T* Y, T* Part,
+                          T* Lb) {
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int nblk = (n + SB - 1) / SB;
+    auto stage_l11 = [&](int k0, int kb) {
+        for (int idx = t; idx < kb * kb; idx += BNT) {
+            const int i = idx % kb, j = idx / kb;
+            Lb[i * SLD + j] = (i >= j) ? L[(k0 + i) + (size_t)(k0 + j) * lda] : T(0);
+        }
+    };
     for (int c0 = 0; c0 < nrhs; c0 += NR) {
         const int nr = min(NR, nrhs - c0);
         for (int idx = t; idx < n * nr; idx += BNT) {
@@ -345,64 +382,106 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
             Y[c * BNMAX + i] = B[i + (size_t)(c0 + c) * ldb];
         }
         __syncthreads();
-        // forward: L y = b (dtrsm Left Lower NoTrans NonUnit: y_k /= l_kk, y_i -= y_k l_ik)
-        for (int k0 = 0; k0 < n; k0 += BNB) {
-            const int kb = min(BNB, n - k0);
-            for (int idx = t; idx < kb * kb; idx += BNT) {
-                const int i = idx % kb, c = idx / kb;
-                Ls[i * BLD + c] = L[(k0 + i) + (size_t)(k0 + c) * lda];
-            }
-            __syncthreads();
-            if (t < nr) {
-                T* y = Y + t * BNMAX + k0;
-                for (int k = 0; k < kb; ++k) {
-                    if (y[k] != T(0)) {
-                        y[k] = y[k] / Ls[k * BLD + k];
-                        for (int i = k + 1; i < kb; ++i) y[i] = __builtin_fma(-y[k], Ls[i * BLD + k], y[i]);
-                    }
-                }
-            }
-            __syncthreads();
-            for (int i = k0 + kb + t; i < n; i += BNT) {
+        // forward: L y = b (dtrsm Left Lower NoTrans NonUnit)
+        for (int b = 0; b < nblk; ++b) {
+            const int k0 = b * SB, kb = min(SB, n - k0);
+            {
+                const int r = t & 31, g = t >> 5;
                 T acc[NR];
 #pragma unroll
-                for (int c = 0; c < NR; ++c) acc[c] = (c < nr) ? Y[c * BNMAX + i] : T(0);
-                for (int k = 0; k < kb; ++k) {
-                    const T lik = L[i + (size_t)(k0 + k) * lda];
+                for (int q = 0; q < NR; ++q) acc[q] = T(0);
+                if (r < kb) {
+#pragma unroll 4
+                    for (int c = g; c < k0; c += 8) {
+                        const T l = L[(k0 + r) + (size_t)c * lda];
 #pragma unroll
-                    for (int c = 0; c < NR; ++c)
-                        if (c < nr) acc[c] = __builtin_fma(-Y[c * BNMAX + k0 + k], lik, acc[c]);
+                        for (int q = 0; q < NR; ++q)
+                            if (q < nr) acc[q] = __builtin_fma(l, Y[q * BNMAX + c], acc[q]);
+                    }
                 }
 #pragma unroll
-                for (int c = 0; c < NR; ++c)
-                    if (c < nr) Y[c * BNMAX + i] = acc[c];
+                for (int q = 0; q < NR; ++q) Part[(g * SB + r) * NR + q] = acc[q];
+            }
+            stage_l11(k0, kb);
+            __syncthreads();
+            if (wave == 0) {
+                T lrow[SB], yv[NR];
+#pragma unroll
+                for (int j = 0; j < SB; ++j) lrow[j] = (lane < kb) ? Lb[lane * SLD + j] : T(0);
+#pragma unroll
+                for (int q = 0; q < NR; ++q) {
+                    T s = T(0);
+                    if (lane < kb && q < nr) {
+                        s = Y[q * BNMAX + k0 + lane];
+#pragma unroll
+                        for (int g = 0; g < 8; ++g) s -= Part[(g * SB + lane) * NR + q];
+                    }
+                    yv[q] = s;
+                }
+#pragma unroll
+                for (int k = 0; k < SB; ++k) {
+                    if (k < kb) {
+#pragma unroll
+                        for (int q = 0; q < NR; ++q) {
+                            if (lane == k) yv[q] = yv[q] / lrow[k];
+                            const T yk = __shfl(yv[q], k);
+                            if (lane > k) yv[q] = __builtin_fma(-lrow[k], yk, yv[q]);
+                        }
+                    }
+                }
+                if (lane < kb)
+#pragma unroll
+                    for (int q = 0; q < NR; ++q)
+                        if (q < nr) Y[q * BNMAX + k0 + lane] = yv[q];
             }
             __syncthreads();
         }
-        // backward: L^T x = y (dtrsm Left Lower Trans NonUnit: dot form, descending rows)
-        const int nblk = (n + BNB - 1) / BNB;
+        // backward: L^T x = y (dtrsm Left Lower Trans NonUnit), blocks from the bottom
         for (int b = nblk - 1; b >= 0; --b) {
-            const int k0 = b * BNB, kb = min(BNB, n - k0);
-            // y_blk(j) -= sum over rows below the block of l(i, k0+j) x(i)
-            for (int pr = t; pr < kb * nr; pr += BNT) {
-                const int j = pr % kb, c = pr / kb;
-                T s = T(0);
-                for (int i = k0 + kb; i < n; ++i)
-                    s = __builtin_fma(L[i + (size_t)(k0 + j) * lda], Y[c * BNMAX + i], s);
-                Y[c * BNMAX + k0 + j] -= s;
-            }
-            for (int idx = t; idx < kb * kb; idx += BNT) {
-                const int i = idx % kb, c = idx / kb;
-                Ls[i * BLD + c] = L[(k0 + i) + (size_t)(k0 + c) * lda];
-            }
-            __syncthreads();
-            if (t < nr) {
-                T* y = Y + t * BNMAX + k0;
-                for (int i = kb - 1; i >= 0; --i) {
-                    T v = y[i];
-                    for (int k = i + 1; k < kb; ++k) v = __builtin_fma(-Ls[k * BLD + i], y[k], v);
-                    y[i] = v / Ls[i * BLD + i];
+            const int k0 = b * SB, kb = min(SB, n - k0), r0 = k0 + kb;
+            for (int j = wave; j < kb; j += 4) {
+                T acc[NR];
+#pragma unroll
+                for (int q = 0; q < NR; ++q) acc[q] = T(0);
+                const T* col = L + (size_t)(k0 + j) * lda;
+#pragma unroll 2
+                for (int i = r0 + lane; i < n; i += 64) {
+                    const T l = col[i];
+#pragma unroll
+                    for (int q = 0; q < NR; ++q)
+                        if (q < nr) acc[q] = __builtin_fma(l, Y[q * BNMAX + i], acc[q]);
                 }
+#pragma unroll
+                for (int q = 0; q < NR; ++q) acc[q] = wave_sum(acc[q]);
+                if (lane == 0)
+#pragma unroll
+                    for (int q = 0; q < NR; ++q) Part[j * NR + q] = acc[q];
+            }
+            stage_l11(k0, kb);
+            __syncthreads();
+            if (wave == 0) {
+                // lane k holds column k of L11: L(i, k) for every i
+                T lcol[SB], xv[NR];
+#pragma unroll
+                for (int i = 0; i < SB; ++i) lcol[i] = (lane < kb) ? Lb[i * SLD + lane] : T(0);
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    xv[q] = (lane < kb && q < nr) ? Y[q * BNMAX + k0 + lane] - Part[lane * NR + q] : T(0);
+#pragma unroll
+                for (int i = SB - 1; i >= 0; --i) {
+                    if (i < kb) {
+#pragma unroll
+                        for (int q = 0; q < NR; ++q) {
+                            if (lane == i) xv[q] = xv[q] / lcol[i];
+                            const T xi = __shfl(xv[q], i);
+                            if (lane < i) xv[q] = __builtin_fma(-lcol[i], xi, xv[q]);
+                        }
+                    }
+                }
+                if (lane < kb)
+#pragma unroll
+                    for (int q = 0; q < NR; ++q)
+                        if (q < nr) Y[q * BNMAX + k0 + lane] = xv[q];
             }
             __syncthreads();
         }
@@ -414,12 +493,14 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
     }
 }
 
+// info (may be null): matrices whose factorization failed are left unsolved.
 template <typename T>
-__global__ __launch_bounds__(BNT) void potrs_vbatched_kernel(const BatchItem* __restrict__ items,
-                                                             int count, const T* const* __restrict__ Ls,
-                                                             T* const* __restrict__ Bs, int* queue) {
-    __shared__ T Y[8 * BNMAX];
-    __shared__ T Lsm[BNB * BLD];
+__global__ __launch_bounds__(BNT, 2) void potrs_vbatched_kernel(
+    const BatchItem* __restrict__ items, int count, const T* const* __restrict__ Ls,
+    T* const* __restrict__ Bs, const int32_t* __restrict__ info, int* queue) {
+    __shared__ T Y[NR * BNMAX];
+    __shared__ T Part[8 * SB * NR];
+    __shared__ T Lb[SB * SLD];
     __shared__ int s_i;
     for (;;) {
         if (threadIdx.x == 0) s_i = atomicAdd(queue, 1);
@@ -428,8 +509,9 @@ __global__ __launch_bounds__(BNT) void potrs_vbatched_kernel(const BatchItem* __
         __syncthreads();
         if (i >= count) break;
         const BatchItem it = items[i];
+        if (info && info[it.index] != 0) continue;
         if (it.n > 0 && it.nrhs > 0)
-            potrs_one<T>(it.n, Ls[it.index], it.lda, Bs[it.index], it.ldb, it.nrhs, Y, Lsm);
+            potrs_one<T>(it.n, Ls[it.index], it.lda, Bs[it.index], it.ldb, it.nrhs, Y, Part, Lb);
     }
 }
 
@@ -438,18 +520,31 @@ __global__ void zero_ints(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Per-device schedules (sorted items) for the factor and the solve, kept between calls: a call
-// with the same sizes and leading dimensions as the previous one of its kind reuses the
-// device schedule without sorting or copying. The host staging buffer is pinned, and its
-// previous copy is waited for before it is refilled.
+// ------------------------------------------------------------------------- host side
+// A schedule: the valid items in decreasing order of n (ties by index) on the device, and
+// the item ranges of the size bins.
 struct Schedule {
     std::vector<int32_t> key;
     BatchItem* dev = nullptr;
-    BatchItem* host = nullptr;  // pinned
+    BatchItem* host = nullptr;  // pinned staging (cached schedules)
     size_t cap = 0;
     int valid = 0;
+    int bin_end[NBIN] = {0, 0, 0};  // bin_end[b]: number of items of order > kBinMax[b-1]
     hipEvent_t copied = nullptr;
+    double flops = 0.0, bytes = 0.0;
 };
+
+// Items of bin b (n in (kBinMax[b-1], kBinMax[b]]) in the descending order: [lo, hi).
+void bin_range(const Schedule& sc, int b, int* lo, int* hi) {
+    // bins in the sorted order: 512, 256, 128
+    *lo = (b == NBIN - 1) ? 0 : sc.bin_end[b + 1];
+    *hi = sc.bin_end[b];
+}
+
+// Per-device cached schedules (factor, solve) and work queue, kept between calls: a call with
+// the same sizes and leading dimensions as the previous one of its kind reuses the device
+// schedule without sorting or copying. The host staging buffer is pinned, and its previous
+// copy is waited for before it is refilled.
 struct BatchArea {
     Schedule sched[2];  // 0: factor, 1: solve
     int* queue = nullptr;
@@ -457,14 +552,25 @@ struct BatchArea {
 std::mutex g_bmu;
 std::map<int, BatchArea> g_batch;
 
+// Variant overrides set by elx_batched_set_variant (the autotuner); nb 0 = the table's.
+elx::tuned::BatchedVariant g_override[2][NBIN] = {};
+
 elx_status_t batch_area(BatchArea** out) {
     int dev = 0;
     ELX_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_bmu);
     BatchArea& a = g_batch[dev];
-    if (!a.queue) ELX_HIP(hipMalloc(&a.queue, sizeof(int) * 64));
+    if (!a.queue) ELX_HIP(hipMalloc(&a.queue, sizeof(int) * kQueueWords));
     *out = &a;
     return ELX_SUCCESS;
+}
+
+void free_schedule(Schedule& sc) {
+    if (sc.copied) (void)hipEventSynchronize(sc.copied);
+    if (sc.dev) (void)hipFree(sc.dev);
+    if (sc.host) (void)hipHostFree(sc.host);
+    if (sc.copied) (void)hipEventDestroy(sc.copied);
+    sc = Schedule();
 }
 
 }  // namespace
@@ -478,13 +584,7 @@ void batched_release() {
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(kv.first);
         BatchArea& a = kv.second;
-        for (Schedule& sc : a.sched) {
-            if (sc.copied) (void)hipEventSynchronize(sc.copied);
-            if (sc.dev) (void)hipFree(sc.dev);
-            if (sc.host) (void)hipHostFree(sc.host);
-            if (sc.copied) (void)hipEventDestroy(sc.copied);
-            sc = Schedule();
-        }
+        for (Schedule& sc : a.sched) free_schedule(sc);
         if (a.queue) (void)hipFree(a.queue);
         a.queue = nullptr;
         (void)hipSetDevice(cur);
@@ -495,9 +595,11 @@ void batched_release() {
 
 namespace {
 
-// Valid items in decreasing order of n (ties by index), on the device.
+// Sorts the valid items, fills sc (device copy through the pinned buffer when `cached`,
+// else a synchronous copy from a temporary host vector).
 elx_status_t schedule(hipStream_t s, Schedule& sc, int64_t count, const int32_t* n,
-                      const int32_t* lda, const int32_t* nrhs, const int32_t* ldb) {
+                      const int32_t* lda, const int32_t* nrhs, const int32_t* ldb, size_t tsize,
+                      bool cached) {
     std::vector<int32_t> key;
     key.reserve((size_t)count * 4);
     for (int64_t i = 0; i < count; ++i) {
@@ -508,40 +610,133 @@ elx_status_t schedule(hipStream_t s, Schedule& sc, int64_t count, const int32_t*
         key.push_back(nrhs ? nrhs[i] : 0);
         key.push_back(ldb ? ldb[i] : 0);
     }
-    if (sc.dev && key == sc.key) return ELX_SUCCESS;
+    if (cached && sc.dev && key == sc.key) return ELX_SUCCESS;
     if (sc.copied) ELX_HIP(hipEventSynchronize(sc.copied));
     if (sc.cap < (size_t)count) {
         if (sc.dev) {
             ELX_HIP(hipDeviceSynchronize());
             ELX_HIP(hipFree(sc.dev));
-            ELX_HIP(hipHostFree(sc.host));
+            if (sc.host) ELX_HIP(hipHostFree(sc.host));
             sc.dev = nullptr;
             sc.host = nullptr;
         }
-        const size_t cap = std::max<size_t>((size_t)count, 1024);
+        const size_t cap = std::max<size_t>((size_t)count, cached ? 1024 : 1);
         ELX_HIP(hipMalloc(&sc.dev, sizeof(BatchItem) * cap));
-        ELX_HIP(hipHostMalloc(reinterpret_cast<void**>(&sc.host), sizeof(BatchItem) * cap, 0));
+        if (cached)
+            ELX_HIP(hipHostMalloc(reinterpret_cast<void**>(&sc.host), sizeof(BatchItem) * cap, 0));
         sc.cap = cap;
     }
-    if (!sc.copied) ELX_HIP(hipEventCreateWithFlags(&sc.copied, hipEventDisableTiming));
     std::vector<int32_t> order((size_t)count);
     for (int64_t i = 0; i < count; ++i) order[(size_t)i] = (int32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return n[x] > n[y]; });
+    std::vector<BatchItem> tmp;
+    BatchItem* h = sc.host;
+    if (!cached) {
+        tmp.resize((size_t)count);
+        h = tmp.data();
+    }
+    sc.flops = 0.0;
+    sc.bytes = 0.0;
+    for (int b = 0; b < NBIN; ++b) sc.bin_end[b] = 0;
     for (int64_t k = 0; k < count; ++k) {
         const int32_t i = order[(size_t)k];
-        BatchItem& it = sc.host[k];
+        BatchItem& it = h[k];
         it.n = n[i];
         it.lda = lda[i];
         it.index = i;
         it.nrhs = nrhs ? nrhs[i] : 0;
         it.ldb = ldb ? ldb[i] : 0;
         it.pad = 0;
+        const double nn = (double)n[i];
+        sc.flops += nn * nn * nn / 3.0;
+        sc.bytes += nn * nn * tsize + 2.0 * nn * (double)it.nrhs * tsize;
+        // in the descending order, the items of order > kBinMax[b-1] are a prefix
+        for (int b = 0; b < NBIN; ++b)
+            if (n[i] > (b == 0 ? -1 : kBinMax[b - 1])) sc.bin_end[b] = (int)(k + 1);
     }
-    ELX_HIP(hipMemcpyAsync(sc.dev, sc.host, sizeof(BatchItem) * (size_t)count,
-                           hipMemcpyHostToDevice, s));
-    ELX_HIP(hipEventRecord(sc.copied, s));
+    if (cached) {
+        if (!sc.copied) ELX_HIP(hipEventCreateWithFlags(&sc.copied, hipEventDisableTiming));
+        ELX_HIP(hipMemcpyAsync(sc.dev, sc.host, sizeof(BatchItem) * (size_t)count,
+                               hipMemcpyHostToDevice, s));
+        ELX_HIP(hipEventRecord(sc.copied, s));
+    } else if (count > 0) {
+        ELX_HIP(hipMemcpy(sc.dev, h, sizeof(BatchItem) * (size_t)count, hipMemcpyHostToDevice));
+    }
     sc.key.swap(key);
     sc.valid = (int)count;
+    return ELX_SUCCESS;
+}
+
+template <typename T>
+elx::tuned::BatchedVariant variant_of(int bin) {
+    const int d = sizeof(T) == 8 ? 0 : 1;
+    const elx::tuned::BatchedVariant o = g_override[d][bin];
+    return o.nb ? o : elx::tuned::kBatchedPotrf[d][bin];
+}
+
+template <typename T, int NM, int NB>
+elx_status_t launch_bin(hipStream_t s, const BatchItem* items, int lo, int hi, T* const* dA,
+                        int32_t* dinfo, int* queue, int occ) {
+    const void* fn = reinterpret_cast<const void*>(&potrf_vbatched_kernel<T, NM, NB>);
+    const size_t lds = sizeof(T) * ((size_t)NM * (NB + 1) + (size_t)NB * (NB + 1));
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+        attr = true;
+    }
+    int per_cu = 0;
+    ELX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BNT, lds));
+    per_cu = std::max(1, std::min(per_cu, occ));
+    const elx::DeviceInfo& di = elx::device_info();
+    const int G = (int)std::min<int64_t>(hi - lo, (int64_t)per_cu * di.num_cus);
+    hipLaunchKernelGGL((potrf_vbatched_kernel<T, NM, NB>), dim3(G), dim3(BNT), lds, s, items, lo,
+                       hi, dA, dinfo, queue);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
+template <typename T>
+elx_status_t launch_factor(hipStream_t s, const Schedule& sc, T* const* dA, int32_t* dinfo,
+                           int* queue) {
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, queue + kQueueFactor, NBIN);
+    ELX_HIP(hipGetLastError());
+    elx::ProfScope prof_(s, elx::kKBatched, sc.flops);
+    for (int b = NBIN - 1; b >= 0; --b) {  // largest orders first
+        int lo = 0, hi = 0;
+        bin_range(sc, b, &lo, &hi);
+        if (hi <= lo) continue;
+        const elx::tuned::BatchedVariant v = variant_of<T>(b);
+        int* q = queue + kQueueFactor + b;
+        const BatchItem* it = sc.dev;
+        if (b == 0) {
+            if (v.nb == 16) ELX_CHECK((launch_bin<T, 128, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else ELX_CHECK((launch_bin<T, 128, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+        } else if (b == 1) {
+            if (v.nb == 16) ELX_CHECK((launch_bin<T, 256, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else ELX_CHECK((launch_bin<T, 256, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+        } else {
+            if (v.nb == 16) ELX_CHECK((launch_bin<T, 512, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else ELX_CHECK((launch_bin<T, 512, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+        }
+    }
+    return ELX_SUCCESS;
+}
+
+template <typename T>
+elx_status_t launch_solve(hipStream_t s, const Schedule& sc, const T* const* dA, T* const* dB,
+                          const int32_t* dinfo, int* queue) {
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, queue + kQueueSolve, 1);
+    ELX_HIP(hipGetLastError());
+    int per_cu = 0;
+    ELX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&potrs_vbatched_kernel<T>), BNT, 0));
+    const elx::DeviceInfo& di = elx::device_info();
+    const int G = (int)std::min<int64_t>(sc.valid, (int64_t)std::max(per_cu, 1) * di.num_cus);
+    elx::ProfScope prof_(s, elx::kKBatchedSolve, sc.bytes);
+    hipLaunchKernelGGL(potrs_vbatched_kernel<T>, dim3(G), dim3(BNT), 0, s, sc.dev, sc.valid, dA, dB,
+                       dinfo, queue + kQueueSolve);
+    ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
 
@@ -554,30 +749,8 @@ elx_status_t potrf_vbatched(hipStream_t s, int uplo, int64_t count, const int32_
     ELX_ARG(n && lda && dA && dinfo);
     BatchArea* a = nullptr;
     ELX_CHECK(batch_area(&a));
-    ELX_CHECK(schedule(s, a->sched[0], count, n, lda, nullptr, nullptr));
-    const int valid = a->sched[0].valid;
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, a->queue, 1);
-    ELX_HIP(hipGetLastError());
-    const size_t lds = sizeof(T) * ((size_t)BNMAX * BLD + 2 * BNB * BLD);
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&potrf_vbatched_kernel<T>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-        attr = true;
-    }
-    int per_cu = 0;
-    ELX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&potrf_vbatched_kernel<T>), BNT, lds));
-    const elx::DeviceInfo& di = elx::device_info();
-    const int G = (int)std::min<int64_t>(count, (int64_t)std::max(per_cu, 1) * di.num_cus);
-    double flops = 0.0;
-    for (int64_t i = 0; i < count; ++i) flops += (double)n[i] * n[i] * n[i] / 3.0;
-    elx::ProfScope prof_(s, elx::kKBatched, flops);
-    hipLaunchKernelGGL(potrf_vbatched_kernel<T>, dim3(G), dim3(BNT), lds, s, a->sched[0].dev, valid,
-                       dA, dinfo, a->queue);
-    ELX_HIP(hipGetLastError());
-    return ELX_SUCCESS;
+    ELX_CHECK(schedule(s, a->sched[0], count, n, lda, nullptr, nullptr, sizeof(T), true));
+    return launch_factor<T>(s, a->sched[0], dA, dinfo, a->queue);
 }
 
 template <typename T>
@@ -590,21 +763,63 @@ elx_status_t potrs_vbatched(hipStream_t s, int uplo, int64_t count, const int32_
     ELX_ARG(n && nrhs && lda && ldb && dA && dB);
     BatchArea* a = nullptr;
     ELX_CHECK(batch_area(&a));
-    ELX_CHECK(schedule(s, a->sched[1], count, n, lda, nrhs, ldb));
-    const int valid = a->sched[1].valid;
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, a->queue + 1, 1);
-    ELX_HIP(hipGetLastError());
-    int per_cu = 0;
-    ELX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&potrs_vbatched_kernel<T>), BNT, 0));
-    const elx::DeviceInfo& di = elx::device_info();
-    const int G = (int)std::min<int64_t>(count, (int64_t)std::max(per_cu, 1) * di.num_cus);
-    double bytes = 0.0;
-    for (int64_t i = 0; i < count; ++i) bytes += (double)n[i] * n[i] * sizeof(T) + 2.0 * n[i] * nrhs[i] * sizeof(T);
-    elx::ProfScope prof_(s, elx::kKBatchedSolve, bytes);
-    hipLaunchKernelGGL(potrs_vbatched_kernel<T>, dim3(G), dim3(BNT), 0, s, a->sched[1].dev, valid, dA,
-                       dB, a->queue + 1);
-    ELX_HIP(hipGetLastError());
+    ELX_CHECK(schedule(s, a->sched[1], count, n, lda, nrhs, ldb, sizeof(T), true));
+    return launch_solve<T>(s, a->sched[1], dA, dB, nullptr, a->queue);
+}
+
+}  // namespace
+
+// A captured factor + solve: its own schedules and queue words, so other batched calls can
+// run between replays.
+struct elx_graph_st {
+    Schedule sched;
+    int* queue = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+};
+
+namespace {
+
+template <typename T>
+elx_status_t posv_graph(hipStream_t s, int uplo, int64_t count, const int32_t* n,
+                        const int32_t* nrhs, T* const* dA, const int32_t* lda, T* const* dB,
+                        const int32_t* ldb, int32_t* dinfo, elx_graph_t* out) {
+    ELX_ARG(out != nullptr);
+    *out = nullptr;
+    ELX_ARG(uplo == elx::kLower);
+    ELX_ARG(count >= 1 && count < (1ll << 30));
+    ELX_ARG(n && nrhs && lda && ldb && dA && dB && dinfo);
+    elx_graph_st* g = new elx_graph_st();
+    auto fail = [&](elx_status_t st) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        free_schedule(g->sched);
+        if (g->queue) (void)hipFree(g->queue);
+        delete g;
+        return st;
+    };
+    if (hipGetDevice(&g->device) != hipSuccess) return fail(ELX_ERROR_HIP);
+    // every allocation and copy happens before the capture
+    elx_status_t st = schedule(s, g->sched, count, n, lda, nrhs, ldb, sizeof(T), false);
+    if (st != ELX_SUCCESS) return fail(st);
+    if (hipMalloc(&g->queue, sizeof(int) * kQueueWords) != hipSuccess) return fail(ELX_ERROR_ALLOC);
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return fail(ELX_ERROR_HIP);
+    hipGraph_t graph = nullptr;
+    st = ELX_SUCCESS;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        st = ELX_ERROR_HIP;
+    } else {
+        st = launch_factor<T>(cs, g->sched, dA, dinfo, g->queue);
+        if (st == ELX_SUCCESS)
+            st = launch_solve<T>(cs, g->sched, const_cast<const T* const*>(dA), dB, dinfo, g->queue);
+        if (hipStreamEndCapture(cs, &graph) != hipSuccess && st == ELX_SUCCESS) st = ELX_ERROR_HIP;
+    }
+    if (st == ELX_SUCCESS && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
+        st = ELX_ERROR_HIP;
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    if (st != ELX_SUCCESS) return fail(st);
+    *out = g;
     return ELX_SUCCESS;
 }
 
@@ -636,6 +851,61 @@ elx_status_t elx_spotrs_vbatched(void* stream, int uplo, int64_t count, const in
                                  const int32_t* lda, float* const* dB_array, const int32_t* ldb) {
     return potrs_vbatched<float>(static_cast<hipStream_t>(stream), uplo, count, n, nrhs, dA_array,
                                  lda, dB_array, ldb);
+}
+
+elx_status_t elx_dposv_vbatched_graph(void* stream, int uplo, int64_t count, const int32_t* n,
+                                      const int32_t* nrhs, double* const* dA_array,
+                                      const int32_t* lda, double* const* dB_array,
+                                      const int32_t* ldb, int32_t* dinfo, elx_graph_t* graph) {
+    return posv_graph<double>(static_cast<hipStream_t>(stream), uplo, count, n, nrhs, dA_array, lda,
+                              dB_array, ldb, dinfo, graph);
+}
+
+elx_status_t elx_sposv_vbatched_graph(void* stream, int uplo, int64_t count, const int32_t* n,
+                                      const int32_t* nrhs, float* const* dA_array,
+                                      const int32_t* lda, float* const* dB_array,
+                                      const int32_t* ldb, int32_t* dinfo, elx_graph_t* graph) {
+    return posv_graph<float>(static_cast<hipStream_t>(stream), uplo, count, n, nrhs, dA_array, lda,
+                             dB_array, ldb, dinfo, graph);
+}
+
+elx_status_t elx_graph_launch(elx_graph_t graph, void* stream) {
+    ELX_ARG(graph != nullptr && graph->exec != nullptr);
+    int dev = 0;
+    ELX_HIP(hipGetDevice(&dev));
+    ELX_ARG(dev == graph->device);
+    ELX_HIP(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
+    return ELX_SUCCESS;
+}
+
+elx_status_t elx_graph_destroy(elx_graph_t graph) {
+    if (!graph) return ELX_SUCCESS;
+    ELX_HIP(hipDeviceSynchronize());
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    free_schedule(graph->sched);
+    if (graph->queue) (void)hipFree(graph->queue);
+    delete graph;
+    return ELX_SUCCESS;
+}
+
+elx_status_t elx_batched_set_variant(int dtype, int bin, int nb, int occ) {
+    ELX_ARG(dtype == 0 || dtype == 1);
+    ELX_ARG(bin >= 0 && bin < NBIN);
+    ELX_ARG(nb == 0 || nb == 16 || nb == 32);
+    ELX_ARG(nb == 0 || (occ >= 1 && occ <= 8));
+    std::lock_guard<std::mutex> lk(g_bmu);
+    g_override[dtype][bin] = elx::tuned::BatchedVariant{nb, nb ? occ : 0};
+    return ELX_SUCCESS;
+}
+
+elx_status_t elx_batched_get_variant(int dtype, int bin, int* nb, int* occ) {
+    ELX_ARG(dtype == 0 || dtype == 1);
+    ELX_ARG(bin >= 0 && bin < NBIN && nb && occ);
+    const elx::tuned::BatchedVariant v =
+        dtype == 0 ? variant_of<double>(bin) : variant_of<float>(bin);
+    *nb = v.nb;
+    *occ = v.occ;
+    return ELX_SUCCESS;
 }
 
 }  // extern "C"

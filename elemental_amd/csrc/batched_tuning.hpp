@@ -1,0 +1,21 @@
+// Variant table of the batched Cholesky factorization, per element type and size bin.
+// Written by elemental_amd/autotune_batched.py from timings on one MI355X (see that script for
+// the workloads and the candidates); edit by running the script again, not by hand.
+#pragma once
+
+namespace elx {
+namespace tuned {
+
+struct BatchedVariant {
+    int nb;   // block-column width of the left-looking factorization (16 or 32)
+    int occ;  // persistent workgroups per CU (at most what the bin's LDS and registers allow)
+};
+
+// [0: double, 1: float][bin 0: n <= 128, 1: n <= 256, 2: n <= 512]
+constexpr BatchedVariant kBatchedPotrf[2][3] = {
+    {{32, 3}, {32, 2}, {32, 1}},
+    {{32, 3}, {32, 2}, {32, 1}},
+};
+
+}  // namespace tuned
+}  // namespace elx

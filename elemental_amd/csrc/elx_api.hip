@@ -219,6 +219,7 @@ elx_status_t release_all() {
     }
     library_gemm_release();
     batched_release();
+    batched_lu_release();
     {
         std::lock_guard<std::mutex> lk(g_pmu);
         prof_drain();
@@ -229,22 +230,10 @@ elx_status_t release_all() {
     return ELX_SUCCESS;
 }
 
-namespace {
-// Everything the library holds (library handles, pinned host buffers, device scratch, events)
-// is released when the process exits, before the HIP runtime's own teardown: the runtime was
-// loaded before this library, so its exit handlers run after this one.
-void release_at_exit() {
-    bool used = false;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        used = !g_info.empty() || !g_scratch.empty();
-    }
-    if (used) (void)release_all();  // a process that never ran a kernel has nothing to free
-}
-struct ExitHook {
-    ExitHook() { std::atexit(release_at_exit); }
-} g_exit_hook;
-}  // namespace
+// No release at process exit: an exit handler of this library can run after the HIP runtime
+// has begun its own teardown (the runtime registers its handlers when it initialises, after
+// this library was loaded), and HIP calls made then hang or crash the process. Callers that
+// want the memory back before exit call elx_finalize.
 
 // Scratch slots
 enum { kSlotPairs = 0, kSlotPerm = 1, kSlotRhs = 2, kSlotInfo = 3, kSlotGate = 4 };
@@ -259,15 +248,16 @@ elx_status_t gate_slot(double** out) {
 namespace {
 
 // General triangular solve: small-kernel diagonal blocks + MFMA GEMM updates. `stop` as in
-// the launchers: a recorded failure makes every step a no-op.
+// the launchers: a recorded failure makes every step a no-op. allow_library false keeps the
+// updates on the hand-written GEMM (no library call, so nothing is allocated).
 template <typename T>
 elx_status_t trsm_blocked(hipStream_t s, int side, int uplo, int orient, int diag, i64 m, i64 n,
                           T alpha, const T* A, i64 lda, T* B, i64 ldb,
-                          const int32_t* stop = nullptr) {
+                          const int32_t* stop = nullptr, bool allow_library = true) {
     if (m <= 0 || n <= 0) return ELX_SUCCESS;
     if (alpha != T(1))
         ELX_CHECK(launch_gemm(s, false, false, m, n, 0, T(0), (const T*)nullptr, 1,
-                              (const T*)nullptr, 1, alpha, B, ldb, 0, stop));
+                              (const T*)nullptr, 1, alpha, B, ldb, 0, stop, allow_library));
     const bool trans = (orient != kNormal);
     const i64 bs = 128;
     if (side == kLeft) {
@@ -286,20 +276,20 @@ elx_status_t trsm_blocked(hipStream_t s, int side, int uplo, int orient, int dia
                 // B[r0:,:] -= M[r0:, b:b+w] B[b:b+w,:]
                 if (!trans)
                     ELX_CHECK(launch_gemm(s, false, false, rows, n, w, T(-1), A + r0 + b * lda, lda,
-                                          B + b, ldb, T(1), B + r0, ldb, 0, stop));
+                                          B + b, ldb, T(1), B + r0, ldb, 0, stop, allow_library));
                 else
                     ELX_CHECK(launch_gemm(s, true, false, rows, n, w, T(-1), A + b + r0 * lda, lda,
-                                          B + b, ldb, T(1), B + r0, ldb, 0, stop));
+                                          B + b, ldb, T(1), B + r0, ldb, 0, stop, allow_library));
             } else {
                 const i64 rows = b;
                 if (rows <= 0) continue;
                 // B[0:b,:] -= M[0:b, b:b+w] B[b:b+w,:]
                 if (!trans)
                     ELX_CHECK(launch_gemm(s, false, false, rows, n, w, T(-1), A + b * lda, lda,
-                                          B + b, ldb, T(1), B, ldb, 0, stop));
+                                          B + b, ldb, T(1), B, ldb, 0, stop, allow_library));
                 else
                     ELX_CHECK(launch_gemm(s, true, false, rows, n, w, T(-1), A + b, lda, B + b,
-                                          ldb, T(1), B, ldb, 0, stop));
+                                          ldb, T(1), B, ldb, 0, stop, allow_library));
             }
         }
         return ELX_SUCCESS;
@@ -319,20 +309,20 @@ elx_status_t trsm_blocked(hipStream_t s, int side, int uplo, int orient, int dia
             // B[:, c0:] -= X[:, b:b+w] M[b:b+w, c0:]
             if (!trans)
                 ELX_CHECK(launch_gemm(s, false, false, m, cols, w, T(-1), B + b * ldb, ldb,
-                                      A + b + c0 * lda, lda, T(1), B + c0 * ldb, ldb, 0, stop));
+                                      A + b + c0 * lda, lda, T(1), B + c0 * ldb, ldb, 0, stop, allow_library));
             else
                 ELX_CHECK(launch_gemm(s, false, true, m, cols, w, T(-1), B + b * ldb, ldb,
-                                      A + c0 + b * lda, lda, T(1), B + c0 * ldb, ldb, 0, stop));
+                                      A + c0 + b * lda, lda, T(1), B + c0 * ldb, ldb, 0, stop, allow_library));
         } else {
             const i64 cols = b;
             if (cols <= 0) continue;
             // B[:, 0:b] -= X[:, b:b+w] M[b:b+w, 0:b]
             if (!trans)
                 ELX_CHECK(launch_gemm(s, false, false, m, cols, w, T(-1), B + b * ldb, ldb, A + b,
-                                      lda, T(1), B, ldb, 0, stop));
+                                      lda, T(1), B, ldb, 0, stop, allow_library));
             else
                 ELX_CHECK(launch_gemm(s, false, true, m, cols, w, T(-1), B + b * ldb, ldb,
-                                      A + b * lda, lda, T(1), B, ldb, 0, stop));
+                                      A + b * lda, lda, T(1), B, ldb, 0, stop, allow_library));
         }
     }
     return ELX_SUCCESS;
@@ -407,6 +397,18 @@ elx_status_t getrf_nopiv_impl(hipStream_t s, i64 m, i64 n, T* A, i64 lda, i64 nb
     return ELX_SUCCESS;
 }
 
+// Row permutation by a swap sequence through work space perm (m int32) and W (m x n values).
+template <typename T>
+elx_status_t permute_rows_with(hipStream_t s, int inverse, const int32_t* dests, i64 nswaps,
+                               i64 m, i64 n, T* B, i64 ldb, int32_t* perm, T* W) {
+    if (m <= 0 || n <= 0 || nswaps <= 0) return ELX_SUCCESS;
+    ELX_CHECK(launch_swaps_to_perm(s, dests, nswaps, m, perm, inverse));
+    ELX_CHECK(launch_gather_rows(s, perm, m, n, B, ldb, W, m));
+    ELX_HIP(hipMemcpy2DAsync(B, sizeof(T) * ldb, W, sizeof(T) * m, sizeof(T) * m, n,
+                             hipMemcpyDeviceToDevice, s));
+    return ELX_SUCCESS;
+}
+
 template <typename T>
 elx_status_t permute_rows_impl(hipStream_t s, int inverse, const int32_t* dests, i64 nswaps,
                                i64 m, i64 n, T* B, i64 ldb) {
@@ -417,11 +419,7 @@ elx_status_t permute_rows_impl(hipStream_t s, int inverse, const int32_t* dests,
     ELX_CHECK(scratch_get(kSlotRhs, sizeof(T) * (size_t)m * (size_t)n, &pb));
     int32_t* perm = static_cast<int32_t*>(pp);
     T* W = static_cast<T*>(pb);
-    ELX_CHECK(launch_swaps_to_perm(s, dests, nswaps, m, perm, inverse));
-    ELX_CHECK(launch_gather_rows(s, perm, m, n, B, ldb, W, m));
-    ELX_HIP(hipMemcpy2DAsync(B, sizeof(T) * ldb, W, sizeof(T) * m, sizeof(T) * m, n,
-                             hipMemcpyDeviceToDevice, s));
-    return ELX_SUCCESS;
+    return permute_rows_with(s, inverse, dests, nswaps, m, n, B, ldb, perm, W);
 }
 
 // El::Cholesky, variant 3 (LowerVariant3.hpp:43-68, UpperVariant3.hpp:48-73), stopped by the
@@ -496,23 +494,42 @@ elx_status_t getrf_nopiv_api(void* stream, int64_t m, int64_t n, T* dA, int64_t 
     return getrf_nopiv_impl<T>(s, m, n, dA, lda, std::min<int64_t>(nb, 128), dInfo);
 }
 
+// lu::SolveAfter's sequence. perm / W (m int32, n x nrhs values) are the row permutation's
+// work space when given (the no-alloc entry point), else the library's scratch slots; with
+// allow_library false every update stays on the hand-written GEMM.
+template <typename T>
+elx_status_t getrs_body(hipStream_t s, int orient, int64_t n, int64_t nrhs, const T* dA,
+                        int64_t lda, const int32_t* dDests, T* dB, int64_t ldb, int32_t* perm,
+                        T* W, bool allow_library) {
+    auto permute = [&](int inverse) -> elx_status_t {
+        if (!dDests) return ELX_SUCCESS;
+        if (perm) return permute_rows_with(s, inverse, dDests, n, n, nrhs, dB, ldb, perm, W);
+        return permute_rows_impl(s, inverse, dDests, n, n, nrhs, dB, ldb);
+    };
+    if (orient == kNormal) {
+        ELX_CHECK(permute(0));
+        ELX_CHECK(trsm_blocked(s, kLeft, kLower, kNormal, kUnit, n, nrhs, T(1), dA, lda, dB, ldb,
+                               nullptr, allow_library));
+        ELX_CHECK(trsm_blocked(s, kLeft, kUpper, kNormal, kNonUnit, n, nrhs, T(1), dA, lda, dB,
+                               ldb, nullptr, allow_library));
+    } else {
+        ELX_CHECK(trsm_blocked(s, kLeft, kUpper, orient, kNonUnit, n, nrhs, T(1), dA, lda, dB, ldb,
+                               nullptr, allow_library));
+        ELX_CHECK(trsm_blocked(s, kLeft, kLower, orient, kUnit, n, nrhs, T(1), dA, lda, dB, ldb,
+                               nullptr, allow_library));
+        ELX_CHECK(permute(1));
+    }
+    return ELX_SUCCESS;
+}
+
 template <typename T>
 elx_status_t getrs_api(void* stream, int orient, int64_t n, int64_t nrhs, const T* dA,
                        int64_t lda, const int32_t* dDests, T* dB, int64_t ldb) {
     ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) && ldb >= std::max<int64_t>(n, 1));
     ELX_ARG(orient >= 0 && orient <= 2);
     if (n == 0 || nrhs == 0) return ELX_SUCCESS;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (orient == kNormal) {
-        if (dDests) ELX_CHECK(permute_rows_impl(s, 0, dDests, n, n, nrhs, dB, ldb));
-        ELX_CHECK(trsm_blocked(s, kLeft, kLower, kNormal, kUnit, n, nrhs, T(1), dA, lda, dB, ldb));
-        ELX_CHECK(trsm_blocked(s, kLeft, kUpper, kNormal, kNonUnit, n, nrhs, T(1), dA, lda, dB, ldb));
-    } else {
-        ELX_CHECK(trsm_blocked(s, kLeft, kUpper, orient, kNonUnit, n, nrhs, T(1), dA, lda, dB, ldb));
-        ELX_CHECK(trsm_blocked(s, kLeft, kLower, orient, kUnit, n, nrhs, T(1), dA, lda, dB, ldb));
-        if (dDests) ELX_CHECK(permute_rows_impl(s, 1, dDests, n, n, nrhs, dB, ldb));
-    }
-    return ELX_SUCCESS;
+    return getrs_body<T>(static_cast<hipStream_t>(stream), orient, n, nrhs, dA, lda, dDests, dB,
+                         ldb, nullptr, nullptr, true);
 }
 
 template <typename T>
@@ -759,6 +776,7 @@ elx_status_t elx_sgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const
                         int64_t lda, const int32_t* dDests, float* dB, int64_t ldb) {
     return getrs_api<float>(stream, orient, n, nrhs, dA, lda, dDests, dB, ldb);
 }
+
 
 elx_status_t elx_dgesv(void* stream, int64_t n, int64_t nrhs, double* dA, int64_t lda,
                        double* dB, int64_t ldb, int64_t nb, int32_t* dDests, int32_t* dInfo) {

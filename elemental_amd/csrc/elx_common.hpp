@@ -248,6 +248,14 @@ elx_status_t library_strsm(hipStream_t s, bool upper, bool trans, bool unit, i64
 void library_gemm_release();
 // Frees the batched solvers' schedules (pinned host and device copies) and queues.
 void batched_release();
+void batched_lu_release();
+// The device-side batched mixed LU-IR (FP16 factors, AUTO refinement, FP64 LU fallback) for
+// systems of order <= 2048; synchronous, stats per system.
+elx_status_t gesv_mixed_batched_device(hipStream_t s, int64_t count, const int32_t* n,
+                                       const int32_t* nrhs, const double* const* A_ptrs,
+                                       const int32_t* lda, const double* const* B_ptrs,
+                                       const int32_t* ldb, double* const* X_ptrs,
+                                       const int32_t* ldx, double tol, elx_ir_stats_t* stats);
 // e (device) with max|S| * 2^e < 2^15; ctl points at two zeroed unsigneds, zero again on return.
 elx_status_t launch_scale_exponent(hipStream_t s, i64 m, i64 n, const float* S, i64 lds,
                                    unsigned* ctl, int* e_dev);

@@ -24,9 +24,10 @@ __device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b, uint32_t c) {
 }
 constexpr int kSysScope = 1 | 16;  // sc0 | sc1
 // Workgroups on one XCD share its L2: stores go through the (write-through) vector L1 to L2,
-// and loads with sc0 set miss the L1 and read L2. Ordinary device memory, coherent among the
+// and loads at device scope (sc1) miss the L1 and read L2 (sc0 alone is workgroup scope and
+// may hit a stale line of the reader's own L1). Ordinary device memory, coherent among the
 // workgroups of that XCD only.
-constexpr int kXcdLoad = 1;   // sc0
+constexpr int kXcdLoad = 16;  // sc1
 constexpr int kXcdStore = 0;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t area(const u4* base, uint32_t count) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<u4*>(base), 0, (int)(count * 16u), 0x00020000);

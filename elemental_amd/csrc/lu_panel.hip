@@ -708,14 +708,12 @@ elx_status_t launch_panel(hipStream_t s, i64 mp, i64 w, T* A, i64 lda, int rowma
 
 namespace elx {
 
-// The mixed LU's panels use the one-XCD exchange unless ELX_PANEL_EXCHANGE=memory selects the
-// all-XCD exchange through uncached memory (an A/B switch between two complete panels).
+// Panels exchange their candidates through uncached memory across all XCDs unless
+// ELX_PANEL_EXCHANGE=xcd selects the one-XCD exchange through L2 for panels of at most 32
+// columns (read at every call, so a process can switch between two complete panels).
 bool xcd_panels_enabled() {
-    static const bool on = [] {
-        const char* v = std::getenv("ELX_PANEL_EXCHANGE");
-        return !(v && std::strcmp(v, "memory") == 0);
-    }();
-    return on;
+    const char* v = std::getenv("ELX_PANEL_EXCHANGE");
+    return v && std::strcmp(v, "xcd") == 0;
 }
 
 template <typename T>

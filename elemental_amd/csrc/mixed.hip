@@ -844,6 +844,14 @@ elx_status_t elx_dgesv_mixed_batched(void* stream, int64_t count, const int32_t*
     ELX_ARG(count >= 0);
     if (count == 0) return ELX_SUCCESS;
     ELX_ARG(n && nrhs && A_ptrs && lda && B_ptrs && ldb && X_ptrs && ldx);
+    // FP16 factors with AUTO refinement and orders up to 2048: every system factored and
+    // refined on the device, one workgroup each, FP64 LU fallback per system
+    bool device_path = (factor_type == ELX_FACTOR_FP16 && refine == ELX_REFINE_AUTO &&
+                        (max_iters <= 0 || max_iters == 30));
+    for (int64_t i = 0; i < count && device_path; ++i) device_path = n[i] >= 0 && n[i] <= 2048;
+    if (device_path)
+        return gesv_mixed_batched_device(static_cast<hipStream_t>(stream), count, n, nrhs, A_ptrs,
+                                         lda, B_ptrs, ldb, X_ptrs, ldx, tol, stats);
     // largest first: the workspaces grow once, to the largest order
     std::vector<int64_t> order((size_t)count);
     for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;

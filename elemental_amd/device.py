@@ -273,7 +273,7 @@ def posv_mixed(A, B, X, uplo=LOWER, factor="fp16", refine="auto", tol=0.0, max_i
 __all__ = [
     "load", "getrf", "getrf_nopiv", "getrf_full", "getrs_full", "getrs", "gesv", "potrf", "potrs", "posv", "gemm", "trsm", "syrk",
     "permute_rows", "ruiz_equil", "symmetric_ruiz_equil", "geom_equil", "gesv_mixed", "posv_mixed", "to_device", "to_host", "colmajor_empty",
-    "Batch", "potrf_vbatched", "potrs_vbatched", "gesv_mixed_batched",
+    "Batch", "potrf_vbatched", "potrs_vbatched", "PosvGraph", "batched_variant", "gesv_mixed_batched",
 ]
 
 
@@ -322,6 +322,45 @@ def potrs_vbatched(factors, rhs, uplo=LOWER, stream=None):
     call(name, _stream(stream), uplo, len(factors), _np_ptr(factors.n), _np_ptr(rhs.cols),
          ctypes_void(factors.dptrs.data_ptr()), _np_ptr(factors.ld),
          ctypes_void(rhs.dptrs.data_ptr()), _np_ptr(rhs.ld))
+
+
+class PosvGraph:
+    """elx_?posv_vbatched_graph: the batched factor + solve of fixed matrices and right-hand
+    sides captured into a HIP graph; launch() replays it, close() frees it."""
+
+    def __init__(self, factors, rhs, uplo=LOWER, stream=None):
+        import ctypes
+        if len(factors) != len(rhs) or len(factors) == 0:
+            raise ValueError("factor and right-hand-side batches must have the same, non-zero length")
+        self.factors, self.rhs = factors, rhs
+        self.info = torch.zeros(len(factors), dtype=torch.int32, device="cuda")
+        self.handle = ctypes.c_void_p()
+        name = "elx_dposv_vbatched_graph" if factors.dtype == torch.float64 else "elx_sposv_vbatched_graph"
+        call(name, _stream(stream), uplo, len(factors), _np_ptr(factors.n), _np_ptr(rhs.cols),
+             ctypes_void(factors.dptrs.data_ptr()), _np_ptr(factors.ld),
+             ctypes_void(rhs.dptrs.data_ptr()), _np_ptr(rhs.ld), ctypes_void(self.info.data_ptr()),
+             ctypes.byref(self.handle))
+
+    def launch(self, stream=None):
+        call("elx_graph_launch", self.handle, _stream(stream))
+        return self.info
+
+    def close(self):
+        if self.handle:
+            call("elx_graph_destroy", self.handle)
+            self.handle = None
+
+
+def batched_variant(dtype, bin_index, nb=None, occ=None):
+    """Reads (nb, occ) of a batched-Cholesky size bin; with nb given, sets it first (nb 0
+    restores the built-in table)."""
+    import ctypes
+    d = 0 if dtype == torch.float64 else 1
+    if nb is not None:
+        call("elx_batched_set_variant", d, bin_index, nb, occ or 0)
+    a, b = ctypes.c_int(), ctypes.c_int()
+    call("elx_batched_get_variant", d, bin_index, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
 
 
 def gesv_mixed_batched(As, Bs, Xs, factor="fp16", refine="auto", tol=0.0, max_iters=0,

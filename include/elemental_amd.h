@@ -303,6 +303,31 @@ elx_status_t elx_spotrs_vbatched(void* stream, int uplo, int64_t count, const in
                                  const int32_t* nrhs, const float* const* dA_array,
                                  const int32_t* lda, float* const* dB_array, const int32_t* ldb);
 
+/* The batched factor + solve (elx_?potrf_vbatched, then elx_?potrs_vbatched on the same
+ * matrices, skipping any matrix whose factorization failed: HPDSolve throws before it
+ * solves) captured once into a HIP graph for this batch: sizes, leading dimensions and the
+ * device pointer arrays are fixed at capture, the matrices' contents may change between
+ * launches. elx_graph_launch replays it on a stream (on the device it was made on);
+ * elx_graph_destroy waits for the device and frees it. No counterpart in the reference. */
+typedef struct elx_graph_st* elx_graph_t;
+elx_status_t elx_dposv_vbatched_graph(void* stream, int uplo, int64_t count, const int32_t* n,
+                                      const int32_t* nrhs, double* const* dA_array,
+                                      const int32_t* lda, double* const* dB_array,
+                                      const int32_t* ldb, int32_t* dinfo, elx_graph_t* graph);
+elx_status_t elx_sposv_vbatched_graph(void* stream, int uplo, int64_t count, const int32_t* n,
+                                      const int32_t* nrhs, float* const* dA_array,
+                                      const int32_t* lda, float* const* dB_array,
+                                      const int32_t* ldb, int32_t* dinfo, elx_graph_t* graph);
+elx_status_t elx_graph_launch(elx_graph_t graph, void* stream);
+elx_status_t elx_graph_destroy(elx_graph_t graph);
+
+/* Tuning hooks of the batched Cholesky (no counterpart in the reference; used by the
+ * autotuner, elemental_amd/autotune_batched.py): the variant -- block-column width nb (16 or
+ * 32) and persistent workgroups per CU occ -- of size bin `bin` (0: n <= 128, 1: n <= 256,
+ * 2: n <= 512) for dtype 0 (double) or 1 (float). nb = 0 restores the built-in table. */
+elx_status_t elx_batched_set_variant(int dtype, int bin, int nb, int occ);
+elx_status_t elx_batched_get_variant(int dtype, int bin, int* nb, int* occ);
+
 #ifdef __cplusplus
 }
 #endif

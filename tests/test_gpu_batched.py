@@ -105,3 +105,88 @@ def test_batched_empty_and_zero_sizes(dev):
     mats = [dev.colmajor_empty(0, 0), dev.to_device(np.eye(3) * 4.0)]
     info = dev.potrf_vbatched(dev.Batch(mats, torch.float64)).cpu().numpy()
     assert (info == 0).all() and np.array_equal(dev.to_host(mats[1]), np.eye(3) * 2.0)
+
+
+@pytest.mark.parametrize("nb", [16, 32])
+def test_batched_cholesky_every_variant(dev, orc, nb):
+    """Each size bin's kernel with both block widths (the autotuner's candidates), one and
+    two or three workgroups per CU: same bars as above, and the non-HPD column exact."""
+    rng = np.random.default_rng(43 + nb)
+    ns = [1, 15, 16, 17, 33, 100, 128, 129, 200, 256, 257, 300, 511, 512]
+    As = [spd(n, rng) for n in ns]
+    As[9][70, 70] = -1e3  # n = 256: not positive definite at column 70
+    try:
+        for occ in (1, 2):
+            for b in range(3):
+                dev.batched_variant(torch.float64, b, nb, min(occ, 3 - b))
+            say("batched variant", nb, occ)
+            mats = [dev.to_device(A) for A in As]
+            info = dev.potrf_vbatched(dev.Batch(mats, torch.float64)).cpu().numpy()
+            for i, (A, n) in enumerate(zip(As, ns)):
+                Fo, info_o = orc.cholesky(A, uplo=0)
+                assert info[i] == info_o, (i, n, info[i], info_o)
+                if info_o:
+                    assert info_o == 71
+                    continue
+                F = dev.to_host(mats[i])
+                assert orc.chol_factor_error(A, np.asfortranarray(F), 0) <= 8 * n * EPS64
+                L, Lo = np.tril(F), np.tril(Fo)
+                assert np.abs(L - Lo).max() <= 64 * n * EPS64 * np.abs(Lo).max(), (i, n)
+    finally:
+        for b in range(3):
+            dev.batched_variant(torch.float64, b, 0)
+    assert dev.batched_variant(torch.float64, 0)[0] in (16, 32)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_graph_replays(dev, orc, dtype):
+    """The captured factor + solve, replayed three times with the inputs restored by eager
+    copies in between, equals the eager potrf_vbatched + potrs_vbatched bit for bit on every
+    replay; a matrix that is not HPD keeps its right-hand side unsolved."""
+    rng = np.random.default_rng(47)
+    ns = [int(v) for v in rng.integers(16, 513, size=40)] + [3, 64, 128, 256, 512]
+    As = [spd(n, rng) for n in ns]
+    bad = 5
+    As[bad][min(10, ns[bad] - 1), min(10, ns[bad] - 1)] = -1e3
+    nr = [int(v) for v in rng.integers(1, 9, size=len(ns))]
+    Bs = [rng.standard_normal((n, r)) for n, r in zip(ns, nr)]
+    A0 = [dev.to_device(A, dtype=dtype) for A in As]
+    B0 = [dev.to_device(B, dtype=dtype) for B in Bs]
+    # eager reference
+    Ae = [dev.to_device(dev.to_host(a), dtype=dtype) for a in A0]
+    Be = [dev.to_device(dev.to_host(b), dtype=dtype) for b in B0]
+    fac_e = dev.Batch(Ae, dtype)
+    info_e = dev.potrf_vbatched(fac_e).cpu().numpy()
+    keep = [i for i in range(len(ns)) if info_e[i] == 0]
+    assert bad not in keep
+    dev.potrs_vbatched(dev.Batch([Ae[i] for i in keep], dtype),
+                       dev.Batch([Be[i] for i in keep], dtype))
+    torch.cuda.synchronize()
+    # graph on its own copies
+    Ag = [dev.to_device(dev.to_host(a), dtype=dtype) for a in A0]
+    Bg = [dev.to_device(dev.to_host(b), dtype=dtype) for b in B0]
+    g = dev.PosvGraph(dev.Batch(Ag, dtype), dev.Batch(Bg, dtype))
+    try:
+        for rep in range(3):
+            say("graph replay", dtype, rep)
+            for a, a0 in zip(Ag, A0):
+                a.copy_(a0)
+            for b, b0 in zip(Bg, B0):
+                b.copy_(b0)
+            torch.cuda.synchronize()
+            info = g.launch().cpu().numpy()
+            torch.cuda.synchronize()
+            assert np.array_equal(info, info_e)
+            for i in range(len(ns)):
+                assert np.array_equal(dev.to_host(Ag[i]), dev.to_host(Ae[i])), (rep, i)
+                assert np.array_equal(dev.to_host(Bg[i]), dev.to_host(Be[i])), (rep, i)
+            assert np.array_equal(dev.to_host(Bg[bad]), dev.to_host(B0[bad]))
+            # eager work of the program between replays
+            dev.potrf_vbatched(dev.Batch([dev.to_device(spd(40, rng), dtype=dtype)], dtype))
+    finally:
+        g.close()
+    for j, i in enumerate(keep[:10]):
+        X = dev.to_host(Bg[i]).astype(np.float64)
+        eta = orc.backward_error(As[i], np.asfortranarray(X), np.asfortranarray(Bs[i]))
+        u = EPS64 if dtype == torch.float64 else EPS32
+        assert eta <= 4 * max(ns[i], 4) * u

@@ -395,7 +395,7 @@ def test_ir_keeps_only_decreasing_candidates(dev, orc):
     x1 = x0 + dev.to_host(dD)
     grow = np.abs(b - A @ x1).max() / np.abs(r0).max()
     say("keep-if-decreasing: residual ratio of the first candidate", grow)
-    assert grow > 2.0, "premise: the first correction must increase the residual"
+    assert grow > 1.05, "premise: the first correction must increase the residual"
     dX = dev.colmajor_empty(n, 1)
     st = dev.gesv_mixed(dA, dB, dX, factor="fp16", refine="ir")
     say(st)

@@ -135,10 +135,13 @@ def test_lu_single_panel_bitwise(dev, orc):
         check_lu(dev, orc, A, nb, expect_exact=True)
 
 
-def test_lu_one_xcd_panels(dev, orc):
-    """Blocksize() <= 32: the panels run on the CUs of one XCD and exchange their candidates
-    through its L2 (up to 1024 rows per CU). Same bars: single panels bit for bit, blocked
-    factorizations with identical pivots."""
+@pytest.mark.parametrize("exchange", ["memory", "xcd"])
+def test_lu_one_xcd_panels(dev, orc, exchange, monkeypatch):
+    """Blocksize() <= 32, with the panel exchange through uncached memory (the default) and
+    with ELX_PANEL_EXCHANGE=xcd, where the panels run on the CUs of one XCD and exchange their
+    candidates through its L2 (up to 1024 rows per CU). Same bars: single panels bit for bit,
+    blocked factorizations with identical pivots."""
+    monkeypatch.setenv("ELX_PANEL_EXCHANGE", exchange)
     orc.rng_seed(18)
     for (m, n, nb) in [(40, 40, 32), (1000, 32, 32), (17000, 16, 16), (32768, 32, 32)]:
         say("one-XCD panel bitwise", m, n, nb)
