@@ -358,11 +358,11 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 // One workgroup: L L^T X = B for one matrix (L from potrf), up to NR right-hand sides at a time
-// in LDS, blocked by SB = 32 rows. Forward: each block's rows first receive the products with
-// the solved part (256 threads as 32 rows x 8 column groups, coalesced down the columns of L),
-// then wave 0 solves the diagonal block with its rows in registers. Backward: each block
-// column's dot products with the solved part below (a wave per column, lanes down the column),
-// then wave 0 solves the transposed diagonal block with its columns in registers.
+// in LDS, blocked by SB = 32 rows. Forward (right-looking): wave 0 solves the diagonal block
+// with its rows in registers, then every row below subtracts the block's products (a row per
+// thread). Backward: each block column's dot products with the solved part below (a wave per
+// column, lanes down the column), then wave 0 solves the transposed diagonal block with its
+// columns in registers.
 constexpr int NR = 8, SB = 32, SLD = SB + 1;
 template <typename T>
 __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T* Y, T* Part,
@@ -382,26 +382,12 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
             Y[c * BNMAX + i] = B[i + (size_t)(c0 + c) * ldb];
         }
         __syncthreads();
-        // forward: L y = b (dtrsm Left Lower NoTrans NonUnit)
+        // forward: L y = b (dtrsm Left Lower NoTrans NonUnit), right-looking by 32-row blocks:
+        // the diagonal block's solve, then its products subtracted from every row below (a
+        // row per thread: the block's 32 columns of L are independent loads, coalesced across
+        // the threads)
         for (int b = 0; b < nblk; ++b) {
             const int k0 = b * SB, kb = min(SB, n - k0);
-            {
-                const int r = t & 31, g = t >> 5;
-                T acc[NR];
-#pragma unroll
-                for (int q = 0; q < NR; ++q) acc[q] = T(0);
-                if (r < kb) {
-#pragma unroll 4
-                    for (int c = g; c < k0; c += 8) {
-                        const T l = L[(k0 + r) + (size_t)c * lda];
-#pragma unroll
-                        for (int q = 0; q < NR; ++q)
-                            if (q < nr) acc[q] = __builtin_fma(l, Y[q * BNMAX + c], acc[q]);
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < NR; ++q) Part[(g * SB + r) * NR + q] = acc[q];
-            }
             stage_l11(k0, kb);
             __syncthreads();
             if (wave == 0) {
@@ -409,15 +395,7 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
 #pragma unroll
                 for (int j = 0; j < SB; ++j) lrow[j] = (lane < kb) ? Lb[lane * SLD + j] : T(0);
 #pragma unroll
-                for (int q = 0; q < NR; ++q) {
-                    T s = T(0);
-                    if (lane < kb && q < nr) {
-                        s = Y[q * BNMAX + k0 + lane];
-#pragma unroll
-                        for (int g = 0; g < 8; ++g) s -= Part[(g * SB + lane) * NR + q];
-                    }
-                    yv[q] = s;
-                }
+                for (int q = 0; q < NR; ++q) yv[q] = (lane < kb && q < nr) ? Y[q * BNMAX + k0 + lane] : T(0);
 #pragma unroll
                 for (int k = 0; k < SB; ++k) {
                     if (k < kb) {
@@ -433,6 +411,23 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
 #pragma unroll
                     for (int q = 0; q < NR; ++q)
                         if (q < nr) Y[q * BNMAX + k0 + lane] = yv[q];
+            }
+            __syncthreads();
+            for (int i = k0 + kb + t; i < n; i += BNT) {
+                T acc[NR];
+#pragma unroll
+                for (int q = 0; q < NR; ++q) acc[q] = (q < nr) ? Y[q * BNMAX + i] : T(0);
+                const T* li = L + i + (size_t)k0 * lda;
+#pragma unroll 8
+                for (int k = 0; k < kb; ++k) {
+                    const T l = li[(size_t)k * lda];
+#pragma unroll
+                    for (int q = 0; q < NR; ++q)
+                        if (q < nr) acc[q] = __builtin_fma(-l, Y[q * BNMAX + k0 + k], acc[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    if (q < nr) Y[q * BNMAX + i] = acc[q];
             }
             __syncthreads();
         }
@@ -499,7 +494,7 @@ __global__ __launch_bounds__(BNT, 2) void potrs_vbatched_kernel(
     const BatchItem* __restrict__ items, int count, const T* const* __restrict__ Ls,
     T* const* __restrict__ Bs, const int32_t* __restrict__ info, int* queue) {
     __shared__ T Y[NR * BNMAX];
-    __shared__ T Part[8 * SB * NR];
+    __shared__ T Part[SB * NR];
     __shared__ T Lb[SB * SLD];
     __shared__ int s_i;
     for (;;) {

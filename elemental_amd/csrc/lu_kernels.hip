@@ -147,6 +147,75 @@ __global__ void gather_rows_kernel(const int32_t* __restrict__ perm, i64 m, i64 
 // The lanes of a vector live in the same wave and LDS operations of a wave complete in
 // order, so a step reads what the previous step wrote without a workgroup barrier.
 // ---------------------------------------------------------------------------------------
+// Left, lower, no transpose, unit diagonal, order m <= 128, many right-hand sides: the LU's
+// U12 = L11^{-1} A12 (Trsm(LEFT,LOWER,NORMAL,UNIT), src/lapack_like/factor/LU.cpp:139) and the
+// row-echelon form's B update. 64 columns per workgroup in LDS, 32-row blocks: per block the
+// block's column panel of L (its rows r0.., 32 columns) is staged, wave 0 solves the diagonal
+// block (a column per lane, 32 values in registers), and all waves subtract its contribution
+// from the rows below. Every entry still receives its updates one FMA at a time in increasing
+// k -- b_i = fma(-x_k, l_ik, b_i), the column-oriented substitution's order -- so the result
+// is bitwise that of trsm_small_kernel and the restated BLAS; the chain is 4 blocks of 32
+// steps instead of 128 dependent steps.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_llnu_kernel(int m, i64 ncols, const T* __restrict__ A,
+                                                        i64 lda, T* B, i64 ldb,
+                                                        const int32_t* __restrict__ stop) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int XL = 65, PL = 33;
+    T* Xs = reinterpret_cast<T*>(smem_raw);  // 128 x 65: Xs[i*XL + c]
+    T* Ps = Xs + 128 * XL;                   // 128 x 33: Ps[i*PL + kk] = A(r0 + i, r0 + kk)
+    if (stop && *stop != 0) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const i64 c0 = (i64)blockIdx.x * 64;
+    const int nc = (int)((ncols - c0) < 64 ? (ncols - c0) : 64);
+    for (int idx = t; idx < m * 64; idx += 256) {
+        const int i = idx % m, c = idx / m;  // down the columns: coalesced
+        Xs[i * XL + c] = (c < nc) ? B[i + (c0 + c) * ldb] : T(0);
+    }
+    for (int r0 = 0; r0 < m; r0 += 32) {
+        const int rb = min(32, m - r0), rows = m - r0;
+        __syncthreads();  // Xs staged, or the previous block's updates done with Ps
+        for (int idx = t; idx < rows * rb; idx += 256) {
+            const int i = idx % rows, kk = idx / rows;
+            Ps[i * PL + kk] = A[(r0 + i) + (i64)(r0 + kk) * lda];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            T x[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) x[i] = (i < rb) ? Xs[(r0 + i) * XL + lane] : T(0);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+#pragma unroll
+                for (int i = k + 1; i < 32; ++i)
+                    if (i < rb) x[i] = __builtin_fma(-x[k], Ps[i * PL + k], x[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+                if (i < rb) Xs[(r0 + i) * XL + lane] = x[i];
+        }
+        __syncthreads();
+        if (rb < rows) {
+            T xb[32];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) xb[k] = (k < rb) ? Xs[(r0 + k) * XL + lane] : T(0);
+            for (int i = rb + wave; i < rows; i += 4) {
+                T acc = Xs[(r0 + i) * XL + lane];
+                const T* pi = Ps + i * PL;
+#pragma unroll
+                for (int k = 0; k < 32; ++k)
+                    if (k < rb) acc = __builtin_fma(-xb[k], pi[k], acc);
+                Xs[(r0 + i) * XL + lane] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < m * 64; idx += 256) {
+        const int i = idx % m, c = idx / m;
+        if (c < nc) B[i + (c0 + c) * ldb] = Xs[i * XL + c];
+    }
+}
+
 constexpr int TP = 16;           // lanes per vector
 constexpr int TNT = 256;         // threads per workgroup
 constexpr int TV = TNT / TP;     // vectors per workgroup
@@ -566,6 +635,22 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     // Left: M = op(A). Right: M = op(A)^T.
     const bool mtrans = right ? !trans : trans;
     const bool lower_eff = lower != mtrans;
+    if (!right && lower && !trans && diag == kUnit && nvec >= 256) {
+        // many right-hand sides: the blocked form (same FMA order per entry)
+        const size_t lds2 = sizeof(T) * (128 * 65 + 128 * 33);
+        ProfScope prof2_(s, kKTrsm, (double)order * (double)order * (double)nvec);
+        static bool attr2 = false;
+        if (!attr2) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_llnu_kernel<T>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+            (void)hipGetLastError();
+            attr2 = true;
+        }
+        hipLaunchKernelGGL(trsm_llnu_kernel<T>, dim3(cdiv(nvec, 64)), dim3(256), lds2, s,
+                           (int)order, nvec, A, lda, B, ldb, stop);
+        ELX_HIP(hipGetLastError());
+        return ELX_SUCCESS;
+    }
     const size_t lds = sizeof(T) * ((size_t)(order + 1) * TV + (size_t)order * (order + 1));
     ProfScope prof_(s, kKTrsm, (double)order * (double)order * (double)nvec);
     static bool attr = false;

@@ -182,9 +182,8 @@ elx_status_t getrf_mixed(Ctx& c) {
                                                     j0, j0 + w, K1));
             const i64 wr = K1 - (j0 + w);
             if (wr > 0) {
-                // U12 = L11^{-1} A12, as U12^T L11^T = A12^T on the stored transposes
-                ELX_CHECK(launch_trsm_small_f32(c.s, kRight, kUpper, kNormal, kUnit, wr, w,
-                                                at(j0, j0), ld, at(j0, j0 + w), ld));
+                // U12 = L11^{-1} A12 (rows of the working matrix), blocked substitution
+                ELX_CHECK(launch_trsm_rows(c.s, w, wr, at(j0, j0), at(j0, j0 + w), ld));
                 ELX_CHECK(lp_update(c, n - (j0 + w), wr, w, at(j0 + w, j0), at(j0, j0 + w),
                                     at(j0 + w, j0 + w), ld, 128));
             }

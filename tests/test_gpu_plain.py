@@ -123,6 +123,40 @@ def test_lu_shapes(dev, orc):
         check_lu(dev, orc, A, nb)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_trsm_many_rhs_blocked_bitwise(dev, orc, dtype):
+    """Left lower unit solves with 256+ right-hand sides (the LU's U12 = L11^-1 A12) run in
+    32-row blocks, each entry updated one FMA at a time in increasing k: bit for bit the
+    restated BLAS, at every order up to 128, ragged column counts and padded leading
+    dimensions; and the U12 block of a two-panel LU equals the oracle's bit for bit."""
+    rng = np.random.default_rng(5)
+    for order, ncols in [(1, 256), (31, 300), (32, 257), (33, 1000), (100, 513), (128, 4000)]:
+        say("trsm llnu", order, ncols, dtype.__name__)
+        T = (np.tril(rng.standard_normal((order, order)), -1) + np.eye(order)).astype(dtype)
+        B = rng.standard_normal((order, ncols)).astype(dtype)
+        dB = dev.to_device(B)
+        dev.trsm(0, 0, 0, 1, 1.0, dev.to_device(T), dB)
+        ref = orc.trsm("L", "L", "N", "U", 1.0, np.asfortranarray(T.astype(np.float64)),
+                       np.asfortranarray(B.astype(np.float64))) if dtype == np.float64 else None
+        got = dev.to_host(dB)
+        if dtype == np.float64:
+            assert np.array_equal(got, ref)
+        else:
+            # float: against the small kernel (under 256 columns) column by column
+            for c0 in range(0, ncols, 200):
+                part = dev.to_device(B[:, c0:c0 + 200])
+                dev.trsm(0, 0, 0, 1, 1.0, dev.to_device(T), part)
+                assert np.array_equal(dev.to_host(part), got[:, c0:c0 + 200])
+    orc.rng_seed(23)
+    A = orc.gaussian(256, 256)
+    F_ref, d_ref, _ = orc.lu(A, bsize=128)
+    dA = dev.to_device(A)
+    d, info = dev.getrf(dA, nb=128)
+    F = dev.to_host(dA)
+    assert int(info.item()) == 0 and np.array_equal(d.cpu().numpy(), d_ref)
+    assert np.array_equal(F[:128, :], F_ref[:128, :]), "first panel and U12 bit for bit"
+
+
 def test_lu_single_panel_bitwise(dev, orc):
     """n <= nb (and tall single panels: one cooperative panel over many workgroups) runs only
     the panel kernel, whose per-entry FMA order is lu::Panel's: the factors are asserted equal
