@@ -53,6 +53,7 @@ _SIGS = {
     "elx_finalize": [],
     "elx_dgetrf": [c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp],
     "elx_dgetrs": [c_vp, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64],
+    "elx_dgetrs_noalloc": [c_vp, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64],
     "elx_dgesv": [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp],
     "elx_dpotrf": [c_vp, c_int, c_i64, c_vp, c_i64, c_i64, c_vp],
     "elx_dpotrs": [c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64],

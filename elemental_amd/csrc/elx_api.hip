@@ -13,6 +13,37 @@
 
 #include "elx_common.hpp"
 
+// Work space of elx_dgetrs_noalloc: one fixed array per GPU, held until the process ends. Its
+// host-side name is not a device address: host code uses only the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[ELX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+elx_status_t noalloc_scratch(unsigned char** out) {
+    int dev = 0;
+    ELX_HIP(hipGetDevice(&dev));
+    ELX_ARG(dev >= 0 && dev < kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            std::fprintf(stderr, "elemental_amd: no device address for the no-alloc scratch: %s\n",
+                         hipGetErrorString(e));
+            return ELX_ERROR_HIP;
+        }
+        g_na_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_na_ptr[dev];
+    return ELX_SUCCESS;
+}
+}  // namespace
+
 namespace elx {
 
 namespace {
@@ -777,6 +808,37 @@ elx_status_t elx_sgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const
     return getrs_api<float>(stream, orient, n, nrhs, dA, lda, dDests, dB, ldb);
 }
 
+// lu::SolveAfter without allocation: the row permutation's work space is the fixed array
+// `scratch` below, reached through its device address; every update runs on the hand-written
+// kernels (no library call). Calls that overlap on one GPU share the array and race.
+elx_status_t elx_dgetrs_noalloc(void* stream, int orient, int64_t n, int64_t nrhs,
+                                const double* dA, int64_t lda, const int32_t* dDests, double* dB,
+                                int64_t ldb) {
+    ELX_ARG(n >= 0 && nrhs >= 0 && lda >= std::max<int64_t>(n, 1) && ldb >= std::max<int64_t>(n, 1));
+    ELX_ARG(orient >= 0 && orient <= 2);
+    if (n == 0 || nrhs == 0) return ELX_SUCCESS;
+    int32_t* perm = nullptr;
+    double* W = nullptr;
+    if (dDests) {
+        const size_t perm_bytes = (sizeof(int32_t) * (size_t)n + 255) / 256 * 256;
+        const double need = (double)perm_bytes + (double)sizeof(double) * (double)n * (double)nrhs;
+        if (need > (double)ELX_NOALLOC_SCRATCH_BYTES) {
+            std::fprintf(stderr,
+                         "elemental_amd: elx_dgetrs_noalloc needs %.0f bytes of work space for "
+                         "n=%lld, nrhs=%lld; its fixed scratch holds %lld bytes "
+                         "(4n rounded up to 256, plus 8 n nrhs)\n",
+                         need, (long long)n, (long long)nrhs,
+                         (long long)ELX_NOALLOC_SCRATCH_BYTES);
+            return ELX_ERROR_INVALID_ARG;
+        }
+        unsigned char* base = nullptr;
+        ELX_CHECK(noalloc_scratch(&base));
+        perm = reinterpret_cast<int32_t*>(base);
+        W = reinterpret_cast<double*>(base + perm_bytes);
+    }
+    return getrs_body<double>(static_cast<hipStream_t>(stream), orient, n, nrhs, dA, lda, dDests,
+                              dB, ldb, perm, W, false);
+}
 
 elx_status_t elx_dgesv(void* stream, int64_t n, int64_t nrhs, double* dA, int64_t lda,
                        double* dB, int64_t ldb, int64_t nb, int32_t* dDests, int32_t* dInfo) {

@@ -115,6 +115,19 @@ elx_status_t elx_dgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const
 elx_status_t elx_sgetrs(void* stream, int orient, int64_t n, int64_t nrhs, const float* dA,
                         int64_t lda, const int32_t* dDests, float* dB, int64_t ldb);
 
+/* elx_dgetrs without any allocation, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Same arguments and results as elx_dgetrs; the row
+ * permutation's work space (4n bytes rounded up to 256, plus 8 n nrhs bytes; none when
+ * dDests is NULL) comes from a fixed device array of ELX_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends, and every update runs on the library's own kernels. Larger
+ * problems return ELX_ERROR_INVALID_ARG. Calls that overlap on one GPU share that array and
+ * race: serialise them (one stream, or events). */
+#define ELX_NOALLOC_SCRATCH_BYTES (32LL << 20)
+elx_status_t elx_dgetrs_noalloc(void* stream, int orient, int64_t n, int64_t nrhs,
+                                const double* dA, int64_t lda, const int32_t* dDests, double* dB,
+                                int64_t ldb);
+
 /* Short-circuited LU on [A|B] followed by one upper Trsm: B := A^{-1} B, A overwritten
  * with its (row-echelon) factors. Replaces lin_solve::Overwrite / RowEchelon --
  * src/lapack_like/solve/Linear.cpp:30-76,176-183 (El::LinearSolve, :212-218, is this after
