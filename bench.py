@@ -5,7 +5,8 @@ Workload (mixed-precision LU-IR, one right-hand side): A = U diag(sigma) V^T wit
 Haar-orthogonal (QR of seeded Gaussians with sign-fixed R), sigma geometric from 1 to
 1/kappa, b ~ N(0,1); FP64 storage in HBM. A step is one complete solve to FP64 normwise
 backward error <= n * 2^-53: fp16 LU (fp16 MFMA trailing updates with fp32 accumulation,
-fp32 panels) and FP64 refinement (IR, switching to GMRES-IR when it stalls), with the
+fp32 panels with tournament pivoting in 16-column blocks, ELX_MIXED_PANEL) and FP64
+refinement (IR, switching to GMRES-IR when it stalls), with the
 residual and the error measured on the device in FP64. The plain FP64 GPU LU + solve of the
 same system (a fresh copy of A per solve, as El::LinearSolve copies A) is timed beside it,
 and the reference's CPU path (the oracle's restatement of El::LU + lu::SolveAfter over the
@@ -129,7 +130,8 @@ def roofline_of(cls, rec, factor):
 # kernel class -> (kernel-name prefixes whose dispatches are the class's launches, prefixes of
 # helper kernels launched inside the same class launch) in a rocprofv3 counter CSV
 CLASS_KERNELS = {
-    "lu_panel": (("void (anonymous namespace)::lu_panel_kernel<float>",), ()),
+    "lu_panel": (("void (anonymous namespace)::calu_panel_kernel<",
+                  "void (anonymous namespace)::lu_panel_kernel<float>"), ()),
     "lpgemm": (("void (anonymous namespace)::lpgemm_kernel",), ()),
     "row_moves": (("(anonymous namespace)::row_moves_rowmajor_kernel",), ()),
     "trsm_small": (("void (anonymous namespace)::trsm_small_kernel<float>",
@@ -807,7 +809,8 @@ def main():
         "config": {"workload": f"LU-IR solve, n={n}, randsvd kappa={args.kappa:g}, 1 RHS, "
                                f"{args.factor} factor, FP64 refinement ({args.refine})",
                    "n": n, "kappa": args.kappa, "nrhs": 1, "factor": args.factor,
-                   "refine": args.refine, "seed": args.seed},
+                   "refine": args.refine, "seed": args.seed,
+                   "panel": os.environ.get("ELX_MIXED_PANEL", "calu16")},
         "converged": ok,
         "backward_error": eta_dev,
         "backward_error_check": eta_chk,

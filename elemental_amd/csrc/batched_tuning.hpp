@@ -1,6 +1,13 @@
 // Variant table of the batched Cholesky factorization, per element type and size bin.
 // Written by elemental_amd/autotune_batched.py from timings on one MI355X (see that script for
 // the workloads and the candidates); edit by running the script again, not by hand.
+// Measured 2026-12-03, chosen variants (median factor time of the bin's workload):
+//   float64 n in [16, 128]: nb 32 occ 3 0.295 ms
+//   float64 n in [129, 256]: nb 32 occ 2 1.705 ms
+//   float64 n in [257, 512]: nb 32 occ 1 7.343 ms
+//   float32 n in [16, 128]: nb 32 occ 3 0.289 ms
+//   float32 n in [129, 256]: nb 32 occ 2 1.362 ms
+//   float32 n in [257, 512]: nb 32 occ 1 7.027 ms
 #pragma once
 
 namespace elx {

@@ -177,6 +177,18 @@ elx_status_t launch_lu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A
                                           bool xcd = false);
 bool lu_panel_xcd_fits(i64 mp, i64 w);
 bool xcd_panels_enabled();
+// Tournament-pivoting panel (communication-avoiding LU) of the mixed-precision factorization,
+// same storage and outputs as launch_lu_panel_f32_rowmajor, factored ws (16 or 32) columns at
+// a time; fits when the rows spread over at most one workgroup per CU, 128 rows each.
+bool calu_panel_fits(i64 mp, i64 w, int ws);
+elx_status_t launch_calu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float* A, i64 lda,
+                                            int32_t* dests, i64 goff, int2* pairs, int32_t* info,
+                                            int ws);
+// Panel of the mixed-precision LU: 0 partial pivoting, 16 or 32 the tournament block width
+// (environment ELX_MIXED_PANEL = partial | calu16 | calu32, read at every solve; calu16 when
+// unset), used for panels of at least mixed_panel_min_rows() rows (partial pivoting below).
+int mixed_panel_mode();
+i64 mixed_panel_min_rows();
 // The row-move list applied to an fp32 matrix stored row by row (row i contiguous at
 // A + i*lda): columns [c0,c1) and [c2,c3) of each moved row, starting at row r0; moves that
 // reach outside the `rows` rows from r0 on are skipped.

@@ -17,6 +17,9 @@
 // earlier step has recorded a failure, every workgroup returns at once and C is left as the
 // reference leaves it when it throws.
 #include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
 
 #include "elx_common.hpp"
 
@@ -193,6 +196,190 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(i64 M, i64 N, i64 K, T alph
     }
 }
 
+// FP64 C := alpha op(A) op(B) + beta C (the LU's Gemm(NORMAL, NORMAL) update, the Cholesky's
+// Herk(LOWER, NORMAL) = A A^T and Herk(UPPER, ADJOINT) = A^T A updates on one triangle), for K a
+// multiple of 16 and operands whose columns start on 16-byte boundaries (even leading
+// dimensions, aligned bases).
+// Same 128x128 tile, 4x4 MFMA accumulators per wave and XCD-aware tile order as gemm_kernel,
+// but with 16-byte operand loads and double-buffered LDS slabs: the next slab is fetched into
+// registers while the current one feeds the MFMAs and is then stored into the other buffer, so
+// each 16-deep step costs one barrier. Edge tiles (partial in m or n) load element by element.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, double alpha,
+                                                        const double* __restrict__ A, i64 lda,
+                                                        const double* __restrict__ B, i64 ldb,
+                                                        double beta, double* __restrict__ C,
+                                                        i64 ldc, int tri, int tiles_m, int tiles_n,
+                                                        const int32_t* __restrict__ stop) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    __shared__ __align__(16) double As[2][BK][BM + PAD];  // As[b][k][i] = sgn A(m0+i, k0+k)
+    __shared__ __align__(16) double Bs[2][BK][BN + PAD];  // Bs[b][k][j] = op(B)(k0+k, n0+j)
+    if (stop && *stop != 0) return;
+    const int total = tiles_m * tiles_n;
+    const int per_xcd = (total + 7) / 8;
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (lin >= total) return;
+    constexpr int GROUP = 8;
+    const int group = lin / (GROUP * tiles_n);
+    const int first_m = group * GROUP;
+    const int gsize = min(tiles_m - first_m, GROUP);
+    const int tm = first_m + (lin % (GROUP * tiles_n)) % gsize;
+    const int tn = (lin % (GROUP * tiles_n)) / gsize;
+    const i64 m0 = (i64)tm * BM, n0 = (i64)tn * BN;
+    if (tri == 1 && n0 > m0 + BM - 1) return;
+    if (tri == 2 && m0 > n0 + BN - 1) return;
+    const bool full = (m0 + BM <= M) && (n0 + BN <= N);
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const bool fold = (beta == 1.0) && (alpha == 1.0 || alpha == -1.0);
+    const double sgn = fold ? alpha : 1.0;
+    d4 acc[4][4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+        const i64 m = m0 + wm + mi * 16 + r16;
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 n = n0 + wn + ni * 16 + kq + 4 * r;
+                const bool ok = fold && (full || (m < M && n < N)) && !(tri == 1 && m < n) &&
+                                !(tri == 2 && m > n);
+                acc[mi][ni][r] = ok ? C[m + n * ldc] : 0.0;
+            }
+    }
+    // slab loads: A as 4 pairs along m per thread (16 k x 64 pairs); B as 4 pairs along k
+    // (NN: 128 n x 8 pairs) or along n (NT: 16 k x 64 pairs)
+    d2 ra[4], rb[4];
+    auto gload = [&](i64 k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = t + NT * u;
+            if (!TA) {
+                const int k = idx >> 6, i2 = (idx & 63) * 2;
+                const i64 m = m0 + i2, kk = k0 + k;
+                if (full) {
+                    ra[u] = *reinterpret_cast<const d2*>(A + m + kk * lda);
+                } else {
+                    ra[u][0] = (m < M) ? A[m + kk * lda] : 0.0;
+                    ra[u][1] = (m + 1 < M) ? A[m + 1 + kk * lda] : 0.0;
+                }
+            } else {
+                const int i = idx >> 3, k2 = (idx & 7) * 2;
+                const i64 m = m0 + i, kk = k0 + k2;
+                if (full) {
+                    ra[u] = *reinterpret_cast<const d2*>(A + kk + m * lda);
+                } else {
+                    ra[u][0] = (m < M) ? A[kk + m * lda] : 0.0;
+                    ra[u][1] = (m < M) ? A[kk + 1 + m * lda] : 0.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = t + NT * u;
+            if (!TB) {
+                const int n = idx >> 3, k2 = (idx & 7) * 2;
+                const i64 nn = n0 + n, kk = k0 + k2;
+                if (full) {
+                    rb[u] = *reinterpret_cast<const d2*>(B + kk + nn * ldb);
+                } else {
+                    rb[u][0] = (nn < N) ? B[kk + nn * ldb] : 0.0;
+                    rb[u][1] = (nn < N) ? B[kk + 1 + nn * ldb] : 0.0;
+                }
+            } else {
+                const int k = idx >> 6, j2 = (idx & 63) * 2;
+                const i64 nn = n0 + j2, kk = k0 + k;
+                if (full) {
+                    rb[u] = *reinterpret_cast<const d2*>(B + nn + kk * ldb);
+                } else {
+                    rb[u][0] = (nn < N) ? B[nn + kk * ldb] : 0.0;
+                    rb[u][1] = (nn + 1 < N) ? B[nn + 1 + kk * ldb] : 0.0;
+                }
+            }
+        }
+    };
+    auto sstore = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = t + NT * u;
+            if (!TA) {
+                const int k = idx >> 6, i2 = (idx & 63) * 2;
+                *reinterpret_cast<d2*>(&As[b][k][i2]) = sgn * ra[u];
+            } else {
+                const int i = idx >> 3, k2 = (idx & 7) * 2;
+                As[b][k2][i] = sgn * ra[u][0];
+                As[b][k2 + 1][i] = sgn * ra[u][1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = t + NT * u;
+            if (!TB) {
+                const int n = idx >> 3, k2 = (idx & 7) * 2;
+                Bs[b][k2][n] = rb[u][0];
+                Bs[b][k2 + 1][n] = rb[u][1];
+            } else {
+                const int k = idx >> 6, j2 = (idx & 63) * 2;
+                *reinterpret_cast<d2*>(&Bs[b][k][j2]) = rb[u];
+            }
+        }
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int cur = 0;
+    for (i64 k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) gload(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double af[4], bfv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                af[q] = As[cur][kk + kq][wm + q * 16 + r16];
+                bfv[q] = Bs[cur][kk + kq][wn + q * 16 + r16];
+            }
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 4; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(bfv[ni], af[mi], acc[mi][ni],
+                                                                       0, 0, 0);
+        }
+        if (more) sstore(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+        const i64 m = m0 + wm + mi * 16 + r16;
+        if (!full && m >= M) continue;
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 n = n0 + wn + ni * 16 + kq + 4 * r;
+                if (!full && n >= N) continue;
+                if (tri == 1 && m < n) continue;
+                if (tri == 2 && m > n) continue;
+                double* c = C + m + n * ldc;
+                const double v = acc[mi][ni][r];
+                if (fold) *c = v;
+                else *c = (beta == 0.0) ? alpha * v : __builtin_fma(alpha, v, beta * *c);
+            }
+        }
+    }
+}
+
+// The hand-written kernels for the large FP64 products as well (environment ELX_DGEMM=own,
+// read at every call); rocBLAS otherwise.
+bool dgemm_own_selected() {
+    const char* v = std::getenv("ELX_DGEMM");
+    return v && std::strcmp(v, "own") == 0;
+}
+
 // C := beta C on the touched triangle (k == 0 or alpha == 0). Grid-stride over all entries.
 template <typename T>
 __global__ void scale_kernel(i64 M, i64 N, T beta, T* C, i64 ldc, int tri,
@@ -234,13 +421,14 @@ elx_status_t gemm_impl(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, T a
     }
     const double flops = (tri ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
     if constexpr (sizeof(T) == 8) {
-        // Large FP64 products are plain library GEMMs: rocBLAS runs the rank-128 trailing
-        // updates of the LU at 46-47 TFLOP/s against 34-35 for gemm_kernel (m = n =
-        // 8192..32640); the symmetric rank-k update of the Cholesky (A == B, one operand
-        // transposed, one triangle of C) goes to dsyrk. In a factorization sequence the
+        // Large FP64 products go to rocBLAS (dgemm, and dsyrk for the symmetric rank-k update:
+        // A == B, one operand transposed, one triangle of C) unless ELX_DGEMM=own: inside the
+        // LU at n = 32768 (leading dimension 2^15) the hand-written kernel ran the trailing
+        // updates in 779 ms against rocBLAS's 558 ms. In a factorization sequence the
         // coefficients come from the device (gate_kernel), so a recorded failure leaves C as it
         // is.
-        if (allow_library && (double)m * (double)n * (double)k >= (double)(1 << 24)) {
+        if (allow_library && !dgemm_own_selected() &&
+            (double)m * (double)n * (double)k >= (double)(1 << 24)) {
             const bool syrk = tri != 0 && A == B && lda == ldb && m == n && ta != tb && (tri == 1) == tb;
             if (tri == 0 || syrk) {
                 const double* coef = nullptr;
@@ -256,6 +444,35 @@ elx_status_t gemm_impl(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, T a
                     return library_dgemm(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, coef);
                 return library_dsyrk(s, tri == 2, ta, n, k, alpha, A, lda, beta, C, ldc, coef);
             }
+        }
+        // 16-byte operand loads and double-buffered slabs where the operands allow them
+        const bool vec = (k % BK == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
+                         (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
+                         (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+        if (vec) {
+            const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+            ELX_ARG(tm * tn < (i64)1 << 30);
+            ProfScope prof_(s, kKDgemm, flops);
+            const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
+            const int itm = (int)tm, itn = (int)tn;
+            const double* dA = reinterpret_cast<const double*>(A);
+            const double* dB = reinterpret_cast<const double*>(B);
+            double* dC = reinterpret_cast<double*>(C);
+            const double al = (double)alpha, be = (double)beta;
+            if (!ta && !tb)
+                hipLaunchKernelGGL((dgemm_fast_kernel<false, false>), dim3(blocks), dim3(NT), 0, s,
+                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
+            else if (!ta && tb)
+                hipLaunchKernelGGL((dgemm_fast_kernel<false, true>), dim3(blocks), dim3(NT), 0, s,
+                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
+            else if (ta && !tb)
+                hipLaunchKernelGGL((dgemm_fast_kernel<true, false>), dim3(blocks), dim3(NT), 0, s,
+                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
+            else
+                hipLaunchKernelGGL((dgemm_fast_kernel<true, true>), dim3(blocks), dim3(NT), 0, s,
+                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
+            ELX_HIP(hipGetLastError());
+            return ELX_SUCCESS;
         }
     }
     const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;

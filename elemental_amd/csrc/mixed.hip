@@ -99,6 +99,8 @@ struct Ctx {
     int32_t* perm;
     int2* pairs;  // (NB/32) x 256
     bool xcd_panels;  // one-XCD 32-column panels where they fit
+    int panel_mode;   // 0 partial pivoting, 16 / 32 tournament pivoting in blocks of that width
+    i64 calu_min_rows;  // tournament panels only from this height on
     int32_t* info;
     int* e_dev;
     unsigned* bits;
@@ -176,14 +178,21 @@ elx_status_t getrf_mixed(Ctx& c) {
         for (i64 j0 = K0; j0 < K1; j0 += P) {
             const i64 w = std::min<i64>(P, K1 - j0);
             int2* pr = c.pairs + ((j0 - K0) / P) * 256;
-            ELX_CHECK(launch_lu_panel_f32_rowmajor(c.s, n - j0, w, at(j0, j0), ld, c.dests + j0,
-                                                   j0, pr, c.info, xcd));
+            if (c.panel_mode != 0 && !xcd && n - j0 >= c.calu_min_rows &&
+                calu_panel_fits(n - j0, w, c.panel_mode))
+                ELX_CHECK(launch_calu_panel_f32_rowmajor(c.s, n - j0, w, at(j0, j0), ld,
+                                                         c.dests + j0, j0, pr, c.info,
+                                                         c.panel_mode));
+            else
+                ELX_CHECK(launch_lu_panel_f32_rowmajor(c.s, n - j0, w, at(j0, j0), ld,
+                                                       c.dests + j0, j0, pr, c.info, xcd));
             ELX_CHECK(launch_row_moves_f32_rowmajor(c.s, pr, (int)(2 * w), W, ld, j0, n - j0, K0,
                                                     j0, j0 + w, K1));
             const i64 wr = K1 - (j0 + w);
             if (wr > 0) {
-                // U12 = L11^{-1} A12 (rows of the working matrix), blocked substitution
-                ELX_CHECK(launch_trsm_rows(c.s, w, wr, at(j0, j0), at(j0, j0 + w), ld));
+                // U12 = L11^{-1} A12, as U12^T L11^T = A12^T on the stored transposes
+                ELX_CHECK(launch_trsm_small_f32(c.s, kRight, kUpper, kNormal, kUnit, wr, w,
+                                                at(j0, j0), ld, at(j0, j0 + w), ld));
                 ELX_CHECK(lp_update(c, n - (j0 + w), wr, w, at(j0 + w, j0), at(j0, j0 + w),
                                     at(j0 + w, j0 + w), ld, 128));
             }
@@ -340,6 +349,8 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     int32_t* idx = nullptr;
     ELX_CHECK(get(kSlotIdx, (size_t)(2 * n + 8 + (c.NB / 32 + 1) * 512), &idx));
     c.xcd_panels = xcd_panels_enabled();
+    c.panel_mode = mixed_panel_mode();
+    c.calu_min_rows = mixed_panel_min_rows();
     c.dests = idx;
     c.perm = idx + n;
     c.info = idx + 2 * n;

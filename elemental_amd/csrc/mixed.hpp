@@ -28,9 +28,6 @@ elx_status_t launch_iota(hipStream_t s, i64 n, int32_t* p);
 // X := Inv * X for an nb-row block X (nb <= 128) of ncols columns stored row by row
 // (X(k, j) at B[j + k*ldb]), in place.
 elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb);
-// U12 = L11^-1 A12 in place, row-by-row storage (L11 unit lower, order <= 128): blocked
-// substitution, 64 columns per workgroup.
-elx_status_t launch_trsm_rows(hipStream_t s, i64 w, i64 ncols, const float* L, float* B, i64 ld);
 // Persistent triangular solve with the factor stored row by row in W (fp32, ldw % 4 == 0,
 // padding columns zero).
 // xchunks: 64 x 16 bytes per 128-row block, uncached; epoch distinguishes this launch's

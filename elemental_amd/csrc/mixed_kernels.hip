@@ -169,70 +169,6 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(i64 n, const float* _
     }
 }
 
-// U12 = L11^-1 A12 in place for the row-by-row working matrix, by blocked substitution:
-// L11 (w x w, unit lower, w <= 128) at L[i*ld + p], A12 (w x ncols) at B[i*ld + c]. A
-// workgroup owns 64 columns: L11 and its 64-column slice of A12 are staged in LDS; per
-// 32-row block, wave 0 solves the diagonal block (a column per lane, the block's 32 values in
-// registers), then all waves subtract the block's contribution from the rows below (a row
-// per wave at a time, the block's solved values held in registers). Every sum is the
-// substitution's own, only grouped by 32-row blocks, so the accuracy is that of the
-// column-by-column solve.
-__global__ __launch_bounds__(256) void trsm_rows_kernel(int w, i64 ncols, const float* __restrict__ L,
-                                                        float* B, i64 ld) {
-    extern __shared__ __align__(16) float sm[];
-    constexpr int LL = 129, XL = 65;
-    float* Ls = sm;              // 128 x 129: Ls[i*LL + p] = L11(i, p)
-    float* Xs = sm + 128 * LL;   // 128 x 65:  Xs[i*XL + c]
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const i64 c0 = (i64)blockIdx.x * 64;
-    const int nc = (int)std::min<i64>(64, ncols - c0);
-    for (int idx = t; idx < w * w; idx += 256) {
-        const int i = idx / w, p = idx - i * w;
-        Ls[i * LL + p] = L[(i64)i * ld + p];
-    }
-    for (int idx = t; idx < w * 64; idx += 256) {
-        const int i = idx >> 6, c = idx & 63;
-        Xs[i * XL + c] = (c < nc) ? B[(i64)i * ld + c0 + c] : 0.0f;
-    }
-    __syncthreads();
-    for (int r0 = 0; r0 < w; r0 += 32) {
-        const int rb = min(32, w - r0);
-        if (wave == 0) {
-            float x[32];
-#pragma unroll
-            for (int i = 0; i < 32; ++i) x[i] = (i < rb) ? Xs[(r0 + i) * XL + lane] : 0.0f;
-#pragma unroll
-            for (int p = 0; p < 32; ++p) {
-#pragma unroll
-                for (int i = p + 1; i < 32; ++i)
-                    if (i < rb) x[i] = __builtin_fmaf(-Ls[(r0 + i) * LL + r0 + p], x[p], x[i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 32; ++i)
-                if (i < rb) Xs[(r0 + i) * XL + lane] = x[i];
-        }
-        __syncthreads();
-        if (r0 + rb < w) {
-            float xb[32];
-#pragma unroll
-            for (int p = 0; p < 32; ++p) xb[p] = (p < rb) ? Xs[(r0 + p) * XL + lane] : 0.0f;
-            for (int i = r0 + rb + wave; i < w; i += 4) {
-                float acc = Xs[i * XL + lane];
-                const float* li = Ls + i * LL + r0;
-#pragma unroll
-                for (int p = 0; p < 32; ++p)
-                    if (p < rb) acc = __builtin_fmaf(-li[p], xb[p], acc);
-                Xs[i * XL + lane] = acc;
-            }
-        }
-        __syncthreads();
-    }
-    for (int idx = t; idx < w * 64; idx += 256) {
-        const int i = idx >> 6, c = idx & 63;
-        if (c < nc) B[(i64)i * ld + c0 + c] = Xs[i * XL + c];
-    }
-}
-
 // X := Inv * X for one 128-row block X (nb x ncols) of a matrix stored row by row: X(k, j) at
 // B[j + k*ldb]. Inv is the 128 x 128 inverse of a diagonal block (column-major, rows and
 // columns >= nb zero). This is the triangular solve L11^{-1} A12 of the blocked LU as a product
@@ -803,24 +739,6 @@ elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv,
     }
     hipLaunchKernelGGL(trmm_inv_kernel, dim3((unsigned)cdiv(ncols, 64)), dim3(256), lds, s, (int)nb,
                        ncols, Inv, B, ldb);
-    ELX_HIP(hipGetLastError());
-    return ELX_SUCCESS;
-}
-
-elx_status_t launch_trsm_rows(hipStream_t s, i64 w, i64 ncols, const float* L, float* B, i64 ld) {
-    ELX_ARG(w >= 0 && w <= 128 && ncols >= 0 && ld >= w);
-    if (w == 0 || ncols == 0) return ELX_SUCCESS;
-    ProfScope prof_(s, kKTrsm, (double)w * (double)w * (double)ncols);
-    const size_t lds = (128 * 129 + 128 * 65) * sizeof(float);
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trsm_rows_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-        attr = true;
-    }
-    hipLaunchKernelGGL(trsm_rows_kernel, dim3((unsigned)cdiv(ncols, 64)), dim3(256), lds, s, (int)w,
-                       ncols, L, B, ld);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
