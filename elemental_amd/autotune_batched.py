@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
 BINS = [(16, 128), (129, 256), (257, 512)]
-MAX_OCC = [3, 2, 1]
+MAX_OCC = [3, 2, 2]
 NBS = [16, 32]
 
 

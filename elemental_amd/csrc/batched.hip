@@ -120,7 +120,9 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Minv) {
     constexpr int LPC = BNT / NB;   // lanes per column of the inverse of L11
     constexpr int RPL = NB / LPC;   // rows per lane of that inverse
     constexpr int SCAP = NM * LD;   // staging capacity (values)
-    constexpr int PV = NM <= 128 ? 16 : 32;  // staged values per thread per chunk
+    // staged values per thread per chunk (16 where the registers are halved for two
+    // workgroups per CU in the largest bin)
+    constexpr int PV = (NM <= 128 || (NM > 256 && NB == 16)) ? 16 : 32;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int lr = lane & 15, lk = lane >> 4;
     T* S = P;
@@ -323,15 +325,17 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Minv) {
     }
 }
 
-template <int NM>
+// Workgroups per CU a bin's kernel is compiled for (registers); LDS may allow fewer: the
+// n <= 512 bin fits two with 16-column blocks (72 KB each in FP64), one with 32.
+template <int NM, int NB>
 constexpr int bin_occ_max() {
-    return NM <= 128 ? 3 : (NM <= 256 ? 2 : 1);
+    return NM <= 128 ? 3 : (NM <= 256 ? 2 : (NB == 16 ? 2 : 1));
 }
 
 // Persistent workgroups of one size bin: items[begin, end) (orders <= NM), pulled through the
 // bin's queue word.
 template <typename T, int NM, int NB>
-__global__ __launch_bounds__(BNT, bin_occ_max<NM>()) void potrf_vbatched_kernel(
+__global__ __launch_bounds__(BNT, (bin_occ_max<NM, NB>())) void potrf_vbatched_kernel(
     const BatchItem* __restrict__ items, int begin, int end, T* const* __restrict__ As,
     int32_t* __restrict__ info, int* queue) {
     extern __shared__ __align__(16) unsigned char smem_raw[];

@@ -61,7 +61,7 @@ constexpr int VU = (WMAX + 2) / 3;         // value units of a full published ro
 constexpr int FULLU = 1 + VU;              // units per published row: header + values
 constexpr int UL = WMAX + 4;               // row stride of the block rows in LDS
 constexpr long long SPIN_LIMIT = 1 << 20;  // bounded waits: every wave exits
-constexpr long long kCaluMinRows = 4096;   // default: tournament panels from this height on
+constexpr long long kCaluMinOrder = 4096;  // default: tournament panels from this order on
 
 template <int WS>
 struct Blk {
@@ -141,16 +141,16 @@ __device__ __forceinline__ void static_for(Fn&& f) {
 template <int WS>
 __device__ __forceinline__ void wave_select(float (&v)[2][WS], bool (&act)[2], int ks, bool keep,
                                             int* order, int (&step)[2]) {
-    // All WS steps run as straight-line code (no branches, so the compiler can overlap one
-    // step's updates with the next step's reduction): rows that are not eliminated get a zero
-    // multiplier; steps past the ks wanted (a narrow block, or fewer candidates) choose rows
-    // the caller ignores, or none once no candidate is left (key 0: no row matches). The
-    // candidate chosen at step k is kept in lane k and stored at the end.
+    // Straight-line steps inside (no divergent branches): rows that are not eliminated get a
+    // zero multiplier, and the candidate chosen at step k is kept in lane k and stored at the
+    // end. Only the wave-uniform exit past ks steps branches.
     const int lane = threadIdx.x & 63;
     step[0] = step[1] = -1;
     int qk = -1;
+    bool live = true;  // wave-uniform: candidates remain
     static_for<0, WS>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
+        if (!live || k >= ks) return;
         uint32_t key = 0u;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -160,7 +160,11 @@ __device__ __forceinline__ void wave_select(float (&v)[2][WS], bool (&act)[2], i
             key = act[i] ? max(key, ki) : key;
         }
         key = wave_max_u32(key);
-        const int q = 255 - (int)(key & 0xFFu);  // 255 (no lane) when key == 0
+        if (key == 0u) {  // no candidate left
+            live = false;
+            return;
+        }
+        const int q = 255 - (int)(key & 0xFFu);
         const int pl = q & 63, ps = q >> 6;
         float u[WS];
         static_for<k, WS>([&](auto jc) {
@@ -174,10 +178,10 @@ __device__ __forceinline__ void wave_select(float (&v)[2][WS], bool (&act)[2], i
         const float inv = (piv != 0.0f) ? __builtin_amdgcn_rcpf(piv) : 0.0f;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const bool chosen = (i == ps) && (lane == pl) && act[i];
-            const bool elim = act[i] && !chosen && (key != 0u);
+            const bool chosen = (i == ps) && (lane == pl);
+            const bool elim = act[i] && !chosen;
             step[i] = chosen ? k : step[i];
-            act[i] = act[i] && !chosen;
+            act[i] = elim;
             const float l = elim ? v[i][k] * inv : 0.0f;
             if (keep) v[i][k] = elim ? l : v[i][k];
             static_for<k + 1, WS>([&](auto jc) {
@@ -187,7 +191,7 @@ __device__ __forceinline__ void wave_select(float (&v)[2][WS], bool (&act)[2], i
         }
         qk = (lane == k) ? q : qk;
     });
-    if (lane < ks) order[lane] = qk;
+    if (lane < ks && qk >= 0) order[lane] = qk;
 }
 
 // LDS layout (byte offsets, 16-byte aligned sections).
@@ -803,16 +807,17 @@ int mixed_panel_mode() {
     return 16;
 }
 
-// Tournament pivoting pays where partial pivoting's per-column exchange spans many workgroups
-// (tall panels), and its factors precondition less well (multipliers above 1), so by default
-// only panels of at least this many rows use it; an explicit ELX_MIXED_PANEL=calu16 / calu32
-// uses it for every panel. ELX_CALU_MIN_ROWS overrides the default height.
-i64 mixed_panel_min_rows() {
+// Tournament pivoting pays where the factorization dominates the solve (large systems, whose
+// panels spread over many workgroups), and its factors precondition less well (multipliers
+// above 1), so by default only systems of at least this order use it; an explicit
+// ELX_MIXED_PANEL=calu16 / calu32 uses it for every system. ELX_CALU_MIN_ORDER overrides the
+// default order.
+i64 mixed_panel_min_order() {
     const char* v = std::getenv("ELX_MIXED_PANEL");
     if (v && (std::strcmp(v, "calu16") == 0 || std::strcmp(v, "calu32") == 0)) return 0;
-    const char* m = std::getenv("ELX_CALU_MIN_ROWS");
+    const char* m = std::getenv("ELX_CALU_MIN_ORDER");
     if (m && *m) return std::max<i64>(0, std::atoll(m));
-    return kCaluMinRows;
+    return kCaluMinOrder;
 }
 
 bool calu_panel_fits(i64 mp, i64 w, int ws) {

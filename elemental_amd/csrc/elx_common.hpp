@@ -186,9 +186,9 @@ elx_status_t launch_calu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float*
                                             int ws);
 // Panel of the mixed-precision LU: 0 partial pivoting, 16 or 32 the tournament block width
 // (environment ELX_MIXED_PANEL = partial | calu16 | calu32, read at every solve; calu16 when
-// unset), used for panels of at least mixed_panel_min_rows() rows (partial pivoting below).
+// unset), used for systems of order at least mixed_panel_min_order() (partial pivoting below).
 int mixed_panel_mode();
-i64 mixed_panel_min_rows();
+i64 mixed_panel_min_order();
 // The row-move list applied to an fp32 matrix stored row by row (row i contiguous at
 // A + i*lda): columns [c0,c1) and [c2,c3) of each moved row, starting at row r0; moves that
 // reach outside the `rows` rows from r0 on are skipped.

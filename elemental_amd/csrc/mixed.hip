@@ -100,7 +100,7 @@ struct Ctx {
     int2* pairs;  // (NB/32) x 256
     bool xcd_panels;  // one-XCD 32-column panels where they fit
     int panel_mode;   // 0 partial pivoting, 16 / 32 tournament pivoting in blocks of that width
-    i64 calu_min_rows;  // tournament panels only from this height on
+    i64 calu_min_order;  // tournament panels only for systems of at least this order
     int32_t* info;
     int* e_dev;
     unsigned* bits;
@@ -178,7 +178,7 @@ elx_status_t getrf_mixed(Ctx& c) {
         for (i64 j0 = K0; j0 < K1; j0 += P) {
             const i64 w = std::min<i64>(P, K1 - j0);
             int2* pr = c.pairs + ((j0 - K0) / P) * 256;
-            if (c.panel_mode != 0 && !xcd && n - j0 >= c.calu_min_rows &&
+            if (c.panel_mode != 0 && !xcd && n >= c.calu_min_order &&
                 calu_panel_fits(n - j0, w, c.panel_mode))
                 ELX_CHECK(launch_calu_panel_f32_rowmajor(c.s, n - j0, w, at(j0, j0), ld,
                                                          c.dests + j0, j0, pr, c.info,
@@ -350,7 +350,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(get(kSlotIdx, (size_t)(2 * n + 8 + (c.NB / 32 + 1) * 512), &idx));
     c.xcd_panels = xcd_panels_enabled();
     c.panel_mode = mixed_panel_mode();
-    c.calu_min_rows = mixed_panel_min_rows();
+    c.calu_min_order = mixed_panel_min_order();
     c.dests = idx;
     c.perm = idx + n;
     c.info = idx + 2 * n;
