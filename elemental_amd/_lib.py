@@ -101,8 +101,8 @@ _SIGS = {
                                  c_vp],
     "elx_graph_launch": [c_vp, c_vp],
     "elx_graph_destroy": [c_vp],
-    "elx_batched_set_variant": [c_int, c_int, c_int, c_int],
-    "elx_batched_get_variant": [c_int, c_int, c_vp, c_vp],
+    "elx_batched_set_variant": [c_int, c_int, c_int, c_int, c_int],
+    "elx_batched_get_variant": [c_int, c_int, c_vp, c_vp, c_vp],
 }
 
 _lib = None

@@ -23,6 +23,10 @@ sys.path.insert(0, os.path.dirname(HERE))
 BINS = [(16, 128), (129, 256), (257, 512)]
 MAX_OCC = [3, 2, 2]
 NBS = [16, 32]
+# kernel kinds per bin: 0 block column staged in LDS, 1 operands read straight into registers
+# (bins 1 and 2; at most three workgroups per CU, as registers allow)
+KINDS = [[0], [0, 1], [0, 1]]
+RD_MAX_OCC = 3
 
 
 def make_batch(dtype, lo, hi, count, seed):
@@ -41,10 +45,10 @@ def make_batch(dtype, lo, hi, count, seed):
     return mats, [int(n) for n in ns]
 
 
-def time_variant(dtype, d, b, nb, occ, mats, saved, reps):
+def time_variant(dtype, d, b, nb, occ, mats, saved, reps, kind=0):
     import torch
     from elemental_amd import device as dev
-    dev.batched_variant(dtype, b, nb, occ)
+    dev.batched_variant(dtype, b, nb, occ, kind)
     batch = dev.Batch(mats, dtype)
     times = []
     for r in range(reps + 1):
@@ -84,22 +88,24 @@ def main():
             for s, m in zip(saved, mats):
                 s.copy_(m)
             best = None
-            for nb in NBS:
-                for occ in range(1, MAX_OCC[b] + 1):
-                    ms = time_variant(dtype, d, b, nb, occ, mats, saved, args.reps)
-                    tf = flops / (ms * 1e-3) / 1e12
-                    report.append({"dtype": str(dtype).split(".")[-1], "bin": [lo, hi], "nb": nb,
-                                   "occ": occ, "ms": round(ms, 4), "tflops": round(tf, 3)})
-                    print(json.dumps(report[-1]), flush=True)
-                    if best is None or ms < best[0]:
-                        best = (ms, nb, occ)
+            for kind in KINDS[b]:
+                for nb in NBS:
+                    for occ in range(1, (RD_MAX_OCC if kind else MAX_OCC[b]) + 1):
+                        ms = time_variant(dtype, d, b, nb, occ, mats, saved, args.reps, kind)
+                        tf = flops / (ms * 1e-3) / 1e12
+                        report.append({"dtype": str(dtype).split(".")[-1], "bin": [lo, hi],
+                                       "kind": kind, "nb": nb, "occ": occ, "ms": round(ms, 4),
+                                       "tflops": round(tf, 3)})
+                        print(json.dumps(report[-1]), flush=True)
+                        if best is None or ms < best[0]:
+                            best = (ms, nb, occ, kind)
             dev.batched_variant(dtype, b, 0)
-            table[d][b] = (best[1], best[2])
+            table[d][b] = (best[1], best[2], best[3])
             del mats, saved
             torch.cuda.empty_cache()
     rows = []
     for d in range(2):
-        rows.append("    {" + ", ".join("{%d, %d}" % v for v in table[d]) + "},")
+        rows.append("    {" + ", ".join("{%d, %d, %d}" % v for v in table[d]) + "},")
     text = """// Variant table of the batched Cholesky factorization, per element type and size bin.
 // Written by elemental_amd/autotune_batched.py from timings on one MI355X (see that script for
 // the workloads and the candidates); edit by running the script again, not by hand.
@@ -111,8 +117,9 @@ namespace elx {
 namespace tuned {
 
 struct BatchedVariant {
-    int nb;   // block-column width of the left-looking factorization (16 or 32)
-    int occ;  // persistent workgroups per CU (at most what the bin's LDS and registers allow)
+    int nb;    // block-column width of the left-looking factorization (16 or 32)
+    int occ;   // persistent workgroups per CU (at most what the bin's LDS and registers allow)
+    int kind;  // 0: block column staged in LDS; 1: operands read straight into registers
 };
 
 // [0: double, 1: float][bin 0: n <= 128, 1: n <= 256, 2: n <= 512]
@@ -123,10 +130,10 @@ constexpr BatchedVariant kBatchedPotrf[2][3] = {
 }  // namespace tuned
 }  // namespace elx
 """ % (time.strftime("%Y-%m-%d"), "\n".join(
-        "//   %s n in [%d, %d]: nb %d occ %d %.3f ms" % (r["dtype"], r["bin"][0], r["bin"][1], r["nb"],
-                                                r["occ"], r["ms"])
-        for r in report if (r["nb"], r["occ"]) == table[0 if r["dtype"] == "float64" else 1][
-            BINS.index(tuple(r["bin"]))]),
+        "//   %s n in [%d, %d]: kind %d nb %d occ %d %.3f ms" % (
+            r["dtype"], r["bin"][0], r["bin"][1], r["kind"], r["nb"], r["occ"], r["ms"])
+        for r in report if (r["nb"], r["occ"], r["kind"]) == table[
+            0 if r["dtype"] == "float64" else 1][BINS.index(tuple(r["bin"]))]),
         "\n".join(rows))
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as f:

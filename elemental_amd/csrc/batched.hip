@@ -25,6 +25,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "batched_tuning.hpp"
@@ -85,6 +86,15 @@ __device__ __forceinline__ double bcast(double v, int src) {
 }
 __device__ __forceinline__ float bcast(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = B .. E-1.
+template <int B, int E, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
 }
 
 // S[kk * SM + r] = A(k0 + r, c0 + kk) for r < m, kk < kc, zero for kc <= kk < kcp: threads
@@ -325,6 +335,188 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Minv) {
     }
 }
 
+// Register-direct form (variant kind 1) of potrf_one for the larger bins: the left-looking
+// update of a block column reads L straight from memory into MFMA operand registers, DEPTH
+// 4-column steps ahead, with no LDS staging and no barrier inside the product; the updated
+// block column goes back to A, the diagonal block is factored by wave 0 from there, and the
+// rows below are read back for their product with the inverse of L11. LDS holds only L11 and
+// its inverse, so several workgroups share a CU and one's serial diagonal factor overlaps
+// another's products.
+template <typename T, int NM, int NB>
+__device__ void potrf_rd_one(int n, T* A, int lda, int32_t* info) {
+    typedef typename Mma<T>::v4 v4;
+    constexpr int MT = NM / 64;   // row tiles per wave
+    constexpr int CT = NB / 16;   // column tiles of a block column
+    constexpr int LD = NB + 1;
+    constexpr int LPC = BNT / NB;
+    constexpr int RPL = NB / LPC;
+    // 4-column steps in flight (block columns are multiples of 16); 2 where the accumulators
+    // take half the registers (FP64, 512 rows x 32 columns)
+    constexpr int DEPTH = (NM * NB * (int)sizeof(T) >= 512 * 32 * 8) ? 2 : 4;
+    __shared__ T Dg[NB * LD];     // L11
+    __shared__ T Minv[NB * LD];   // Minv(k, j) = inv(L11)(j, k)
+    __shared__ int s_fail;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0), m = n - k0;
+        const int nrt = (m + 15) >> 4;
+        // (1) C = A(k0:n, k0:k0+kb) - L(k0:n, 0:k0) L(k0:k0+kb, 0:k0)^T; wave w owns row tiles
+        // w, w+4, ...
+        v4 acc[MT][CT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = (wave + 4 * u) * 16 + Mma<T>::drow(lane, r);
+                    const int col = ct * 16 + lr;
+                    acc[u][ct][r] = (row < m && col < kb) ? A[(k0 + row) + (size_t)(k0 + col) * lda]
+                                                          : T(0);
+                }
+        if (k0 > 0) {
+            T ra[DEPTH][MT], rb[DEPTH][CT];
+            auto load = [&](auto dc, int kk) {
+                constexpr int d = decltype(dc)::value;
+                const T* col = A + (size_t)(kk + lk) * lda + k0;
+#pragma unroll
+                for (int u = 0; u < MT; ++u) {
+                    const int row = (wave + 4 * u) * 16 + lr;
+                    ra[d][u] = (row < m) ? -col[row] : T(0);
+                }
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    const int c = ct * 16 + lr;
+                    rb[d][ct] = (c < kb) ? col[c] : T(0);
+                }
+            };
+            static_for<0, DEPTH>([&](auto dc) {
+                if (4 * decltype(dc)::value < k0) load(dc, 4 * decltype(dc)::value);
+            });
+            for (int kk = 0; kk < k0; kk += 4 * DEPTH) {
+                static_for<0, DEPTH>([&](auto dc) {
+                    constexpr int d = decltype(dc)::value;
+                    const int ks = kk + 4 * d;
+                    if (ks < k0) {
+#pragma unroll
+                        for (int u = 0; u < MT; ++u) {
+                            if (wave + 4 * u < nrt) {
+#pragma unroll
+                                for (int ct = 0; ct < CT; ++ct)
+                                    acc[u][ct] = Mma<T>::mma(ra[d][u], rb[d][ct], acc[u][ct]);
+                            }
+                        }
+                        if (ks + 4 * DEPTH < k0) load(dc, ks + 4 * DEPTH);
+                    }
+                });
+            }
+        }
+        // (2) the updated block column back to A (on and below the diagonal only)
+#pragma unroll
+        for (int u = 0; u < MT; ++u)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = (wave + 4 * u) * 16 + Mma<T>::drow(lane, r);
+                    const int col = ct * 16 + lr;
+                    if (row < m && col < kb && row >= col)
+                        A[(k0 + row) + (size_t)(k0 + col) * lda] = acc[u][ct][r];
+                }
+        __syncthreads();
+        // (3) the diagonal block, unblocked, by wave 0 (lane i holds row i; entries above the
+        // diagonal are never read into the result), as in potrf_one
+        if (wave == 0) {
+            T rw[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                rw[c] = (lane < kb && c <= lane) ? A[(k0 + lane) + (size_t)(k0 + c) * lda] : T(0);
+            int fail = -1;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (j < kb && fail < 0) {
+                    const T ajj = bcast(rw[j], j);
+                    if (ajj <= T(0)) {
+                        fail = j;
+                    } else {
+                        const T alpha = fsqrt(ajj);
+                        const T inv = T(1) / alpha;
+                        rw[j] = (lane == j) ? alpha : ((lane > j) ? rw[j] * inv : rw[j]);
+                        const T xi = rw[j];
+#pragma unroll
+                        for (int l = j + 1; l < NB; ++l) rw[l] = __builtin_fma(xi, -bcast(rw[j], l), rw[l]);
+                    }
+                }
+            }
+            if (lane < kb) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    Dg[lane * LD + c] = rw[c];
+                    if (c <= lane) A[(k0 + lane) + (size_t)(k0 + c) * lda] = rw[c];
+                }
+            }
+            if (lane == 0) s_fail = fail;
+        }
+        __syncthreads();
+        const int fail = s_fail;
+        if (fail >= 0) {
+            // NonHPDMatrixException at global column k0+fail: the diagonal block's lower
+            // triangle after the finished steps, the rows below updated by the earlier block
+            // columns (both already in A)
+            if (t == 0) *info = k0 + fail + 1;
+            __syncthreads();
+            return;
+        }
+        if (m > NB) {
+            // (4) inv(L11) by all threads, as in potrf_one
+            const int c = t / LPC, g = t % LPC;
+            T y[RPL];
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) y[q] = (g + LPC * q == c) ? T(1) : T(0);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int qk = k / LPC, gk = k % LPC;
+                if (g == gk) y[qk] = y[qk] / Dg[k * LD + k];
+                const T yk = __shfl(y[qk], (lane & ~(LPC - 1)) | gk);
+#pragma unroll
+                for (int q = 0; q < RPL; ++q) {
+                    const int i = g + LPC * q;
+                    if (i > k) y[q] = __builtin_fma(-Dg[i * LD + k], yk, y[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) Minv[c * LD + g + LPC * q] = y[q];
+            __syncthreads();
+            // (5) rows below: X = A21 L11^{-T} as one MFMA product with the inverse; each wave
+            // reads its row tiles completely before it overwrites them
+            for (int rt = CT + wave; rt < nrt; rt += 4) {
+                v4 acc2[CT];
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) acc2[ct] = v4{T(0), T(0), T(0), T(0)};
+                const int row = rt * 16 + lr;
+                T a[NB / 4];
+#pragma unroll
+                for (int kk = 0; kk < NB; kk += 4)
+                    a[kk / 4] = (row < m) ? A[(k0 + row) + (size_t)(k0 + kk + lk) * lda] : T(0);
+#pragma unroll
+                for (int kk = 0; kk < NB; kk += 4)
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct)
+                        acc2[ct] = Mma<T>::mma(a[kk / 4], Minv[(kk + lk) * LD + ct * 16 + lr], acc2[ct]);
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int rw2 = rt * 16 + Mma<T>::drow(lane, r);
+                        if (rw2 < m) A[(k0 + rw2) + (size_t)(k0 + ct * 16 + lr) * lda] = acc2[ct][r];
+                    }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Workgroups per CU a bin's kernel is compiled for (registers); LDS may allow fewer: the
 // n <= 512 bin fits two with 16-column blocks (72 KB each in FP64), one with 32.
 template <int NM, int NB>
@@ -351,6 +543,26 @@ __global__ __launch_bounds__(BNT, (bin_occ_max<NM, NB>())) void potrf_vbatched_k
         const BatchItem it = items[i];
         if (threadIdx.x == 0) info[it.index] = 0;
         if (it.n > 0) potrf_one<T, NM, NB>(it.n, As[it.index], it.lda, info + it.index, P, Minv);
+    }
+}
+
+// Register-direct persistent workgroups of one size bin (variant kind 1), compiled for two
+// workgroups per CU.
+constexpr int kRdOcc = 2;
+template <typename T, int NM, int NB>
+__global__ __launch_bounds__(BNT, kRdOcc) void potrf_rd_vbatched_kernel(
+    const BatchItem* __restrict__ items, int begin, int end, T* const* __restrict__ As,
+    int32_t* __restrict__ info, int* queue) {
+    __shared__ int s_i;
+    for (;;) {
+        if (threadIdx.x == 0) s_i = begin + atomicAdd(queue, 1);
+        __syncthreads();
+        const int i = s_i;
+        __syncthreads();
+        if (i >= end) break;
+        const BatchItem it = items[i];
+        if (threadIdx.x == 0) info[it.index] = 0;
+        if (it.n > 0) potrf_rd_one<T, NM, NB>(it.n, As[it.index], it.lda, info + it.index);
     }
 }
 
@@ -695,6 +907,21 @@ elx_status_t launch_bin(hipStream_t s, const BatchItem* items, int lo, int hi, T
     return ELX_SUCCESS;
 }
 
+template <typename T, int NM, int NB>
+elx_status_t launch_rd_bin(hipStream_t s, const BatchItem* items, int lo, int hi, T* const* dA,
+                           int32_t* dinfo, int* queue, int occ) {
+    const void* fn = reinterpret_cast<const void*>(&potrf_rd_vbatched_kernel<T, NM, NB>);
+    int per_cu = 0;
+    ELX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BNT, 0));
+    per_cu = std::max(1, std::min(per_cu, occ));
+    const elx::DeviceInfo& di = elx::device_info();
+    const int G = (int)std::min<int64_t>(hi - lo, (int64_t)per_cu * di.num_cus);
+    hipLaunchKernelGGL((potrf_rd_vbatched_kernel<T, NM, NB>), dim3(G), dim3(BNT), 0, s, items, lo,
+                       hi, dA, dinfo, queue);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
 template <typename T>
 elx_status_t launch_factor(hipStream_t s, const Schedule& sc, T* const* dA, int32_t* dinfo,
                            int* queue) {
@@ -712,11 +939,23 @@ elx_status_t launch_factor(hipStream_t s, const Schedule& sc, T* const* dA, int3
             if (v.nb == 16) ELX_CHECK((launch_bin<T, 128, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
             else ELX_CHECK((launch_bin<T, 128, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
         } else if (b == 1) {
-            if (v.nb == 16) ELX_CHECK((launch_bin<T, 256, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
-            else ELX_CHECK((launch_bin<T, 256, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            if (v.kind == 1 && v.nb == 16)
+                ELX_CHECK((launch_rd_bin<T, 256, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else if (v.kind == 1)
+                ELX_CHECK((launch_rd_bin<T, 256, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else if (v.nb == 16)
+                ELX_CHECK((launch_bin<T, 256, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else
+                ELX_CHECK((launch_bin<T, 256, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
         } else {
-            if (v.nb == 16) ELX_CHECK((launch_bin<T, 512, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
-            else ELX_CHECK((launch_bin<T, 512, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            if (v.kind == 1 && v.nb == 16)
+                ELX_CHECK((launch_rd_bin<T, 512, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else if (v.kind == 1)
+                ELX_CHECK((launch_rd_bin<T, 512, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else if (v.nb == 16)
+                ELX_CHECK((launch_bin<T, 512, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else
+                ELX_CHECK((launch_bin<T, 512, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
         }
     }
     return ELX_SUCCESS;
@@ -887,23 +1126,26 @@ elx_status_t elx_graph_destroy(elx_graph_t graph) {
     return ELX_SUCCESS;
 }
 
-elx_status_t elx_batched_set_variant(int dtype, int bin, int nb, int occ) {
+elx_status_t elx_batched_set_variant(int dtype, int bin, int nb, int occ, int kind) {
     ELX_ARG(dtype == 0 || dtype == 1);
     ELX_ARG(bin >= 0 && bin < NBIN);
     ELX_ARG(nb == 0 || nb == 16 || nb == 32);
     ELX_ARG(nb == 0 || (occ >= 1 && occ <= 8));
+    ELX_ARG(nb == 0 || kind == 0 || (kind == 1 && bin >= 1));
     std::lock_guard<std::mutex> lk(g_bmu);
-    g_override[dtype][bin] = elx::tuned::BatchedVariant{nb, nb ? occ : 0};
+    g_override[dtype][bin] =
+        elx::tuned::BatchedVariant{nb, nb ? occ : 0, nb ? kind : 0};
     return ELX_SUCCESS;
 }
 
-elx_status_t elx_batched_get_variant(int dtype, int bin, int* nb, int* occ) {
+elx_status_t elx_batched_get_variant(int dtype, int bin, int* nb, int* occ, int* kind) {
     ELX_ARG(dtype == 0 || dtype == 1);
-    ELX_ARG(bin >= 0 && bin < NBIN && nb && occ);
+    ELX_ARG(bin >= 0 && bin < NBIN && nb && occ && kind);
     const elx::tuned::BatchedVariant v =
         dtype == 0 ? variant_of<double>(bin) : variant_of<float>(bin);
     *nb = v.nb;
     *occ = v.occ;
+    *kind = v.kind;
     return ELX_SUCCESS;
 }
 

@@ -351,16 +351,17 @@ class PosvGraph:
             self.handle = None
 
 
-def batched_variant(dtype, bin_index, nb=None, occ=None):
-    """Reads (nb, occ) of a batched-Cholesky size bin; with nb given, sets it first (nb 0
-    restores the built-in table)."""
+def batched_variant(dtype, bin_index, nb=None, occ=None, kind=0):
+    """Reads (nb, occ, kind) of a batched-Cholesky size bin; with nb given, sets it first (nb 0
+    restores the built-in table; kind 1 = operands read straight into registers, bins 1-2)."""
     import ctypes
     d = 0 if dtype == torch.float64 else 1
     if nb is not None:
-        call("elx_batched_set_variant", d, bin_index, nb, occ or 0)
-    a, b = ctypes.c_int(), ctypes.c_int()
-    call("elx_batched_get_variant", d, bin_index, ctypes.byref(a), ctypes.byref(b))
-    return a.value, b.value
+        call("elx_batched_set_variant", d, bin_index, nb, occ or 0, kind)
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    call("elx_batched_get_variant", d, bin_index, ctypes.byref(a), ctypes.byref(b),
+         ctypes.byref(c))
+    return a.value, b.value, c.value
 
 
 def gesv_mixed_batched(As, Bs, Xs, factor="fp16", refine="auto", tol=0.0, max_iters=0,

@@ -107,19 +107,22 @@ def test_batched_empty_and_zero_sizes(dev):
     assert (info == 0).all() and np.array_equal(dev.to_host(mats[1]), np.eye(3) * 2.0)
 
 
+@pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("nb", [16, 32])
-def test_batched_cholesky_every_variant(dev, orc, nb):
+def test_batched_cholesky_every_variant(dev, orc, nb, kind):
     """Each size bin's kernel with both block widths (the autotuner's candidates), one and
-    two or three workgroups per CU: same bars as above, and the non-HPD column exact."""
-    rng = np.random.default_rng(43 + nb)
+    two or three workgroups per CU, both kernel kinds (the register-direct one in bins 1 and 2):
+    same bars as above, and the non-HPD column exact."""
+    rng = np.random.default_rng(43 + nb + kind)
     ns = [1, 15, 16, 17, 33, 100, 128, 129, 200, 256, 257, 300, 511, 512]
     As = [spd(n, rng) for n in ns]
     As[9][70, 70] = -1e3  # n = 256: not positive definite at column 70
     try:
         for occ in (1, 2):
             for b in range(3):
-                dev.batched_variant(torch.float64, b, nb, min(occ, 3 - b))
-            say("batched variant", nb, occ)
+                dev.batched_variant(torch.float64, b, nb, min(occ, 3 - b) if not kind else occ,
+                                    kind if b >= 1 else 0)
+            say("batched variant", nb, occ, kind)
             mats = [dev.to_device(A) for A in As]
             info = dev.potrf_vbatched(dev.Batch(mats, torch.float64)).cpu().numpy()
             for i, (A, n) in enumerate(zip(As, ns)):
@@ -136,6 +139,35 @@ def test_batched_cholesky_every_variant(dev, orc, nb):
         for b in range(3):
             dev.batched_variant(torch.float64, b, 0)
     assert dev.batched_variant(torch.float64, 0)[0] in (16, 32)
+
+
+@pytest.mark.parametrize("nb", [16, 32])
+def test_batched_register_direct_float32(dev, orc, nb):
+    """The register-direct kernel kind in FP32 (bins 1 and 2, one and two workgroups per CU):
+    factor error within 8 n u32 of the oracle's bar, the non-HPD column exact."""
+    rng = np.random.default_rng(61 + nb)
+    ns = [129, 160, 200, 255, 256, 257, 300, 384, 511, 512]
+    As = [spd(n, rng) for n in ns]
+    As[6][90, 90] = -1e3  # n = 300: not positive definite at column 90
+    try:
+        for occ in (1, 2):
+            for b in (1, 2):
+                dev.batched_variant(torch.float32, b, nb, occ, 1)
+            say("batched fp32 register-direct", nb, occ)
+            mats = [dev.to_device(A, dtype=torch.float32) for A in As]
+            info = dev.potrf_vbatched(dev.Batch(mats, torch.float32)).cpu().numpy()
+            for i, (A, n) in enumerate(zip(As, ns)):
+                _, info_o = orc.cholesky(A, uplo=0)
+                assert info[i] == info_o, (i, n, info[i], info_o)
+                if info_o:
+                    assert info_o == 91
+                    continue
+                F = dev.to_host(mats[i]).astype(np.float64)
+                err = orc.chol_factor_error(A, np.asfortranarray(F), 0)
+                assert err <= 8 * n * EPS32, (i, n, err)
+    finally:
+        for b in range(3):
+            dev.batched_variant(torch.float32, b, 0)
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
