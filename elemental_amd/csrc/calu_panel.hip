@@ -37,6 +37,7 @@
 // then falls back to FP64).
 #include <algorithm>
 #include <climits>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -62,6 +63,7 @@ constexpr int FULLU = 1 + VU;              // units per published row: header + 
 constexpr int UL = WMAX + 4;               // row stride of the block rows in LDS
 constexpr long long SPIN_LIMIT = 1 << 20;  // bounded waits: every wave exits
 constexpr long long kCaluMinOrder = 4096;  // default: tournament panels from this order on
+constexpr long long kCaluMaxRhs = 4;      // ... and for at most this many right-hand sides
 
 template <int WS>
 struct Blk {
@@ -812,6 +814,18 @@ int mixed_panel_mode() {
 // above 1), so by default only systems of at least this order use it; an explicit
 // ELX_MIXED_PANEL=calu16 / calu32 uses it for every system. ELX_CALU_MIN_ORDER overrides the
 // default order.
+// Right-hand sides up to which the default uses tournament panels: each IR/GMRES step costs
+// about 3.3 ms per column at n = 32768 and the tournament's factors need about five more of
+// them per column, against about 80 ms saved in the panels (ELX_CALU_MAX_RHS overrides; an
+// explicit ELX_MIXED_PANEL=calu16 / calu32 has no limit).
+i64 mixed_panel_max_rhs() {
+    const char* v = std::getenv("ELX_MIXED_PANEL");
+    if (v && (std::strcmp(v, "calu16") == 0 || std::strcmp(v, "calu32") == 0)) return INT64_MAX;
+    const char* m = std::getenv("ELX_CALU_MAX_RHS");
+    if (m && *m) return std::max<i64>(0, std::atoll(m));
+    return kCaluMaxRhs;
+}
+
 i64 mixed_panel_min_order() {
     const char* v = std::getenv("ELX_MIXED_PANEL");
     if (v && (std::strcmp(v, "calu16") == 0 || std::strcmp(v, "calu32") == 0)) return 0;

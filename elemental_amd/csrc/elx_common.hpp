@@ -189,6 +189,7 @@ elx_status_t launch_calu_panel_f32_rowmajor(hipStream_t s, i64 mp, i64 w, float*
 // unset), used for systems of order at least mixed_panel_min_order() (partial pivoting below).
 int mixed_panel_mode();
 i64 mixed_panel_min_order();
+i64 mixed_panel_max_rhs();  // and for at most this many right-hand sides
 // The row-move list applied to an fp32 matrix stored row by row (row i contiguous at
 // A + i*lda): columns [c0,c1) and [c2,c3) of each moved row, starting at row r0; moves that
 // reach outside the `rows` rows from r0 on are skipped.

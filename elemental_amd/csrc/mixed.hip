@@ -351,6 +351,9 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     c.xcd_panels = xcd_panels_enabled();
     c.panel_mode = mixed_panel_mode();
     c.calu_min_order = mixed_panel_min_order();
+    // the refinement's cost grows with the right-hand sides and the tournament's weaker
+    // factors need more steps: beyond a few columns partial pivoting pays
+    if (nrhs > mixed_panel_max_rhs()) c.panel_mode = 0;
     c.dests = idx;
     c.perm = idx + n;
     c.info = idx + 2 * n;
