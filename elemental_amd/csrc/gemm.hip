@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "elx_common.hpp"
 
@@ -196,59 +197,24 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(i64 M, i64 N, i64 K, T alph
     }
 }
 
-// FP64 C := alpha op(A) op(B) + beta C (the LU's Gemm(NORMAL, NORMAL) update, the Cholesky's
-// Herk(LOWER, NORMAL) = A A^T and Herk(UPPER, ADJOINT) = A^T A updates on one triangle), for K a
-// multiple of 16 and operands whose columns start on 16-byte boundaries (even leading
-// dimensions, aligned bases).
-// Same 128x128 tile, 4x4 MFMA accumulators per wave and XCD-aware tile order as gemm_kernel,
-// but with 16-byte operand loads and double-buffered LDS slabs: the next slab is fetched into
-// registers while the current one feeds the MFMAs and is then stored into the other buffer, so
-// each 16-deep step costs one barrier. Edge tiles (partial in m or n) load element by element.
-template <bool TA, bool TB>
-__global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, double alpha,
-                                                        const double* __restrict__ A, i64 lda,
-                                                        const double* __restrict__ B, i64 ldb,
-                                                        double beta, double* __restrict__ C,
-                                                        i64 ldc, int tri, int tiles_m, int tiles_n,
-                                                        const int32_t* __restrict__ stop) {
+// One 128x128 tile of dgemm_fast_kernel; FULL: the tile lies inside C, every load and store
+// unchecked (a compile-time branch, so the K loop holds no per-tile conditions).
+template <bool TA, bool TB, bool FULL>
+__device__ __forceinline__ void dgemm_fast_tile(i64 M, i64 N, i64 K, double alpha,
+                                                const double* __restrict__ A, i64 lda,
+                                                const double* __restrict__ B, i64 ldb,
+                                                double beta, double* __restrict__ C, i64 ldc,
+                                                int tri, i64 m0, i64 n0,
+                                                double (*As)[BK][BM + PAD],
+                                                double (*Bs)[BK][BN + PAD]) {
     typedef double d2 __attribute__((ext_vector_type(2)));
-    __shared__ __align__(16) double As[2][BK][BM + PAD];  // As[b][k][i] = sgn A(m0+i, k0+k)
-    __shared__ __align__(16) double Bs[2][BK][BN + PAD];  // Bs[b][k][j] = op(B)(k0+k, n0+j)
-    if (stop && *stop != 0) return;
-    const int total = tiles_m * tiles_n;
-    const int per_xcd = (total + 7) / 8;
-    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (lin >= total) return;
-    constexpr int GROUP = 8;
-    const int group = lin / (GROUP * tiles_n);
-    const int first_m = group * GROUP;
-    const int gsize = min(tiles_m - first_m, GROUP);
-    const int tm = first_m + (lin % (GROUP * tiles_n)) % gsize;
-    const int tn = (lin % (GROUP * tiles_n)) / gsize;
-    const i64 m0 = (i64)tm * BM, n0 = (i64)tn * BN;
-    if (tri == 1 && n0 > m0 + BM - 1) return;
-    if (tri == 2 && m0 > n0 + BN - 1) return;
-    const bool full = (m0 + BM <= M) && (n0 + BN <= N);
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     const int r16 = lane & 15, kq = lane >> 4;
     const bool fold = (beta == 1.0) && (alpha == 1.0 || alpha == -1.0);
-    const double sgn = fold ? alpha : 1.0;
+    const bool neg = fold && alpha < 0.0;  // alpha folded into the A operand (exact)
     d4 acc[4][4];
-#pragma unroll
-    for (int mi = 0; mi < 4; ++mi) {
-        const i64 m = m0 + wm + mi * 16 + r16;
-#pragma unroll
-        for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const i64 n = n0 + wn + ni * 16 + kq + 4 * r;
-                const bool ok = fold && (full || (m < M && n < N)) && !(tri == 1 && m < n) &&
-                                !(tri == 2 && m > n);
-                acc[mi][ni][r] = ok ? C[m + n * ldc] : 0.0;
-            }
-    }
     // slab loads: A as 4 pairs along m per thread (16 k x 64 pairs); B as 4 pairs along k
     // (NN: 128 n x 8 pairs) or along n (NT: 16 k x 64 pairs)
     d2 ra[4], rb[4];
@@ -259,7 +225,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
             if (!TA) {
                 const int k = idx >> 6, i2 = (idx & 63) * 2;
                 const i64 m = m0 + i2, kk = k0 + k;
-                if (full) {
+                if (FULL) {
                     ra[u] = *reinterpret_cast<const d2*>(A + m + kk * lda);
                 } else {
                     ra[u][0] = (m < M) ? A[m + kk * lda] : 0.0;
@@ -268,7 +234,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
             } else {
                 const int i = idx >> 3, k2 = (idx & 7) * 2;
                 const i64 m = m0 + i, kk = k0 + k2;
-                if (full) {
+                if (FULL) {
                     ra[u] = *reinterpret_cast<const d2*>(A + kk + m * lda);
                 } else {
                     ra[u][0] = (m < M) ? A[kk + m * lda] : 0.0;
@@ -282,7 +248,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
             if (!TB) {
                 const int n = idx >> 3, k2 = (idx & 7) * 2;
                 const i64 nn = n0 + n, kk = k0 + k2;
-                if (full) {
+                if (FULL) {
                     rb[u] = *reinterpret_cast<const d2*>(B + kk + nn * ldb);
                 } else {
                     rb[u][0] = (nn < N) ? B[kk + nn * ldb] : 0.0;
@@ -291,7 +257,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
             } else {
                 const int k = idx >> 6, j2 = (idx & 63) * 2;
                 const i64 nn = n0 + j2, kk = k0 + k;
-                if (full) {
+                if (FULL) {
                     rb[u] = *reinterpret_cast<const d2*>(B + nn + kk * ldb);
                 } else {
                     rb[u][0] = (nn < N) ? B[nn + kk * ldb] : 0.0;
@@ -306,11 +272,11 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
             const int idx = t + NT * u;
             if (!TA) {
                 const int k = idx >> 6, i2 = (idx & 63) * 2;
-                *reinterpret_cast<d2*>(&As[b][k][i2]) = sgn * ra[u];
+                *reinterpret_cast<d2*>(&As[b][k][i2]) = neg ? -ra[u] : ra[u];
             } else {
                 const int i = idx >> 3, k2 = (idx & 7) * 2;
-                As[b][k2][i] = sgn * ra[u][0];
-                As[b][k2 + 1][i] = sgn * ra[u][1];
+                As[b][k2][i] = neg ? -ra[u][0] : ra[u][0];
+                As[b][k2 + 1][i] = neg ? -ra[u][1] : ra[u][1];
             }
         }
 #pragma unroll
@@ -326,7 +292,33 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
             }
         }
     };
+    // the first slab's loads go out before C's, so that storing the slab waits for them alone
+    // (loads complete in order) and C's 64 loads per thread stay in flight into the first steps
     gload(0);
+    if (FULL && fold && tri == 0) {
+        // the whole tile, no per-entry conditions (the LU's trailing update)
+        const double* c0 = C + m0 + wm + r16 + (n0 + wn + kq) * ldc;
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[mi][ni][r] = c0[mi * 16 + (ni * 16 + 4 * r) * ldc];
+    } else {
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) {
+            const i64 m = m0 + wm + mi * 16 + r16;
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const i64 n = n0 + wn + ni * 16 + kq + 4 * r;
+                    const bool ok = fold && (FULL || (m < M && n < N)) &&
+                                    !(tri == 1 && m < n) && !(tri == 2 && m > n);
+                    acc[mi][ni][r] = ok ? C[m + n * ldc] : 0.0;
+                }
+        }
+    }
     sstore(0);
     __syncthreads();
     int cur = 0;
@@ -355,13 +347,13 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
         const i64 m = m0 + wm + mi * 16 + r16;
-        if (!full && m >= M) continue;
+        if (!FULL && m >= M) continue;
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const i64 n = n0 + wn + ni * 16 + kq + 4 * r;
-                if (!full && n >= N) continue;
+                if (!FULL && n >= N) continue;
                 if (tri == 1 && m < n) continue;
                 if (tri == 2 && m > n) continue;
                 double* c = C + m + n * ldc;
@@ -371,6 +363,45 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
             }
         }
     }
+}
+
+
+// FP64 C := alpha op(A) op(B) + beta C (the LU's Gemm(NORMAL, NORMAL) update, the Cholesky's
+// Herk(LOWER, NORMAL) = A A^T and Herk(UPPER, ADJOINT) = A^T A updates on one triangle), for K a
+// multiple of 16 and operands whose columns start on 16-byte boundaries (even leading
+// dimensions, aligned bases).
+// Same 128x128 tile, 4x4 MFMA accumulators per wave and XCD-aware tile order as gemm_kernel,
+// but with 16-byte operand loads and double-buffered LDS slabs: the next slab is fetched into
+// registers while the current one feeds the MFMAs and is then stored into the other buffer, so
+// each 16-deep step costs one barrier. Edge tiles (partial in m or n) load element by element.
+template <bool TA, bool TB, bool FULL>
+__global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, double alpha,
+                                                        const double* __restrict__ A, i64 lda,
+                                                        const double* __restrict__ B, i64 ldb,
+                                                        double beta, double* __restrict__ C,
+                                                        i64 ldc, int tri, int tiles_m, int tiles_n,
+                                                        const int32_t* __restrict__ stop) {
+    if (stop && *stop != 0) return;
+    const int total = tiles_m * tiles_n;
+    const int per_xcd = (total + 7) / 8;
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (lin >= total) return;
+    constexpr int GROUP = 8;
+    const int group = lin / (GROUP * tiles_n);
+    const int first_m = group * GROUP;
+    const int gsize = min(tiles_m - first_m, GROUP);
+    const int tm = first_m + (lin % (GROUP * tiles_n)) % gsize;
+    const int tn = (lin % (GROUP * tiles_n)) / gsize;
+    const i64 m0 = (i64)tm * BM, n0 = (i64)tn * BN;
+    if (tri == 1 && n0 > m0 + BM - 1) return;
+    if (tri == 2 && m0 > n0 + BN - 1) return;
+    // As[b][k][i] = sgn op(A)(m0+i, k0+k), Bs[b][k][j] = op(B)(k0+k, n0+j), double-buffered
+    __shared__ __align__(16) double As[2][BK][BM + PAD];
+    __shared__ __align__(16) double Bs[2][BK][BN + PAD];
+    // FULL: launched over the tiles inside C only; otherwise over every tile, the edge tiles
+    // done here and the full ones left to the FULL launch
+    if (!FULL && (m0 + BM <= M) && (n0 + BN <= N)) return;
+    dgemm_fast_tile<TA, TB, FULL>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri, m0, n0, As, Bs);
 }
 
 // The hand-written kernels for the large FP64 products as well (environment ELX_DGEMM=own,
@@ -450,27 +481,35 @@ elx_status_t gemm_impl(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, T a
                          (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
                          (reinterpret_cast<uintptr_t>(B) % 16 == 0);
         if (vec) {
-            const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
-            ELX_ARG(tm * tn < (i64)1 << 30);
             ProfScope prof_(s, kKDgemm, flops);
-            const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
-            const int itm = (int)tm, itn = (int)tn;
             const double* dA = reinterpret_cast<const double*>(A);
             const double* dB = reinterpret_cast<const double*>(B);
             double* dC = reinterpret_cast<double*>(C);
             const double al = (double)alpha, be = (double)beta;
-            if (!ta && !tb)
-                hipLaunchKernelGGL((dgemm_fast_kernel<false, false>), dim3(blocks), dim3(NT), 0, s,
-                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
-            else if (!ta && tb)
-                hipLaunchKernelGGL((dgemm_fast_kernel<false, true>), dim3(blocks), dim3(NT), 0, s,
-                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
-            else if (ta && !tb)
-                hipLaunchKernelGGL((dgemm_fast_kernel<true, false>), dim3(blocks), dim3(NT), 0, s,
-                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
-            else
-                hipLaunchKernelGGL((dgemm_fast_kernel<true, true>), dim3(blocks), dim3(NT), 0, s,
-                                   m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, itm, itn, stop);
+            // interior tiles (unchecked), then, if C has partial edge tiles, the checked launch
+            const i64 fm = m / BM, fn = n / BN;
+            const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+            ELX_ARG(tm * tn < (i64)1 << 30);
+            auto go = [&](auto full_tag, i64 gm, i64 gn) {
+                constexpr bool F = decltype(full_tag)::value;
+                if (gm <= 0 || gn <= 0) return;
+                const unsigned blocks = (unsigned)(((gm * gn + 7) / 8) * 8);
+                const int igm = (int)gm, ign = (int)gn;
+                if (!ta && !tb)
+                    hipLaunchKernelGGL((dgemm_fast_kernel<false, false, F>), dim3(blocks), dim3(NT), 0,
+                                       s, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, igm, ign, stop);
+                else if (!ta && tb)
+                    hipLaunchKernelGGL((dgemm_fast_kernel<false, true, F>), dim3(blocks), dim3(NT), 0,
+                                       s, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, igm, ign, stop);
+                else if (ta && !tb)
+                    hipLaunchKernelGGL((dgemm_fast_kernel<true, false, F>), dim3(blocks), dim3(NT), 0,
+                                       s, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, igm, ign, stop);
+                else
+                    hipLaunchKernelGGL((dgemm_fast_kernel<true, true, F>), dim3(blocks), dim3(NT), 0,
+                                       s, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, tri, igm, ign, stop);
+            };
+            go(std::integral_constant<bool, true>{}, fm, fn);
+            if (fm != tm || fn != tn) go(std::integral_constant<bool, false>{}, tm, tn);
             ELX_HIP(hipGetLastError());
             return ELX_SUCCESS;
         }
