@@ -88,6 +88,12 @@ __device__ __forceinline__ float bcast(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
+// A matrix in global memory. The matrices' pointers come from a device array, so without the
+// address space the compiler emits flat loads, which also count as LDS operations: every wait
+// for an LDS read would then wait for the next chunk's loads still in flight as well.
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+
 // Compile-time loop: f(std::integral_constant<int, i>) for i = B .. E-1.
 template <int B, int E, typename Fn>
 __device__ __forceinline__ void static_for(Fn&& f) {
@@ -100,7 +106,7 @@ __device__ __forceinline__ void static_for(Fn&& f) {
 // S[kk * SM + r] = A(k0 + r, c0 + kk) for r < m, kk < kc, zero for kc <= kk < kcp: threads
 // walk the (row, column) pairs column by column (coalesced), 16 loads in flight per thread.
 template <typename T>
-__device__ __forceinline__ void stage_cols(const T* A, int lda, int k0, int m, int c0, int kc,
+__device__ __forceinline__ void stage_cols(const gmem<T>* A, int lda, int k0, int m, int c0, int kc,
                                            int kcp, int SM, T* S) {
     const int total = m * kcp;
     for (int base = threadIdx.x; base < total; base += 16 * BNT) {
@@ -122,7 +128,7 @@ __device__ __forceinline__ void stage_cols(const T* A, int lda, int k0, int m, i
 // One workgroup: the lower Cholesky factor of one matrix of order n <= NM, in place. P is
 // NM x (NB+1) values of LDS (the block column, also the staging area), Minv (NB+1) x NB.
 template <typename T, int NM, int NB>
-__device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Minv) {
+__device__ void potrf_one(int n, gmem<T>* A, int lda, int32_t* info, T* P, T* Minv) {
     typedef typename Mma<T>::v4 v4;
     constexpr int MT = NM / 64;     // row tiles per wave (NM/16 tiles over 4 waves)
     constexpr int CT = NB / 16;     // column tiles of a block column
@@ -343,7 +349,7 @@ __device__ void potrf_one(int n, T* A, int lda, int32_t* info, T* P, T* Minv) {
 // its inverse, so several workgroups share a CU and one's serial diagonal factor overlaps
 // another's products.
 template <typename T, int NM, int NB>
-__device__ void potrf_rd_one(int n, T* A, int lda, int32_t* info) {
+__device__ void potrf_rd_one(int n, gmem<T>* A, int lda, int32_t* info) {
     typedef typename Mma<T>::v4 v4;
     constexpr int MT = NM / 64;   // row tiles per wave
     constexpr int CT = NB / 16;   // column tiles of a block column
@@ -379,7 +385,7 @@ __device__ void potrf_rd_one(int n, T* A, int lda, int32_t* info) {
             T ra[DEPTH][MT], rb[DEPTH][CT];
             auto load = [&](auto dc, int kk) {
                 constexpr int d = decltype(dc)::value;
-                const T* col = A + (size_t)(kk + lk) * lda + k0;
+                const gmem<T>* col = A + (size_t)(kk + lk) * lda + k0;
 #pragma unroll
                 for (int u = 0; u < MT; ++u) {
                     const int row = (wave + 4 * u) * 16 + lr;
@@ -517,6 +523,340 @@ __device__ void potrf_rd_one(int n, T* A, int lda, int32_t* info) {
     }
 }
 
+// Wide-panel form (variant kind 2) for the larger bins. The 16- or 32-column block columns
+// above stream every earlier column once per block column, at 4-8 flops per byte read, which
+// bounds the n <= 512 bin by memory; here the left-looking update runs over 64-column panels
+// (16 flops per byte). A panel is updated in passes of up to 256 rows, the pass holding the
+// diagonal block last: a 16-column chunk of the earlier columns, for the pass's rows and for the
+// panel's first 64 rows, goes through LDS while the next chunk's loads are in flight in
+// registers. The accumulators hold the panel transposed (MFMA rows = panel columns), which is
+// the operand layout of the product with the inverse, so the last pass's rows below the
+// diagonal block never leave the registers. The 64 x 64 diagonal block is factored in LDS in
+// 16-column steps (wave 0 factors each 16 x 16 block as potrf_one does, all threads invert it,
+// the other blocks are updated with MFMA); its inverse is assembled from the four block
+// inverses by MFMA (wave j forms block column j), and the rows below are multiplied by the
+// transposed inverse, as the narrower forms do with their diagonal blocks.
+template <typename T>
+struct Wide {
+    static constexpr int W = 64, H = 256, KC = 16;
+    static constexpr int LSA = H + 4, LSB = W + 4;  // strides of the staged chunk (k-major)
+    static constexpr int LDD = W + 1;               // row stride of L11 and its inverse
+    static constexpr int L16 = 17;                  // row stride of the 16 x 16 block inverses
+    static constexpr int stage = KC * (LSA + LSB);
+    static constexpr int diag = 2 * W * LDD + 4 * 16 * L16;
+    static constexpr size_t lds = sizeof(T) * (size_t)(stage > diag ? stage : diag);
+};
+
+template <typename T, int NM>
+__device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* smem) {
+    typedef typename Mma<T>::v4 v4;
+    typedef Wide<T> C;
+    constexpr int W = C::W, H = C::H, KC = C::KC, LSA = C::LSA, LSB = C::LSB, LDD = C::LDD,
+                  L16 = C::L16;
+    constexpr int MTH = H / 64;       // row tiles per wave in a pass
+    constexpr int CTW = W / 16;       // column tiles of a panel
+    constexpr int BV = W * KC / BNT;  // staged values per thread of the panel's first rows
+    static_assert(H == BNT && NM <= 2 * H, "one pass row per thread, at most two passes");
+    T* SA = smem;                 // chunk: pass rows, SA[kk * LSA + r]
+    T* SB = smem + KC * LSA;      // chunk: the panel's first W rows, SB[kk * LSB + r]
+    T* D = smem;                  // L11 at D[i * LDD + j] (once the chunks are done)
+    T* Minv = smem + W * LDD;     // Minv[k * LDD + j] = inv(L11)(j, k)
+    T* M16 = smem + 2 * W * LDD;  // M16[b * 16 * L16 + k * L16 + j] = inv(L_bb)(j, k)
+    __shared__ int s_fail;
+    for (int K0 = 0; K0 < n; K0 += W) {
+        // the thread index through an opaque copy, once per panel: the many per-lane offsets
+        // and conditions derived from it are otherwise hoisted out of the panel loop to the
+        // kernel's start, where they do not fit in registers
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+        const int lr = lane & 15, lk = lane >> 4;
+        const int kw = min(W, n - K0), m = n - K0;
+        gmem<T>* Ap = A + K0 + (size_t)K0 * lda;  // Ap[r + c * lda] = A(K0 + r, K0 + c)
+        // (1) the panel minus L(K0:n, 0:K0) L(K0:K0+kw, 0:K0)^T, pass by pass (only the last
+        // pass for the first panel: the rows below are A's). acc[ct][u] holds the entries
+        // (row r0 + (wave + 4u)*16 + lr, column ct*16 + drow(lane, r)); entries above the
+        // diagonal start at zero (never read into the result).
+        v4 acc[CTW][MTH];
+        for (int p = K0 > 0 ? (m - 1) / H : 0; p >= 0; --p) {
+            const int r0 = p * H;
+#pragma unroll
+            for (int ct = 0; ct < CTW; ++ct)
+#pragma unroll
+                for (int u = 0; u < MTH; ++u)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = r0 + (wave + 4 * u) * 16 + lr;
+                        const int col = ct * 16 + Mma<T>::drow(lane, r);
+                        acc[ct][u][r] = (row < m && col < kw && row >= col)
+                                            ? Ap[row + (size_t)col * lda] : T(0);
+                    }
+            if (K0 > 0) {
+                const int arow = r0 + t, brow = lane, bc = wave * BV;
+                T va[KC], vb[BV];
+                // column bases are the same for every lane (scalar registers), the lane adds
+                // its row
+                auto load = [&](int kc) {
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) {
+                        const gmem<T>* col = A + K0 + (size_t)(kc + j) * lda;
+                        va[j] = (arow < m) ? col[arow] : T(0);
+                    }
+#pragma unroll
+                    for (int j = 0; j < BV; ++j) {
+                        const gmem<T>* col = A + K0 + (size_t)(kc + bc + j) * lda;
+                        vb[j] = (brow < kw) ? col[brow] : T(0);
+                    }
+                };
+                load(0);
+                for (int kc = 0; kc < K0; kc += KC) {  // K0 is a multiple of KC
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) SA[j * LSA + t] = va[j];
+#pragma unroll
+                    for (int j = 0; j < BV; ++j) SB[(bc + j) * LSB + brow] = vb[j];
+                    __syncthreads();
+                    if (kc + KC < K0) load(kc + KC);
+#pragma unroll
+                    for (int kk = 0; kk < KC; kk += 4) {
+                        T a[CTW];
+#pragma unroll
+                        for (int ct = 0; ct < CTW; ++ct) a[ct] = -SB[(kk + lk) * LSB + ct * 16 + lr];
+#pragma unroll
+                        for (int u = 0; u < MTH; ++u) {
+                            const int rt = wave + 4 * u;
+                            if (r0 + rt * 16 < m) {
+                                const T b = SA[(kk + lk) * LSA + rt * 16 + lr];
+#pragma unroll
+                                for (int ct = 0; ct < CTW; ++ct)
+                                    if (ct * 16 < kw) acc[ct][u] = Mma<T>::mma(a[ct], b, acc[ct][u]);
+                            }
+                        }
+                    }
+                }
+                __syncthreads();  // the chunk area becomes L11
+            }
+            // the diagonal block's rows to LDS; a pass below the last one back to A, through
+            // an opaque copy of the panel pointer (the compiler would otherwise keep the 64
+            // load addresses live across the product instead of recomputing them)
+            gmem<T>* Aw = Ap;
+            asm volatile("" : "+s"(Aw));
+#pragma unroll
+            for (int ct = 0; ct < CTW; ++ct)
+#pragma unroll
+                for (int u = 0; u < MTH; ++u)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = r0 + (wave + 4 * u) * 16 + lr;
+                        const int col = ct * 16 + Mma<T>::drow(lane, r);
+                        if (row < W) D[row * LDD + col] = acc[ct][u][r];
+                        else if (p > 0 && row < m && col < kw) Aw[row + (size_t)col * lda] = acc[ct][u][r];
+                    }
+        }
+        __syncthreads();
+        // (2) L11 in 16-column steps; the inverse of each 16 x 16 diagonal block to M16 (all
+        // four when rows below exist: they make the inverse of L11)
+        const bool below = m > kw;  // then kw == W
+        int fail = -1;
+        for (int c = 0; c < kw; c += 16) {
+            const int cb = min(16, kw - c);
+            if (wave == 0) {
+                T rw[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) rw[j] = (lane < cb && j < cb) ? D[(c + lane) * LDD + c + j] : T(0);
+                int f = -1;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    if (j < cb && f < 0) {
+                        const T ajj = bcast(rw[j], j);
+                        if (ajj <= T(0)) {
+                            f = j;
+                        } else {
+                            const T alpha = fsqrt(ajj);
+                            const T inv = T(1) / alpha;
+                            rw[j] = (lane == j) ? alpha : ((lane > j) ? rw[j] * inv : rw[j]);
+                            const T xi = rw[j];
+#pragma unroll
+                            for (int l = j + 1; l < 16; ++l) rw[l] = __builtin_fma(xi, -bcast(rw[j], l), rw[l]);
+                        }
+                    }
+                }
+                if (lane < cb)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) D[(c + lane) * LDD + c + j] = rw[j];
+                if (lane == 0) s_fail = f;
+            }
+            __syncthreads();
+            if (s_fail >= 0) {
+                fail = c + s_fail;
+                break;
+            }
+            const bool last = c + 16 >= kw;
+            if (last && !below) break;
+            T* Mb = M16 + (c >> 4) * 16 * L16;
+            {
+                // 16 lanes per column, in the column order of potrf_one's inverse
+                const int cc = t >> 4, g = t & 15;
+                T y = (g == cc) ? T(1) : T(0);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    if (g == k) y = y / D[(c + k) * LDD + c + k];
+                    const T yk = __shfl(y, (lane & ~15) | k);
+                    if (g > k) y = __builtin_fma(-D[(c + g) * LDD + c + k], yk, y);
+                }
+                Mb[cc * L16 + g] = y;
+            }
+            __syncthreads();
+            if (last) break;
+            const int nt = (kw + 15) >> 4, c1 = (c >> 4) + 1;
+            {
+                // the block's rows below inside L11: X = L(rows, c:c+16) inv(L_cc)^T
+                const int ti = c1 + wave;
+                if (ti < nt) {
+                    v4 x = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                    for (int kk = 0; kk < 16; kk += 4)
+                        x = Mma<T>::mma(D[(ti * 16 + lr) * LDD + c + kk + lk], Mb[(kk + lk) * L16 + lr], x);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) D[(ti * 16 + Mma<T>::drow(lane, r)) * LDD + c + lr] = x[r];
+                }
+            }
+            __syncthreads();
+            {
+                // the rest of L11: tile (i, j), c1 <= j <= i < nt, minus X_i X_j^T
+                int pair = 0;
+                for (int i = c1; i < nt; ++i)
+                    for (int j = c1; j <= i; ++j, ++pair) {
+                        if ((pair & 3) != wave) continue;
+                        v4 a4;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) a4[r] = D[(i * 16 + Mma<T>::drow(lane, r)) * LDD + j * 16 + lr];
+#pragma unroll
+                        for (int kk = 0; kk < 16; kk += 4)
+                            a4 = Mma<T>::mma(-D[(i * 16 + lr) * LDD + c + kk + lk],
+                                             D[(j * 16 + lr) * LDD + c + kk + lk], a4);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) D[(i * 16 + Mma<T>::drow(lane, r)) * LDD + j * 16 + lr] = a4[r];
+                    }
+            }
+            __syncthreads();
+        }
+        // L11's lower triangle to A
+        for (int idx = t; idx < W * W; idx += BNT) {
+            const int i = idx & (W - 1), c = idx / W;
+            if (i < kw && c < kw && i >= c) Ap[i + (size_t)c * lda] = D[i * LDD + c];
+        }
+        if (fail >= 0) {
+            // NonHPDMatrixException at global column K0 + fail. A then holds the finished steps
+            // of L11, the rest of the panel as updated by the earlier panels.
+            if (below && K0 > 0) {
+#pragma unroll
+                for (int ct = 0; ct < CTW; ++ct)
+#pragma unroll
+                    for (int u = 1; u < MTH; ++u)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = (wave + 4 * u) * 16 + lr;
+                            if (row < m) Ap[row + (size_t)(ct * 16 + Mma<T>::drow(lane, r)) * lda] = acc[ct][u][r];
+                        }
+            }
+            if (t == 0) *info = K0 + fail + 1;
+            __syncthreads();
+            return;
+        }
+        if (below) {
+            // (3) inv(L11): wave J forms block column J from the block inverses,
+            // X_JJ = inv(L_JJ), X_IJ = -inv(L_II) sum_{K=J}^{I-1} L_IK X_KJ; X[I] holds the
+            // entries (16I + drow(lane, r), 16J + lr), the operand layout of the next product
+            static_for<0, CTW>([&](auto jc) {
+                constexpr int J = decltype(jc)::value;
+                if (wave != J) return;
+                v4 X[CTW];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[J][r] = M16[J * 16 * L16 + lr * L16 + Mma<T>::drow(lane, r)];
+                static_for<J + 1, CTW>([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    v4 s = v4{T(0), T(0), T(0), T(0)};
+                    static_for<J, I>([&](auto kc) {
+                        constexpr int K = decltype(kc)::value;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            s = Mma<T>::mma(D[(16 * I + lr) * LDD + 16 * K + Mma<T>::drow(lane, r)], X[K][r], s);
+                    });
+                    v4 x = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        x = Mma<T>::mma(-M16[I * 16 * L16 + Mma<T>::drow(lane, r) * L16 + lr], s[r], x);
+                    X[I] = x;
+                });
+                static_for<0, CTW>([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        T v = T(0);
+                        if constexpr (I >= J) v = X[I][r];
+                        Minv[(16 * J + lr) * LDD + 16 * I + Mma<T>::drow(lane, r)] = v;
+                    }
+                });
+            });
+            __syncthreads();
+            // (4) rows below: X^T = inv(L11) P21^T, the panel's transposed tiles as the second
+            // operand; first the last pass's rows, from the registers
+#pragma unroll
+            for (int u = 1; u < MTH; ++u) {
+                const int rt = wave + 4 * u;
+                if (rt * 16 >= m) continue;
+                v4 x[CTW];
+#pragma unroll
+                for (int ct = 0; ct < CTW; ++ct) x[ct] = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int kt = 0; kt < CTW; ++kt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+#pragma unroll
+                        for (int ct = 0; ct < CTW; ++ct)
+                            x[ct] = Mma<T>::mma(Minv[(kt * 16 + Mma<T>::drow(lane, r)) * LDD + ct * 16 + lr],
+                                                acc[kt][u][r], x[ct]);
+                const int row = rt * 16 + lr;
+                if (row < m)
+#pragma unroll
+                    for (int ct = 0; ct < CTW; ++ct)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            Ap[row + (size_t)(ct * 16 + Mma<T>::drow(lane, r)) * lda] = x[ct][r];
+            }
+            // then the earlier pass's rows (n > 256 + K0), from A
+            for (int rt = H / 16 + wave; rt * 16 < m; rt += 4) {
+                const int row = rt * 16 + lr;
+                T bv[CTW][4];
+#pragma unroll
+                for (int kt = 0; kt < CTW; ++kt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        bv[kt][r] = (row < m) ? Ap[row + (size_t)(kt * 16 + Mma<T>::drow(lane, r)) * lda] : T(0);
+                v4 x[CTW];
+#pragma unroll
+                for (int ct = 0; ct < CTW; ++ct) x[ct] = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int kt = 0; kt < CTW; ++kt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+#pragma unroll
+                        for (int ct = 0; ct < CTW; ++ct)
+                            x[ct] = Mma<T>::mma(Minv[(kt * 16 + Mma<T>::drow(lane, r)) * LDD + ct * 16 + lr],
+                                                bv[kt][r], x[ct]);
+                if (row < m)
+#pragma unroll
+                    for (int ct = 0; ct < CTW; ++ct)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            Ap[row + (size_t)(ct * 16 + Mma<T>::drow(lane, r)) * lda] = x[ct][r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Workgroups per CU a bin's kernel is compiled for (registers); LDS may allow fewer: the
 // n <= 512 bin fits two with 16-column blocks (72 KB each in FP64), one with 32.
 template <int NM, int NB>
@@ -542,7 +882,7 @@ __global__ __launch_bounds__(BNT, (bin_occ_max<NM, NB>())) void potrf_vbatched_k
         if (i >= end) break;
         const BatchItem it = items[i];
         if (threadIdx.x == 0) info[it.index] = 0;
-        if (it.n > 0) potrf_one<T, NM, NB>(it.n, As[it.index], it.lda, info + it.index, P, Minv);
+        if (it.n > 0) potrf_one<T, NM, NB>(it.n, (gmem<T>*)As[it.index], it.lda, info + it.index, P, Minv);
     }
 }
 
@@ -562,7 +902,28 @@ __global__ __launch_bounds__(BNT, kRdOcc) void potrf_rd_vbatched_kernel(
         if (i >= end) break;
         const BatchItem it = items[i];
         if (threadIdx.x == 0) info[it.index] = 0;
-        if (it.n > 0) potrf_rd_one<T, NM, NB>(it.n, As[it.index], it.lda, info + it.index);
+        if (it.n > 0) potrf_rd_one<T, NM, NB>(it.n, (gmem<T>*)As[it.index], it.lda, info + it.index);
+    }
+}
+
+// Wide-panel persistent workgroups of one size bin (variant kind 2), two per CU.
+constexpr int kWideOcc = 2;
+template <typename T, int NM>
+__global__ __launch_bounds__(BNT, kWideOcc) void potrf_wide_vbatched_kernel(
+    const BatchItem* __restrict__ items, int begin, int end, T* const* __restrict__ As,
+    int32_t* __restrict__ info, int* queue) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* sm = reinterpret_cast<T*>(smem_raw);
+    __shared__ int s_i;
+    for (;;) {
+        if (threadIdx.x == 0) s_i = begin + atomicAdd(queue, 1);
+        __syncthreads();
+        const int i = s_i;
+        __syncthreads();
+        if (i >= end) break;
+        const BatchItem it = items[i];
+        if (threadIdx.x == 0) info[it.index] = 0;
+        if (it.n > 0) potrf_wide_one<T, NM>(it.n, (gmem<T>*)As[it.index], it.lda, info + it.index, sm);
     }
 }
 
@@ -922,6 +1283,28 @@ elx_status_t launch_rd_bin(hipStream_t s, const BatchItem* items, int lo, int hi
     return ELX_SUCCESS;
 }
 
+template <typename T, int NM>
+elx_status_t launch_wide_bin(hipStream_t s, const BatchItem* items, int lo, int hi, T* const* dA,
+                             int32_t* dinfo, int* queue, int occ) {
+    const void* fn = reinterpret_cast<const void*>(&potrf_wide_vbatched_kernel<T, NM>);
+    const size_t lds = Wide<T>::lds;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+        attr = true;
+    }
+    int per_cu = 0;
+    ELX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BNT, lds));
+    per_cu = std::max(1, std::min(per_cu, occ));
+    const elx::DeviceInfo& di = elx::device_info();
+    const int G = (int)std::min<int64_t>(hi - lo, (int64_t)per_cu * di.num_cus);
+    hipLaunchKernelGGL((potrf_wide_vbatched_kernel<T, NM>), dim3(G), dim3(BNT), lds, s, items, lo,
+                       hi, dA, dinfo, queue);
+    ELX_HIP(hipGetLastError());
+    return ELX_SUCCESS;
+}
+
 template <typename T>
 elx_status_t launch_factor(hipStream_t s, const Schedule& sc, T* const* dA, int32_t* dinfo,
                            int* queue) {
@@ -939,7 +1322,9 @@ elx_status_t launch_factor(hipStream_t s, const Schedule& sc, T* const* dA, int3
             if (v.nb == 16) ELX_CHECK((launch_bin<T, 128, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
             else ELX_CHECK((launch_bin<T, 128, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
         } else if (b == 1) {
-            if (v.kind == 1 && v.nb == 16)
+            if (v.kind == 2)
+                ELX_CHECK((launch_wide_bin<T, 256>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else if (v.kind == 1 && v.nb == 16)
                 ELX_CHECK((launch_rd_bin<T, 256, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
             else if (v.kind == 1)
                 ELX_CHECK((launch_rd_bin<T, 256, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
@@ -948,7 +1333,9 @@ elx_status_t launch_factor(hipStream_t s, const Schedule& sc, T* const* dA, int3
             else
                 ELX_CHECK((launch_bin<T, 256, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
         } else {
-            if (v.kind == 1 && v.nb == 16)
+            if (v.kind == 2)
+                ELX_CHECK((launch_wide_bin<T, 512>(s, it, lo, hi, dA, dinfo, q, v.occ)));
+            else if (v.kind == 1 && v.nb == 16)
                 ELX_CHECK((launch_rd_bin<T, 512, 16>(s, it, lo, hi, dA, dinfo, q, v.occ)));
             else if (v.kind == 1)
                 ELX_CHECK((launch_rd_bin<T, 512, 32>(s, it, lo, hi, dA, dinfo, q, v.occ)));
@@ -1129,9 +1516,10 @@ elx_status_t elx_graph_destroy(elx_graph_t graph) {
 elx_status_t elx_batched_set_variant(int dtype, int bin, int nb, int occ, int kind) {
     ELX_ARG(dtype == 0 || dtype == 1);
     ELX_ARG(bin >= 0 && bin < NBIN);
-    ELX_ARG(nb == 0 || nb == 16 || nb == 32);
+    ELX_ARG(nb == 0 || nb == 16 || nb == 32 || nb == 64);
     ELX_ARG(nb == 0 || (occ >= 1 && occ <= 8));
-    ELX_ARG(nb == 0 || kind == 0 || (kind == 1 && bin >= 1));
+    ELX_ARG(nb == 0 || (kind == 0 && nb != 64) || (kind == 1 && bin >= 1 && nb != 64) ||
+            (kind == 2 && bin >= 1 && nb == 64));
     std::lock_guard<std::mutex> lk(g_bmu);
     g_override[dtype][bin] =
         elx::tuned::BatchedVariant{nb, nb ? occ : 0, nb ? kind : 0};

@@ -353,7 +353,8 @@ class PosvGraph:
 
 def batched_variant(dtype, bin_index, nb=None, occ=None, kind=0):
     """Reads (nb, occ, kind) of a batched-Cholesky size bin; with nb given, sets it first (nb 0
-    restores the built-in table; kind 1 = operands read straight into registers, bins 1-2)."""
+    restores the built-in table; kind 1 = operands read straight into registers, kind 2 =
+    64-column panels (nb 64), both in bins 1-2)."""
     import ctypes
     d = 0 if dtype == torch.float64 else 1
     if nb is not None:

@@ -336,10 +336,11 @@ elx_status_t elx_graph_destroy(elx_graph_t graph);
 
 /* Tuning hooks of the batched Cholesky (no counterpart in the reference; used by the
  * autotuner, elemental_amd/autotune_batched.py): the variant -- block-column width nb (16 or
- * 32), persistent workgroups per CU occ, and kernel kind (0: block column staged in LDS; 1:
- * operands read straight into registers, bins 1 and 2 only) -- of size bin `bin` (0:
- * n <= 128, 1: n <= 256, 2: n <= 512) for dtype 0 (double) or 1 (float). nb = 0 restores the
- * built-in table. */
+ * 32; 64 for kind 2), persistent workgroups per CU occ, and kernel kind (0: block column
+ * staged in LDS; 1: operands read straight into registers; 2: 64-column panels with the
+ * diagonal block factored and inverted in LDS; kinds 1 and 2 in bins 1 and 2 only) -- of size
+ * bin `bin` (0: n <= 128, 1: n <= 256, 2: n <= 512) for dtype 0 (double) or 1 (float).
+ * nb = 0 restores the built-in table. */
 elx_status_t elx_batched_set_variant(int dtype, int bin, int nb, int occ, int kind);
 elx_status_t elx_batched_get_variant(int dtype, int bin, int* nb, int* occ, int* kind);
 

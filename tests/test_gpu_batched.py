@@ -141,6 +141,44 @@ def test_batched_cholesky_every_variant(dev, orc, nb, kind):
     assert dev.batched_variant(torch.float64, 0)[0] in (16, 32)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_wide_panels(dev, orc, dtype):
+    """The wide-panel kind (64-column panels, bins 1 and 2) with one and two workgroups per CU:
+    sizes around the panel width and the 256-row pass, one matrix not positive definite in its
+    first panel and one in its fourth; same bars as above."""
+    rng = np.random.default_rng(47)
+    ns = [129, 150, 191, 192, 193, 255, 256, 257, 300, 319, 320, 321, 383, 384, 385, 448, 500,
+          511, 512]
+    As = [spd(n, rng) for n in ns]
+    As[2][40, 40] = -1e3    # n = 191: column 40, first panel
+    As[17][200, 200] = -1e3  # n = 511: column 200, fourth panel
+    u = EPS64 if dtype == torch.float64 else EPS32
+    try:
+        for occ in (1, 2):
+            for b in (1, 2):
+                dev.batched_variant(dtype, b, 64, occ, 2)
+            say("wide panels", dtype, occ)
+            mats = [dev.to_device(A, dtype=dtype) for A in As]
+            info = dev.potrf_vbatched(dev.Batch(mats, dtype)).cpu().numpy()
+            for i, (A, n) in enumerate(zip(As, ns)):
+                Fo, info_o = orc.cholesky(A, uplo=0)
+                assert info[i] == info_o, (i, n, info[i], info_o)
+                if info_o:
+                    assert info_o in (41, 201)
+                    continue
+                F = dev.to_host(mats[i]).astype(np.float64)
+                assert orc.chol_factor_error(A, np.asfortranarray(F), 0) <= 8 * n * u, (i, n)
+                # the strict upper triangle is never written
+                Au = A if dtype == torch.float64 else A.astype(np.float32).astype(np.float64)
+                assert np.array_equal(np.triu(F, 1), np.triu(Au, 1)), (i, n)
+                if dtype == torch.float64:
+                    L, Lo = np.tril(F), np.tril(Fo)
+                    assert np.abs(L - Lo).max() <= 64 * n * u * np.abs(Lo).max(), (i, n)
+    finally:
+        for b in range(3):
+            dev.batched_variant(dtype, b, 0)
+
+
 @pytest.mark.parametrize("nb", [16, 32])
 def test_batched_register_direct_float32(dev, orc, nb):
     """The register-direct kernel kind in FP32 (bins 1 and 2, one and two workgroups per CU):
