@@ -538,11 +538,12 @@ __device__ void potrf_rd_one(int n, gmem<T>* A, int lda, int32_t* info) {
 // transposed inverse, as the narrower forms do with their diagonal blocks.
 template <typename T>
 struct Wide {
-    static constexpr int W = 64, H = 256, KC = 16;
+    static constexpr int W = 64, H = 256, KC = 8;
     static constexpr int LSA = H + 4, LSB = W + 4;  // strides of the staged chunk (k-major)
     static constexpr int LDD = W + 1;               // row stride of L11 and its inverse
     static constexpr int L16 = 17;                  // row stride of the 16 x 16 block inverses
-    static constexpr int stage = KC * (LSA + LSB);
+    static constexpr int buf = KC * (LSA + LSB);    // one chunk buffer (two of them)
+    static constexpr int stage = 2 * buf;
     static constexpr int diag = 2 * W * LDD + 4 * 16 * L16;
     static constexpr size_t lds = sizeof(T) * (size_t)(stage > diag ? stage : diag);
 };
@@ -557,8 +558,8 @@ __device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* sme
     constexpr int CTW = W / 16;       // column tiles of a panel
     constexpr int BV = W * KC / BNT;  // staged values per thread of the panel's first rows
     static_assert(H == BNT && NM <= 2 * H, "one pass row per thread, at most two passes");
-    T* SA = smem;                 // chunk: pass rows, SA[kk * LSA + r]
-    T* SB = smem + KC * LSA;      // chunk: the panel's first W rows, SB[kk * LSB + r]
+    // chunk buffer b: the pass rows at smem + b * buf (kk * LSA + r), then the panel's first W
+    // rows (kk * LSB + r)
     T* D = smem;                  // L11 at D[i * LDD + j] (once the chunks are done)
     T* Minv = smem + W * LDD;     // Minv[k * LDD + j] = inv(L11)(j, k)
     T* M16 = smem + 2 * W * LDD;  // M16[b * 16 * L16 + k * L16 + j] = inv(L_bb)(j, k)
@@ -593,30 +594,37 @@ __device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* sme
                     }
             if (K0 > 0) {
                 const int arow = r0 + t, brow = lane, bc = wave * BV;
-                T va[KC], vb[BV];
-                // column bases are the same for every lane (scalar registers), the lane adds
-                // its row
-                auto load = [&](int kc) {
+                // chunk c (columns c*KC ..) is loaded into registers R[c & 1] two chunks
+                // ahead, stored to buffer c & 1 after chunk c-1 is multiplied: one barrier per
+                // chunk, two chunks of loads in flight
+                T va[2][KC], vb[2][BV];
+                auto load = [&](auto pc, int c) {
+                    constexpr int P = decltype(pc)::value;
 #pragma unroll
                     for (int j = 0; j < KC; ++j) {
-                        const gmem<T>* col = A + K0 + (size_t)(kc + j) * lda;
-                        va[j] = (arow < m) ? col[arow] : T(0);
+                        // column bases are the same for every lane (scalar registers)
+                        const gmem<T>* col = A + K0 + (size_t)(c * KC + j) * lda;
+                        va[P][j] = (arow < m) ? col[arow] : T(0);
                     }
 #pragma unroll
                     for (int j = 0; j < BV; ++j) {
-                        const gmem<T>* col = A + K0 + (size_t)(kc + bc + j) * lda;
-                        vb[j] = (brow < kw) ? col[brow] : T(0);
+                        const gmem<T>* col = A + K0 + (size_t)(c * KC + bc + j) * lda;
+                        vb[P][j] = (brow < kw) ? col[brow] : T(0);
                     }
                 };
-                load(0);
-                for (int kc = 0; kc < K0; kc += KC) {  // K0 is a multiple of KC
-                    __syncthreads();
+                auto store = [&](auto pc) {
+                    constexpr int P = decltype(pc)::value;
+                    T* SA = smem + P * C::buf;
+                    T* SB = SA + KC * LSA;
 #pragma unroll
-                    for (int j = 0; j < KC; ++j) SA[j * LSA + t] = va[j];
+                    for (int j = 0; j < KC; ++j) SA[j * LSA + t] = va[P][j];
 #pragma unroll
-                    for (int j = 0; j < BV; ++j) SB[(bc + j) * LSB + brow] = vb[j];
-                    __syncthreads();
-                    if (kc + KC < K0) load(kc + KC);
+                    for (int j = 0; j < BV; ++j) SB[(bc + j) * LSB + brow] = vb[P][j];
+                };
+                auto multiply = [&](auto pc) {
+                    constexpr int P = decltype(pc)::value;
+                    const T* SA = smem + P * C::buf;
+                    const T* SB = SA + KC * LSA;
 #pragma unroll
                     for (int kk = 0; kk < KC; kk += 4) {
                         T a[CTW];
@@ -633,8 +641,25 @@ __device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* sme
                             }
                         }
                     }
+                };
+                const int nch = K0 / KC;  // K0 is a multiple of W, so at least 8 chunks
+                load(std::integral_constant<int, 0>{}, 0);
+                load(std::integral_constant<int, 1>{}, 1);
+                store(std::integral_constant<int, 0>{});
+                __syncthreads();
+                // step c: chunk c+2 into registers, chunk c multiplied from buffer c & 1,
+                // chunk c+1 to the other buffer (multiplied by every wave at step c-1)
+                auto step = [&](auto pc, int c) {
+                    if (c + 2 < nch) load(pc, c + 2);
+                    multiply(pc);
+                    if (c + 1 < nch) store(std::integral_constant<int, 1 - decltype(pc)::value>{});
+                    __syncthreads();
+                };
+                for (int c = 0; c < nch; c += 2) {
+                    step(std::integral_constant<int, 0>{}, c);
+                    if (c + 1 < nch) step(std::integral_constant<int, 1>{}, c + 1);
                 }
-                __syncthreads();  // the chunk area becomes L11
+                // (the last step's barrier: the chunk area becomes L11)
             }
             // the diagonal block's rows to LDS; a pass below the last one back to A, through
             // an opaque copy of the panel pointer (the compiler would otherwise keep the 64
@@ -654,19 +679,24 @@ __device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* sme
                     }
         }
         __syncthreads();
-        // (2) L11 in 16-column steps; the inverse of each 16 x 16 diagonal block to M16 (all
-        // four when rows below exist: they make the inverse of L11)
+        // (2) L11 in 16-column steps: wave 0 factors the 16 x 16 diagonal block (lane i holds
+        // row i) and inverts it into M16 (lane j forms column j of the inverse by forward
+        // substitution); all four inverses are kept when rows below exist (they make the
+        // inverse of L11)
         const bool below = m > kw;  // then kw == W
         int fail = -1;
         for (int c = 0; c < kw; c += 16) {
             const int cb = min(16, kw - c);
+            const bool last = c + 16 >= kw;
+            T* Mb = M16 + (c >> 4) * 16 * L16;
             if (wave == 0) {
-                T rw[16];
+                T rw[16], dinv[16];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) rw[j] = (lane < cb && j < cb) ? D[(c + lane) * LDD + c + j] : T(0);
                 int f = -1;
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
+                    dinv[j] = T(0);
                     if (j < cb && f < 0) {
                         const T ajj = bcast(rw[j], j);
                         if (ajj <= T(0)) {
@@ -674,6 +704,7 @@ __device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* sme
                         } else {
                             const T alpha = fsqrt(ajj);
                             const T inv = T(1) / alpha;
+                            dinv[j] = inv;
                             rw[j] = (lane == j) ? alpha : ((lane > j) ? rw[j] * inv : rw[j]);
                             const T xi = rw[j];
 #pragma unroll
@@ -684,6 +715,20 @@ __device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* sme
                 if (lane < cb)
 #pragma unroll
                     for (int j = 0; j < 16; ++j) D[(c + lane) * LDD + c + j] = rw[j];
+                if (f < 0 && (!last || below)) {
+                    // y = inv(L_cc) e_lane: y_k = (e_k - sum_{i<k} l_ki y_i) / l_kk
+                    T y[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        T sk = (k == lane) ? T(1) : T(0);
+#pragma unroll
+                        for (int i = 0; i < k; ++i) sk = __builtin_fma(-bcast(rw[i], k), y[i], sk);
+                        y[k] = sk * dinv[k];
+                    }
+                    if (lane < 16)
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) Mb[lane * L16 + k] = y[k];
+                }
                 if (lane == 0) s_fail = f;
             }
             __syncthreads();
@@ -691,22 +736,6 @@ __device__ void potrf_wide_one(int n, gmem<T>* A, int lda, int32_t* info, T* sme
                 fail = c + s_fail;
                 break;
             }
-            const bool last = c + 16 >= kw;
-            if (last && !below) break;
-            T* Mb = M16 + (c >> 4) * 16 * L16;
-            {
-                // 16 lanes per column, in the column order of potrf_one's inverse
-                const int cc = t >> 4, g = t & 15;
-                T y = (g == cc) ? T(1) : T(0);
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    if (g == k) y = y / D[(c + k) * LDD + c + k];
-                    const T yk = __shfl(y, (lane & ~15) | k);
-                    if (g > k) y = __builtin_fma(-D[(c + g) * LDD + c + k], yk, y);
-                }
-                Mb[cc * L16 + g] = y;
-            }
-            __syncthreads();
             if (last) break;
             const int nt = (kw + 15) >> 4, c1 = (c >> 4) + 1;
             {
