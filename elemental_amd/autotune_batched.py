@@ -3,8 +3,9 @@ writes the variant table the library compiles in (csrc/batched_tuning.hpp).
 
 For each element type (float64, float32) and size bin (n <= 128, n <= 256, n <= 512) the
 workload is a batch of HPD matrices G G^T / n + I with n uniform over the bin's range (the
-batched configuration's distribution restricted to the bin), and the candidates are the
-block-column widths 16 and 32 with 1 .. (the bin's maximum) persistent workgroups per CU.
+batched configuration's distribution restricted to the bin), and the candidates are the kernel
+kinds with their block-column widths (16 and 32; 64-column panels for kind 2) and 1 .. (the
+bin's maximum) persistent workgroups per CU.
 Each candidate factors the batch `--reps` times (A restored from a copy before each run, the
 factorization alone timed with HIP events); the smallest median wins.
 
@@ -24,9 +25,19 @@ BINS = [(16, 128), (129, 256), (257, 512)]
 MAX_OCC = [3, 2, 2]
 NBS = [16, 32]
 # kernel kinds per bin: 0 block column staged in LDS, 1 operands read straight into registers
-# (bins 1 and 2; at most three workgroups per CU, as registers allow)
-KINDS = [[0], [0, 1], [0, 1]]
+# (bins 1 and 2; at most three workgroups per CU, as registers allow), 2 64-column panels
+# (bins 1 and 2; at most two workgroups per CU)
+KINDS = [[0], [0, 1, 2], [0, 1, 2]]
 RD_MAX_OCC = 3
+WIDE_MAX_OCC = 2
+
+
+def candidates(b, kind):
+    """(nb, occ) pairs of a kernel kind in bin b."""
+    if kind == 2:
+        return [(64, occ) for occ in range(1, WIDE_MAX_OCC + 1)]
+    top = RD_MAX_OCC if kind == 1 else MAX_OCC[b]
+    return [(nb, occ) for nb in NBS for occ in range(1, top + 1)]
 
 
 def make_batch(dtype, lo, hi, count, seed):
@@ -89,16 +100,15 @@ def main():
                 s.copy_(m)
             best = None
             for kind in KINDS[b]:
-                for nb in NBS:
-                    for occ in range(1, (RD_MAX_OCC if kind else MAX_OCC[b]) + 1):
-                        ms = time_variant(dtype, d, b, nb, occ, mats, saved, args.reps, kind)
-                        tf = flops / (ms * 1e-3) / 1e12
-                        report.append({"dtype": str(dtype).split(".")[-1], "bin": [lo, hi],
-                                       "kind": kind, "nb": nb, "occ": occ, "ms": round(ms, 4),
-                                       "tflops": round(tf, 3)})
-                        print(json.dumps(report[-1]), flush=True)
-                        if best is None or ms < best[0]:
-                            best = (ms, nb, occ, kind)
+                for nb, occ in candidates(b, kind):
+                    ms = time_variant(dtype, d, b, nb, occ, mats, saved, args.reps, kind)
+                    tf = flops / (ms * 1e-3) / 1e12
+                    report.append({"dtype": str(dtype).split(".")[-1], "bin": [lo, hi],
+                                   "kind": kind, "nb": nb, "occ": occ, "ms": round(ms, 4),
+                                   "tflops": round(tf, 3)})
+                    print(json.dumps(report[-1]), flush=True)
+                    if best is None or ms < best[0]:
+                        best = (ms, nb, occ, kind)
             dev.batched_variant(dtype, b, 0)
             table[d][b] = (best[1], best[2], best[3])
             del mats, saved
@@ -117,9 +127,10 @@ namespace elx {
 namespace tuned {
 
 struct BatchedVariant {
-    int nb;    // block-column width of the left-looking factorization (16 or 32)
+    int nb;    // block-column width of the left-looking factorization (16 or 32; 64 for kind 2)
     int occ;   // persistent workgroups per CU (at most what the bin's LDS and registers allow)
-    int kind;  // 0: block column staged in LDS; 1: operands read straight into registers
+    int kind;  // 0: block column staged in LDS; 1: operands read straight into registers;
+               // 2: 64-column panels, diagonal block factored and inverted in LDS
 };
 
 // [0: double, 1: float][bin 0: n <= 128, 1: n <= 256, 2: n <= 512]
