@@ -15,6 +15,7 @@ host BLAS) on a bounded sample.
 Prints one JSON line. `value` = mixed-precision solves/s.
 """
 import argparse
+import contextlib
 import ctypes
 import json
 import os
@@ -431,17 +432,23 @@ def hpd_solve(args):
     out = {"n": n, "nrhs": nrhs, "matrix": "G G^T/n + I", "flops_factor": n ** 3 / 3.0}
     Ac = dev.colmajor_empty(n, n)
     Xc = dev.colmajor_empty(n, nrhs)
-    times = []
-    for rep in range(args.hpd_steps + 1):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        Ac.copy_(A)
-        Xc.copy_(B)
-        dev.posv(Ac, Xc)
-        torch.cuda.synchronize()
-        if rep:
-            times.append(time.perf_counter() - t0)
-    out["plain_fp64_ms"] = round(1e3 * sum(times) / len(times), 2)
+    t_by = {}
+    for gemm in ("own", "library"):
+        with gemm_env("ELX_DGEMM", gemm):
+            times = []
+            for rep in range(args.hpd_steps + 1):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                Ac.copy_(A)
+                Xc.copy_(B)
+                dev.posv(Ac, Xc)
+                torch.cuda.synchronize()
+                if rep:
+                    times.append(time.perf_counter() - t0)
+            t_by[gemm] = sum(times) / len(times)
+    out["plain_fp64_ms"] = round(1e3 * min(t_by.values()), 2)
+    out["plain_fp64_ms_own_gemm"] = round(1e3 * t_by["own"], 2)
+    out["plain_fp64_ms_rocblas_gemm"] = round(1e3 * t_by["library"], 2)
     out["plain_fp64_backward_error"] = backward_error(A, Xc, B)
     del Ac
     torch.cuda.empty_cache()
@@ -522,6 +529,24 @@ def batched_lu_ir(args):
     del As, Bs, Xs
     torch.cuda.empty_cache()
     return out
+
+
+@contextlib.contextmanager
+def gemm_env(name, value):
+    """Runs the block with environment variable `name` set to `value` ("own" removes it: the
+    library's own kernels are the default)."""
+    old = os.environ.get(name)
+    if value == "own":
+        os.environ.pop(name, None)
+    else:
+        os.environ[name] = value
+    try:
+        yield
+    finally:
+        if old is None:
+            os.environ.pop(name, None)
+        else:
+            os.environ[name] = old
 
 
 def batched_cholesky(args, lib):
@@ -699,19 +724,29 @@ def main():
     if not args.skip_plain:
         Ac = dev.colmajor_empty(n, n)
         xb = dev.colmajor_empty(n, 1)
-        times = []
-        for rep in range(args.plain_steps + 1):
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            Ac.copy_(A)
-            xb.copy_(b)
-            d, info = dev.getrf(Ac)
-            dev.getrs(Ac, d, xb)
-            torch.cuda.synchronize()
-            if rep > 0:
-                times.append(time.perf_counter() - t1)
-        t_plain = sum(times) / len(times)
+        # the plain solve on this library's FP64 GEMM (the default) and on rocBLAS's
+        # (ELX_DGEMM=library); the speedup is quoted against the faster of the two
+        t_by = {}
+        for gemm in ("own", "library"):
+            with gemm_env("ELX_DGEMM", gemm):
+                times = []
+                for rep in range(args.plain_steps + 1):
+                    torch.cuda.synchronize()
+                    t1 = time.perf_counter()
+                    Ac.copy_(A)
+                    xb.copy_(b)
+                    d, info = dev.getrf(Ac)
+                    dev.getrs(Ac, d, xb)
+                    torch.cuda.synchronize()
+                    if rep > 0:
+                        times.append(time.perf_counter() - t1)
+                t_by[gemm] = sum(times) / len(times)
+        fastest = min(t_by, key=t_by.get)
+        t_plain = t_by[fastest]
         plain = {"solves_per_s": round(1.0 / t_plain, 4), "ms_per_solve": round(1e3 * t_plain, 2),
+                 "faster_gemm": fastest,
+                 "ms_per_solve_own_gemm": round(1e3 * t_by["own"], 2),
+                 "ms_per_solve_rocblas_gemm": round(1e3 * t_by["library"], 2),
                  "backward_error": backward_error(A, xb, b), "info": int(info.item())}
         if not args.no_profile_pass:
             # kernel classes of one profiled plain FP64 solve
@@ -727,6 +762,15 @@ def main():
         del Ac
         torch.cuda.empty_cache()
         print(f"[bench] plain fp64 {plain}", file=sys.stderr, flush=True)
+        # for reference: the mixed solve with rocBLAS's gemm_ex for the rank-NB updates
+        # (ELX_LOWP_GEMM=library) in place of this library's kernel, one warmup and one step
+        with gemm_env("ELX_LOWP_GEMM", "library"):
+            mixed_solve()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            mixed_solve()
+            torch.cuda.synchronize()
+            plain["mixed_ms_with_rocblas_rank_nb_update"] = round(1e3 * (time.perf_counter() - t1), 2)
 
     kernels = {}
     roof = None

@@ -404,11 +404,11 @@ __global__ __launch_bounds__(NT, 2) void dgemm_fast_kernel(i64 M, i64 N, i64 K, 
     dgemm_fast_tile<TA, TB, FULL>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri, m0, n0, As, Bs);
 }
 
-// The hand-written kernels for the large FP64 products as well (environment ELX_DGEMM=own,
-// read at every call); rocBLAS otherwise.
+// The large FP64 products on rocBLAS instead of the hand-written kernels: environment
+// ELX_DGEMM=library, read at every call.
 bool dgemm_own_selected() {
     const char* v = std::getenv("ELX_DGEMM");
-    return v && std::strcmp(v, "own") == 0;
+    return !(v && std::strcmp(v, "library") == 0);
 }
 
 // C := beta C on the touched triangle (k == 0 or alpha == 0). Grid-stride over all entries.
@@ -452,12 +452,10 @@ elx_status_t gemm_impl(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, T a
     }
     const double flops = (tri ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
     if constexpr (sizeof(T) == 8) {
-        // Large FP64 products go to rocBLAS (dgemm, and dsyrk for the symmetric rank-k update:
-        // A == B, one operand transposed, one triangle of C) unless ELX_DGEMM=own: inside the
-        // LU at n = 32768 (leading dimension 2^15) the hand-written kernel ran the trailing
-        // updates in 779 ms against rocBLAS's 558 ms. In a factorization sequence the
-        // coefficients come from the device (gate_kernel), so a recorded failure leaves C as it
-        // is.
+        // With ELX_DGEMM=library, large FP64 products go to rocBLAS (dgemm, and dsyrk for the
+        // symmetric rank-k update: A == B, one operand transposed, one triangle of C). In a
+        // factorization sequence the coefficients come from the device (gate_kernel), so a
+        // recorded failure leaves C as it is.
         if (allow_library && !dgemm_own_selected() &&
             (double)m * (double)n * (double)k >= (double)(1 << 24)) {
             const bool syrk = tri != 0 && A == B && lda == ldb && m == n && ta != tb && (tri == 1) == tb;

@@ -20,7 +20,14 @@
 
 namespace {
 
-constexpr int BM = 256, BN = 256, BK = 32, LDK = BK + 8, NT = 512;
+constexpr int BM = 256, BN = 256, BK = 32, NT = 512;
+// K-slab depth of the kernel: 32, or 64 for the rank-NB updates (K a multiple of 64: half the
+// barriers and slab hand-offs per flop); LDS rows padded by 8 halves so the 16-byte fragment
+// reads of 16 consecutive rows hit disjoint banks
+template <int BKV>
+constexpr int ldk() {
+    return BKV + 8;
+}
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -47,20 +54,21 @@ __device__ __forceinline__ f4 mfma(u4 a, u4 b, f4 c) {
                                                        __builtin_bit_cast(b8, b), c, 0, 0, 0);
 }
 
-// One K-slab of a K-major operand: rows r0..r0+255, k0..k0+31; thread t loads 32 bytes of
-// row t/2 (halves 16*(t%2) .. +15). Rows >= R load zeros. The operand's K extent is padded
-// to a multiple of 32 with zeros by the pack kernels, so k never runs past the row.
+// One K-slab of a K-major operand: rows r0..r0+255, k0..k0+BKV-1; thread t loads BKV bytes
+// of row t/2 (halves (BKV/2)*(t%2) ..). Rows >= R load zeros. The operand's K extent is padded
+// to a multiple of BKV with zeros by the pack kernels, so k never runs past the row.
+template <int BKV>
 __device__ __forceinline__ void load_slab(const uint16_t* __restrict__ P, i64 ld, i64 r0, i64 R,
-                                          i64 k0, u4 (&r)[2]) {
+                                          i64 k0, u4 (&r)[BKV / 16]) {
     const int t = threadIdx.x;
     const i64 row = r0 + (t >> 1);
     if (row < R) {
-        const u4* src = reinterpret_cast<const u4*>(P + row * ld + k0 + 16 * (t & 1));
-        r[0] = src[0];
-        r[1] = src[1];
+        const u4* src = reinterpret_cast<const u4*>(P + row * ld + k0 + (BKV / 2) * (t & 1));
+#pragma unroll
+        for (int q = 0; q < BKV / 16; ++q) r[q] = src[q];
     } else {
-        r[0] = u4{0, 0, 0, 0};
-        r[1] = u4{0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < BKV / 16; ++q) r[q] = u4{0, 0, 0, 0};
     }
 }
 
@@ -101,21 +109,25 @@ __device__ __forceinline__ void load_slab_f32(const float* __restrict__ P, i64 l
     }
 }
 
-__device__ __forceinline__ void store_slab(uint16_t* S, const u4 (&r)[2]) {
+template <int BKV>
+__device__ __forceinline__ void store_slab(uint16_t* S, const u4 (&r)[BKV / 16]) {
     const int t = threadIdx.x;
-    u4* dst = reinterpret_cast<u4*>(S + (t >> 1) * LDK + 16 * (t & 1));
-    dst[0] = r[0];
-    dst[1] = r[1];
+    u4* dst = reinterpret_cast<u4*>(S + (t >> 1) * ldk<BKV>() + (BKV / 2) * (t & 1));
+#pragma unroll
+    for (int q = 0; q < BKV / 16; ++q) dst[q] = r[q];
 }
 
-// BF32: B is the FP32 operand itself (element (n, k) at B[n*ldb + k]), converted while staged.
-template <int PREC, bool BF32>
+// BF32: B is the FP32 operand itself (element (n, k) at B[n*ldb + k]), converted while staged
+// (32-deep slabs only).
+template <int PREC, bool BF32, int BKV>
 __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float alpha,
                                                     const uint16_t* __restrict__ A, i64 lda,
                                                     const void* __restrict__ Bv, i64 ldb,
                                                     float* __restrict__ C, i64 ldc,
                                                     const int* ea_dev, const int* eb_dev,
                                                     int tiles_m, int tiles_n) {
+    static_assert(!BF32 || BKV == 32, "the FP32 operand is staged in 32-deep slabs");
+    constexpr int LDK = ldk<BKV>(), RV = BKV / 16;
     __shared__ __align__(16) uint16_t As[2][BM * LDK];
     __shared__ __align__(16) uint16_t Bs[2][BN * LDK];
 
@@ -143,36 +155,39 @@ __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float a
 
     const uint16_t* B = static_cast<const uint16_t*>(Bv);
     const float* Bf = static_cast<const float*>(Bv);
-    u4 ra[2], rb[2];
-    load_slab(A, lda, m0, M, 0, ra);
+    u4 ra[RV], rb[RV];
+    load_slab<BKV>(A, lda, m0, M, 0, ra);
     if constexpr (BF32) load_slab_f32<PREC>(Bf, ldb, n0, N, 0, K, rb);
-    else load_slab(B, ldb, n0, N, 0, rb);
-    store_slab(As[0], ra);
-    store_slab(Bs[0], rb);
+    else load_slab<BKV>(B, ldb, n0, N, 0, rb);
+    store_slab<BKV>(As[0], ra);
+    store_slab<BKV>(Bs[0], rb);
     __syncthreads();
     int cur = 0;
-    for (i64 k0 = 0; k0 < K; k0 += BK) {
-        const bool more = (k0 + BK < K);
+    for (i64 k0 = 0; k0 < K; k0 += BKV) {
+        const bool more = (k0 + BKV < K);
         if (more) {
-            load_slab(A, lda, m0, M, k0 + BK, ra);
-            if constexpr (BF32) load_slab_f32<PREC>(Bf, ldb, n0, N, k0 + BK, K, rb);
-            else load_slab(B, ldb, n0, N, k0 + BK, rb);
+            load_slab<BKV>(A, lda, m0, M, k0 + BKV, ra);
+            if constexpr (BF32) load_slab_f32<PREC>(Bf, ldb, n0, N, k0 + BKV, K, rb);
+            else load_slab<BKV>(B, ldb, n0, N, k0 + BKV, rb);
         }
         const uint16_t* as = As[cur];
         const uint16_t* bs = Bs[cur];
-        u4 af[8];
 #pragma unroll
-        for (int mi = 0; mi < 8; ++mi)
-            af[mi] = *reinterpret_cast<const u4*>(as + (wm + mi * 16 + r16) * LDK + 8 * kq);
+        for (int ks = 0; ks < BKV; ks += 32) {
+            u4 af[8];
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-            const u4 bf = *reinterpret_cast<const u4*>(bs + (wn + ni * 16 + r16) * LDK + 8 * kq);
+            for (int mi = 0; mi < 8; ++mi)
+                af[mi] = *reinterpret_cast<const u4*>(as + (wm + mi * 16 + r16) * LDK + ks + 8 * kq);
 #pragma unroll
-            for (int mi = 0; mi < 8; ++mi) acc[ni][mi] = mfma<PREC>(bf, af[mi], acc[ni][mi]);
+            for (int ni = 0; ni < 4; ++ni) {
+                const u4 bf = *reinterpret_cast<const u4*>(bs + (wn + ni * 16 + r16) * LDK + ks + 8 * kq);
+#pragma unroll
+                for (int mi = 0; mi < 8; ++mi) acc[ni][mi] = mfma<PREC>(bf, af[mi], acc[ni][mi]);
+            }
         }
         if (more) {
-            store_slab(As[cur ^ 1], ra);
-            store_slab(Bs[cur ^ 1], rb);
+            store_slab<BKV>(As[cur ^ 1], ra);
+            store_slab<BKV>(Bs[cur ^ 1], rb);
         }
         __syncthreads();
         cur ^= 1;
@@ -317,15 +332,19 @@ elx_status_t launch_lpgemm_packed(hipStream_t s, int prec, i64 m, i64 n, i64 k, 
             ldb >= ((k + BK - 1) / BK) * BK);
     const i64 tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     ELX_ARG(tm * tn < ((i64)1 << 30));
-    ProfScope prof_(s, kKLpgemm, 2.0 * (double)m * (double)n * (double)k);
     const unsigned blocks = (unsigned)(((tm * tn + 7) / 8) * 8);
     const void* Bv = B;
-    if (prec == 0)
-        hipLaunchKernelGGL((lpgemm_kernel<0, false>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
-                           A, lda, Bv, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
-    else
-        hipLaunchKernelGGL((lpgemm_kernel<1, false>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
-                           A, lda, Bv, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn);
+    // 64-deep slabs for the rank-NB updates (their K extent is a multiple of 64)
+    const bool deep = k >= 512 && k % 64 == 0;
+    ProfScope prof_(s, kKLpgemm, 2.0 * (double)m * (double)n * (double)k);
+#define ELX_LPG(P, D)                                                                              \
+    hipLaunchKernelGGL((lpgemm_kernel<P, false, D>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha, \
+                       A, lda, Bv, ldb, C, ldc, ea_dev, eb_dev, (int)tm, (int)tn)
+    if (prec == 0 && deep) ELX_LPG(0, 64);
+    else if (prec == 0) ELX_LPG(0, 32);
+    else if (deep) ELX_LPG(1, 64);
+    else ELX_LPG(1, 32);
+#undef ELX_LPG
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
@@ -343,11 +362,11 @@ elx_status_t launch_lpgemm_bf32(hipStream_t s, int prec, i64 m, i64 n, i64 k, fl
     const void* Bv = B;
     const int* eb = nullptr;
     if (prec == 0)
-        hipLaunchKernelGGL((lpgemm_kernel<0, true>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
-                           A, lda, Bv, ldb, C, ldc, ea_dev, eb, (int)tm, (int)tn);
+        hipLaunchKernelGGL((lpgemm_kernel<0, true, 32>), dim3(blocks), dim3(NT), 0, s, m, n, k,
+                           alpha, A, lda, Bv, ldb, C, ldc, ea_dev, eb, (int)tm, (int)tn);
     else
-        hipLaunchKernelGGL((lpgemm_kernel<1, true>), dim3(blocks), dim3(NT), 0, s, m, n, k, alpha,
-                           A, lda, Bv, ldb, C, ldc, ea_dev, eb, (int)tm, (int)tn);
+        hipLaunchKernelGGL((lpgemm_kernel<1, true, 32>), dim3(blocks), dim3(NT), 0, s, m, n, k,
+                           alpha, A, lda, Bv, ldb, C, ldc, ea_dev, eb, (int)tm, (int)tn);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }

@@ -10,6 +10,7 @@
 // those semantics.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -114,6 +115,13 @@ struct Ctx {
 // fp16/bf16 (U scaled by a power of two for fp16). FP32-accurate form: both operands split
 // into fp16 high and low parts, C^T -= Uh^T Lh^T + 2^-11 (Uh^T Ll^T + Ul^T Lh^T), the small
 // products first.
+// The rank-NB updates on rocBLAS (gemm_ex) instead of the hand-written kernel: environment
+// ELX_LOWP_GEMM=library, read at every update.
+bool lowp_library_selected() {
+    const char* v = std::getenv("ELX_LOWP_GEMM");
+    return v && std::strcmp(v, "library") == 0;
+}
+
 elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float* Ut, float* Ct,
                        i64 ld, i64 ldpack) {
     if (m <= 0 || n <= 0 || k <= 0) return ELX_SUCCESS;
@@ -136,8 +144,8 @@ elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float
         ELX_CHECK(launch_pack(c.s, 0, true, m, k, kpad, Lt, ld, 1.0f, nullptr, c.Ll, ldpack, true));
         ELX_CHECK(launch_pack(c.s, 0, false, n, k, kpad, Ut, ld, 1.0f, e, c.Ul, ldpack, true));
     }
-    if (k >= 512) {
-        // the rank-NB update: a plain library GEMM (rocBLAS), same packs and scaling
+    if (k >= 512 && lowp_library_selected()) {
+        // the rank-NB update as a plain library GEMM (rocBLAS), same packs and scaling
         ELX_CHECK(launch_gemm_alphas(c.s, e, 1, c.alphas));
         if (split) {
             ELX_CHECK(library_gemm_lowp(c.s, 0, n, m, kpad, c.alphas + 1, c.Uh, ldpack, c.Ll,
