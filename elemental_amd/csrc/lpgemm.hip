@@ -196,30 +196,38 @@ __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float a
     // Lane holds D(i = n, j = m) with n = n0+wn+ni*16+4*kq+r, m = m0+wm+mi*16+r16.
     const int ea = ea_dev ? *ea_dev : 0, eb = eb_dev ? *eb_dev : 0;
     const float a = alpha * __builtin_ldexpf(1.0f, -(ea + eb));
-    // C += a * acc, 32 values per lane at a time: all 32 loads are issued before any store,
-    // so they are in flight together (a load may not move above a store to C that could
-    // alias it, which would make each read-modify-write wait out a full memory round trip).
+    // C += a * acc, in CVN column groups of 32 values per lane: all of a round's loads are
+    // issued before any of its stores, so they are in flight together (a load may not move
+    // above a store to C that could alias it, which would make each read-modify-write wait out
+    // a full memory round trip). The 64-deep form has the registers of its larger operand
+    // slabs free by now and takes two groups per round.
+    constexpr int CVN = BKV == 64 ? 2 : 1;
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-        float cv[4][8];
+    for (int n0i = 0; n0i < 4; n0i += CVN) {
+        float cv[CVN][4][8];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const i64 n = n0 + wn + ni * 16 + 4 * kq + r;
+        for (int g = 0; g < CVN; ++g)
 #pragma unroll
-            for (int mi = 0; mi < 8; ++mi) {
-                const i64 m = m0 + wm + mi * 16 + r16;
-                cv[r][mi] = (n < N && m < M) ? C[m + n * ldc] : 0.0f;
+            for (int r = 0; r < 4; ++r) {
+                const i64 n = n0 + wn + (n0i + g) * 16 + 4 * kq + r;
+#pragma unroll
+                for (int mi = 0; mi < 8; ++mi) {
+                    const i64 m = m0 + wm + mi * 16 + r16;
+                    cv[g][r][mi] = (n < N && m < M) ? C[m + n * ldc] : 0.0f;
+                }
             }
-        }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const i64 n = n0 + wn + ni * 16 + 4 * kq + r;
+        for (int g = 0; g < CVN; ++g)
 #pragma unroll
-            for (int mi = 0; mi < 8; ++mi) {
-                const i64 m = m0 + wm + mi * 16 + r16;
-                if (n < N && m < M) C[m + n * ldc] = __builtin_fmaf(a, acc[ni][mi][r], cv[r][mi]);
+            for (int r = 0; r < 4; ++r) {
+                const i64 n = n0 + wn + (n0i + g) * 16 + 4 * kq + r;
+#pragma unroll
+                for (int mi = 0; mi < 8; ++mi) {
+                    const i64 m = m0 + wm + mi * 16 + r16;
+                    if (n < N && m < M)
+                        C[m + n * ldc] = __builtin_fmaf(a, acc[n0i + g][mi][r], cv[g][r][mi]);
+                }
             }
-        }
     }
 }
 
