@@ -62,6 +62,32 @@ def test_factor_alone_is_a_preconditioner(dev, orc, factor):
     assert eta < 50 * u_low
 
 
+@pytest.mark.parametrize("factor", ["fp16", "bf16", "fp32"])
+def test_rank_nb_update_kernel_matches_library(dev, orc, factor, monkeypatch):
+    """The rank-1024 trailing update of the mixed LU (n = 3000: two of them) on the
+    hand-written 64-deep-slab MFMA kernel and on rocBLAS gemm_ex (ELX_LOWP_GEMM=library): the
+    same pivots and one-shot solves of the same quality. Both accumulate in FP32 from the same
+    fp16/bf16 packs, so only the order of the FP32 additions differs."""
+    n = 3000
+    A = randsvd(n, 1e2, 27)
+    b = np.random.default_rng(28).standard_normal((n, 1))
+    etas = {}
+    for gemm in ("own", "library"):
+        if gemm == "library":
+            monkeypatch.setenv("ELX_LOWP_GEMM", "library")
+        else:
+            monkeypatch.delenv("ELX_LOWP_GEMM", raising=False)
+        dA, dB = dev.to_device(A), dev.to_device(b)
+        dX = dev.colmajor_empty(n, 1)
+        st = dev.gesv_mixed(dA, dB, dX, factor=factor, refine="none")
+        assert st["status"] in (0, 5) and st["factor_info"] == 0
+        etas[gemm] = orc.backward_error(A, dev.to_host(dX), b)
+    say(factor, "one-shot backward error own / rocBLAS", etas)
+    u_low = {"fp16": 2.0 ** -11, "bf16": 2.0 ** -8, "fp32": 2.0 ** -22}[factor]
+    assert etas["own"] < 50 * u_low
+    assert etas["own"] <= 2.0 * etas["library"] and etas["library"] <= 2.0 * etas["own"]
+
+
 def test_mixed_multiple_rhs(dev, orc):
     n = 700
     A = randsvd(n, 1e2, 3)
