@@ -454,13 +454,13 @@ elx_status_t gemm_impl(hipStream_t s, bool ta, bool tb, i64 m, i64 n, i64 k, T a
     if constexpr (sizeof(T) == 8) {
         // Large FP64 products go to rocBLAS (dgemm, and dsyrk for the symmetric rank-k update:
         // A == B, one operand transposed, one triangle of C) with ELX_DGEMM=library, and so do
-        // the narrow ones (fewer than 128 rows or columns, e.g. the residual of a block of
-        // right-hand sides, 32768 x 64 x 32768): there every 128 x 128 tile of the hand-written
-        // kernel is an edge tile, and with only m/128 of them each runs the whole K extent
-        // (a 64-right-hand-side refinement took 1.88 s instead of 1.53 s). In a factorization
+        // the narrow ones with a long K (fewer than 128 rows or columns and K >= 4096, e.g. the
+        // residual of a block of right-hand sides, 32768 x 64 x 32768): there every 128 x 128
+        // tile of the hand-written kernel is an edge tile, and only m/128 workgroups each run
+        // the whole K extent (1.4 ms more per residual at that shape). In a factorization
         // sequence the coefficients come from the device (gate_kernel), so a recorded failure
         // leaves C as it is.
-        const bool narrow = std::min(m, n) < BM;
+        const bool narrow = std::min(m, n) < BM && k >= 4096;
         if (allow_library && (!dgemm_own_selected() || narrow) &&
             (double)m * (double)n * (double)k >= (double)(1 << 24)) {
             const bool syrk = tri != 0 && A == B && lda == ldb && m == n && ta != tb && (tri == 1) == tb;
