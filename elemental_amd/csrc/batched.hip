@@ -971,8 +971,8 @@ __device__ __forceinline__ T wave_sum(T v) {
 // columns in registers.
 constexpr int NR = 8, SB = 32, SLD = SB + 1;
 template <typename T>
-__device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T* Y, T* Part,
-                          T* Lb) {
+__device__ void potrs_one(int n, const gmem<T>* L, int lda, gmem<T>* B, int ldb, int nrhs, T* Y,
+                          T* Part, T* Lb) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int nblk = (n + SB - 1) / SB;
     auto stage_l11 = [&](int k0, int kb) {
@@ -1008,7 +1008,7 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
 #pragma unroll
                         for (int q = 0; q < NR; ++q) {
                             if (lane == k) yv[q] = yv[q] / lrow[k];
-                            const T yk = __shfl(yv[q], k);
+                            const T yk = bcast(yv[q], k);
                             if (lane > k) yv[q] = __builtin_fma(-lrow[k], yk, yv[q]);
                         }
                     }
@@ -1023,7 +1023,7 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
                 T acc[NR];
 #pragma unroll
                 for (int q = 0; q < NR; ++q) acc[q] = (q < nr) ? Y[q * BNMAX + i] : T(0);
-                const T* li = L + i + (size_t)k0 * lda;
+                const gmem<T>* li = L + i + (size_t)k0 * lda;
 #pragma unroll 8
                 for (int k = 0; k < kb; ++k) {
                     const T l = li[(size_t)k * lda];
@@ -1044,7 +1044,7 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
                 T acc[NR];
 #pragma unroll
                 for (int q = 0; q < NR; ++q) acc[q] = T(0);
-                const T* col = L + (size_t)(k0 + j) * lda;
+                const gmem<T>* col = L + (size_t)(k0 + j) * lda;
 #pragma unroll 2
                 for (int i = r0 + lane; i < n; i += 64) {
                     const T l = col[i];
@@ -1074,7 +1074,7 @@ __device__ void potrs_one(int n, const T* L, int lda, T* B, int ldb, int nrhs, T
 #pragma unroll
                         for (int q = 0; q < NR; ++q) {
                             if (lane == i) xv[q] = xv[q] / lcol[i];
-                            const T xi = __shfl(xv[q], i);
+                            const T xi = bcast(xv[q], i);
                             if (lane < i) xv[q] = __builtin_fma(-lcol[i], xi, xv[q]);
                         }
                     }
@@ -1112,7 +1112,8 @@ __global__ __launch_bounds__(BNT, 2) void potrs_vbatched_kernel(
         const BatchItem it = items[i];
         if (info && info[it.index] != 0) continue;
         if (it.n > 0 && it.nrhs > 0)
-            potrs_one<T>(it.n, Ls[it.index], it.lda, Bs[it.index], it.ldb, it.nrhs, Y, Part, Lb);
+            potrs_one<T>(it.n, (const gmem<T>*)Ls[it.index], it.lda, (gmem<T>*)Bs[it.index], it.ldb,
+                         it.nrhs, Y, Part, Lb);
     }
 }
 
