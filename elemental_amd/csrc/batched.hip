@@ -971,8 +971,8 @@ __device__ __forceinline__ T wave_sum(T v) {
 // columns in registers.
 constexpr int NR = 8, SB = 32, SLD = SB + 1;
 template <typename T>
-__device__ void potrs_one(int n, const gmem<T>* L, int lda, gmem<T>* B, int ldb, int nrhs, T* Y,
-                          T* Part, T* Lb) {
+__device__ __forceinline__ void potrs_one(int n, const gmem<T>* L, int lda, gmem<T>* B, int ldb,
+                                          int nrhs, T* Y, T* Part, T* Lb) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int nblk = (n + SB - 1) / SB;
     auto stage_l11 = [&](int k0, int kb) {
