@@ -468,6 +468,17 @@ __global__ __launch_bounds__(NT) void lu_factor_kernel(const LuItem* __restrict_
 // ------------------------------------------------------------------ refinement
 // Work space of one refining workgroup (FP64, NMAX entries per vector): XP, R, D (NRC vectors
 // each), V (RESTART + 1), Z (RESTART).
+// Lane src's value in every lane (src uniform), through the scalar unit: the diagonal-block
+// solves' broadcast, one readlane (two for FP64) instead of a cross-lane LDS permute.
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
 template <int NMAX>
 struct RefWs {
     static constexpr size_t per_slot = (size_t)(3 * NRC + 2 * RESTART + 1) * NMAX;
@@ -501,7 +512,7 @@ struct Refiner {
                 constexpr int k = decltype(kk)::value;
 #pragma unroll
                 for (int q = 0; q < NRC; ++q) {
-                    const T yk = __shfl(yv[q], k);
+                    const T yk = lane_bcast(yv[q], k);
                     if (r > k) yv[q] = __builtin_fma(-row[k], yk, yv[q]);
                 }
             });
@@ -511,7 +522,7 @@ struct Refiner {
 #pragma unroll
                 for (int q = 0; q < NRC; ++q) {
                     if (r == k) yv[q] = live ? yv[q] / row[k] : yv[q];
-                    const T yk = __shfl(yv[q], k);
+                    const T yk = lane_bcast(yv[q], k);
                     if (r < k) yv[q] = __builtin_fma(-row[k], yk, yv[q]);
                 }
             });
