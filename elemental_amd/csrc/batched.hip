@@ -1002,14 +1002,19 @@ __device__ __forceinline__ void potrs_one(int n, const gmem<T>* L, int lda, gmem
                 for (int j = 0; j < SB; ++j) lrow[j] = (lane < kb) ? Lb[lane * SLD + j] : T(0);
 #pragma unroll
                 for (int q = 0; q < NR; ++q) yv[q] = (lane < kb && q < nr) ? Y[q * BNMAX + k0 + lane] : T(0);
+                // reciprocals of the diagonal, one division per lane, broadcast per step
+                const T dinv = (lane < kb) ? T(1) / Lb[lane * SLD + lane] : T(0);
 #pragma unroll
                 for (int k = 0; k < SB; ++k) {
                     if (k < kb) {
+                        const T dk = bcast(dinv, k);
 #pragma unroll
                         for (int q = 0; q < NR; ++q) {
-                            if (lane == k) yv[q] = yv[q] / lrow[k];
-                            const T yk = bcast(yv[q], k);
-                            if (lane > k) yv[q] = __builtin_fma(-lrow[k], yk, yv[q]);
+                            if (q < nr) {
+                                if (lane == k) yv[q] = yv[q] * dk;
+                                const T yk = bcast(yv[q], k);
+                                if (lane > k) yv[q] = __builtin_fma(-lrow[k], yk, yv[q]);
+                            }
                         }
                     }
                 }
@@ -1068,14 +1073,18 @@ __device__ __forceinline__ void potrs_one(int n, const gmem<T>* L, int lda, gmem
 #pragma unroll
                 for (int q = 0; q < NR; ++q)
                     xv[q] = (lane < kb && q < nr) ? Y[q * BNMAX + k0 + lane] - Part[lane * NR + q] : T(0);
+                const T dinv = (lane < kb) ? T(1) / Lb[lane * SLD + lane] : T(0);
 #pragma unroll
                 for (int i = SB - 1; i >= 0; --i) {
                     if (i < kb) {
+                        const T di = bcast(dinv, i);
 #pragma unroll
                         for (int q = 0; q < NR; ++q) {
-                            if (lane == i) xv[q] = xv[q] / lcol[i];
-                            const T xi = bcast(xv[q], i);
-                            if (lane < i) xv[q] = __builtin_fma(-lcol[i], xi, xv[q]);
+                            if (q < nr) {
+                                if (lane == i) xv[q] = xv[q] * di;
+                                const T xi = bcast(xv[q], i);
+                                if (lane < i) xv[q] = __builtin_fma(-lcol[i], xi, xv[q]);
+                            }
                         }
                     }
                 }
