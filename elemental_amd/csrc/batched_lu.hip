@@ -138,7 +138,7 @@ struct FactorLds<double> {
 // One system: W (ldw x ldw, row by row) = LU of fp(A) with partial pivoting. Returns the
 // 1-based column of an exactly zero (or NaN) pivot, 0 on success.
 template <typename T, int NT, int NMAX>
-__device__ int factor_one(const LuItem& it, T* W, int32_t* piv, unsigned char* dyn) {
+__device__ __forceinline__ int factor_one(const LuItem& it, T* W, int32_t* piv, unsigned char* dyn) {
     constexpr int PB = LuCfg<T>::PB;
     constexpr int RPT = NMAX / NT;  // panel rows per thread
     constexpr int NW = NT / 64;
@@ -533,7 +533,7 @@ struct Refiner {
     }
 
     // dst(:, q) = M^-1 src(:, q) for q < nq; src/dst columns at stride ssd / dsd (FP64).
-    __device__ void precond(const double* src, int ssd, double* dst, int dsd, int nq) {
+    __device__ __forceinline__ void precond(const double* src, int ssd, double* dst, int dsd, int nq) {
         const int t = threadIdx.x;
         for (int idx = t; idx < NRC * ldw; idx += NT) {
             const int q = idx / ldw, i = idx % ldw;
