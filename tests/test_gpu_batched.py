@@ -158,8 +158,24 @@ def test_batched_wide_panels(dev, orc, dtype):
             for b in (1, 2):
                 dev.batched_variant(dtype, b, 64, occ, 2)
             say("wide panels", dtype, occ)
-            mats = [dev.to_device(A, dtype=dtype) for A in As]
+            # every third matrix a view with 9 padding rows below it (leading dimension n + 9),
+            # the padding holding 7.0: it must come back untouched
+            mats, pads = [], []
+            for i, A in enumerate(As):
+                if i % 3 == 0:
+                    n = A.shape[0]
+                    big = dev.colmajor_empty(n + 9, n, dtype=dtype)
+                    big.fill_(7.0)
+                    big[:n].copy_(dev.to_device(A, dtype=dtype))
+                    mats.append(big[:n])
+                    pads.append(big)
+                else:
+                    mats.append(dev.to_device(A, dtype=dtype))
+                    pads.append(None)
             info = dev.potrf_vbatched(dev.Batch(mats, dtype)).cpu().numpy()
+            for big, A in zip(pads, As):
+                if big is not None:
+                    assert (dev.to_host(big)[A.shape[0]:] == 7.0).all()
             for i, (A, n) in enumerate(zip(As, ns)):
                 Fo, info_o = orc.cholesky(A, uplo=0)
                 assert info[i] == info_o, (i, n, info[i], info_o)
