@@ -1008,13 +1008,13 @@ __device__ __forceinline__ void potrs_one(int n, const gmem<T>* L, int lda, gmem
                 for (int k = 0; k < SB; ++k) {
                     if (k < kb) {
                         const T dk = bcast(dinv, k);
+                        // every column slot, branch-free (slots >= nr hold zeros): a guard per
+                        // slot would split the step chain into small branch blocks
 #pragma unroll
                         for (int q = 0; q < NR; ++q) {
-                            if (q < nr) {
-                                if (lane == k) yv[q] = yv[q] * dk;
-                                const T yk = bcast(yv[q], k);
-                                if (lane > k) yv[q] = __builtin_fma(-lrow[k], yk, yv[q]);
-                            }
+                            if (lane == k) yv[q] = yv[q] * dk;
+                            const T yk = bcast(yv[q], k);
+                            if (lane > k) yv[q] = __builtin_fma(-lrow[k], yk, yv[q]);
                         }
                     }
                 }
@@ -1080,11 +1080,9 @@ __device__ __forceinline__ void potrs_one(int n, const gmem<T>* L, int lda, gmem
                         const T di = bcast(dinv, i);
 #pragma unroll
                         for (int q = 0; q < NR; ++q) {
-                            if (q < nr) {
-                                if (lane == i) xv[q] = xv[q] * di;
-                                const T xi = bcast(xv[q], i);
-                                if (lane < i) xv[q] = __builtin_fma(-lcol[i], xi, xv[q]);
-                            }
+                            if (lane == i) xv[q] = xv[q] * di;
+                            const T xi = bcast(xv[q], i);
+                            if (lane < i) xv[q] = __builtin_fma(-lcol[i], xi, xv[q]);
                         }
                     }
                 }
