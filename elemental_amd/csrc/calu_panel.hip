@@ -345,7 +345,7 @@ __global__ __launch_bounds__(NT) void calu_panel_kernel(CaluArgs p) {
             nsurv = misc[3];
             survp = comp;
         }
-        // round 2 (or the only round): wave 0 over at most 128 candidates
+        // second merge round (or the only one): wave 0 over at most 128 candidates
         if (wave == 0) {
             float v[2][WS];
             bool act[2];

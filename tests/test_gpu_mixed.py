@@ -1,6 +1,6 @@
 """Mixed-precision LU + FP64 iterative refinement on the GPU.
 
-Bar (north star): every solve reports FP64 normwise backward error <= n * 2^-53, checked
+Bar: every solve reports FP64 normwise backward error <= n * 2^-53, checked
 here independently with the oracle's double-double residual; the factorization alone
 (no refinement) must be a useful preconditioner (backward error well below 1)."""
 import numpy as np
