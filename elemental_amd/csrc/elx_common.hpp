@@ -149,8 +149,11 @@ elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, in
 elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, int diag, i64 m,
                                i64 n, const float* A, i64 lda, float* B, i64 ldb,
                                const int32_t* stop = nullptr);
+// maxbits (may be null): the largest |entry| of the result is folded into *maxbits (bits of a
+// non-negative float, atomic max), for the exponent of the low-precision update that follows.
 elx_status_t launch_trsm_small_f32(hipStream_t s, int side, int uplo, int orient, int diag,
-                                   i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb);
+                                   i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb,
+                                   unsigned* maxbits = nullptr);
 
 // General triangular solve (blocked by 128 with GEMM updates for larger triangles).
 elx_status_t dtrsm(hipStream_t s, int side, int uplo, int orient, int diag, i64 m, i64 n,
@@ -234,9 +237,12 @@ elx_status_t launch_lpgemm_bf32(hipStream_t s, int prec, i64 m, i64 n, i64 k, fl
                                 i64 ldc, const int* ea_dev);
 // H(r,k) = cvt(sign * 2^e * S(r,k)), S(r,k) = S[r + k*lds] (trans false) or S[k + r*lds];
 // lo (fp16 only): the low part cvt((v - float(cvt(v))) * 2^11) of v = sign * 2^e * S(r,k).
+// mx_in (may be null): e = scale_exp_of of the maximum folded into *mx_in by S's producer
+// instead of *e_dev; the pack then stores e at e_out and clears *mx_clear.
 elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
                          const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
-                         i64 ldh, bool lo = false);
+                         i64 ldh, bool lo = false, const unsigned* mx_in = nullptr,
+                         unsigned* mx_clear = nullptr, int* e_out = nullptr);
 // C(m x n, column-major) += alpha * A B^T with A (m x k) and B (n x k) fp16 (prec 0) or
 // bf16 (prec 1) packs (row r at A + r*lda), FP32 C, as a rocBLAS gemm_ex call; alpha_dev and
 // beta_dev are device floats.

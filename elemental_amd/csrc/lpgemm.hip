@@ -240,6 +240,18 @@ __global__ __launch_bounds__(NT) void lpgemm_kernel(i64 M, i64 N, i64 K, float a
 // (TRANS = true). The column-major case goes through a 64x64 LDS transpose so both the reads
 // and the K-major writes are coalesced.
 // ---------------------------------------------------------------------------------------
+// e such that max * 2^e < 2^15 (fp16 headroom for K-term fp32 accumulation), clamped.
+__device__ __forceinline__ int scale_exp_of(float mx) {
+    int e = 0;
+    if (mx > 0.0f && isfinite(mx)) {
+        int p;
+        (void)frexpf(mx, &p);  // mx < 2^p
+        e = 15 - p;
+        e = max(-100, min(100, e));
+    }
+    return e;
+}
+
 template <int PREC, bool TRANS, bool LO>
 __device__ __forceinline__ uint16_t pack_one(float v) {
     typedef typename HalfT<PREC>::T HT;
@@ -248,12 +260,25 @@ __device__ __forceinline__ uint16_t pack_one(float v) {
     return __builtin_bit_cast(uint16_t, h);
 }
 
+// mx_in (may be null): the exponent comes from the largest magnitude folded into *mx_in by the
+// kernel that produced S (instead of from e_dev); workgroup (0, 0) then stores it at e_out for
+// the GEMM and clears *mx_clear, the word the next producer folds into.
 template <int PREC, bool TRANS, bool LO>
 __global__ __launch_bounds__(256) void pack_kernel(i64 R, i64 K, i64 Kpad, const float* __restrict__ S,
                                                    i64 lds, float sign, const int* e_dev,
-                                                   uint16_t* __restrict__ H, i64 ldh) {
+                                                   uint16_t* __restrict__ H, i64 ldh,
+                                                   const unsigned* mx_in, unsigned* mx_clear,
+                                                   int* e_out) {
     __shared__ float tile[64][65];
-    const float sc = sign * __builtin_ldexpf(1.0f, e_dev ? *e_dev : 0);
+    int e = e_dev ? *e_dev : 0;
+    if (mx_in) {
+        e = scale_exp_of(__uint_as_float(*mx_in));
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+            *e_out = e;
+            *mx_clear = 0u;
+        }
+    }
+    const float sc = sign * __builtin_ldexpf(1.0f, e);
     const i64 r0 = (i64)blockIdx.x * 64, k0 = (i64)blockIdx.y * 64;
     const int t = threadIdx.x;
     if (!TRANS) {
@@ -279,18 +304,6 @@ __global__ __launch_bounds__(256) void pack_kernel(i64 R, i64 K, i64 Kpad, const
             }
         }
     }
-}
-
-// e such that max * 2^e < 2^15 (fp16 headroom for K-term fp32 accumulation), clamped.
-__device__ __forceinline__ int scale_exp_of(float mx) {
-    int e = 0;
-    if (mx > 0.0f && isfinite(mx)) {
-        int p;
-        (void)frexpf(mx, &p);  // mx < 2^p
-        e = 15 - p;
-        e = max(-100, min(100, e));
-    }
-    return e;
 }
 
 // e_out = scale_exp_of(max |S(i,j)|) over an m x n column-major block. Workgroups take whole
@@ -381,7 +394,8 @@ elx_status_t launch_lpgemm_bf32(hipStream_t s, int prec, i64 m, i64 n, i64 k, fl
 
 elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 Kpad,
                          const float* S, i64 lds, float sign, const int* e_dev, uint16_t* H,
-                         i64 ldh, bool lo) {
+                         i64 ldh, bool lo, const unsigned* mx_in, unsigned* mx_clear,
+                         int* e_out) {
     if (R <= 0 || Kpad <= 0) return ELX_SUCCESS;
     ELX_ARG(ldh >= Kpad && (prec == 0 || !lo));
     dim3 grid(cdiv(R, 64), cdiv(Kpad, 64));
@@ -389,7 +403,7 @@ elx_status_t launch_pack(hipStream_t s, int prec, bool trans, i64 R, i64 K, i64 
     ProfScope prof_(s, kKPack, (double)R * (double)Kpad * 6.0);
 #define ELX_PACK(P, TR, LO_)                                                                   \
     hipLaunchKernelGGL((pack_kernel<P, TR, LO_>), grid, dim3(256), 0, s, R, K, Kpad, S, lds, sign, \
-                       e_dev, H, ldh)
+                       e_dev, H, ldh, mx_in, mx_clear, e_out)
     if (prec == 0) {
         if (lo) {
             if (trans) ELX_PACK(0, true, true);

@@ -220,11 +220,20 @@ constexpr int TP = 16;           // lanes per vector
 constexpr int TNT = 256;         // threads per workgroup
 constexpr int TV = TNT / TP;     // vectors per workgroup
 
+// maxbits (may be null): the largest |entry| written is folded into *maxbits (the bits of a
+// non-negative float, atomic max), for the exponent of a low-precision product that follows.
+__device__ __forceinline__ void fold_absmax(float mx, unsigned* maxbits) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(mx));
+}
+
 template <typename T>
 __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const T* __restrict__ A,
                                                          i64 lda, T* B, i64 ldb, int right,
                                                          int lower_eff, int mtrans, int unit,
-                                                         const int32_t* __restrict__ stop) {
+                                                         const int32_t* __restrict__ stop,
+                                                         unsigned* maxbits) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     if (stop && *stop != 0) return;
     // Both LDS arrays have a leading dimension of m + 1, so transposed stagings (consecutive
@@ -321,6 +330,7 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
         }
     }
     __syncthreads();
+    float mx = 0.0f;
     for (int idx = t; idx < m * TV; idx += TNT) {
         int i, vv;
         if (!right) {
@@ -334,8 +344,10 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
         if (gv < nvec) {
             if (right) B[gv + (i64)i * ldb] = x[vv * xm + i];
             else B[i + gv * ldb] = x[vv * xm + i];
+            if (maxbits) mx = fmaxf(mx, (float)fabs(x[vv * xm + i]));
         }
     }
+    if (maxbits) fold_absmax(mx, maxbits);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -624,7 +636,7 @@ elx_status_t launch_gather_rows(hipStream_t s, const int32_t* perm, i64 m, i64 n
 template <typename T>
 static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orient, int diag,
                                     i64 m, i64 n, const T* A, i64 lda, T* B, i64 ldb,
-                                    const int32_t* stop) {
+                                    const int32_t* stop, unsigned* maxbits = nullptr) {
     if (m <= 0 || n <= 0) return ELX_SUCCESS;
     const bool right = (side == kRight);
     const i64 order = right ? n : m;
@@ -635,7 +647,7 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     // Left: M = op(A). Right: M = op(A)^T.
     const bool mtrans = right ? !trans : trans;
     const bool lower_eff = lower != mtrans;
-    if (!right && lower && !trans && diag == kUnit && nvec >= 256) {
+    if (!right && lower && !trans && diag == kUnit && nvec >= 256 && !maxbits) {
         // many right-hand sides: the blocked form (same FMA order per entry)
         const size_t lds2 = sizeof(T) * (128 * 65 + 128 * 33);
         ProfScope prof2_(s, kKTrsm, (double)order * (double)order * (double)nvec);
@@ -663,7 +675,7 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
     }
     hipLaunchKernelGGL(trsm_small_kernel<T>, dim3(cdiv(nvec, TV)), dim3(TNT), lds, s, (int)order,
                        nvec, A, lda, B, ldb, right ? 1 : 0, lower_eff ? 1 : 0, mtrans ? 1 : 0,
-                       diag == kUnit ? 1 : 0, stop);
+                       diag == kUnit ? 1 : 0, stop, maxbits);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
@@ -681,8 +693,10 @@ elx_status_t launch_trsm_small(hipStream_t s, int side, int uplo, int orient, in
 }
 
 elx_status_t launch_trsm_small_f32(hipStream_t s, int side, int uplo, int orient, int diag,
-                                   i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb) {
-    return trsm_small_impl<float>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb, nullptr);
+                                   i64 m, i64 n, const float* A, i64 lda, float* B, i64 ldb,
+                                   unsigned* maxbits) {
+    return trsm_small_impl<float>(s, side, uplo, orient, diag, m, n, A, lda, B, ldb, nullptr,
+                                  maxbits);
 }
 
 elx_status_t launch_potf2_f64(hipStream_t s, int uplo, i64 n, double* A, i64 lda, i64 goff,

@@ -105,6 +105,7 @@ struct Ctx {
     int32_t* info;
     int* e_dev;
     unsigned* bits;
+    unsigned* ring;  // two words: the U block maxima folded in by the TRSM / TRMM producing them
     float* alphas;  // device alphas of the library GEMM (3 floats)
 };
 
@@ -122,12 +123,22 @@ bool lowp_library_selected() {
     return v && std::strcmp(v, "library") == 0;
 }
 
+// umax (may be null): one of c.ring's words, into which the kernel that produced U folded its
+// largest magnitude; the U pack then takes the exponent from it (no reduction launch) and clears
+// the other ring word, which the next producer folds into.
 elx_status_t lp_update(Ctx& c, i64 m, i64 n, i64 k, const float* Lt, const float* Ut, float* Ct,
-                       i64 ld, i64 ldpack) {
+                       i64 ld, i64 ldpack, unsigned* umax = nullptr) {
     if (m <= 0 || n <= 0 || k <= 0) return ELX_SUCCESS;
     const i64 kpad = round_up(k, 32);
     const bool split = (c.prec == 2);
     const int hp = split ? 0 : c.prec;  // element type of the MFMA operands
+    const bool l_direct0 = !split && k < 512;
+    if (umax && hp == 0 && l_direct0) {
+        unsigned* other = (umax == c.ring) ? c.ring + 1 : c.ring;
+        ELX_CHECK(launch_pack(c.s, 0, false, n, k, kpad, Ut, ld, 1.0f, nullptr, c.Uh, ldpack,
+                              false, umax, other, c.e_dev));
+        return launch_lpgemm_bf32(c.s, 0, n, m, k, -1.0f, c.Uh, ldpack, Lt, ld, Ct, ld, c.e_dev);
+    }
     const int* e = nullptr;
     if (hp == 0) {
         // c.bits: zeroed with the solve's control words, left zero by each exponent launch
@@ -178,6 +189,12 @@ elx_status_t getrf_mixed(Ctx& c) {
     // A(i, j) of the working matrix
     auto at = [&](i64 i, i64 j) { return W + j + i * ld; };
     ELX_HIP(hipMemsetAsync(c.info, 0, sizeof(int32_t), c.s));
+    // the producers of the rank-128 updates' U blocks fold their maxima into c.ring[rp], the
+    // pack of that update clears c.ring[rp ^ 1] for the next one (both start at zero; with bf16
+    // or split operands no exponent is taken and the words stay zero)
+    const bool fold = c.prec == 0;
+    int rp = 0;
+    if (fold) ELX_HIP(hipMemsetAsync(c.ring, 0, 2 * sizeof(unsigned), c.s));
     for (i64 K0 = 0; K0 < n; K0 += c.NB) {
         const i64 nbk = std::min(c.NB, n - K0);
         const i64 K1 = K0 + nbk;
@@ -199,10 +216,12 @@ elx_status_t getrf_mixed(Ctx& c) {
             const i64 wr = K1 - (j0 + w);
             if (wr > 0) {
                 // U12 = L11^{-1} A12, as U12^T L11^T = A12^T on the stored transposes
+                unsigned* um = fold ? c.ring + rp : nullptr;
                 ELX_CHECK(launch_trsm_small_f32(c.s, kRight, kUpper, kNormal, kUnit, wr, w,
-                                                at(j0, j0), ld, at(j0, j0 + w), ld));
+                                                at(j0, j0), ld, at(j0, j0 + w), ld, um));
                 ELX_CHECK(lp_update(c, n - (j0 + w), wr, w, at(j0 + w, j0), at(j0, j0 + w),
-                                    at(j0 + w, j0 + w), ld, 128));
+                                    at(j0 + w, j0 + w), ld, 128, um));
+                rp ^= fold ? 1 : 0;
             }
         }
         // the outer block's swaps on the columns outside it, in panel order
@@ -220,12 +239,15 @@ elx_status_t getrf_mixed(Ctx& c) {
         // U12 = L11^{-1} A12, blocked by 128, each block by its diagonal inverse
         for (i64 jb = K0; jb < K1; jb += 128) {
             const i64 wb = std::min<i64>(128, K1 - jb);
-            ELX_CHECK(launch_trmm_inv(c.s, wb, n2, c.Linv + (jb / 128) * 128 * 128, at(jb, K1),
-                                      ld));
             const i64 below = K1 - (jb + wb);
-            if (below > 0)
+            unsigned* um = (fold && below > 0) ? c.ring + rp : nullptr;
+            ELX_CHECK(launch_trmm_inv(c.s, wb, n2, c.Linv + (jb / 128) * 128 * 128, at(jb, K1),
+                                      ld, um));
+            if (below > 0) {
                 ELX_CHECK(lp_update(c, below, n2, wb, at(jb + wb, jb), at(jb, K1),
-                                    at(jb + wb, K1), ld, 128));
+                                    at(jb + wb, K1), ld, 128, um));
+                rp ^= fold ? 1 : 0;
+            }
         }
         // A22 -= L21 U12 (rank nbk)
         ELX_CHECK(lp_update(c, n - K1, n2, nbk, at(K1, K0), at(K0, K1), at(K1, K1), ld,
@@ -355,7 +377,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     ELX_CHECK(get(kSlotLinv, (size_t)nblk * 128 * 128, &c.Linv));
     ELX_CHECK(get(kSlotUinv, (size_t)nblk * 128 * 128, &c.Uinv));
     int32_t* idx = nullptr;
-    ELX_CHECK(get(kSlotIdx, (size_t)(2 * n + 8 + (c.NB / 32 + 1) * 512), &idx));
+    ELX_CHECK(get(kSlotIdx, (size_t)(2 * n + 16 + (c.NB / 32 + 1) * 512), &idx));
     c.xcd_panels = xcd_panels_enabled();
     c.panel_mode = mixed_panel_mode();
     c.calu_min_order = mixed_panel_min_order();
@@ -368,7 +390,8 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
     c.e_dev = idx + 2 * n + 1;
     c.bits = reinterpret_cast<unsigned*>(idx + 2 * n + 2);
     c.alphas = reinterpret_cast<float*>(idx + 2 * n + 4);
-    c.pairs = reinterpret_cast<int2*>(idx + 2 * n + 8);
+    c.ring = reinterpret_cast<unsigned*>(idx + 2 * n + 8);
+    c.pairs = reinterpret_cast<int2*>(idx + 2 * n + 16);
     double* vec = nullptr;  // r, norms(4), anorm
     ELX_CHECK(get(kSlotVec, (size_t)(n + 8), &vec));
     double* r = vec;
@@ -387,7 +410,7 @@ static elx_status_t mixed_solve(int kind, void* stream, int64_t n, int64_t nrhs,
 
     // ---- factorization ----
     Timer tf(s);
-    ELX_HIP(hipMemsetAsync(idx + 2 * n, 0, sizeof(int32_t) * 8, s));
+    ELX_HIP(hipMemsetAsync(idx + 2 * n, 0, sizeof(int32_t) * 16, s));
     ELX_CHECK(launch_convert(s, n, dA, lda, c.W, c.ldw, part, nch));
     ELX_HIP(hipMemsetAsync(anorm_d, 0, sizeof(double), s));
     ELX_CHECK(launch_rowabs_max(s, n, nch, part, anorm_d));

@@ -27,7 +27,10 @@ elx_status_t launch_chol_to_lu(hipStream_t s, i64 n, float* W, i64 ldw, float* d
 elx_status_t launch_iota(hipStream_t s, i64 n, int32_t* p);
 // X := Inv * X for an nb-row block X (nb <= 128) of ncols columns stored row by row
 // (X(k, j) at B[j + k*ldb]), in place.
-elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb);
+// maxbits (may be null): the largest |entry| of the result folded into *maxbits (atomic max of
+// the bits of a non-negative float).
+elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb,
+                             unsigned* maxbits = nullptr);
 // Persistent triangular solve with the factor stored row by row in W (fp32, ldw % 4 == 0,
 // padding columns zero).
 // xchunks: 64 x 16 bytes per 128-row block, uncached; epoch distinguishes this launch's

@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(i64 n, const float* _
 // them back in place. Thread (jg, ig) forms an 8 x 4 block of the result: rows 8*ig..,
 // columns 4*jg.., so the 16 lanes of a row group write 256 contiguous bytes.
 __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const float* __restrict__ Inv,
-                                                       float* B, i64 ldb) {
+                                                       float* B, i64 ldb, unsigned* maxbits) {
     typedef float f4v __attribute__((ext_vector_type(4)));
     extern __shared__ __align__(16) float sm[];
     float* Is = sm;               // 128 x 128, Is[i + k*128]
@@ -209,6 +209,7 @@ __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const 
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fmaf(iv[a], bv[b], acc[a][b]);
     }
+    float mx = 0.0f;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         const int i = 8 * ig + a;
@@ -216,8 +217,17 @@ __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const 
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int j = 4 * jg + b;
-            if (j < nc) B[(c0 + j) + i * ldb] = acc[a][b];
+            if (j < nc) {
+                B[(c0 + j) + i * ldb] = acc[a][b];
+                mx = fmaxf(mx, fabsf(acc[a][b]));
+            }
         }
+    }
+    if (maxbits) {
+        // the largest |entry| written, folded into *maxbits for the next update's exponent
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        if ((t & 63) == 0) atomicMax(maxbits, __float_as_uint(mx));
     }
 }
 
@@ -725,7 +735,8 @@ elx_status_t launch_iota(hipStream_t s, i64 n, int32_t* p) {
     return ELX_SUCCESS;
 }
 
-elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb) {
+elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv, float* B, i64 ldb,
+                             unsigned* maxbits) {
     ELX_ARG(nb >= 0 && nb <= 128 && ncols >= 0 && ldb >= nb);
     if (nb == 0 || ncols == 0) return ELX_SUCCESS;
     ProfScope prof_(s, kKTrsm, (double)nb * (double)nb * (double)ncols);
@@ -738,7 +749,7 @@ elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv,
         attr = true;
     }
     hipLaunchKernelGGL(trmm_inv_kernel, dim3((unsigned)cdiv(ncols, 64)), dim3(256), lds, s, (int)nb,
-                       ncols, Inv, B, ldb);
+                       ncols, Inv, B, ldb, maxbits);
     ELX_HIP(hipGetLastError());
     return ELX_SUCCESS;
 }
