@@ -221,11 +221,16 @@ constexpr int TNT = 256;         // threads per workgroup
 constexpr int TV = TNT / TP;     // vectors per workgroup
 
 // maxbits (may be null): the largest |entry| written is folded into *maxbits (the bits of a
-// non-negative float, atomic max), for the exponent of a low-precision product that follows.
+// non-negative float, atomic max), for the exponent of a low-precision product that follows;
+// one atomic per workgroup (called by every thread of the TNT-thread workgroup).
 __device__ __forceinline__ void fold_absmax(float mx, unsigned* maxbits) {
+    __shared__ float wmx[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(mx));
+    if ((threadIdx.x & 63) == 0) wmx[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(maxbits, __float_as_uint(fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3]))));
 }
 
 template <typename T>

@@ -224,10 +224,15 @@ __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const 
         }
     }
     if (maxbits) {
-        // the largest |entry| written, folded into *maxbits for the next update's exponent
+        // the largest |entry| written, folded into *maxbits for the next update's exponent:
+        // one atomic per workgroup (the grid's atomics on the one word serialise)
+        __shared__ float wmx[4];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-        if ((t & 63) == 0) atomicMax(maxbits, __float_as_uint(mx));
+        if ((t & 63) == 0) wmx[t >> 6] = mx;
+        __syncthreads();
+        if (t == 0)
+            atomicMax(maxbits, __float_as_uint(fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3]))));
     }
 }
 
