@@ -174,55 +174,67 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(i64 n, const float* _
 // columns >= nb zero). This is the triangular solve L11^{-1} A12 of the blocked LU as a product
 // with the precomputed inverse: every column's result depends only on that column, so a
 // workgroup reads its 64 columns (contiguous in every row) into LDS, multiplies, and writes
-// them back in place. Thread (jg, ig) forms an 8 x 4 block of the result: rows 8*ig..,
-// columns 4*jg.., so the 16 lanes of a row group write 256 contiguous bytes.
+// them back in place, the product on the FP32 matrix cores (v_mfma_f32_16x16x4f32); the 16
+// lanes of a result row group write 64 contiguous bytes.
+constexpr int kTrmmLdi = 132, kTrmmLdb = 68;  // padded LDS strides (no bank conflicts)
 __global__ __launch_bounds__(256) void trmm_inv_kernel(int nb, i64 ncols, const float* __restrict__ Inv,
                                                        float* B, i64 ldb, unsigned* maxbits) {
     typedef float f4v __attribute__((ext_vector_type(4)));
     extern __shared__ __align__(16) float sm[];
-    float* Is = sm;               // 128 x 128, Is[i + k*128]
-    float* Bs = sm + 128 * 128;   // 128 x 64, Bs[k*64 + j]
+    float* Is = sm;                        // 128 x 128, Is[i + k*kTrmmLdi] = Inv(i, k)
+    float* Bs = sm + 128 * kTrmmLdi;       // 128 x 64, Bs[k*kTrmmLdb + j]
     const int t = threadIdx.x;
     const i64 c0 = (i64)blockIdx.x * 64;
     const int nc = (int)std::min<i64>(64, ncols - c0);
-    for (int idx = t; idx < 128 * 128 / 4; idx += 256)
-        reinterpret_cast<f4v*>(Is)[idx] = reinterpret_cast<const f4v*>(Inv)[idx];
+    for (int idx = t; idx < 128 * 128 / 4; idx += 256) {
+        const int i4 = idx & 31, k = idx >> 5;
+        *reinterpret_cast<f4v*>(Is + 4 * i4 + k * kTrmmLdi) = reinterpret_cast<const f4v*>(Inv)[idx];
+    }
     for (int idx = t; idx < 128 * 64; idx += 256) {
         const int j = idx & 63, k = idx >> 6;
-        Bs[k * 64 + j] = (k < nb && j < nc) ? B[(c0 + j) + k * ldb] : 0.0f;
+        Bs[k * kTrmmLdb + j] = (k < nb && j < nc) ? B[(c0 + j) + k * ldb] : 0.0f;
     }
     __syncthreads();
-    const int jg = t & 15, ig = t >> 4;  // columns 4*jg.., rows 8*ig..
-    float acc[8][4];
+    // X = Inv B on the FP32 matrix cores: wave w owns rows 32w .. 32w+31 (two 16-row tiles) and
+    // the 64 columns (four tiles); MFMA operands Inv(i, k) with lane i = l%16, k = l/16, and
+    // B(k, j) with j = l%16
+    const int lane = t & 63, wave = t >> 6, l16 = lane & 15, lq = lane >> 4;
+    f4v acc[2][4];
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
-#pragma unroll 4
-    for (int k = 0; k < 128; ++k) {
-        const f4v i0 = *reinterpret_cast<const f4v*>(Is + k * 128 + 8 * ig);
-        const f4v i1 = *reinterpret_cast<const f4v*>(Is + k * 128 + 8 * ig + 4);
-        const f4v bv = *reinterpret_cast<const f4v*>(Bs + k * 64 + 4 * jg);
-        const float iv[8] = {i0[0], i0[1], i0[2], i0[3], i1[0], i1[1], i1[2], i1[3]};
+        for (int ct = 0; ct < 4; ++ct) acc[u][ct] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+    for (int k0 = 0; k0 < 128; k0 += 4) {
+        const int k = k0 + lq;
+        float a[2], b[4];
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+        for (int u = 0; u < 2; ++u) a[u] = Is[(32 * wave + 16 * u + l16) + k * kTrmmLdi];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fmaf(iv[a], bv[b], acc[a][b]);
+        for (int ct = 0; ct < 4; ++ct) b[ct] = Bs[k * kTrmmLdb + 16 * ct + l16];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+                acc[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[ct], acc[u][ct], 0, 0, 0);
     }
+    // lane l, register r: row 16*tile + 4*(l/16) + r, column 16*ct + l%16
     float mx = 0.0f;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const int i = 8 * ig + a;
-        if (i >= nb) continue;
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int j = 4 * jg + b;
-            if (j < nc) {
-                B[(c0 + j) + i * ldb] = acc[a][b];
-                mx = fmaxf(mx, fabsf(acc[a][b]));
+        for (int r = 0; r < 4; ++r) {
+            const int i = 32 * wave + 16 * u + 4 * lq + r;
+            if (i >= nb) continue;
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const int j = 16 * ct + l16;
+                if (j < nc) {
+                    B[(c0 + j) + i * ldb] = acc[u][ct][r];
+                    mx = fmaxf(mx, fabsf(acc[u][ct][r]));
+                }
             }
         }
-    }
     if (maxbits) {
         // the largest |entry| written, folded into *maxbits for the next update's exponent:
         // one atomic per workgroup (the grid's atomics on the one word serialise)
@@ -745,7 +757,7 @@ elx_status_t launch_trmm_inv(hipStream_t s, i64 nb, i64 ncols, const float* Inv,
     ELX_ARG(nb >= 0 && nb <= 128 && ncols >= 0 && ldb >= nb);
     if (nb == 0 || ncols == 0) return ELX_SUCCESS;
     ProfScope prof_(s, kKTrsm, (double)nb * (double)nb * (double)ncols);
-    const size_t lds = (128 * 128 + 128 * 64) * sizeof(float);
+    const size_t lds = (128 * kTrmmLdi + 128 * kTrmmLdb) * sizeof(float);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trmm_inv_kernel),
