@@ -156,63 +156,145 @@ __global__ void gather_rows_kernel(const int32_t* __restrict__ perm, i64 m, i64 
 // k -- b_i = fma(-x_k, l_ik, b_i), the column-oriented substitution's order -- so the result
 // is bitwise that of trsm_small_kernel and the restated BLAS; the chain is 4 blocks of 32
 // steps instead of 128 dependent steps.
+// One 32-row block of trsm_llnu_kernel: wave 0 solves the diagonal block, then every wave
+// subtracts its contribution from the rows below, two rows per pass. FULL (rb == 32) drops the
+// per-row guards, so the fully unrolled loops carry no run-time branch and their LDS reads are
+// issued in batches rather than one round trip per FMA.
+constexpr int kLlnuXL = 65, kLlnuPL = 33;
+template <typename T, bool FULL>
+__device__ __forceinline__ void llnu_block(int r0, int rb, int rows, int lane, int wave, T* Xs,
+                                           const T* Ps, const T* __restrict__ A, i64 lda) {
+    constexpr int XL = kLlnuXL, PL = kLlnuPL;
+    if (wave == 0) {
+        T x[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) x[i] = (FULL || i < rb) ? Xs[(r0 + i) * XL + lane] : T(0);
+        // full blocks read the diagonal block's entries straight from A: their addresses are
+        // uniform, so they come through the scalar cache into SGPRs, and the 496 FMAs wait on
+        // no LDS round trips (one wave per SIMD here: nothing else would hide them)
+        const T* Ad = A + r0 + (i64)r0 * lda;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+#pragma unroll
+            for (int i = k + 1; i < 32; ++i)
+                if (FULL || i < rb)
+                    x[i] = __builtin_fma(-x[k], FULL ? Ad[i + (i64)k * lda] : Ps[i * PL + k], x[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            if (FULL || i < rb) Xs[(r0 + i) * XL + lane] = x[i];
+    }
+    __syncthreads();
+    if (rb < rows) {
+        T xb[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) xb[k] = (FULL || k < rb) ? Xs[(r0 + k) * XL + lane] : T(0);
+        int i = rb + wave;
+        for (; i + 4 < rows; i += 8) {
+            T a0 = Xs[(r0 + i) * XL + lane], a1 = Xs[(r0 + i + 4) * XL + lane];
+            const T* p0 = Ps + i * PL;
+            const T* p1 = p0 + 4 * PL;
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                if (FULL || k < rb) {
+                    a0 = __builtin_fma(-xb[k], p0[k], a0);
+                    a1 = __builtin_fma(-xb[k], p1[k], a1);
+                }
+            Xs[(r0 + i) * XL + lane] = a0;
+            Xs[(r0 + i + 4) * XL + lane] = a1;
+        }
+        if (i < rows) {
+            T acc = Xs[(r0 + i) * XL + lane];
+            const T* pi = Ps + i * PL;
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                if (FULL || k < rb) acc = __builtin_fma(-xb[k], pi[k], acc);
+            Xs[(r0 + i) * XL + lane] = acc;
+        }
+    }
+}
+
+// Left, lower, no transpose, unit diagonal, order m <= 128, many right-hand sides: the LU's
+// U12 = L11^{-1} A12 (Trsm(LEFT,LOWER,NORMAL,UNIT), src/lapack_like/factor/LU.cpp:139) and the
+// row-echelon form's B update. 64 columns per workgroup in LDS, 32-row blocks: per block the
+// block's column panel of L (its rows r0.., 32 columns) is staged, wave 0 solves the diagonal
+// block (a column per lane, 32 values in registers), and all waves subtract its contribution
+// from the rows below. Every entry still receives its updates one FMA at a time in increasing
+// k -- b_i = fma(-x_k, l_ik, b_i), the column-oriented substitution's order -- so the result
+// is bitwise that of trsm_small_kernel and the restated BLAS; the chain is 4 blocks of 32
+// steps instead of 128 dependent steps.
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_llnu_kernel(int m, i64 ncols, const T* __restrict__ A,
                                                         i64 lda, T* B, i64 ldb,
                                                         const int32_t* __restrict__ stop) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int XL = 65, PL = 33;
-    T* Xs = reinterpret_cast<T*>(smem_raw);  // 128 x 65: Xs[i*XL + c]
-    T* Ps = Xs + 128 * XL;                   // 128 x 33: Ps[i*PL + kk] = A(r0 + i, r0 + kk)
+    constexpr int XL = kLlnuXL, PL = kLlnuPL;
+    // Ps first: its entries' LDS addresses (< 34 KB) fit the 16-bit immediate offset of
+    // ds_read, so the unrolled loops address them from one register instead of materialising
+    // an address per entry
+    T* Ps = reinterpret_cast<T*>(smem_raw);  // 128 x 33: Ps[i*PL + kk] = A(r0 + i, r0 + kk)
+    T* Xs = Ps + 128 * PL;                   // 128 x 65: Xs[i*XL + c]
     if (stop && *stop != 0) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const i64 c0 = (i64)blockIdx.x * 64;
     const int nc = (int)((ncols - c0) < 64 ? (ncols - c0) : 64);
-    for (int idx = t; idx < m * 64; idx += 256) {
-        const int i = idx % m, c = idx / m;  // down the columns: coalesced
-        Xs[i * XL + c] = (c < nc) ? B[i + (c0 + c) * ldb] : T(0);
+    // Staging loads are issued in unrolled batches before their LDS stores: with one wave per
+    // SIMD nothing else hides a global load's latency, so a load-store-load loop would pay it
+    // once per element.
+    if (m == 128) {
+        T v[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int idx = t + 256 * u, i = idx & 127, c = idx >> 7;  // down the columns
+            v[u] = (c < nc) ? B[i + (c0 + c) * ldb] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int idx = t + 256 * u;
+            Xs[(idx & 127) * XL + (idx >> 7)] = v[u];
+        }
+    } else {
+        for (int idx = t; idx < m * 64; idx += 256) {
+            const int i = idx % m, c = idx / m;  // down the columns: coalesced
+            Xs[i * XL + c] = (c < nc) ? B[i + (c0 + c) * ldb] : T(0);
+        }
     }
     for (int r0 = 0; r0 < m; r0 += 32) {
         const int rb = min(32, m - r0), rows = m - r0;
         __syncthreads();  // Xs staged, or the previous block's updates done with Ps
-        for (int idx = t; idx < rows * rb; idx += 256) {
-            const int i = idx % rows, kk = idx / rows;
-            Ps[i * PL + kk] = A[(r0 + i) + (i64)(r0 + kk) * lda];
-        }
-        __syncthreads();
-        if (wave == 0) {
-            T x[32];
+        {
+            // thread t stages row t % 128 of the block's column panel, every other column
+            const int i = t & 127, kh = t >> 7;
+            if (i < rows) {
+                T v[16];
 #pragma unroll
-            for (int i = 0; i < 32; ++i) x[i] = (i < rb) ? Xs[(r0 + i) * XL + lane] : T(0);
+                for (int u = 0; u < 16; ++u) {
+                    const int kk = kh + 2 * u;
+                    v[u] = (kk < rb) ? A[(r0 + i) + (i64)(r0 + kk) * lda] : T(0);
+                }
 #pragma unroll
-            for (int k = 0; k < 32; ++k) {
-#pragma unroll
-                for (int i = k + 1; i < 32; ++i)
-                    if (i < rb) x[i] = __builtin_fma(-x[k], Ps[i * PL + k], x[i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 32; ++i)
-                if (i < rb) Xs[(r0 + i) * XL + lane] = x[i];
-        }
-        __syncthreads();
-        if (rb < rows) {
-            T xb[32];
-#pragma unroll
-            for (int k = 0; k < 32; ++k) xb[k] = (k < rb) ? Xs[(r0 + k) * XL + lane] : T(0);
-            for (int i = rb + wave; i < rows; i += 4) {
-                T acc = Xs[(r0 + i) * XL + lane];
-                const T* pi = Ps + i * PL;
-#pragma unroll
-                for (int k = 0; k < 32; ++k)
-                    if (k < rb) acc = __builtin_fma(-xb[k], pi[k], acc);
-                Xs[(r0 + i) * XL + lane] = acc;
+                for (int u = 0; u < 16; ++u) {
+                    const int kk = kh + 2 * u;
+                    if (kk < rb) Ps[i * PL + kk] = v[u];
+                }
             }
         }
+        __syncthreads();
+        if (rb == 32) llnu_block<T, true>(r0, rb, rows, lane, wave, Xs, Ps, A, lda);
+        else llnu_block<T, false>(r0, rb, rows, lane, wave, Xs, Ps, A, lda);
     }
     __syncthreads();
-    for (int idx = t; idx < m * 64; idx += 256) {
-        const int i = idx % m, c = idx / m;
-        if (c < nc) B[i + (c0 + c) * ldb] = Xs[i * XL + c];
+    if (m == 128) {
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int idx = t + 256 * u, i = idx & 127, c = idx >> 7;
+            if (c < nc) B[i + (c0 + c) * ldb] = Xs[i * XL + c];
+        }
+    } else {
+        for (int idx = t; idx < m * 64; idx += 256) {
+            const int i = idx % m, c = idx / m;
+            if (c < nc) B[i + (c0 + c) * ldb] = Xs[i * XL + c];
+        }
     }
 }
 
@@ -638,6 +720,67 @@ elx_status_t launch_gather_rows(hipStream_t s, const int32_t* perm, i64 m, i64 n
     return gather_rows_impl<float>(s, perm, m, n, Bin, ldin, Bout, ldout);
 }
 
+// One right-hand side, left side, column-oriented substitution (M = op(A) lower, forward;
+// or M = A upper, backward): lu::SolveAfter's diagonal blocks (src/lapack_like/solve/LU.cpp
+// Trsm(LEFT,LOWER,NORMAL,UNIT) and Trsm(LEFT,UPPER,NORMAL,NON_UNIT)) at one column. The
+// block is staged in LDS by the whole workgroup; then one wave solves, lane q holding rows q
+// and q + 64: x_j reaches every lane by readlane (no cross-lane shuffle through LDS), and each
+// lane reads its two entries of M's column j, which do not depend on x and are issued ahead.
+// Every entry receives the same FMAs in the same order as trsm_small_kernel's
+// column-oriented path (b_i = fma(-m_ij, x_j, b_i) in step order, x_j = b_j / m_jj for a
+// non-unit diagonal), so the two agree bit for bit.
+template <typename T>
+__global__ __launch_bounds__(256) void trsv_col_kernel(int m, const T* __restrict__ A, i64 lda,
+                                                       T* B, int lower_eff, int mtrans, int unit,
+                                                       const int32_t* __restrict__ stop) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    if (stop && *stop != 0) return;
+    T* Ms = reinterpret_cast<T*>(smem_raw);  // Ms[i + j*128] = op(A)(i, j)
+    const int t = threadIdx.x;
+    for (int idx = t; idx < m * m; idx += 256) {
+        const int i = idx % m, j = idx / m;  // walk A's columns: coalesced reads
+        Ms[mtrans ? (j + i * 128) : (i + j * 128)] = A[i + (i64)j * lda];
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    const int q = t;
+    const bool h0 = q < m, h1 = q + 64 < m;
+    T x0 = h0 ? B[q] : T(0), x1 = h1 ? B[q + 64] : T(0);
+    if (lower_eff) {
+        for (int j = 0; j < m; ++j) {
+            const T* col = Ms + j * 128;
+            const T m0 = col[q], m1 = col[q + 64];
+            T xj = bcast_lane(j < 64 ? x0 : x1, j & 63);
+            if (!unit) {
+                xj = xj / col[j];
+                if (q == (j & 63)) {
+                    if (j < 64) x0 = xj;
+                    else x1 = xj;
+                }
+            }
+            if (h0 && q > j) x0 = __builtin_fma(-m0, xj, x0);
+            if (h1 && q + 64 > j) x1 = __builtin_fma(-m1, xj, x1);
+        }
+    } else {
+        for (int j = m - 1; j >= 0; --j) {
+            const T* col = Ms + j * 128;
+            const T m0 = col[q], m1 = col[q + 64];
+            T xj = bcast_lane(j < 64 ? x0 : x1, j & 63);
+            if (!unit) {
+                xj = xj / col[j];
+                if (q == (j & 63)) {
+                    if (j < 64) x0 = xj;
+                    else x1 = xj;
+                }
+            }
+            if (q < j) x0 = __builtin_fma(-m0, xj, x0);
+            if (h1 && q + 64 < j) x1 = __builtin_fma(-m1, xj, x1);
+        }
+    }
+    if (h0) B[q] = x0;
+    if (h1) B[q + 64] = x1;
+}
+
 template <typename T>
 static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orient, int diag,
                                     i64 m, i64 n, const T* A, i64 lda, T* B, i64 ldb,
@@ -665,6 +808,22 @@ static elx_status_t trsm_small_impl(hipStream_t s, int side, int uplo, int orien
         }
         hipLaunchKernelGGL(trsm_llnu_kernel<T>, dim3(cdiv(nvec, 64)), dim3(256), lds2, s,
                            (int)order, nvec, A, lda, B, ldb, stop);
+        ELX_HIP(hipGetLastError());
+        return ELX_SUCCESS;
+    }
+    if (!right && nvec == 1 && (lower_eff || !mtrans) && !maxbits) {
+        // one right-hand side (the solve's diagonal blocks): same FMA order per entry
+        const size_t lds1 = sizeof(T) * 128 * 128;
+        ProfScope prof1_(s, kKTrsm, (double)order * (double)order);
+        static bool attr1 = false;
+        if (!attr1) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_col_kernel<T>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+            (void)hipGetLastError();
+            attr1 = true;
+        }
+        hipLaunchKernelGGL(trsv_col_kernel<T>, dim3(1), dim3(256), lds1, s, (int)order, A, lda,
+                           B, lower_eff ? 1 : 0, mtrans ? 1 : 0, diag == kUnit ? 1 : 0, stop);
         ELX_HIP(hipGetLastError());
         return ELX_SUCCESS;
     }

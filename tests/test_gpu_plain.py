@@ -70,20 +70,21 @@ def test_trsm_small_bitwise_and_blocked(dev, orc, side):
         for uplo in (0, 1):
             for orient in (0, 1):
                 for diag in (0, 1):
-                    say("trsm", sides[side], order, uplo, orient, diag)
                     T = rng.standard_normal((order, order)) + 4 * np.eye(order)
-                    other = 45
-                    B = rng.standard_normal((order, other) if side == 0 else (other, order))
-                    dB = dev.to_device(B)
-                    dev.trsm(side, uplo, orient, diag, 1.0, dev.to_device(T), dB)
-                    got = dev.to_host(dB)
-                    ref = orc.trsm("LR"[side], "LU"[uplo], "NT"[orient], "NU"[diag], 1.0,
-                                   np.asfortranarray(T), np.asfortranarray(B))
-                    if order <= 128:
-                        assert np.array_equal(got, ref), "small triangular solve must match the restated BLAS bit for bit"
-                    else:
-                        scale = np.abs(ref).max()
-                        assert np.abs(got - ref).max() <= 1e-11 * scale
+                    # 45 right-hand sides, and one (the solve's one-column kernel on the left)
+                    for other in (45, 1):
+                        say("trsm", sides[side], order, uplo, orient, diag, other)
+                        B = rng.standard_normal((order, other) if side == 0 else (other, order))
+                        dB = dev.to_device(B)
+                        dev.trsm(side, uplo, orient, diag, 1.0, dev.to_device(T), dB)
+                        got = dev.to_host(dB)
+                        ref = orc.trsm("LR"[side], "LU"[uplo], "NT"[orient], "NU"[diag], 1.0,
+                                       np.asfortranarray(T), np.asfortranarray(B))
+                        if order <= 128:
+                            assert np.array_equal(got, ref), "small triangular solve must match the restated BLAS bit for bit"
+                        else:
+                            scale = np.abs(ref).max()
+                            assert np.abs(got - ref).max() <= 1e-11 * scale
 
 
 def check_lu(dev, orc, A, nb=128, expect_exact=False):
