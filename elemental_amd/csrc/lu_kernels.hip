@@ -157,28 +157,40 @@ __global__ void gather_rows_kernel(const int32_t* __restrict__ perm, i64 m, i64 
 // is bitwise that of trsm_small_kernel and the restated BLAS; the chain is 4 blocks of 32
 // steps instead of 128 dependent steps.
 // One 32-row block of trsm_llnu_kernel: wave 0 solves the diagonal block, then every wave
-// subtracts its contribution from the rows below, two rows per pass. FULL (rb == 32) drops the
-// per-row guards, so the fully unrolled loops carry no run-time branch and their LDS reads are
-// issued in batches rather than one round trip per FMA.
+// subtracts its contribution from the rows below, four (then two, then one) rows per pass.
+// FULL (rb == 32) drops the per-row guards, so the fully unrolled loops carry no run-time
+// branch and their LDS reads are issued in batches rather than one round trip per FMA.
 constexpr int kLlnuXL = 65, kLlnuPL = 33;
 template <typename T, bool FULL>
 __device__ __forceinline__ void llnu_block(int r0, int rb, int rows, int lane, int wave, T* Xs,
-                                           const T* Ps, const T* __restrict__ A, i64 lda) {
+                                           const T* Ps) {
     constexpr int XL = kLlnuXL, PL = kLlnuPL;
     if (wave == 0) {
         T x[32];
 #pragma unroll
         for (int i = 0; i < 32; ++i) x[i] = (FULL || i < rb) ? Xs[(r0 + i) * XL + lane] : T(0);
-        // full blocks read the diagonal block's entries straight from A: their addresses are
-        // uniform, so they come through the scalar cache into SGPRs, and the 496 FMAs wait on
-        // no LDS round trips (one wave per SIMD here: nothing else would hide them)
-        const T* Ad = A + r0 + (i64)r0 * lda;
+        if (FULL) {
+            // column k + 1's entries are read while step k's FMAs run (one wave per SIMD here:
+            // nothing else hides an LDS round trip)
+            T cur[32], nxt[32];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) {
+            for (int i = 1; i < 32; ++i) cur[i] = Ps[i * PL];
 #pragma unroll
-            for (int i = k + 1; i < 32; ++i)
-                if (FULL || i < rb)
-                    x[i] = __builtin_fma(-x[k], FULL ? Ad[i + (i64)k * lda] : Ps[i * PL + k], x[i]);
+            for (int k = 0; k < 32; ++k) {
+#pragma unroll
+                for (int i = k + 2; i < 32; ++i) nxt[i] = Ps[i * PL + k + 1];
+#pragma unroll
+                for (int i = k + 1; i < 32; ++i) x[i] = __builtin_fma(-x[k], cur[i], x[i]);
+#pragma unroll
+                for (int i = k + 2; i < 32; ++i) cur[i] = nxt[i];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+#pragma unroll
+                for (int i = k + 1; i < 32; ++i)
+                    if (i < rb) x[i] = __builtin_fma(-x[k], Ps[i * PL + k], x[i]);
+            }
         }
 #pragma unroll
         for (int i = 0; i < 32; ++i)
@@ -190,6 +202,20 @@ __device__ __forceinline__ void llnu_block(int r0, int rb, int rows, int lane, i
 #pragma unroll
         for (int k = 0; k < 32; ++k) xb[k] = (FULL || k < rb) ? Xs[(r0 + k) * XL + lane] : T(0);
         int i = rb + wave;
+        for (; i + 12 < rows; i += 16) {
+            T a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = Xs[(r0 + i + 4 * u) * XL + lane];
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                if (FULL || k < rb) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        a[u] = __builtin_fma(-xb[k], Ps[(i + 4 * u) * PL + k], a[u]);
+                }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Xs[(r0 + i + 4 * u) * XL + lane] = a[u];
+        }
         for (; i + 4 < rows; i += 8) {
             T a0 = Xs[(r0 + i) * XL + lane], a1 = Xs[(r0 + i + 4) * XL + lane];
             const T* p0 = Ps + i * PL;
@@ -280,8 +306,8 @@ __global__ __launch_bounds__(256) void trsm_llnu_kernel(int m, i64 ncols, const 
             }
         }
         __syncthreads();
-        if (rb == 32) llnu_block<T, true>(r0, rb, rows, lane, wave, Xs, Ps, A, lda);
-        else llnu_block<T, false>(r0, rb, rows, lane, wave, Xs, Ps, A, lda);
+        if (rb == 32) llnu_block<T, true>(r0, rb, rows, lane, wave, Xs, Ps);
+        else llnu_block<T, false>(r0, rb, rows, lane, wave, Xs, Ps);
     }
     __syncthreads();
     if (m == 128) {
