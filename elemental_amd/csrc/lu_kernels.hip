@@ -763,9 +763,27 @@ __global__ __launch_bounds__(256) void trsv_col_kernel(int m, const T* __restric
     if (stop && *stop != 0) return;
     T* Ms = reinterpret_cast<T*>(smem_raw);  // Ms[i + j*128] = op(A)(i, j)
     const int t = threadIdx.x;
-    for (int idx = t; idx < m * m; idx += 256) {
-        const int i = idx % m, j = idx / m;  // walk A's columns: coalesced reads
-        Ms[mtrans ? (j + i * 128) : (i + j * 128)] = A[i + (i64)j * lda];
+    if (m == 128) {
+        // loads issued 32 at a time before their LDS stores (the block is rarely in cache)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            T v[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + 256 * (32 * h + u), i = idx & 127, j = idx >> 7;
+                v[u] = A[i + (i64)j * lda];
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + 256 * (32 * h + u), i = idx & 127, j = idx >> 7;
+                Ms[mtrans ? (j + i * 128) : (i + j * 128)] = v[u];
+            }
+        }
+    } else {
+        for (int idx = t; idx < m * m; idx += 256) {
+            const int i = idx % m, j = idx / m;  // walk A's columns: coalesced reads
+            Ms[mtrans ? (j + i * 128) : (i + j * 128)] = A[i + (i64)j * lda];
+        }
     }
     __syncthreads();
     if (t >= 64) return;
