@@ -359,12 +359,31 @@ __global__ __launch_bounds__(TNT) void trsm_small_kernel(int m, i64 nvec, const 
     const int v = t / TP, q = t % TP;
     const i64 v0 = (i64)blockIdx.x * TV;
     // The triangle is staged in LDS once, so each of the m dependent steps below waits on
-    // LDS rather than on the cache hierarchy.
-    for (int idx = t; idx < m * m; idx += TNT) {
-        const int i = idx % m, j = idx / m;  // walk A's columns: coalesced reads
-        Ms[mtrans ? (j + i * xm) : (i + j * xm)] = A[i + (i64)j * lda];
+    // LDS rather than on the cache hierarchy. At order 128 the loads are issued 32 at a time
+    // before their LDS stores (a load-store loop would wait out each load's latency in turn).
+    if (m == 128) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            T v[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + TNT * (32 * h + u), i = idx & 127, j = idx >> 7;
+                v[u] = A[i + (i64)j * lda];
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + TNT * (32 * h + u), i = idx & 127, j = idx >> 7;
+                Ms[mtrans ? (j + i * xm) : (i + j * xm)] = v[u];
+            }
+        }
+    } else {
+        for (int idx = t; idx < m * m; idx += TNT) {
+            const int i = idx % m, j = idx / m;  // walk A's columns: coalesced reads
+            Ms[mtrans ? (j + i * xm) : (i + j * xm)] = A[i + (i64)j * lda];
+        }
     }
     // Coalesced staging: left side walks down each column, right side across rows.
+#pragma unroll 8
     for (int idx = t; idx < m * TV; idx += TNT) {
         int i, vv;
         if (!right) {
