@@ -258,9 +258,28 @@ __global__ __launch_bounds__(NT) void calu_panel_kernel(CaluArgs p) {
         for (int c = t; c < W; c += NT) p.dests[c] = (int32_t)(p.goff + c);
         return;
     }
-    for (int idx = t; idx < rows * W; idx += NT) {
-        const int j = idx % W, r = idx / W;
-        tile[r * LD + j] = p.A[(row0 + r) * p.lda + j];
+    if (W == 128 && rows == 128) {
+        // the full tile: loads issued 32 at a time before their LDS stores (a load-store loop
+        // would wait out each load's latency in turn)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float v[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + NT * (32 * h + u), j = idx & 127, r = idx >> 7;
+                v[u] = p.A[(row0 + r) * p.lda + j];
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + NT * (32 * h + u), j = idx & 127, r = idx >> 7;
+                tile[r * LD + j] = v[u];
+            }
+        }
+    } else {
+        for (int idx = t; idx < rows * W; idx += NT) {
+            const int j = idx % W, r = idx / W;
+            tile[r * LD + j] = p.A[(row0 + r) * p.lda + j];
+        }
     }
     for (int r = t; r < RMAX; r += NT) lbl[r] = (r < rows) ? (int)(row0 + r) : INT_MIN;
     if (t < 16) misc[t] = 0;
@@ -718,9 +737,17 @@ __global__ __launch_bounds__(NT) void calu_panel_kernel(CaluArgs p) {
         return;
     }
     // rows to their final positions; the moved-row list for the other columns
-    for (int idx = t; idx < rows * W; idx += NT) {
-        const int j = idx % W, r = idx / W;
-        p.A[(i64)lbl[r] * p.lda + j] = tile[r * LD + j];
+    if (W == 128 && rows == 128) {
+#pragma unroll 16
+        for (int u = 0; u < 64; ++u) {
+            const int idx = t + NT * u, j = idx & 127, r = idx >> 7;
+            p.A[(i64)lbl[r] * p.lda + j] = tile[r * LD + j];
+        }
+    } else {
+        for (int idx = t; idx < rows * W; idx += NT) {
+            const int j = idx % W, r = idx / W;
+            p.A[(i64)lbl[r] * p.lda + j] = tile[r * LD + j];
+        }
     }
     if (stop_k0 >= 0 && g == 0)
         for (int c = stop_k0 + t; c < W; c += NT) p.dests[c] = (int32_t)(p.goff + c);
