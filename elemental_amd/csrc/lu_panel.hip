@@ -250,7 +250,26 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         return;
     }
 
-    if (p.rowmajor) {
+    if (rows == 128 && W == 128) {
+        // the full tile: loads issued 32 at a time before their LDS stores (a load-store loop
+        // would wait out each load's latency in turn); consecutive threads walk a row (row
+        // major) or a column (column major)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            T v[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + NT * (32 * h + u), a = idx & 127, b = idx >> 7;
+                v[u] = p.rowmajor ? p.A[(row0 + b) * p.lda + a] : p.A[row0 + a + (i64)b * p.lda];
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int idx = t + NT * (32 * h + u), a = idx & 127, b = idx >> 7;
+                if (p.rowmajor) tile[b * LD + a] = v[u];
+                else tile[a * LD + b] = v[u];
+            }
+        }
+    } else if (p.rowmajor) {
         for (int idx = t; idx < rows * W; idx += NT) {
             const int j = idx % W, r = idx / W;
             tile[r * LD + j] = p.A[(row0 + r) * p.lda + j];
@@ -603,7 +622,14 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(PanelArgs<T> p) {
         return;
     }
     // Rows to their final positions; the moved-row list for the other columns.
-    if (p.rowmajor) {
+    if (rows == 128 && W == 128) {
+#pragma unroll 16
+        for (int u = 0; u < 64; ++u) {
+            const int idx = t + NT * u, a = idx & 127, b = idx >> 7;
+            if (p.rowmajor) p.A[(i64)lbl[b] * p.lda + a] = tile[b * LD + a];
+            else p.A[(i64)lbl[a] + (i64)b * p.lda] = tile[a * LD + b];
+        }
+    } else if (p.rowmajor) {
         for (int idx = t; idx < rows * W; idx += NT) {
             const int j = idx % W, r = idx / W;
             p.A[(i64)lbl[r] * p.lda + j] = tile[r * LD + j];
