@@ -90,27 +90,28 @@ __global__ __launch_bounds__(512) void row_moves_rowmajor_kernel(const int2* __r
 // thread replays them on an LDS copy of the index vector when it fits, else in global
 // memory.
 // ---------------------------------------------------------------------------------------
+constexpr int kSwapChunk = 1024;  // swap destinations staged in LDS per round
 __global__ __launch_bounds__(256) void swaps_to_perm_kernel(const int32_t* __restrict__ dests,
                                                             i64 nswaps, i64 size, int32_t* perm,
                                                             int inverse, int in_lds) {
     extern __shared__ int32_t sp[];
+    __shared__ int32_t dq[kSwapChunk];
     const int t = threadIdx.x;
     int32_t* p = in_lds ? sp : perm;
     for (i64 i = t; i < size; i += 256) p[i] = (int32_t)i;
-    __syncthreads();
-    if (t == 0) {
-        if (!inverse) {
-            for (i64 j = 0; j < nswaps; ++j) {
-                const i64 d = dests[j];
-                if (d != j && d >= 0 && d < size) {
-                    const int32_t a = p[j];
-                    p[j] = p[d];
-                    p[d] = a;
-                }
-            }
-        } else {
-            for (i64 j = nswaps - 1; j >= 0; --j) {
-                const i64 d = dests[j];
+    // The swaps are applied one after another by one thread; the whole workgroup first stages
+    // each chunk of destinations (in application order) in LDS, so the dependent chain waits
+    // on no global load.
+    for (i64 c0 = 0; c0 < nswaps; c0 += kSwapChunk) {
+        const int cn = (int)((nswaps - c0) < kSwapChunk ? (nswaps - c0) : kSwapChunk);
+        __syncthreads();  // p initialised, or the previous chunk applied
+        for (int u = t; u < cn; u += 256) dq[u] = dests[inverse ? (nswaps - 1 - (c0 + u)) : (c0 + u)];
+        __syncthreads();
+        if (t == 0) {
+#pragma unroll 4
+            for (int u = 0; u < cn; ++u) {
+                const i64 j = inverse ? (nswaps - 1 - (c0 + u)) : (c0 + u);
+                const i64 d = dq[u];
                 if (d != j && d >= 0 && d < size) {
                     const int32_t a = p[j];
                     p[j] = p[d];
@@ -724,12 +725,12 @@ elx_status_t launch_swaps_to_perm(hipStream_t s, const int32_t* dests, i64 nswap
     if (size <= 0) return ELX_SUCCESS;
     const DeviceInfo& di = device_info();
     const size_t need = sizeof(int32_t) * (size_t)size;
-    const int in_lds = need <= (size_t)di.max_lds_per_block ? 1 : 0;
+    const size_t dyn_max = (size_t)di.max_lds_per_block - sizeof(int32_t) * kSwapChunk;
+    const int in_lds = need <= dyn_max ? 1 : 0;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&swaps_to_perm_kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    di.max_lds_per_block);
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_max);
         (void)hipGetLastError();
         attr = true;
     }
